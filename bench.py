@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark of the rf=2 AMR inter-level transfer path on one MI355X.
+
+Step = one ghost-shell fill (3 points) of the whole uneven level: 512 fine
+boxes with extents from {8,...,64} per dimension, mixed VC/CC centring,
+natural-family orders 1/3/5, 10 FP64 grid functions, replayed as one
+captured hipGraph around the persistent batched kernel.  Inputs are resident
+in HBM before the timed region.
+
+Prints ONE JSON line.  `value` = fine points written per second (all boxes,
+all grid functions).  `roofline.achieved` = algorithmic bytes per launch
+(8 B per fine point written + 8 B per distinct coarse point of each box's
+shell CoarseBoxes, per grid function) / average launch time from HIP events
+on the launch stream.  `extra` carries the plain per-box path on the same
+level and the single-level (64 x 32^3) operators.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner first")
+    ap.add_argument("--cpu-boxes", type=int, default=48)
+    return ap.parse_args()
+
+
+def timed_events(fn, steps, stream):
+    import torch
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return [a.elapsed_time(b) for a, b in ev]
+
+
+def bench_uneven(args, stream, log):
+    import torch
+    from carpetx_amd import workloads
+    from carpetx_amd.batched import Graph, WorkList
+
+    t0 = time.time()
+    wlz = workloads.uneven_512(seed=0)
+    wl = WorkList()
+    for b in wlz["boxes"]:
+        wl.append_shell(b["op"], b["crse"], 0, b["fine"], 0, wlz["ncomp"], b["interior"])
+    with torch.cuda.stream(stream):
+        wl.upload(stream)
+        if args.tune:
+            wl.autotune(reps=3, stream=stream)
+    st = wl.stats()
+    log("uneven_512: %d patches, %d tiles, %.3e fine pts, %.3f GB algorithmic (setup %.1fs)"
+        % (st["patches"], st["tiles"], st["fine_points"], st["algo_bytes"] / 1e9,
+           time.time() - t0))
+
+    with torch.cuda.stream(stream):
+        graph = Graph(lambda: wl.run(stream), stream)
+        for _ in range(args.warmup):
+            graph.launch(stream)
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        times = timed_events(lambda: graph.launch(stream), args.steps, stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_start
+    ms = sum(times) / len(times)
+    res = dict(ms=ms, wall_ms=1e3 * wall / args.steps, stats=st)
+
+    # plain per-box path on the same level (one launch per slab)
+    nper = max(2, min(10, args.steps // 5))
+    with torch.cuda.stream(stream):
+        wl.run_per_patch(stream)
+        torch.cuda.synchronize()
+        tp = timed_events(lambda: wl.run_per_patch(stream), nper, stream)
+    res["per_box_ms"] = sum(tp) / len(tp)
+    return wlz, wl, res
+
+
+def bench_level64(args, stream, log):
+    import torch
+    from carpetx_amd import workloads
+    from carpetx_amd.batched import Graph, WorkList
+    lv = workloads.level_64()
+    out = {}
+    for name, d in lv.items():
+        wl = WorkList()
+        for fab, reg in zip(d["fines"], d["regions"]):
+            wl.append(d["op"], d["crse"], 0, fab, 0, d["ncomp"], reg)
+        with torch.cuda.stream(stream):
+            wl.upload(stream)
+            g = Graph(lambda: wl.run(stream), stream)
+            for _ in range(3):
+                g.launch(stream)
+            tb = timed_events(lambda: g.launch(stream), 20, stream)
+            wl.run_per_patch(stream)
+            tp = timed_events(lambda: wl.run_per_patch(stream), 10, stream)
+        st = wl.stats()
+        mb, mp = sum(tb) / len(tb), sum(tp) / len(tp)
+        out[name] = dict(
+            fine_pts=st["fine_points"], algo_MB=round(st["algo_bytes"] / 1e6, 2),
+            batched_ms=round(mb, 4), per_box_ms=round(mp, 4),
+            batched_pts_per_s=st["fine_points"] / (mb * 1e-3),
+            per_box_pts_per_s=st["fine_points"] / (mp * 1e-3),
+            batched_hbm_frac=round(st["algo_bytes"] / (mb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        log("level_64 %s: batched %.4f ms, per-box %.4f ms" % (name, mb, mp))
+    return out
+
+
+def cpu_baseline(wlz, nboxes, log):
+    """The oracle's OpenMP build (reference CI flags) on the shells of the
+    first `nboxes` boxes of the same level, host copies of the same data."""
+    import numpy as np
+    import oracle
+    from carpetx_amd.workloads import shell_slabs
+    items = []
+    npts = 0
+    for b in wlz["boxes"][:nboxes]:
+        op_d = b["op"]
+        op = oracle.lookup(op_d.prolongation_type, op_d.prolongation_order, op_d.indextype,
+                           kind="fast")
+        fab = b["fine"]
+        slabs = shell_slabs(fab.box, b["interior"])
+        # coarse sub-box covering every slab's CoarseBox
+        clo = [1 << 30] * 3
+        chi = [-(1 << 30)] * 3
+        for s in slabs:
+            lo, hi = oracle.coarse_box(op, s.lo, s.hi)
+            clo = [min(clo[d], lo[d]) for d in range(3)]
+            chi = [max(chi[d], hi[d]) for d in range(3)]
+        crse_dev = b["crse"]
+        sub = crse_dev.t[:, clo[2] - crse_dev.box.lo[2]:chi[2] - crse_dev.box.lo[2] + 1,
+                         clo[1] - crse_dev.box.lo[1]:chi[1] - crse_dev.box.lo[1] + 1,
+                         clo[0] - crse_dev.box.lo[0]:chi[0] - crse_dev.box.lo[0] + 1]
+        ch = oracle.Fab(clo, chi, fab.ncomp, np.ascontiguousarray(sub.cpu().numpy()))
+        fh = oracle.Fab(fab.box.lo, fab.box.hi, fab.ncomp)
+        for s in slabs:
+            items.append((op, ch, fh, s.lo, s.hi, fab.ncomp))
+            npts += s.numPts * fab.ncomp
+    keep = items  # noqa: F841 (host arrays must outlive the calls)
+    oracle.prolongate_many(items, kind="fast")  # warm-up
+    reps = []
+    t_total = time.perf_counter()
+    while len(reps) < 3 or (time.perf_counter() - t_total < 10 and len(reps) < 10):
+        t0 = time.perf_counter()
+        oracle.prolongate_many(items, kind="fast")
+        reps.append(time.perf_counter() - t0)
+    med = sorted(reps)[len(reps) // 2]
+    threads = oracle.num_threads("fast")
+    log("cpu baseline: %d boxes, %.3e pts, median %.3fs over %d runs, %d threads"
+        % (nboxes, npts, med, len(reps), threads))
+    return dict(value=npts / med, unit="fine pts/s", cores=threads, kind="port",
+                sample="ghost shells of the first %d of the 512 boxes, 10 GF, %d fine pts; "
+                       "oracle/liboracle_fast.so (-O3 -ffp-contract=fast -march=x86-64-v3 "
+                       "-fopenmp), median of %d runs" % (nboxes, npts, len(reps)))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the transfer would shard); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    def log(msg):
+        print("[bench] " + msg, file=sys.stderr, flush=True)
+
+    stream = torch.cuda.Stream()
+    wlz, wl, res = bench_uneven(args, stream, log)
+    st = res["stats"]
+    ms = res["ms"]
+    value = st["fine_points"] / (ms * 1e-3)
+    achieved = st["algo_bytes"] / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "fine pts/s + % of HBM peak, FP64, batched vs per-box, fwd and adjoint",
+        "value": value,
+        "unit": "fine pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded sin-product fields + 1e-6 noise, resident in HBM)",
+        "config": {
+            "workload": "uneven_512: ghost-shell fill (3 pts) of 512 boxes, extents "
+                        "{8..64}^3, mixed VC/CC, natural o1/3/5, 10 GF, one hipGraph",
+            "patches": st["patches"], "tiles": st["tiles"],
+            "fine_points": st["fine_points"], "algo_bytes": st["algo_bytes"],
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        },
+        "extra": {
+            "per_box_ms": round(res["per_box_ms"], 4),
+            "per_box_pts_per_s": st["fine_points"] / (res["per_box_ms"] * 1e-3),
+            "batched_over_per_box": round(res["per_box_ms"] / ms, 2),
+            "wall_ms_per_step": round(res["wall_ms"], 4),
+        },
+    }
+    if not args.no_extra:
+        line["extra"]["level_64"] = bench_level64(args, stream, log)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(wlz, args.cpu_boxes, log)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,107 @@
+"""Batched prolongation: a device-resident work list and hipGraph capture.
+
+Replaces the reference's loop of FillPatchInterp -> interp() calls over
+fine patches (CarpetX/src/fillpatch.cxx:116-119,179-180,213-216) by one
+persistent kernel over every (patch, component, tile) of a level.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import check, i3
+from .interp import _stream
+
+
+class WorkList:
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        check("cxp_worklist_create", _lib.lib().cxp_worklist_create(ctypes.byref(self.h)))
+        self._keep = []  # FArrayBoxes referenced by the list stay alive
+
+    def close(self):
+        if self.h:
+            _lib.lib().cxp_worklist_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def append(self, op, crse, crse_comp, fine, fine_comp, ncomp, region):
+        c, f = crse.c(), fine.c()
+        check("cxp_worklist_append", _lib.lib().cxp_worklist_append(
+            self.h, op.handle, ctypes.byref(c), crse_comp, ctypes.byref(f),
+            fine_comp, ncomp, i3(region.lo), i3(region.hi)))
+        self._keep += [crse, fine]
+
+    def append_shell(self, op, crse, crse_comp, fine, fine_comp, ncomp, interior):
+        """Ghost shell of ``fine``: its box minus ``interior``, as 6 slabs."""
+        c, f = crse.c(), fine.c()
+        check("cxp_worklist_append_shell", _lib.lib().cxp_worklist_append_shell(
+            self.h, op.handle, ctypes.byref(c), crse_comp, ctypes.byref(f),
+            fine_comp, ncomp, i3(interior.lo), i3(interior.hi)))
+        self._keep += [crse, fine]
+
+    def upload(self, stream=None):
+        check("cxp_worklist_upload", _lib.lib().cxp_worklist_upload(self.h, _stream(stream)))
+
+    def stats(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        check("cxp_worklist_stats", _lib.lib().cxp_worklist_stats(
+            self.h, *[ctypes.byref(x) for x in v]))
+        return dict(patches=v[0].value, tiles=v[1].value, fine_points=v[2].value,
+                    algo_bytes=v[3].value)
+
+    def run(self, stream=None):
+        check("cxp_prolongate_batched", _lib.lib().cxp_prolongate_batched(
+            self.h, _stream(stream)))
+
+    def run_per_patch(self, stream=None):
+        check("cxp_prolongate_per_patch", _lib.lib().cxp_prolongate_per_patch(
+            self.h, _stream(stream)))
+
+    def autotune(self, reps=3, stream=None):
+        check("cxp_autotune", _lib.lib().cxp_autotune(self.h, reps, _stream(stream)))
+
+
+class Graph:
+    """A captured fill sequence (hipGraph), replayed with ``launch``."""
+
+    def __init__(self, fn, stream):
+        self.h = ctypes.c_void_p()
+        s = _stream(stream)
+        check("cxp_graph_capture_begin", _lib.lib().cxp_graph_capture_begin(s))
+        try:
+            fn()
+        except Exception:
+            g = ctypes.c_void_p()
+            _lib.lib().cxp_graph_capture_end(s, ctypes.byref(g))
+            if g:
+                _lib.lib().cxp_graph_destroy(g)
+            raise
+        check("cxp_graph_capture_end", _lib.lib().cxp_graph_capture_end(s, ctypes.byref(self.h)))
+
+    def launch(self, stream=None):
+        check("cxp_graph_launch", _lib.lib().cxp_graph_launch(self.h, _stream(stream)))
+
+    def close(self):
+        if self.h:
+            _lib.lib().cxp_graph_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tune_save(path):
+    check("cxp_tune_save", _lib.lib().cxp_tune_save(path.encode()))
+
+
+def tune_load(path):
+    check("cxp_tune_load", _lib.lib().cxp_tune_load(path.encode()))

@@ -1,0 +1,942 @@
+// C ABI of libcxp.so (declarations and reference citations in include/cxp.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cxp.h"
+#include "kernels.hpp"
+#include "ops.hpp"
+#include "adjoint.hpp"
+
+using namespace cxp;
+
+struct cxp_op_s {
+  OpDesc d;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Operator handles: one immutable entry per (family, order, indextype).
+
+std::once_flag g_ops_once;
+cxp_op_s g_ops[9][8][8];
+bool g_ops_valid[9][8][8];
+
+void init_ops() {
+  for (int f = 0; f < 9; ++f)
+    for (int o = 0; o < 8; ++o)
+      for (int c = 0; c < 8; ++c) {
+        const int it[3] = {(c >> 2) & 1, (c >> 1) & 1, c & 1};
+        OpDesc d;
+        g_ops_valid[f][o][c] = op_lookup(kFamilies[f], o, it, &d) == 0;
+        if (g_ops_valid[f][o][c])
+          g_ops[f][o][c].d = d;
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Validation (the reference's asserts, as status codes)
+
+bool fab_ok(const cxp_fab *f) {
+  if (!f || !f->data || f->ncomp < 1 || f->comp_stride < 0)
+    return false;
+  for (int d = 0; d < 3; ++d)
+    if (f->hi[d] < f->lo[d])
+      return false;
+  const int64_t vol = int64_t(f->hi[0] - f->lo[0] + 1) *
+                      (f->hi[1] - f->lo[1] + 1) * (f->hi[2] - f->lo[2] + 1);
+  return f->ncomp == 1 || f->comp_stride >= vol;
+}
+
+bool box_in(const int lo[3], const int hi[3], const int blo[3],
+            const int bhi[3]) {
+  for (int d = 0; d < 3; ++d)
+    if (lo[d] < blo[d] || hi[d] > bhi[d])
+      return false;
+  return true;
+}
+
+DFab dfab(const cxp_fab *f, int comp) {
+  DFab v;
+  v.p = f->data + (int64_t)comp * f->comp_stride;
+  for (int d = 0; d < 3; ++d)
+    v.lo[d] = f->lo[d];
+  v.sy = f->hi[0] - f->lo[0] + 1;
+  v.sz = (long long)v.sy * (f->hi[1] - f->lo[1] + 1);
+  return v;
+}
+
+int check_prolong(const cxp_op_s *op, const cxp_fab *crse, int crse_comp,
+                  const cxp_fab *fine, int fine_comp, int ncomp,
+                  const int rlo[3], const int rhi[3], int per_group) {
+  if (!op || !rlo || !rhi || !fab_ok(crse) || !fab_ok(fine) || ncomp < 1)
+    return CXP_ERR_ARG;
+  if (crse_comp < 0 || fine_comp < 0 || crse_comp + ncomp > crse->ncomp ||
+      fine_comp + ncomp > fine->ncomp)
+    return CXP_ERR_COMP;
+  for (int d = 0; d < 3; ++d)
+    if (rhi[d] < rlo[d])
+      return CXP_ERR_ARG;
+  if (!box_in(rlo, rhi, fine->lo, fine->hi))
+    return CXP_ERR_REGION;
+  int clo[3], chi[3];
+  op_coarse_box(op->d, rlo, rhi, clo, chi);
+  if (!box_in(clo, chi, crse->lo, crse->hi))
+    return CXP_ERR_COARSE;
+  if (per_group && ncomp > 16)
+    return CXP_ERR_NCOMP;
+  return CXP_OK;
+}
+
+int hip_status(hipError_t e) { return e == hipSuccess ? CXP_OK : CXP_ERR_HIP; }
+
+// ---------------------------------------------------------------------------
+// Tile-shape selection
+
+struct TileCfg {
+  int tx, ty, ring;
+  int lx, ly, lr;
+};
+// Tile shapes (fine TX x TY per plane, ring depth R); every one fits the
+// kernel's LDS budget (checked at first use).
+const TileCfg kCfgs[] = {
+    {64, 8, 8, 6, 3, 3},   {64, 4, 8, 6, 2, 3},  {32, 16, 8, 5, 4, 3},
+    {32, 8, 8, 5, 3, 3},   {16, 32, 8, 4, 5, 3}, {16, 16, 8, 4, 4, 3},
+    {8, 64, 8, 3, 6, 3},   {8, 32, 8, 3, 5, 3},  {4, 128, 8, 2, 7, 3},
+    {4, 64, 8, 2, 6, 3},   {64, 4, 16, 6, 2, 4}, {32, 8, 16, 5, 3, 4},
+    {16, 16, 16, 4, 4, 4}, {8, 32, 16, 3, 5, 4}, {4, 64, 16, 2, 6, 4},
+};
+constexpr int kNumTileConfigs = sizeof(kCfgs) / sizeof(kCfgs[0]);
+
+int cfg_code(int cfg) {
+  return kCfgs[cfg].lx | (kCfgs[cfg].ly << 4) | (kCfgs[cfg].lr << 8);
+}
+
+bool cfgs_checked() {
+  for (int c = 0; c < kNumTileConfigs; ++c) {
+    const Geom g = make_geom(kCfgs[c].lx, kCfgs[c].ly, kCfgs[c].lr);
+    if (geom_lds_doubles(g) > kLdsDoubles || g.npt % CXP_NT != 0 ||
+        (1 << kCfgs[c].lx) != kCfgs[c].tx || (1 << kCfgs[c].ly) != kCfgs[c].ty ||
+        (1 << kCfgs[c].lr) != kCfgs[c].ring)
+      return false;
+  }
+  return true;
+}
+const bool kCfgsOk = cfgs_checked();
+
+// Size class of a patch: thin direction (extent <= 4) and a coarse x-size
+// bucket.  The tuner keys on (sids, class).
+int size_class(const int rlo[3], const int rhi[3]) {
+  const int n[3] = {rhi[0] - rlo[0] + 1, rhi[1] - rlo[1] + 1,
+                    rhi[2] - rlo[2] + 1};
+  int thin = 0;
+  if (n[0] <= 4)
+    thin = 1;
+  else if (n[1] <= 4)
+    thin = 2;
+  else if (n[2] <= 4)
+    thin = 3;
+  int xb = n[0] <= 8 ? 0 : n[0] <= 16 ? 1 : n[0] <= 32 ? 2 : 3;
+  return thin * 4 + xb;
+}
+
+bool cfg_fits(int cfg, const SepTask &t) {
+  const TileCfg &c = kCfgs[cfg];
+  for (int d = 0; d < 3; ++d)
+    if (t.whi[d] - t.wlo[d] + 1 > 9)
+      return false;
+  return t.whi[2] - t.wlo[2] + 1 <= c.ring;
+}
+
+int default_cfg(const SepTask &t) {
+  const int nx = t.rhi[0] - t.rlo[0] + 1;
+  const int ny = t.rhi[1] - t.rlo[1] + 1;
+  if (t.whi[2] - t.wlo[2] + 1 > 8)
+    return nx > 32 ? 10 : nx > 16 ? 11 : nx > 8 ? 12 : nx > 4 ? 13 : 14;
+  if (nx <= 4)
+    return ny > 64 ? 8 : 9;
+  if (nx <= 8)
+    return ny > 32 ? 6 : 7;
+  if (nx <= 16)
+    return ny > 16 ? 4 : 5;
+  if (nx <= 32)
+    return ny > 8 ? 2 : 3;
+  return ny <= 4 ? 1 : 0;
+}
+
+using TuneKey = std::tuple<int, int, int, int>;
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tune;  // -> (cfg, zchunk)
+
+TuneKey tune_key(const SepTask &t) {
+  return TuneKey(t.sid[0], t.sid[1], t.sid[2], size_class(t.rlo, t.rhi));
+}
+
+void choose_cfg(const SepTask &t, int &cfg, int &zchunk) {
+  cfg = default_cfg(t);
+  zchunk = 16;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(tune_key(t));
+  if (it != g_tune.end() && cfg_fits(it->second.first, t)) {
+    cfg = it->second.first;
+    zchunk = it->second.second;
+  }
+}
+
+SepTask make_sep_task(const OpDesc &op, const cxp_fab *crse, int crse_comp,
+                      const cxp_fab *fine, int fine_comp, const int rlo[3],
+                      const int rhi[3]) {
+  SepTask t;
+  t.crse = dfab(crse, crse_comp);
+  t.fine = dfab(fine, fine_comp);
+  for (int d = 0; d < 3; ++d) {
+    t.rlo[d] = rlo[d];
+    t.rhi[d] = rhi[d];
+    t.sid[d] = op.sid[d];
+    t.wlo[d] = sid_wlo(op.sid[d]);
+    t.whi[d] = sid_whi(op.sid[d]);
+  }
+  t.ccs = crse->comp_stride;
+  t.fcs = fine->comp_stride;
+  return t;
+}
+
+PointTask make_point_task(const OpDesc &op, const cxp_fab *crse, int crse_comp,
+                          const cxp_fab *fine, int fine_comp, int ncomp,
+                          const int rlo[3], const int rhi[3]) {
+  PointTask t;
+  t.crse = dfab(crse, crse_comp);
+  t.fine = dfab(fine, fine_comp);
+  for (int d = 0; d < 3; ++d) {
+    t.rlo[d] = rlo[d];
+    t.rhi[d] = rhi[d];
+    t.sid[d] = op.sid[d];
+  }
+  t.fb_per_var = op_fallback_per_var(op);
+  t.fb_per_group = op_fallback_per_group(op);
+  t.ncomp = ncomp;
+  t.ccs = crse->comp_stride;
+  t.fcs = fine->comp_stride;
+  return t;
+}
+
+int launch_point(const PointTask &t, int ncomp, int per_group,
+                 hipStream_t s) {
+  const long long n = (long long)(t.rhi[0] - t.rlo[0] + 1) *
+                      (t.rhi[1] - t.rlo[1] + 1) * (t.rhi[2] - t.rlo[2] + 1);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (per_group)
+    hipLaunchKernelGGL(k_point_per_group, dim3(nb), dim3(256), 0, s, t);
+  else
+    hipLaunchKernelGGL(k_point_per_var, dim3(nb, ncomp), dim3(256), 0, s, t);
+  return hip_status(hipGetLastError());
+}
+
+int launch_sep_box(const SepTask &t, int ncomp, hipStream_t s) {
+  int cfg, zc;
+  choose_cfg(t, cfg, zc);
+  const TileCfg &c = kCfgs[cfg];
+  const int nx = t.rhi[0] - t.rlo[0] + 1;
+  const int ny = t.rhi[1] - t.rlo[1] + 1;
+  const int nz = t.rhi[2] - t.rlo[2] + 1;
+  const int ntx = (nx + c.tx - 1) / c.tx;
+  const int nty = (ny + c.ty - 1) / c.ty;
+  // per-box: shorten z-chunks until the launch has enough workgroups
+  int zchunk = std::max(2, std::min(nz, zc));
+  while (zchunk > 8 && (long long)ntx * nty * ((nz + zchunk - 1) / zchunk) *
+                               ncomp < 1024)
+    zchunk = (zchunk + 1) / 2;
+  const int nzc = (nz + zchunk - 1) / zchunk;
+  hipLaunchKernelGGL(k_sep_box, dim3(ntx * nty * nzc, ncomp), dim3(CXP_NT), 0,
+                     s, t, c.lx, c.ly, c.lr, ntx, nty, zchunk);
+  return hip_status(hipGetLastError());
+}
+
+int prolongate_impl(const cxp_op_s *op, const cxp_fab *crse, int crse_comp,
+                    const cxp_fab *fine, int fine_comp, int ncomp,
+                    const int rlo[3], const int rhi[3], int per_group,
+                    hipStream_t s) {
+  if (op_is_separable(op->d, per_group != 0)) {
+    // per-var and per-group coincide for operators without shifts/fallback
+    const SepTask t =
+        make_sep_task(op->d, crse, crse_comp, fine, fine_comp, rlo, rhi);
+    return launch_sep_box(t, ncomp, s);
+  }
+  const PointTask t = make_point_task(op->d, crse, crse_comp, fine, fine_comp,
+                                      ncomp, rlo, rhi);
+  return launch_point(t, ncomp, per_group, s);
+}
+
+} // namespace
+
+// ===========================================================================
+
+struct cxp_worklist_s {
+  struct Patch {
+    const cxp_op_s *op;
+    cxp_fab crse, fine;
+    int crse_comp, fine_comp, ncomp;
+    int rlo[3], rhi[3];
+  };
+  std::vector<Patch> patches;
+  std::vector<SepTask> tasks;
+  std::vector<SepTile> tiles;
+  std::vector<int> point_patches;  // indices of non-separable patches
+  SepTask *d_tasks = nullptr;
+  SepTile *d_tiles = nullptr;
+  size_t cap_tasks = 0, cap_tiles = 0;
+  bool uploaded = false;
+  int grid = 0;
+  int64_t fine_points = 0, algo_bytes = 0;
+};
+
+struct cxp_graph_s {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+namespace {
+
+// Union of the coarse points read for one patch.
+int64_t patch_coarse_points(const cxp_worklist_s::Patch &p) {
+  int clo[3], chi[3];
+  op_coarse_box(p.op->d, p.rlo, p.rhi, clo, chi);
+  return int64_t(chi[0] - clo[0] + 1) * (chi[1] - clo[1] + 1) *
+         (chi[2] - clo[2] + 1);
+}
+
+void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
+                 const TuneKey *only_key = nullptr) {
+  wl->tasks.clear();
+  wl->tiles.clear();
+  wl->point_patches.clear();
+  for (size_t n = 0; n < wl->patches.size(); ++n) {
+    const auto &p = wl->patches[n];
+    if (!op_is_separable(p.op->d, false)) {
+      wl->point_patches.push_back((int)n);
+      continue;
+    }
+    SepTask t = make_sep_task(p.op->d, &p.crse, p.crse_comp, &p.fine,
+                              p.fine_comp, p.rlo, p.rhi);
+    if (only_key && tune_key(t) != *only_key)
+      continue;
+    int cfg, zc;
+    choose_cfg(t, cfg, zc);
+    if (only_key) {
+      cfg = only_class_key_cfg >> 8;
+      zc = only_class_key_cfg & 0xff;
+    }
+    const int task = (int)wl->tasks.size();
+    wl->tasks.push_back(t);
+    const TileCfg &c = kCfgs[cfg];
+    for (int comp = 0; comp < p.ncomp; ++comp)
+      for (int z0 = p.rlo[2]; z0 <= p.rhi[2]; z0 += zc)
+        for (int y0 = p.rlo[1]; y0 <= p.rhi[1]; y0 += c.ty)
+          for (int x0 = p.rlo[0]; x0 <= p.rhi[0]; x0 += c.tx) {
+            SepTile tl;
+            tl.task = task;
+            tl.comp = comp;
+            tl.x0 = x0;
+            tl.y0 = y0;
+            tl.z0 = z0;
+            tl.z1 = std::min(p.rhi[2], z0 + zc - 1);
+            tl.cfg = cfg_code(cfg);
+            tl.pad = 0;
+            wl->tiles.push_back(tl);
+          }
+  }
+}
+
+int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
+  if (wl->tasks.size() > wl->cap_tasks) {
+    if (wl->d_tasks)
+      (void)hipFree(wl->d_tasks);
+    wl->d_tasks = nullptr;
+    wl->cap_tasks = wl->tasks.size() * 2;
+    if (hipMalloc(&wl->d_tasks, wl->cap_tasks * sizeof(SepTask)) != hipSuccess)
+      return CXP_ERR_HIP;
+  }
+  if (wl->tiles.size() > wl->cap_tiles) {
+    if (wl->d_tiles)
+      (void)hipFree(wl->d_tiles);
+    wl->d_tiles = nullptr;
+    wl->cap_tiles = wl->tiles.size() * 2;
+    if (hipMalloc(&wl->d_tiles, wl->cap_tiles * sizeof(SepTile)) != hipSuccess)
+      return CXP_ERR_HIP;
+  }
+  if (!wl->tasks.empty() &&
+      hipMemcpyAsync(wl->d_tasks, wl->tasks.data(),
+                     wl->tasks.size() * sizeof(SepTask),
+                     hipMemcpyHostToDevice, s) != hipSuccess)
+    return CXP_ERR_HIP;
+  if (!wl->tiles.empty() &&
+      hipMemcpyAsync(wl->d_tiles, wl->tiles.data(),
+                     wl->tiles.size() * sizeof(SepTile),
+                     hipMemcpyHostToDevice, s) != hipSuccess)
+    return CXP_ERR_HIP;
+  // the host vectors must stay alive until the copy completed
+  if (hipStreamSynchronize(s) != hipSuccess)
+    return CXP_ERR_HIP;
+  return CXP_OK;
+}
+
+int persistent_grid(int ntiles) {
+  static int cus = 0, per_cu = 0;
+  if (!cus) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cus = prop.multiProcessorCount;
+    if (cus <= 0)
+      cus = 256;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sep_batched,
+                                                     CXP_NT, 0) == hipSuccess &&
+        nb > 0)
+      per_cu = nb;
+    else
+      per_cu = 2;
+  }
+  int g = cus * per_cu;
+  g -= g % 8;
+  return std::max(1, std::min(ntiles, g));
+}
+
+int launch_batched(cxp_worklist_s *wl, hipStream_t s) {
+  if (!wl->tiles.empty()) {
+    hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), 0, s,
+                       wl->d_tasks, wl->d_tiles, (int)wl->tiles.size());
+    if (hipGetLastError() != hipSuccess)
+      return CXP_ERR_HIP;
+  }
+  for (int n : wl->point_patches) {
+    const auto &p = wl->patches[n];
+    const PointTask t = make_point_task(p.op->d, &p.crse, p.crse_comp,
+                                        &p.fine, p.fine_comp, p.ncomp, p.rlo,
+                                        p.rhi);
+    const int e = launch_point(t, p.ncomp, 0, s);
+    if (e)
+      return e;
+  }
+  return CXP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int cxp_version(void) { return 1; }
+
+const char *cxp_status_string(int st) {
+  switch (st) {
+  case CXP_OK: return "ok";
+  case CXP_ERR_TYPE: return "unsupported prolongation type";
+  case CXP_ERR_ORDER: return "unsupported prolongation order for this type";
+  case CXP_ERR_ARG: return "invalid argument";
+  case CXP_ERR_REGION: return "fine region is not contained in the fine FAB";
+  case CXP_ERR_COARSE: return "coarse FAB does not cover CoarseBox(region)";
+  case CXP_ERR_NCOMP: return "per-group prolongation supports at most 16 components";
+  case CXP_ERR_RATIO: return "refinement ratio must be 2";
+  case CXP_ERR_HIP: return "HIP runtime error";
+  case CXP_ERR_COMP: return "component range outside the FAB";
+  case CXP_ERR_STATE: return "object used in the wrong state";
+  case CXP_ERR_UNSUPPORTED: return "operation not supported for this operator";
+  }
+  return "unknown status";
+}
+
+int cxp_op_lookup(const char *type, int order, const int indextype[3],
+                  cxp_op *out) {
+  if (!type || !indextype || !out)
+    return CXP_ERR_ARG;
+  std::call_once(g_ops_once, init_ops);
+  OpDesc d;
+  const int e = op_lookup(type, order, indextype, &d);
+  if (e == -1)
+    return CXP_ERR_TYPE;
+  if (e == -2)
+    return CXP_ERR_ORDER;
+  if (e)
+    return CXP_ERR_ARG;
+  const int c = (indextype[0] << 2) | (indextype[1] << 1) | indextype[2];
+  *out = &g_ops[d.family][order][c];
+  return CXP_OK;
+}
+
+int cxp_op_describe(cxp_op op, int cent[3], int intp[3], int order[3],
+                    int ghosts[3], int *fb) {
+  if (!op)
+    return CXP_ERR_ARG;
+  for (int d = 0; d < 3; ++d) {
+    const SidInfo s = sid_info(op->d.sid[d]);
+    if (cent)
+      cent[d] = s.cent;
+    if (intp)
+      intp[d] = (op->d.family == 5 && s.cent == CENT_VC) ? INTP_HERMITE : s.intp;
+    if (order)
+      order[d] = s.order;
+    if (ghosts)
+      ghosts[d] = s.ghosts;
+  }
+  if (fb)
+    *fb = op->d.fb;
+  return CXP_OK;
+}
+
+int cxp_coarse_box(cxp_op op, const int fine_lo[3], const int fine_hi[3],
+                   int crse_lo[3], int crse_hi[3]) {
+  if (!op || !fine_lo || !fine_hi || !crse_lo || !crse_hi)
+    return CXP_ERR_ARG;
+  op_coarse_box(op->d, fine_lo, fine_hi, crse_lo, crse_hi);
+  return CXP_OK;
+}
+
+int cxp_prolongate(cxp_op op, const cxp_fab *crse, int crse_comp,
+                   const cxp_fab *fine, int fine_comp, int ncomp,
+                   const int region_lo[3], const int region_hi[3],
+                   int per_group, void *stream) {
+  if (!kCfgsOk)
+    return CXP_ERR_STATE;
+  const int e = check_prolong(op, crse, crse_comp, fine, fine_comp, ncomp,
+                              region_lo, region_hi, per_group);
+  if (e)
+    return e;
+  return prolongate_impl(op, crse, crse_comp, fine, fine_comp, ncomp,
+                         region_lo, region_hi, per_group,
+                         (hipStream_t)stream);
+}
+
+int cxp_restrict(const int indextype[3], const cxp_fab *fine, int fine_comp,
+                 const cxp_fab *crse, int crse_comp, int ncomp,
+                 const int rlo[3], const int rhi[3], void *stream) {
+  if (!indextype || !rlo || !rhi || !fab_ok(fine) || !fab_ok(crse) ||
+      ncomp < 1)
+    return CXP_ERR_ARG;
+  if (fine_comp < 0 || crse_comp < 0 || fine_comp + ncomp > fine->ncomp ||
+      crse_comp + ncomp > crse->ncomp)
+    return CXP_ERR_COMP;
+  RestrictTask t;
+  for (int d = 0; d < 3; ++d) {
+    if (indextype[d] != 0 && indextype[d] != 1)
+      return CXP_ERR_ARG;
+    if (rhi[d] < rlo[d])
+      return CXP_ERR_ARG;
+    t.cc[d] = indextype[d];
+    t.rlo[d] = rlo[d];
+    t.rhi[d] = rhi[d];
+  }
+  if (!box_in(rlo, rhi, crse->lo, crse->hi))
+    return CXP_ERR_REGION;
+  for (int d = 0; d < 3; ++d)
+    if (2 * rlo[d] < fine->lo[d] || 2 * rhi[d] + t.cc[d] > fine->hi[d])
+      return CXP_ERR_COARSE;
+  t.fine = dfab(fine, fine_comp);
+  t.crse = dfab(crse, crse_comp);
+  t.fcs = fine->comp_stride;
+  t.ccs = crse->comp_stride;
+  const long long n = (long long)(rhi[0] - rlo[0] + 1) * (rhi[1] - rlo[1] + 1) *
+                      (rhi[2] - rlo[2] + 1);
+  hipLaunchKernelGGL(k_restrict, dim3((unsigned)((n + 255) / 256), ncomp),
+                     dim3(256), 0, (hipStream_t)stream, t);
+  return hip_status(hipGetLastError());
+}
+
+int cxp_restrict_adjoint(const int indextype[3], const cxp_fab *crse_bar,
+                         int crse_comp, const cxp_fab *fine_bar, int fine_comp,
+                         int ncomp, const int rlo[3], const int rhi[3],
+                         void *stream) {
+  if (!indextype || !rlo || !rhi || !fab_ok(fine_bar) || !fab_ok(crse_bar) ||
+      ncomp < 1)
+    return CXP_ERR_ARG;
+  if (fine_comp < 0 || crse_comp < 0 || fine_comp + ncomp > fine_bar->ncomp ||
+      crse_comp + ncomp > crse_bar->ncomp)
+    return CXP_ERR_COMP;
+  RestrictTask t;
+  for (int d = 0; d < 3; ++d) {
+    if (indextype[d] != 0 && indextype[d] != 1)
+      return CXP_ERR_ARG;
+    if (rhi[d] < rlo[d])
+      return CXP_ERR_ARG;
+    t.cc[d] = indextype[d];
+    t.rlo[d] = rlo[d];
+    t.rhi[d] = rhi[d];
+  }
+  if (!box_in(rlo, rhi, crse_bar->lo, crse_bar->hi))
+    return CXP_ERR_REGION;
+  for (int d = 0; d < 3; ++d)
+    if (2 * rlo[d] < fine_bar->lo[d] || 2 * rhi[d] + t.cc[d] > fine_bar->hi[d])
+      return CXP_ERR_COARSE;
+  t.fine = dfab(fine_bar, fine_comp);
+  t.crse = dfab(crse_bar, crse_comp);
+  t.fcs = fine_bar->comp_stride;
+  t.ccs = crse_bar->comp_stride;
+  long long n = 1;
+  for (int d = 0; d < 3; ++d)
+    n *= 2 * (rhi[d] - rlo[d]) + 1 + t.cc[d];
+  hipLaunchKernelGGL(k_restrict_adjoint,
+                     dim3((unsigned)((n + 255) / 256), ncomp), dim3(256), 0,
+                     (hipStream_t)stream, t);
+  return hip_status(hipGetLastError());
+}
+
+int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
+                           const cxp_fab *crse_bar, int crse_comp,
+                           const cxp_fab *fine_bar, int fine_comp, int ncomp,
+                           const int rlo[3], const int rhi[3], int per_group,
+                           void *stream) {
+  const int e = check_prolong(op, crse_bar, crse_comp, fine_bar, fine_comp,
+                              ncomp, rlo, rhi, per_group);
+  if (e)
+    return e;
+  return adjoint_launch(op->d, crse_in, crse_bar, crse_comp, fine_bar,
+                        fine_comp, ncomp, rlo, rhi, per_group,
+                        (hipStream_t)stream);
+}
+
+// ---- worklists
+
+int cxp_worklist_create(cxp_worklist *out) {
+  if (!out)
+    return CXP_ERR_ARG;
+  *out = new cxp_worklist_s();
+  return CXP_OK;
+}
+
+int cxp_worklist_destroy(cxp_worklist wl) {
+  if (!wl)
+    return CXP_ERR_ARG;
+  if (wl->d_tasks)
+    (void)hipFree(wl->d_tasks);
+  if (wl->d_tiles)
+    (void)hipFree(wl->d_tiles);
+  delete wl;
+  return CXP_OK;
+}
+
+int cxp_worklist_clear(cxp_worklist wl) {
+  if (!wl)
+    return CXP_ERR_ARG;
+  wl->patches.clear();
+  wl->uploaded = false;
+  wl->fine_points = wl->algo_bytes = 0;
+  return CXP_OK;
+}
+
+int cxp_worklist_append(cxp_worklist wl, cxp_op op, const cxp_fab *crse,
+                        int crse_comp, const cxp_fab *fine, int fine_comp,
+                        int ncomp, const int rlo[3], const int rhi[3]) {
+  if (!wl)
+    return CXP_ERR_ARG;
+  const int e = check_prolong(op, crse, crse_comp, fine, fine_comp, ncomp, rlo,
+                              rhi, 0);
+  if (e)
+    return e;
+  cxp_worklist_s::Patch p;
+  p.op = op;
+  p.crse = *crse;
+  p.fine = *fine;
+  p.crse_comp = crse_comp;
+  p.fine_comp = fine_comp;
+  p.ncomp = ncomp;
+  for (int d = 0; d < 3; ++d) {
+    p.rlo[d] = rlo[d];
+    p.rhi[d] = rhi[d];
+  }
+  wl->patches.push_back(p);
+  wl->uploaded = false;
+  const int64_t npts = int64_t(rhi[0] - rlo[0] + 1) * (rhi[1] - rlo[1] + 1) *
+                       (rhi[2] - rlo[2] + 1);
+  wl->fine_points += npts * ncomp;
+  wl->algo_bytes += 8 * ncomp * (npts + patch_coarse_points(p));
+  return CXP_OK;
+}
+
+int cxp_worklist_append_shell(cxp_worklist wl, cxp_op op, const cxp_fab *crse,
+                              int crse_comp, const cxp_fab *fine,
+                              int fine_comp, int ncomp, const int ilo[3],
+                              const int ihi[3]) {
+  if (!wl || !fine || !ilo || !ihi || !op)
+    return CXP_ERR_ARG;
+  const int *glo = fine->lo, *ghi = fine->hi;
+  for (int d = 0; d < 3; ++d)
+    if (ilo[d] < glo[d] || ihi[d] > ghi[d] || ilo[d] > ihi[d])
+      return CXP_ERR_REGION;
+  // six slabs: z full (x,y grown), y (x grown, z interior), x (y,z interior)
+  struct Slab {
+    int lo[3], hi[3];
+  };
+  std::vector<Slab> slabs;
+  auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
+    if (x0 <= x1 && y0 <= y1 && z0 <= z1)
+      slabs.push_back({{x0, y0, z0}, {x1, y1, z1}});
+  };
+  add(glo[0], ghi[0], glo[1], ghi[1], glo[2], ilo[2] - 1);
+  add(glo[0], ghi[0], glo[1], ghi[1], ihi[2] + 1, ghi[2]);
+  add(glo[0], ghi[0], glo[1], ilo[1] - 1, ilo[2], ihi[2]);
+  add(glo[0], ghi[0], ihi[1] + 1, ghi[1], ilo[2], ihi[2]);
+  add(glo[0], ilo[0] - 1, ilo[1], ihi[1], ilo[2], ihi[2]);
+  add(ihi[0] + 1, ghi[0], ilo[1], ihi[1], ilo[2], ihi[2]);
+  // validate all slabs before appending any
+  for (const Slab &sl : slabs) {
+    const int e = check_prolong(op, crse, crse_comp, fine, fine_comp, ncomp,
+                                sl.lo, sl.hi, 0);
+    if (e)
+      return e;
+  }
+  const int64_t bytes_before = wl->algo_bytes;
+  for (const Slab &sl : slabs) {
+    const int e = cxp_worklist_append(wl, op, crse, crse_comp, fine, fine_comp,
+                                      ncomp, sl.lo, sl.hi);
+    if (e)
+      return e;
+  }
+  // algorithmic bytes count coarse points once per box over the union of the
+  // slabs' coarse boxes: recount them with a mask
+  int64_t fine_pts = 0;
+  for (const Slab &sl : slabs)
+    fine_pts += int64_t(sl.hi[0] - sl.lo[0] + 1) * (sl.hi[1] - sl.lo[1] + 1) *
+                (sl.hi[2] - sl.lo[2] + 1);
+  int ulo[3] = {1 << 30, 1 << 30, 1 << 30}, uhi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+  std::vector<std::pair<std::vector<int>, std::vector<int>>> cboxes;
+  for (const Slab &sl : slabs) {
+    int clo[3], chi[3];
+    op_coarse_box(op->d, sl.lo, sl.hi, clo, chi);
+    cboxes.push_back({{clo[0], clo[1], clo[2]}, {chi[0], chi[1], chi[2]}});
+    for (int d = 0; d < 3; ++d) {
+      ulo[d] = std::min(ulo[d], clo[d]);
+      uhi[d] = std::max(uhi[d], chi[d]);
+    }
+  }
+  const int ux = uhi[0] - ulo[0] + 1, uy = uhi[1] - ulo[1] + 1,
+            uz = uhi[2] - ulo[2] + 1;
+  std::vector<unsigned char> mask((size_t)ux * uy * uz, 0);
+  for (const auto &cb : cboxes)
+    for (int k = cb.first[2]; k <= cb.second[2]; ++k)
+      for (int j = cb.first[1]; j <= cb.second[1]; ++j)
+        for (int i = cb.first[0]; i <= cb.second[0]; ++i)
+          mask[((size_t)(k - ulo[2]) * uy + (j - ulo[1])) * ux + (i - ulo[0])] = 1;
+  int64_t ncoarse = 0;
+  for (unsigned char m : mask)
+    ncoarse += m;
+  wl->algo_bytes = bytes_before + 8 * (int64_t)ncomp * (fine_pts + ncoarse);
+  return CXP_OK;
+}
+
+int cxp_worklist_upload(cxp_worklist wl, void *stream) {
+  if (!wl)
+    return CXP_ERR_ARG;
+  build_tiles(wl);
+  const int e = upload_lists(wl, (hipStream_t)stream);
+  if (e)
+    return e;
+  wl->grid = persistent_grid((int)wl->tiles.size());
+  wl->uploaded = true;
+  return CXP_OK;
+}
+
+int cxp_worklist_stats(cxp_worklist wl, int64_t *npatches, int64_t *ntiles,
+                       int64_t *fine_points, int64_t *algo_bytes) {
+  if (!wl)
+    return CXP_ERR_ARG;
+  if (npatches)
+    *npatches = (int64_t)wl->patches.size();
+  if (ntiles)
+    *ntiles = (int64_t)wl->tiles.size();
+  if (fine_points)
+    *fine_points = wl->fine_points;
+  if (algo_bytes)
+    *algo_bytes = wl->algo_bytes;
+  return CXP_OK;
+}
+
+int cxp_prolongate_batched(cxp_worklist wl, void *stream) {
+  if (!wl)
+    return CXP_ERR_ARG;
+  if (!wl->uploaded)
+    return CXP_ERR_STATE;
+  return launch_batched(wl, (hipStream_t)stream);
+}
+
+int cxp_prolongate_per_patch(cxp_worklist wl, void *stream) {
+  if (!wl)
+    return CXP_ERR_ARG;
+  for (const auto &p : wl->patches) {
+    const int e = prolongate_impl(p.op, &p.crse, p.crse_comp, &p.fine,
+                                  p.fine_comp, p.ncomp, p.rlo, p.rhi, 0,
+                                  (hipStream_t)stream);
+    if (e)
+      return e;
+  }
+  return CXP_OK;
+}
+
+// ---- graphs
+
+int cxp_graph_capture_begin(void *stream) {
+  if (!stream)
+    return CXP_ERR_ARG;  // the legacy default stream cannot be captured
+  return hip_status(hipStreamBeginCapture((hipStream_t)stream,
+                                          hipStreamCaptureModeThreadLocal));
+}
+
+int cxp_graph_capture_end(void *stream, cxp_graph *out) {
+  if (!stream || !out)
+    return CXP_ERR_ARG;
+  hipGraph_t g;
+  if (hipStreamEndCapture((hipStream_t)stream, &g) != hipSuccess)
+    return CXP_ERR_HIP;
+  hipGraphExec_t ex;
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return CXP_ERR_HIP;
+  }
+  *out = new cxp_graph_s{g, ex};
+  return CXP_OK;
+}
+
+int cxp_graph_launch(cxp_graph g, void *stream) {
+  if (!g)
+    return CXP_ERR_ARG;
+  return hip_status(hipGraphLaunch(g->exec, (hipStream_t)stream));
+}
+
+int cxp_graph_destroy(cxp_graph g) {
+  if (!g)
+    return CXP_ERR_ARG;
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  delete g;
+  return CXP_OK;
+}
+
+// ---- autotuner
+
+int cxp_tile_config_count(void) { return kNumTileConfigs; }
+
+int cxp_tile_config(int cfg, int *tx, int *ty, int *ring) {
+  if (cfg < 0 || cfg >= kNumTileConfigs)
+    return CXP_ERR_ARG;
+  if (tx)
+    *tx = kCfgs[cfg].tx;
+  if (ty)
+    *ty = kCfgs[cfg].ty;
+  if (ring)
+    *ring = kCfgs[cfg].ring;
+  return CXP_OK;
+}
+
+int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
+  if (!wl || reps < 1)
+    return CXP_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  // distinct keys in the list
+  std::vector<TuneKey> keys;
+  for (const auto &p : wl->patches) {
+    if (!op_is_separable(p.op->d, false))
+      continue;
+    const SepTask t = make_sep_task(p.op->d, &p.crse, p.crse_comp, &p.fine,
+                                    p.fine_comp, p.rlo, p.rhi);
+    const TuneKey k = tune_key(t);
+    if (std::find(keys.begin(), keys.end(), k) == keys.end())
+      keys.push_back(k);
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return CXP_ERR_HIP;
+  const int zchunks[3] = {8, 16, 32};
+  int status = CXP_OK;
+  for (const TuneKey &k : keys) {
+    float best = 1e30f;
+    int best_cfg = -1, best_zc = 16;
+    for (int cfg = 0; cfg < kNumTileConfigs && !status; ++cfg)
+      for (int zi = 0; zi < 3 && !status; ++zi) {
+        const int zc = zchunks[zi];
+        build_tiles(wl, (cfg << 8) | zc, &k);
+        bool fits = !wl->tasks.empty();
+        for (const SepTask &t : wl->tasks)
+          fits = fits && cfg_fits(cfg, t);
+        if (!fits)
+          continue;
+        if ((status = upload_lists(wl, s)))
+          break;
+        wl->grid = persistent_grid((int)wl->tiles.size());
+        float tot = 0;
+        for (int r = 0; r <= reps; ++r) {
+          (void)hipEventRecord(e0, s);
+          hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), 0, s,
+                             wl->d_tasks, wl->d_tiles, (int)wl->tiles.size());
+          (void)hipEventRecord(e1, s);
+          if (hipEventSynchronize(e1) != hipSuccess) {
+            status = CXP_ERR_HIP;
+            break;
+          }
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (r > 0)
+            tot += ms;  // first launch is a warm-up
+        }
+        if (!status && tot < best) {
+          best = tot;
+          best_cfg = cfg;
+          best_zc = zc;
+        }
+      }
+    if (best_cfg >= 0) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[k] = {best_cfg, best_zc};
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  wl->uploaded = false;
+  if (status)
+    return status;
+  return cxp_worklist_upload(wl, stream);
+}
+
+int cxp_tune_save(const char *path) {
+  if (!path)
+    return CXP_ERR_ARG;
+  FILE *f = std::fopen(path, "w");
+  if (!f)
+    return CXP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  std::fprintf(f, "# sid_x sid_y sid_z size_class cfg zchunk\n");
+  for (const auto &kv : g_tune)
+    std::fprintf(f, "%d %d %d %d %d %d\n", std::get<0>(kv.first),
+                 std::get<1>(kv.first), std::get<2>(kv.first),
+                 std::get<3>(kv.first), kv.second.first, kv.second.second);
+  std::fclose(f);
+  return CXP_OK;
+}
+
+int cxp_tune_load(const char *path) {
+  if (!path)
+    return CXP_ERR_ARG;
+  FILE *f = std::fopen(path, "r");
+  if (!f)
+    return CXP_ERR_ARG;
+  char line[256];
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#')
+      continue;
+    int a, b, c, d, cfg, zc;
+    if (std::sscanf(line, "%d %d %d %d %d %d", &a, &b, &c, &d, &cfg, &zc) == 6 &&
+        cfg >= 0 && cfg < kNumTileConfigs && zc >= 2)
+      g_tune[TuneKey(a, b, c, d)] = {cfg, zc};
+  }
+  std::fclose(f);
+  return CXP_OK;
+}
+
+} // extern "C"

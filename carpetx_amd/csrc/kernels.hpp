@@ -1,0 +1,611 @@
+// Device kernels of the rf=2 transfer path (gfx950, FP64).
+//
+//  * sep_tile: separable prolongation of one fine tile.  The workgroup owns
+//    a TX x TY fine (x, y) tile and marches through coarse z-planes:
+//      coarse plane tile + halo  -> LDS                  (one HBM read each)
+//      x-pass (fine x, coarse y) -> LDS
+//      y-pass (fine x, fine y)   -> LDS ring of the last R planes
+//      z-pass from the ring      -> fine FAB, stores coalesced along x.
+//    Each pass evaluates the reference 1-D operator with its own summation
+//    form, in the order call_stencil_3d uses (x innermost), so results are
+//    bit-identical to the reference's per-point tensor product when the
+//    library is compiled without FMA contraction.
+//  * point kernels: ENO shift search and linear fallback need the whole
+//    3-D stencil per fine point (prolongate_3d_rf2_impl.hxx:1393-1671 and
+//    :1836-2148), so those operators run one thread per fine point.
+//  * restriction: injection in VC directions, mean of the two children in CC
+//    directions (AMReX average_down family called by Restrict,
+//    CarpetX/src/schedule.cxx:2791-2816).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "stencil1d.hpp"
+
+namespace cxp {
+
+// One component of a FAB, addressed in the level's global index space.
+struct DFab {
+  double *p;     // element (lo) of this component
+  int lo[3];
+  int sy;        // x-row stride
+  long long sz;  // plane stride
+};
+
+__device__ __forceinline__ long long didx(const DFab &f, int i, int j, int k) {
+  return (long long)(i - f.lo[0]) + (long long)(j - f.lo[1]) * f.sy +
+         (long long)(k - f.lo[2]) * f.sz;
+}
+
+// A separable prolongation task: one (patch, component).
+struct SepTask {
+  DFab crse;
+  DFab fine;
+  int rlo[3], rhi[3];
+  int sid[3];
+  int wlo[3], whi[3];  // union stencil window per direction
+  long long ccs, fcs;  // component strides (per-box launches)
+};
+
+// A work tile of the batched list.
+struct SepTile {
+  int task;
+  int comp;  // component offset applied with ccs/fcs
+  int x0, y0;
+  int z0, z1;
+  int cfg;  // tile shape: lx | ly << 4 | lr << 8
+  int pad;
+};
+
+#define CXP_LINEAR_SIDS(X)                                                     \
+  X(SID_VPOLY1) X(SID_VPOLY3) X(SID_VPOLY5) X(SID_VPOLY7) X(SID_CPOLY0)        \
+  X(SID_CPOLY1) X(SID_CPOLY2) X(SID_CPOLY3) X(SID_CPOLY4) X(SID_CPOLY5)        \
+  X(SID_CPOLY6) X(SID_CPOLY7) X(SID_VCONS0) X(SID_VCONS1) X(SID_CCONS0)        \
+  X(SID_CCONS1) X(SID_CCONS2) X(SID_CCONS3) X(SID_CCONS4) X(SID_CCONS5)        \
+  X(SID_CCONS6) X(SID_CCONS7) X(SID_MINMOD)
+
+// Runtime tile geometry.  TX and TY are powers of two (x-lanes of a wave stay
+// on one fine row when TX >= 64); R is the ring depth in coarse planes.
+struct Geom {
+  int lx, ly, lr;    // log2 of TX, TY, R
+  int cxs, cys;      // LDS capacity of the coarse tile (row stride, rows)
+  int npt;           // TX * TY
+};
+
+__host__ __device__ inline Geom make_geom(int lx, int ly, int lr) {
+  Geom g;
+  g.lx = lx;
+  g.ly = ly;
+  g.lr = lr;
+  g.cxs = (1 << lx) / 2 + 10;  // fine tile / 2 + 1 + 8 halo, padded
+  g.cys = (1 << ly) / 2 + 10;
+  g.npt = 1 << (lx + ly);
+  return g;
+}
+__host__ __device__ inline int geom_lds_doubles(const Geom &g) {
+  return g.cys * g.cxs + g.cys * (1 << g.lx) + (1 << g.lr) * g.npt;
+}
+
+// LDS budget per workgroup (doubles): sized for 512 fine points per plane
+// with an 8-plane ring, or 256 points with a 16-plane ring.
+constexpr int kLdsDoubles = 5632;
+#define CXP_NT 256
+
+template <int SID>
+__device__ __forceinline__ void sep_xpass(const Geom &g,
+                                          const double *__restrict__ Cs,
+                                          double *__restrict__ Xs, int CY,
+                                          int x0, int TXa, int ci0, int tid) {
+  const int TX = 1 << g.lx;
+  for (int idx = tid; idx < (CY << g.lx); idx += CXP_NT) {
+    const int cy = idx >> g.lx;
+    const int fi = idx & (TX - 1);
+    if (fi < TXa) {
+      const int fx = x0 + fi;
+      const int ic = fx >> 1;
+      const int off = fx & 1;
+      const double *row = Cs + cy * g.cxs + (ic - ci0);
+      Xs[idx] = Op1<SID>::apply([&](int o) { return row[o]; }, off);
+    }
+  }
+}
+
+template <int SID>
+__device__ __forceinline__ void sep_ypass(const Geom &g,
+                                          const double *__restrict__ Xs,
+                                          double *__restrict__ ring, int kp,
+                                          int y0, int TXa, int TYa, int cj0,
+                                          int tid) {
+  const int TX = 1 << g.lx;
+  double *slot = ring + (kp & ((1 << g.lr) - 1)) * g.npt;
+  for (int p = tid; p < g.npt; p += CXP_NT) {
+    const int fj = p >> g.lx;
+    const int fi = p & (TX - 1);
+    if (fi < TXa && fj < TYa) {
+      const int fy = y0 + fj;
+      const int jc = fy >> 1;
+      const int off = fy & 1;
+      const double *col = Xs + ((jc - cj0) << g.lx) + fi;
+      slot[p] = Op1<SID>::apply([&](int o) { return col[o << g.lx]; }, off);
+    }
+  }
+}
+
+template <int SID>
+__device__ __forceinline__ void sep_zpass(const Geom &g,
+                                          const double *__restrict__ ring,
+                                          const DFab &fine, int kc, int z0,
+                                          int z1, int x0, int y0, int TXa,
+                                          int TYa, int tid) {
+  const int TX = 1 << g.lx;
+  const int rmask = (1 << g.lr) - 1;
+#pragma unroll
+  for (int off = 0; off < 2; ++off) {
+    const int fz = 2 * kc + off;
+    if (fz < z0 || fz > z1)
+      continue;
+    for (int p = tid; p < g.npt; p += CXP_NT) {
+      const int fj = p >> g.lx;
+      const int fi = p & (TX - 1);
+      if (fi < TXa && fj < TYa) {
+        const double v = Op1<SID>::apply(
+            [&](int o) { return ring[((kc + o) & rmask) * g.npt + p]; }, off);
+        fine.p[didx(fine, x0 + fi, y0 + fj, fz)] = v;
+      }
+    }
+  }
+}
+
+// Process one tile [x0, x0+TX) x [y0, y0+TY) x [z0, z1] of a task, clipped
+// to the task's fine region.  All threads of the block must call it.
+__device__ inline void sep_tile(const Geom &g, const SepTask &t,
+                                long long comp, int x0, int y0, int z0, int z1,
+                                double *smem) {
+  double *Cs = smem;
+  double *Xs = Cs + g.cys * g.cxs;
+  double *ring = Xs + (g.cys << g.lx);
+  const int tid = threadIdx.x;
+  const int TXa = min(1 << g.lx, t.rhi[0] - x0 + 1);
+  const int TYa = min(1 << g.ly, t.rhi[1] - y0 + 1);
+  const int ci0 = (x0 >> 1) + t.wlo[0];
+  const int ci1 = ((x0 + TXa - 1) >> 1) + t.whi[0];
+  const int cj0 = (y0 >> 1) + t.wlo[1];
+  const int cj1 = ((y0 + TYa - 1) >> 1) + t.whi[1];
+  const int CX = ci1 - ci0 + 1;
+  const int CY = cj1 - cj0 + 1;
+  const int kf = (z0 >> 1) + t.wlo[2];
+  const int kl = (z1 >> 1) + t.whi[2];
+  const int kc0 = z0 >> 1;
+  const int kc1 = z1 >> 1;
+
+  DFab crse = t.crse;
+  crse.p += comp * t.ccs;
+  DFab fine = t.fine;
+  fine.p += comp * t.fcs;
+
+  const int sx = t.sid[0], sy = t.sid[1], sz = t.sid[2];
+  const int ncl = CX * CY;
+
+  for (int kp = kf; kp <= kl; ++kp) {
+    // coarse plane tile -> LDS
+    const double *src = crse.p + didx(crse, ci0, cj0, kp);
+    for (int idx = tid; idx < ncl; idx += CXP_NT) {
+      const int cy = idx / CX;
+      const int cx = idx - cy * CX;
+      Cs[cy * g.cxs + cx] = src[(long long)cy * crse.sy + cx];
+    }
+    __syncthreads();
+    switch (sx) {
+#define CXP_X(S)                                                               \
+  case S:                                                                      \
+    sep_xpass<S>(g, Cs, Xs, CY, x0, TXa, ci0, tid);                            \
+    break;
+      CXP_LINEAR_SIDS(CXP_X)
+#undef CXP_X
+    }
+    __syncthreads();
+    switch (sy) {
+#define CXP_Y(S)                                                               \
+  case S:                                                                      \
+    sep_ypass<S>(g, Xs, ring, kp, y0, TXa, TYa, cj0, tid);                     \
+    break;
+      CXP_LINEAR_SIDS(CXP_Y)
+#undef CXP_Y
+    }
+    const int kc = kp - t.whi[2];
+    if (kc >= kc0 && kc <= kc1) {
+      switch (sz) {
+#define CXP_Z(S)                                                               \
+  case S:                                                                      \
+    sep_zpass<S>(g, ring, fine, kc, z0, z1, x0, y0, TXa, TYa, tid);            \
+    break;
+        CXP_LINEAR_SIDS(CXP_Z)
+#undef CXP_Z
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Per-box launch: grid.x = tiles (x fastest, then y, then z-chunks),
+// grid.y = component.  The tile shape is uniform over the launch.
+__global__ void __launch_bounds__(CXP_NT)
+    k_sep_box(SepTask t, int lx, int ly, int lr, int ntx, int nty,
+              int zchunk) {
+  __shared__ double smem[kLdsDoubles];
+  const Geom g = make_geom(lx, ly, lr);
+  const int b = blockIdx.x;
+  const int tx = b % ntx;
+  const int ty = (b / ntx) % nty;
+  const int tz = b / (ntx * nty);
+  const int x0 = t.rlo[0] + (tx << lx);
+  const int y0 = t.rlo[1] + (ty << ly);
+  const int z0 = t.rlo[2] + tz * zchunk;
+  const int z1 = min(t.rhi[2], z0 + zchunk - 1);
+  sep_tile(g, t, blockIdx.y, x0, y0, z0, z1, smem);
+}
+
+// Batched persistent launch: the grid walks the device-resident tile list;
+// each tile carries its task and tile shape (cfg packs lx | ly<<4 | lr<<8).
+// Workgroups are dispatched round-robin over the 8 XCDs, so the list index
+// is remapped to give each XCD a contiguous run of tiles (neighbouring tiles
+// share coarse halo, which then stays in one L2).
+__global__ void __launch_bounds__(CXP_NT)
+    k_sep_batched(const SepTask *__restrict__ tasks,
+                  const SepTile *__restrict__ tiles, int ntiles) {
+  __shared__ double smem[kLdsDoubles];
+  const int nblk = gridDim.x;
+  const int bid = blockIdx.x;
+  int vb = bid;
+  if (nblk >= 8 && nblk % 8 == 0)
+    vb = (bid % 8) * (nblk / 8) + bid / 8;
+  for (int n = vb; n < ntiles; n += nblk) {
+    const SepTile tl = tiles[n];
+    const SepTask t = tasks[tl.task];
+    const Geom g = make_geom(tl.cfg & 15, (tl.cfg >> 4) & 15, (tl.cfg >> 8) & 15);
+    sep_tile(g, t, tl.comp, tl.x0, tl.y0, tl.z0, tl.z1, smem);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Point kernels for ENO / fallback operators
+
+struct PointTask {
+  DFab crse;
+  DFab fine;
+  int rlo[3], rhi[3];
+  int sid[3];
+  int fb_per_var, fb_per_group;
+  int ncomp;
+  long long ccs, fcs;
+};
+
+__device__ __forceinline__ double crse_at(const DFab &c, int i, int j, int k) {
+  return c.p[didx(c, i, j, k)];
+}
+
+// Tensor product with shifts (call_stencil_3d; x innermost).
+__device__ inline double pt_tensor(const DFab &c, const int sid[3],
+                                   const int shift[3], const int off[3],
+                                   const int ic[3]) {
+  int ylo, yhi, zlo, zhi;
+  stencil_radius(sid[1], shift[1], off[1], ylo, yhi);
+  stencil_radius(sid[2], shift[2], off[2], zlo, zhi);
+  double yb[9];
+  for (int dk = zlo; dk <= zhi; ++dk) {
+    double xb[9];
+    for (int dj = ylo; dj <= yhi; ++dj) {
+      const double *row = c.p + didx(c, ic[0], ic[1] + dj, ic[2] + dk);
+      xb[dj + 4] = apply1d(sid[0], [&](int o) { return row[o]; }, shift[0],
+                           off[0]);
+    }
+    yb[dk + 4] = apply1d(sid[1], [&](int o) { return xb[o + 4]; }, shift[1],
+                         off[1]);
+  }
+  return apply1d(sid[2], [&](int o) { return yb[o + 4]; }, shift[2], off[2]);
+}
+
+// Second undivided difference (CC ENO 2; weights +1 -2 +1, :750-773).
+__device__ __forceinline__ double undiv2(double m, double c, double p) {
+  double dd = -2.0 * c;
+  dd += 1.0 * (m + p);
+  return dd;
+}
+
+// ENO shift search (:1417-1517 per-var; per-group takes the max over comps).
+__device__ inline void pt_eno_search(const PointTask &t, long long comp0,
+                                     int ncomp, const int off[3],
+                                     const int ic[3], int shift[3]) {
+  int use[3], ms[3];
+  for (int d = 0; d < 3; ++d) {
+    use[d] = t.sid[d] == SID_ENO2;
+    ms[d] = use[d] ? 1 : 0;
+  }
+  shift[0] = shift[1] = shift[2] = 0;
+  double min_dd = __builtin_inf();
+  for (int sk = -ms[2]; sk <= ms[2]; ++sk)
+    for (int sj = -ms[1]; sj <= ms[1]; ++sj)
+      for (int si = -ms[0]; si <= ms[0]; ++si) {
+        const int sv[3] = {si, sj, sk};
+        int r[3][2];
+        for (int d = 0; d < 3; ++d) {
+          stencil_radius(t.sid[d], sv[d], off[d], r[d][0], r[d][1]);
+          if (use[d]) {
+            r[d][0] -= sv[d];
+            r[d][1] -= sv[d];
+          }
+        }
+        double ddx = 0, ddy = 0, ddz = 0;
+        for (int cc = 0; cc < ncomp; ++cc) {
+          DFab c = t.crse;
+          c.p += (comp0 + cc) * t.ccs;
+          if (use[0])
+            for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
+              for (int dj = r[1][0]; dj <= r[1][1]; ++dj) {
+                const int i = ic[0] + si, j = ic[1] + sj + dj, k = ic[2] + sk + dk;
+                ddx = fmax(ddx, fabs(undiv2(crse_at(c, i - 1, j, k),
+                                            crse_at(c, i, j, k),
+                                            crse_at(c, i + 1, j, k))));
+              }
+          if (use[1])
+            for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
+              for (int di = r[0][0]; di <= r[0][1]; ++di) {
+                const int i = ic[0] + si + di, j = ic[1] + sj, k = ic[2] + sk + dk;
+                ddy = fmax(ddy, fabs(undiv2(crse_at(c, i, j - 1, k),
+                                            crse_at(c, i, j, k),
+                                            crse_at(c, i, j + 1, k))));
+              }
+          if (use[2])
+            for (int dj = r[1][0]; dj <= r[1][1]; ++dj)
+              for (int di = r[0][0]; di <= r[0][1]; ++di) {
+                const int i = ic[0] + si + di, j = ic[1] + sj + dj, k = ic[2] + sk;
+                ddz = fmax(ddz, fabs(undiv2(crse_at(c, i, j, k - 1),
+                                            crse_at(c, i, j, k),
+                                            crse_at(c, i, j, k + 1))));
+              }
+        }
+        // 1 + sqrt(DBL_EPSILON) * (|si| + |sj| + |sk|); sqrt(eps) = 2^-26
+        const double penalty =
+            1.0 + 0x1p-26 * (double)(abs(si) + abs(sj) + abs(sk));
+        const double dd = penalty * fmax(fmax(ddx, ddy), ddz);
+        if (dd < min_dd) {
+          min_dd = dd;
+          shift[0] = si;
+          shift[1] = sj;
+          shift[2] = sk;
+        }
+      }
+}
+
+__device__ inline void pt_minmax(const DFab &c, const int sr[3][2],
+                                 const int ic[3], double &mn, double &mx) {
+  for (int dk = sr[2][0]; dk <= sr[2][1]; ++dk)
+    for (int dj = sr[1][0]; dj <= sr[1][1]; ++dj)
+      for (int di = sr[0][0]; di <= sr[0][1]; ++di) {
+        const double v = crse_at(c, ic[0] + di, ic[1] + dj, ic[2] + dk);
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+      }
+}
+
+// Slope sign change along d (:1609-1664): s0 = c(ic+1) - c(ic) compared with
+// each difference from srad[d][0]+2 up.
+__device__ inline bool pt_slope_change(const DFab &c, int d, const int sr[3][2],
+                                       const int ic[3]) {
+  const int a = d == 0 ? 1 : 0;
+  const int b = d == 2 ? 1 : 2;
+  bool need = false;
+  for (int tb = sr[b][0]; tb <= sr[b][1]; ++tb)
+    for (int ta = sr[a][0]; ta <= sr[a][1]; ++ta) {
+      int p[3];
+      p[a] = ic[a] + ta;
+      p[b] = ic[b] + tb;
+      p[d] = ic[d];
+      const double c0 = crse_at(c, p[0], p[1], p[2]);
+      p[d] = ic[d] + 1;
+      const double s0 = crse_at(c, p[0], p[1], p[2]) - c0;
+      for (int tt = sr[d][0] + 2; tt <= sr[d][1]; ++tt) {
+        p[d] = ic[d] + tt;
+        const double v1 = crse_at(c, p[0], p[1], p[2]);
+        p[d] = ic[d] + tt - 1;
+        const double v0 = crse_at(c, p[0], p[1], p[2]);
+        const double s = v1 - v0;
+        need |= s * s0 < 0;
+      }
+    }
+  return need;
+}
+
+__global__ void __launch_bounds__(256) k_point_per_var(PointTask t) {
+  const int nx = t.rhi[0] - t.rlo[0] + 1;
+  const int ny = t.rhi[1] - t.rlo[1] + 1;
+  const int nz = t.rhi[2] - t.rlo[2] + 1;
+  const long long npts = (long long)nx * ny * nz;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= npts)
+    return;
+  const long long comp = blockIdx.y;
+  const int i = t.rlo[0] + (int)(gid % nx);
+  const int j = t.rlo[1] + (int)((gid / nx) % ny);
+  const int k = t.rlo[2] + (int)(gid / ((long long)nx * ny));
+  const int f[3] = {i, j, k};
+  int ic[3], off[3], shift[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    ic[d] = f[d] >> 1;
+    off[d] = f[d] & 1;
+  }
+  DFab c = t.crse;
+  c.p += comp * t.ccs;
+  const bool any_shift =
+      t.sid[0] == SID_ENO2 || t.sid[1] == SID_ENO2 || t.sid[2] == SID_ENO2;
+  if (any_shift)
+    pt_eno_search(t, comp, 1, off, ic, shift);
+  const double val = pt_tensor(c, t.sid, shift, off, ic);
+  double res = val;
+  if (t.fb_per_var) {
+    int sl[3];
+    for (int d = 0; d < 3; ++d)
+      sl[d] = sid_linear_per_var(t.sid[d]);
+    const int zero[3] = {0, 0, 0};
+    const double val_lin = pt_tensor(c, sl, zero, off, ic);
+    int sr[3][2];
+    for (int d = 0; d < 3; ++d)
+      stencil_radius(t.sid[d], shift[d], off[d], sr[d][0], sr[d][1]);
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    pt_minmax(c, sr, ic, mn, mx);
+    bool need = val < mn || val > mx;
+    for (int d = 0; d < 3; ++d) {
+      const SidInfo s = sid_info(t.sid[d]);
+      if (s.intp == INTP_CONS || s.intp == INTP_ENO)
+        need |= pt_slope_change(c, d, sr, ic);
+    }
+    res = need ? val_lin : val;
+  }
+  DFab fo = t.fine;
+  fo.p += comp * t.fcs;
+  fo.p[didx(fo, i, j, k)] = res;
+}
+
+__global__ void __launch_bounds__(256) k_point_per_group(PointTask t) {
+  const int nx = t.rhi[0] - t.rlo[0] + 1;
+  const int ny = t.rhi[1] - t.rlo[1] + 1;
+  const int nz = t.rhi[2] - t.rlo[2] + 1;
+  const long long npts = (long long)nx * ny * nz;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= npts)
+    return;
+  const int i = t.rlo[0] + (int)(gid % nx);
+  const int j = t.rlo[1] + (int)((gid / nx) % ny);
+  const int k = t.rlo[2] + (int)(gid / ((long long)nx * ny));
+  const int f[3] = {i, j, k};
+  int ic[3], off[3], shift[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    ic[d] = f[d] >> 1;
+    off[d] = f[d] & 1;
+  }
+  const int nc = t.ncomp;
+  const bool any_shift =
+      t.sid[0] == SID_ENO2 || t.sid[1] == SID_ENO2 || t.sid[2] == SID_ENO2;
+  if (any_shift)
+    pt_eno_search(t, 0, nc, off, ic, shift);
+  double vals[16];
+  for (int cc = 0; cc < nc; ++cc) {
+    DFab c = t.crse;
+    c.p += cc * t.ccs;
+    vals[cc] = pt_tensor(c, t.sid, shift, off, ic);
+  }
+  bool need = false;
+  if (t.fb_per_group) {
+    int sr[3][2];
+    for (int d = 0; d < 3; ++d)
+      stencil_radius(t.sid[d], shift[d], off[d], sr[d][0], sr[d][1]);
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    for (int cc = 0; cc < nc; ++cc) {
+      DFab c = t.crse;
+      c.p += cc * t.ccs;
+      pt_minmax(c, sr, ic, mn, mx);  // accumulates across comps (:2010-2024)
+      need |= vals[cc] < mn || vals[cc] > mx;
+    }
+    for (int d = 0; d < 3; ++d)
+      if (sid_info(t.sid[d]).intp == INTP_CONS)
+        for (int cc = 0; cc < nc; ++cc) {
+          DFab c = t.crse;
+          c.p += cc * t.ccs;
+          need |= pt_slope_change(c, d, sr, ic);
+        }
+  }
+  int sl[3];
+  for (int d = 0; d < 3; ++d)
+    sl[d] = sid_linear_per_group(t.sid[d]);
+  const int zero[3] = {0, 0, 0};
+  for (int cc = 0; cc < nc; ++cc) {
+    double v = vals[cc];
+    if (need) {
+      DFab c = t.crse;
+      c.p += cc * t.ccs;
+      v = pt_tensor(c, sl, zero, off, ic);
+    }
+    DFab fo = t.fine;
+    fo.p += cc * t.fcs;
+    fo.p[didx(fo, i, j, k)] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Restriction and its adjoint
+
+struct RestrictTask {
+  DFab fine;
+  DFab crse;
+  int rlo[3], rhi[3];  // coarse region
+  int cc[3];           // 1 for cell-centred directions
+  long long fcs, ccs;
+};
+
+__global__ void __launch_bounds__(256) k_restrict(RestrictTask t) {
+  const int nx = t.rhi[0] - t.rlo[0] + 1;
+  const int ny = t.rhi[1] - t.rlo[1] + 1;
+  const int nz = t.rhi[2] - t.rlo[2] + 1;
+  const long long n = (long long)nx * ny * nz;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n)
+    return;
+  const long long comp = blockIdx.y;
+  const int i = t.rlo[0] + (int)(gid % nx);
+  const int j = t.rlo[1] + (int)((gid / nx) % ny);
+  const int k = t.rlo[2] + (int)(gid / ((long long)nx * ny));
+  DFab f = t.fine;
+  f.p += comp * t.fcs;
+  DFab c = t.crse;
+  c.p += comp * t.ccs;
+  const int ncc = t.cc[0] + t.cc[1] + t.cc[2];
+  double v;
+  if (ncc == 0) {
+    v = f.p[didx(f, 2 * i, 2 * j, 2 * k)];
+  } else {
+    double s = 0;
+    for (int kk = 0; kk <= t.cc[2]; ++kk)
+      for (int jj = 0; jj <= t.cc[1]; ++jj)
+        for (int ii = 0; ii <= t.cc[0]; ++ii)
+          s += f.p[didx(f, 2 * i + ii, 2 * j + jj, 2 * k + kk)];
+    v = s * (1.0 / (double)(1 << ncc));
+  }
+  c.p[didx(c, i, j, k)] = v;
+}
+
+// fine_bar += R^T crse_bar over the children of the coarse region; each fine
+// point has at most one parent, so the gather needs no atomics.
+__global__ void __launch_bounds__(256) k_restrict_adjoint(RestrictTask t) {
+  // iterate over fine points covered by the coarse region
+  int flo[3], fn[3];
+  for (int d = 0; d < 3; ++d) {
+    flo[d] = 2 * t.rlo[d];
+    fn[d] = 2 * (t.rhi[d] - t.rlo[d]) + 1 + t.cc[d];
+  }
+  const long long n = (long long)fn[0] * fn[1] * fn[2];
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n)
+    return;
+  const long long comp = blockIdx.y;
+  const int fi = flo[0] + (int)(gid % fn[0]);
+  const int fj = flo[1] + (int)((gid / fn[0]) % fn[1]);
+  const int fk = flo[2] + (int)(gid / ((long long)fn[0] * fn[1]));
+  const int f[3] = {fi, fj, fk};
+  int ci[3];
+  for (int d = 0; d < 3; ++d) {
+    if (!t.cc[d] && (f[d] & 1))
+      return;  // odd VC points are not read by injection
+    ci[d] = f[d] >> 1;
+  }
+  const int ncc = t.cc[0] + t.cc[1] + t.cc[2];
+  DFab fb = t.fine;
+  fb.p += comp * t.fcs;
+  DFab cb = t.crse;
+  cb.p += comp * t.ccs;
+  const double w = 1.0 / (double)(1 << ncc);
+  fb.p[didx(fb, fi, fj, fk)] += w * cb.p[didx(cb, ci[0], ci[1], ci[2])];
+}
+
+} // namespace cxp

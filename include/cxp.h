@@ -1,0 +1,162 @@
+/*
+ * cxp.h -- C ABI of the MI355X-native rf=2 AMR inter-level transfer library
+ * (libcxp.so).  Plain C: device pointers, sizes and status codes; streams
+ * are hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Each entry point names the reference interface it replaces.  All device
+ * work is enqueued on the given stream and the call returns without
+ * synchronising, like the reference's ParallelFor launches
+ * (CarpetX/src/prolongate_3d_rf2_impl.hxx:1685-1688).
+ *
+ * Data layout: a FAB is AMReX's FArrayBox layout -- x fastest, then y, z,
+ * then component, indices in the level's global index space, boxes inclusive
+ * [lo, hi] (CarpetX/src/prolongate_3d_rf2_impl.hxx:1339-1348,1776-1779).
+ */
+#ifndef CXP_H
+#define CXP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference aborts (assert / CCTK_VERROR) where these are
+ * returned. */
+#define CXP_OK 0
+#define CXP_ERR_TYPE -1        /* unknown prolongation_type (driver.cxx:620) */
+#define CXP_ERR_ORDER -2       /* unsupported order for type (driver.cxx:623) */
+#define CXP_ERR_ARG -3         /* null pointer, bad indextype, ncomp < 1 ... */
+#define CXP_ERR_REGION -4      /* fine_region not inside fine FAB (:1300-1302) */
+#define CXP_ERR_COARSE -5      /* crse FAB does not cover CoarseBox(region) */
+#define CXP_ERR_NCOMP -6       /* per-group mode with ncomp > 16 (:1707-1708) */
+#define CXP_ERR_RATIO -7       /* refinement ratio other than 2 (:1293-1294) */
+#define CXP_ERR_HIP -8         /* a HIP runtime call failed */
+#define CXP_ERR_COMP -9        /* component range outside the FAB */
+#define CXP_ERR_STATE -10      /* worklist / graph used in the wrong state */
+#define CXP_ERR_UNSUPPORTED -11 /* operator has no adjoint of this kind */
+
+/* Centring and interpolation codes (prolongate_3d_rf2.hxx:15-31). */
+#define CXP_VC 0
+#define CXP_CC 1
+#define CXP_POLY 0
+#define CXP_HERMITE 1
+#define CXP_CONS 2
+#define CXP_ENO 3
+#define CXP_MINMOD 4
+
+typedef struct cxp_fab {
+  double *data;        /* device pointer to component 0 at (lo) */
+  int lo[3];
+  int hi[3];
+  int64_t comp_stride; /* elements between components (AMReX: box volume) */
+  int ncomp;           /* number of components stored */
+} cxp_fab;
+
+typedef const struct cxp_op_s *cxp_op; /* immutable, thread-safe, never freed */
+
+int cxp_version(void);
+const char *cxp_status_string(int status);
+
+/* get_interpolator(type, order, indextype) -- CarpetX/src/driver.cxx:593-642.
+ * indextype[d]: 0 vertex (VC), 1 cell (CC). */
+int cxp_op_lookup(const char *type, int order, const int indextype[3],
+                  cxp_op *out);
+
+/* Per-direction description: centring, interpolation, order, required
+ * ghosts; fb = 1 for the FB_LINEAR families
+ * (prolongate_3d_rf2.hxx:64-75 indextype()/interpolation()/order()). */
+int cxp_op_describe(cxp_op op, int cent[3], int intp[3], int order[3],
+                    int ghosts[3], int *fb);
+
+/* prolongate_3d_rf2::CoarseBox(fine, ratio=2) -- impl.hxx:1226-1254. */
+int cxp_coarse_box(cxp_op op, const int fine_lo[3], const int fine_hi[3],
+                   int crse_lo[3], int crse_hi[3]);
+
+/* prolongate_3d_rf2::interp(crse, crse_comp, fine, fine_comp, ncomp,
+ * fine_region, ratio, ...) -- impl.hxx:2167-2188 (per_group selects
+ * interp_per_group, the `prolongate_per_group` parameter).  Writes exactly
+ * fine_region of components [fine_comp, fine_comp+ncomp); reads only
+ * CoarseBox(fine_region) of crse. */
+int cxp_prolongate(cxp_op op, const cxp_fab *crse, int crse_comp,
+                   const cxp_fab *fine, int fine_comp, int ncomp,
+                   const int region_lo[3], const int region_hi[3],
+                   int per_group, void *stream);
+
+/* Restrict() -> AMReX average_down_nodal/_edges/_faces/average_down by the
+ * number of cell-centred directions (CarpetX/src/schedule.cxx:2791-2816):
+ * injection in VC directions, mean of the two children in CC directions,
+ * over the coarse region [crse_lo, crse_hi]. */
+int cxp_restrict(const int indextype[3], const cxp_fab *fine, int fine_comp,
+                 const cxp_fab *crse, int crse_comp, int ncomp,
+                 const int crse_region_lo[3], const int crse_region_hi[3],
+                 void *stream);
+
+/* Adjoint of cxp_prolongate (no reference counterpart; the reference has no
+ * adjoints): crse_bar += P^T fine_bar over CoarseBox(region), in gather form
+ * over coarse points, no atomics.  crse_in is the forward input, required
+ * for operators whose linearisation depends on the data (MINMOD, ENO,
+ * fallback), ignored otherwise. */
+int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
+                           const cxp_fab *crse_bar, int crse_comp,
+                           const cxp_fab *fine_bar, int fine_comp, int ncomp,
+                           const int region_lo[3], const int region_hi[3],
+                           int per_group, void *stream);
+
+/* Adjoint of cxp_restrict: fine_bar += R^T crse_bar. */
+int cxp_restrict_adjoint(const int indextype[3], const cxp_fab *crse_bar,
+                         int crse_comp, const cxp_fab *fine_bar, int fine_comp,
+                         int ncomp, const int crse_region_lo[3],
+                         const int crse_region_hi[3], void *stream);
+
+/* ---- Batched path: a device-resident work list walked by one persistent
+ * kernel.  Replaces the loop of FillPatchInterp -> interp calls over fine
+ * patches (CarpetX/src/fillpatch.cxx:116-119,179-180,213-216). */
+typedef struct cxp_worklist_s *cxp_worklist;
+
+int cxp_worklist_create(cxp_worklist *out);
+int cxp_worklist_destroy(cxp_worklist wl);
+int cxp_worklist_clear(cxp_worklist wl);
+/* Adds one interp() call; same arguments and checks as cxp_prolongate. */
+int cxp_worklist_append(cxp_worklist wl, cxp_op op, const cxp_fab *crse,
+                        int crse_comp, const cxp_fab *fine, int fine_comp,
+                        int ncomp, const int region_lo[3],
+                        const int region_hi[3]);
+/* Adds the ghost shell of a fine box: the fine FAB's box minus interior
+ * [int_lo, int_hi], as six slabs (z-low/high full, y-low/high, x-low/high). */
+int cxp_worklist_append_shell(cxp_worklist wl, cxp_op op, const cxp_fab *crse,
+                              int crse_comp, const cxp_fab *fine,
+                              int fine_comp, int ncomp, const int int_lo[3],
+                              const int int_hi[3]);
+/* Builds tiles (using the tuned tile shapes) and copies the list to the
+ * device; must precede cxp_prolongate_batched. */
+int cxp_worklist_upload(cxp_worklist wl, void *stream);
+int cxp_worklist_stats(cxp_worklist wl, int64_t *npatches, int64_t *ntiles,
+                       int64_t *fine_points, int64_t *algo_bytes);
+/* One launch of the persistent kernel over the whole list (plus one
+ * point-kernel launch per ENO/fallback patch). */
+int cxp_prolongate_batched(cxp_worklist wl, void *stream);
+/* The plain path over the same list: one cxp_prolongate launch per patch. */
+int cxp_prolongate_per_patch(cxp_worklist wl, void *stream);
+
+/* ---- hipGraph capture of a fill sequence. */
+typedef struct cxp_graph_s *cxp_graph;
+int cxp_graph_capture_begin(void *stream);
+int cxp_graph_capture_end(void *stream, cxp_graph *out);
+int cxp_graph_launch(cxp_graph g, void *stream);
+int cxp_graph_destroy(cxp_graph g);
+
+/* ---- Tile-shape autotuner.  Times every tile shape on the patches of each
+ * (operator, size class) of the list and keeps the fastest; the table is
+ * process-wide and can be saved/loaded as text. */
+int cxp_autotune(cxp_worklist wl, int reps, void *stream);
+int cxp_tune_save(const char *path);
+int cxp_tune_load(const char *path);
+int cxp_tile_config_count(void);
+int cxp_tile_config(int cfg, int *tx, int *ty, int *ring);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

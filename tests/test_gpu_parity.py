@@ -1,0 +1,115 @@
+"""GPU parity: libcxp.so through the C ABI against the oracle on the same
+seeded inputs.
+
+Contract: relative 1e-13 against the CPU restatement (differences from FMA
+reordering only).  The library and the strict oracle are both built without
+FMA contraction and evaluate the reference's summation forms in the same
+order, so the tests demand more: bit-identical results
+(``np.array_equal``), with the 1e-13 bound reported alongside.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from carpetx_amd.interp import Box, FArrayBox, get_interpolator
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-13  # the contract
+
+
+def all_ops():
+    for fam in oracle.FAMILIES:
+        for order in oracle.FAMILY_ORDERS[fam]:
+            for c in range(8):
+                yield fam, order, ((c >> 2) & 1, (c >> 1) & 1, c & 1)
+
+
+def run_case(fam, order, it, rlo, rhi, ncomp=1, per_group=False, seed=0,
+             pad=(1, 2, 0), crse_comp=0, fine_comp=0, extra_comp=0):
+    rng = np.random.default_rng(seed)
+    op_o = oracle.lookup(fam, order, it)
+    P = get_interpolator(fam, order, it)
+    clo, chi = oracle.coarse_box(op_o, rlo, rhi)
+    cbox = Box([clo[d] - pad[d] for d in range(3)], [chi[d] + pad[(d + 1) % 3] for d in range(3)])
+    fbox = Box([rlo[d] - pad[(d + 2) % 3] for d in range(3)], [rhi[d] + pad[d] for d in range(3)])
+    nc_c = ncomp + crse_comp + extra_comp
+    nc_f = ncomp + fine_comp + extra_comp
+    cdata = rng.uniform(-1, 1, size=(nc_c,) + tuple(reversed(cbox.size)))
+    crse_h = oracle.Fab(cbox.lo, cbox.hi, nc_c, cdata)
+    sentinel = 1234.5
+    fine_h = oracle.Fab(fbox.lo, fbox.hi, nc_f, np.full((nc_f,) + tuple(reversed(fbox.size)), sentinel))
+    oracle.prolongate(op_o, crse_h, fine_h, rlo, rhi, ncomp=ncomp, per_group=per_group,
+                      crse_comp=crse_comp, fine_comp=fine_comp)
+    crse = FArrayBox(cbox, nc_c, tensor=torch.from_numpy(cdata).cuda())
+    fine = FArrayBox(fbox, nc_f, tensor=torch.full((nc_f,) + tuple(reversed(fbox.size)), sentinel,
+                                                    dtype=torch.float64, device="cuda"))
+    P.interp(crse, crse_comp, fine, fine_comp, ncomp, Box(rlo, rhi), per_group=per_group)
+    torch.cuda.synchronize()
+    got = fine.t.cpu().numpy()
+    return got, fine_h.arr
+
+
+def assert_parity(got, ref, what):
+    scale = max(1.0, float(np.max(np.abs(ref[np.isfinite(ref)]))))
+    err = float(np.max(np.abs(got - ref)))
+    assert err <= RTOL * scale, "%s: max err %g" % (what, err)
+    assert np.array_equal(got, ref), "%s: not bit-identical (max err %g)" % (what, err)
+
+
+@pytest.mark.parametrize("fam", oracle.FAMILIES)
+def test_every_operator_small_box(fam):
+    """All registered operators of a family, all 8 centrings, on a region with
+    odd and negative bounds; the sentinel outside the region must survive."""
+    for f, order, it in all_ops():
+        if f != fam:
+            continue
+        pgs = (False, True)
+        for pg in pgs:
+            got, ref = run_case(fam, order, it, (-3, 1, -2), (8, 9, 6), ncomp=2,
+                                per_group=pg, seed=hash((fam, order, it)) & 0xffff)
+            assert_parity(got, ref, "%s o%d %s pg=%d" % (fam, order, it, pg))
+
+
+@pytest.mark.parametrize("fam,order,it", [
+    ("interpolate", 3, (0, 0, 0)), ("natural", 5, (0, 0, 0)),
+    ("natural", 5, (1, 1, 1)), ("conservative", 1, (1, 1, 1)),
+    ("natural", 7, (1, 0, 1)), ("minmod", 3, (1, 1, 1)),
+    ("ddf", 7, (0, 1, 1)), ("natural", 1, (1, 0, 0)),
+])
+def test_multi_tile_regions(fam, order, it):
+    """Regions spanning many tiles and z-chunks, odd starts, all three
+    directions longer than a tile."""
+    got, ref = run_case(fam, order, it, (-7, 3, -9), (70, 40, 36), ncomp=1, seed=7)
+    assert_parity(got, ref, "%s o%d %s" % (fam, order, it))
+
+
+@pytest.mark.parametrize("fam,order,it", [
+    ("eno", 3, (1, 1, 1)), ("poly-cons3lfb", 5, (1, 0, 1)), ("poly-eno3lfb", 3, (0, 1, 1)),
+])
+def test_point_kernel_ops_larger(fam, order, it):
+    for pg in (False, True):
+        got, ref = run_case(fam, order, it, (-5, 0, 1), (30, 21, 17), ncomp=3,
+                            per_group=pg, seed=11)
+        assert_parity(got, ref, "%s o%d %s pg=%d" % (fam, order, it, pg))
+
+
+def test_component_offsets():
+    """crse_comp / fine_comp select components inside multi-component FABs."""
+    got, ref = run_case("natural", 3, (0, 1, 0), (0, 0, 0), (17, 12, 9), ncomp=2,
+                        crse_comp=1, fine_comp=2, extra_comp=1, seed=3)
+    assert_parity(got, ref, "component offsets")
+
+
+def test_thin_slabs_every_orientation():
+    """Ghost-shell-like slabs 1..3 points thick in each direction."""
+    for it in [(0, 0, 0), (1, 1, 1), (0, 1, 0)]:
+        for d in range(3):
+            for th in (1, 2, 3):
+                lo = [-3, -3, -3]
+                hi = [40, 33, 29]
+                hi[d] = lo[d] + th - 1
+                got, ref = run_case("natural", 5, it, tuple(lo), tuple(hi), ncomp=1,
+                                    seed=th + 10 * d)
+                assert_parity(got, ref, "slab it=%s d=%d th=%d" % (it, d, th))

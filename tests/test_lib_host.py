@@ -1,0 +1,120 @@
+"""CPU-only checks of libcxp.so: it loads, exports every symbol include/cxp.h
+declares, and its host-side logic (operator registry, CoarseBox, argument
+validation) agrees with the oracle.  No device computation is called."""
+import ctypes
+import itertools
+
+import pytest
+
+import oracle
+from carpetx_amd import _lib
+from carpetx_amd._lib import i3
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _lib.header_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the ctypes signature table covers them all
+    assert set(names) <= set(_lib._SIGS), set(names) - set(_lib._SIGS)
+
+
+def _lookup(t, o, it):
+    h = ctypes.c_void_p()
+    st = _lib.lib().cxp_op_lookup(t.encode(), o, i3(it), ctypes.byref(h))
+    return st, h
+
+
+def test_registry_matches_oracle():
+    """get_interpolator (driver.cxx:593-642): same families, same orders,
+    same per-direction operators, same errors."""
+    L = _lib.lib()
+    for fam in oracle.FAMILIES:
+        for order in range(0, 9):
+            for c in range(8):
+                it = ((c >> 2) & 1, (c >> 1) & 1, c & 1)
+                st, h = _lookup(fam, order, it)
+                try:
+                    op = oracle.lookup(fam, order, it)
+                except ValueError:
+                    assert st == -2, (fam, order)
+                    continue
+                assert st == 0, (fam, order, it, st)
+                cent, intp, ordr, gh = (ctypes.c_int * 3)(), (ctypes.c_int * 3)(), \
+                    (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+                fb = ctypes.c_int()
+                assert L.cxp_op_describe(h, cent, intp, ordr, gh, ctypes.byref(fb)) == 0
+                assert tuple(cent) == tuple(op.cent)
+                assert tuple(intp) == tuple(op.intp), (fam, order, it)
+                assert tuple(ordr) == tuple(op.order), (fam, order, it)
+                assert fb.value == op.fb
+                for d in range(3):
+                    assert gh[d] == oracle.required_ghosts(op.cent[d], op.intp[d], op.order[d])
+    assert _lookup("no-such-type", 1, (0, 0, 0))[0] == -1
+    assert _lookup("eno", 1, (0, 0, 0))[0] == -2
+
+
+def test_handles_are_stable():
+    a = _lookup("natural", 3, (1, 0, 1))[1]
+    b = _lookup("natural", 3, (1, 0, 1))[1]
+    assert a.value == b.value
+
+
+def test_coarse_box_matches_oracle_exhaustive():
+    """CoarseBox with AMReX coarsen semantics (impl.hxx:1226-1254): floor for
+    lo, nodal hi rounded up, grown by required_ghosts; negative lo included."""
+    L = _lib.lib()
+    for fam, order, c in [("natural", 5, 0), ("natural", 5, 7), ("interpolate", 3, 5),
+                          ("eno", 3, 3), ("minmod", 1, 6), ("conservative", 0, 2),
+                          ("poly-cons3lfb", 5, 1)]:
+        it = ((c >> 2) & 1, (c >> 1) & 1, c & 1)
+        h = _lookup(fam, order, it)[1]
+        op = oracle.lookup(fam, order, it)
+        for lo, hi in itertools.product(range(-5, 4), range(-3, 6)):
+            if hi < lo:
+                continue
+            flo, fhi = (lo, lo + 1, -lo), (hi, hi + 2, -lo + 3)
+            clo, chi = i3((0, 0, 0)), i3((0, 0, 0))
+            assert L.cxp_coarse_box(h, i3(flo), i3(fhi), clo, chi) == 0
+            assert (tuple(clo), tuple(chi)) == oracle.coarse_box(op, flo, fhi)
+            # closed form
+            for d in range(3):
+                g = oracle.required_ghosts(op.cent[d], op.intp[d], op.order[d])
+                clo_cf = flo[d] // 2 - g
+                chi_cf = (-(-fhi[d] // 2) if op.cent[d] == 0 else fhi[d] // 2) + g
+                assert (clo[d], chi[d]) == (clo_cf, chi_cf)
+
+
+def test_validation_errors_without_device_work():
+    """The reference's asserts become status codes; all of these are caught
+    before any device call (the data pointer is never dereferenced)."""
+    L = _lib.lib()
+    h = _lookup("natural", 3, (0, 0, 0))[1]
+    fake = ctypes.c_void_p(0x1000)
+
+    def fab(lo, hi, ncomp=1):
+        f = _lib.CxpFab()
+        f.data = fake
+        f.lo, f.hi = i3(lo), i3(hi)
+        f.comp_stride = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+        f.ncomp = ncomp
+        return f
+
+    crse = fab((-4, -4, -4), (20, 20, 20))
+    fine = fab((0, 0, 0), (15, 15, 15))
+    def call(region_lo, region_hi, ncomp=1, per_group=0, crse_=crse, fine_=fine, op=h):
+        return L.cxp_prolongate(op, ctypes.byref(crse_), 0, ctypes.byref(fine_), 0,
+                                ncomp, i3(region_lo), i3(region_hi), per_group, None)
+    assert call((0, 0, 0), (16, 15, 15)) == -4          # region outside fine FAB
+    small = fab((0, 0, 0), (9, 9, 9))
+    assert call((0, 0, 0), (15, 15, 15), crse_=small) == -5  # CoarseBox not covered
+    assert call((0, 0, 0), (15, 15, 15), ncomp=2) == -9   # components
+    big_c = fab((-4, -4, -4), (20, 20, 20), 17)
+    big_f = fab((0, 0, 0), (15, 15, 15), 17)
+    assert call((0, 0, 0), (15, 15, 15), ncomp=17, per_group=1, crse_=big_c,
+                fine_=big_f) == -6                       # per-group ncomp > 16
+    assert call((3, 0, 0), (2, 15, 15)) == -3            # empty region
+    assert call((0, 0, 0), (15, 15, 15), op=None) == -3
+    assert L.cxp_status_string(-5) == b"coarse FAB does not cover CoarseBox(region)"
