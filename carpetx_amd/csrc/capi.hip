@@ -123,7 +123,9 @@ int cfg_code(int cfg) {
 bool cfgs_checked() {
   for (int c = 0; c < kNumTileConfigs; ++c) {
     const Geom g = make_geom(kCfgs[c].lx, kCfgs[c].ly, kCfgs[c].lr);
+    const int max_coarse = (kCfgs[c].tx / 2 + 9) * (kCfgs[c].ty / 2 + 9);
     if (geom_lds_doubles(g) > kLdsDoubles || g.npt % CXP_NT != 0 ||
+        max_coarse > CXP_PRE * CXP_NT || max_coarse > g.cxs * g.cys ||
         (1 << kCfgs[c].lx) != kCfgs[c].tx || (1 << kCfgs[c].ly) != kCfgs[c].ty ||
         (1 << kCfgs[c].lr) != kCfgs[c].ring)
       return false;

@@ -92,6 +92,9 @@ __host__ __device__ inline int geom_lds_doubles(const Geom &g) {
 // with an 8-plane ring, or 256 points with a 16-plane ring.
 constexpr int kLdsDoubles = 5632;
 #define CXP_NT 256
+// Coarse values per thread and plane: the largest coarse tile of any shape
+// in use is (TX/2 + 10) x (TY/2 + 10) <= 12 x 74 = 888 <= 4 * 256.
+#define CXP_PRE 4
 
 template <int SID>
 __device__ __forceinline__ void sep_xpass(const Geom &g,
@@ -188,13 +191,38 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
   const int sx = t.sid[0], sy = t.sid[1], sz = t.sid[2];
   const int ncl = CX * CY;
 
+  // Each thread moves up to CXP_PRE coarse values per plane; the next
+  // plane's loads are issued before the current plane is processed so their
+  // latency overlaps the three passes.
+  int soff[CXP_PRE], loff[CXP_PRE];
+#pragma unroll
+  for (int r = 0; r < CXP_PRE; ++r) {
+    const int idx = tid + r * CXP_NT;
+    const int cy = idx / CX;
+    const int cx = idx - cy * CX;
+    soff[r] = idx < ncl ? cy * g.cxs + cx : -1;
+    loff[r] = cy * crse.sy + cx;
+  }
+  double pre[CXP_PRE];
+  {
+    const double *src = crse.p + didx(crse, ci0, cj0, kf);
+#pragma unroll
+    for (int r = 0; r < CXP_PRE; ++r)
+      pre[r] = soff[r] >= 0 ? src[loff[r]] : 0.0;
+  }
+
   for (int kp = kf; kp <= kl; ++kp) {
-    // coarse plane tile -> LDS
-    const double *src = crse.p + didx(crse, ci0, cj0, kp);
-    for (int idx = tid; idx < ncl; idx += CXP_NT) {
-      const int cy = idx / CX;
-      const int cx = idx - cy * CX;
-      Cs[cy * g.cxs + cx] = src[(long long)cy * crse.sy + cx];
+    // coarse plane tile -> LDS, then prefetch the next plane
+#pragma unroll
+    for (int r = 0; r < CXP_PRE; ++r)
+      if (soff[r] >= 0)
+        Cs[soff[r]] = pre[r];
+    if (kp < kl) {
+      const double *src = crse.p + didx(crse, ci0, cj0, kp + 1);
+#pragma unroll
+      for (int r = 0; r < CXP_PRE; ++r)
+        if (soff[r] >= 0)
+          pre[r] = src[loff[r]];
     }
     __syncthreads();
     switch (sx) {

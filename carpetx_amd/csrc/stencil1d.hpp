@@ -151,7 +151,7 @@ template <int SID> struct Op1;
 
 template <int SID, int P> struct VPolyOp {
   template <class F>
-  __host__ __device__ __attribute__((always_inline)) static double
+  __host__ __device__ __attribute__((always_inline)) static constexpr double
   apply(const F &c, int off) {
     if (off == 0)
       return c(0);
@@ -167,7 +167,7 @@ template <int SID, int P> struct VPolyOp {
 
 template <int SID, int P> struct CPolyOp {
   template <class F>
-  __host__ __device__ __attribute__((always_inline)) static double
+  __host__ __device__ __attribute__((always_inline)) static constexpr double
   apply(const F &c, int off) {
     constexpr int N = P + 1;
     constexpr int i0 = N / 2;
@@ -187,11 +187,11 @@ template <int SID, int P> struct CPolyOp {
 
 template <int SID, int P> struct CConsOp {
   template <class F>
-  __host__ __device__ __attribute__((always_inline)) static double
+  __host__ __device__ __attribute__((always_inline)) static constexpr double
   apply(const F &c, int off) {
     constexpr int N = P + 1;
     constexpr int i0 = N / 2;
-    double y;
+    double y = 0;
     if constexpr (P % 2 == 0) {
       y = W<SID>::c(P / 2) * c(P / 2 - i0);
       if (off == 0) {
@@ -243,7 +243,7 @@ template <> struct Op1<SID_CCONS7> : CConsOp<SID_CCONS7, 7> {};
 // VC CONS order 0: copy at off 0 (as 0 + 1*c), nothing (0) at off 1.
 template <> struct Op1<SID_VCONS0> {
   template <class F>
-  __host__ __device__ __attribute__((always_inline)) static double
+  __host__ __device__ __attribute__((always_inline)) static constexpr double
   apply(const F &c, int off) {
     double y = 0;
     if (off == 0)
@@ -254,7 +254,7 @@ template <> struct Op1<SID_VCONS0> {
 // VC CONS order 1: copy at off 0, midpoint average at off 1.
 template <> struct Op1<SID_VCONS1> {
   template <class F>
-  __host__ __device__ __attribute__((always_inline)) static double
+  __host__ __device__ __attribute__((always_inline)) static constexpr double
   apply(const F &c, int off) {
     double y = 0;
     if (off == 0) {

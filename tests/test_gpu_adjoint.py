@@ -1,0 +1,151 @@
+"""GPU adjoints through the C ABI.
+
+Checks (no reference counterpart exists; these pin the transpose from the
+forward operator):
+* dot-product identity <P u, v> == <u, P^T v> with the GPU forward, every
+  linear operator family and centring;
+* an explicit dense P^T on a small sub-box: columns of P are the oracle's
+  forward applied to coarse unit vectors;
+* MINMOD (piecewise linear): the dense Jacobian of the oracle at u, taken by
+  central differences (exact inside one limiter branch), transposed.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from carpetx_amd.interp import Box, FArrayBox, get_interpolator
+
+pytestmark = pytest.mark.gpu
+
+
+def _fabs(op, rlo, rhi, ncomp, rng):
+    P = get_interpolator(*op)
+    cb = P.CoarseBox(Box(rlo, rhi))
+    u = rng.uniform(-1, 1, size=(ncomp,) + tuple(reversed(cb.size)))
+    fbox = Box(rlo, rhi)
+    v = rng.uniform(-1, 1, size=(ncomp,) + tuple(reversed(fbox.size)))
+    return P, cb, fbox, u, v
+
+
+LINEAR = [("interpolate", 1), ("interpolate", 3), ("interpolate", 5), ("interpolate", 7),
+          ("natural", 1), ("natural", 3), ("natural", 5), ("natural", 7),
+          ("ddf", 3), ("ddf", 5), ("conservative", 0), ("conservative", 1), ("hermite", 5)]
+
+
+@pytest.mark.parametrize("fam,order", LINEAR)
+def test_dot_product_identity(fam, order):
+    rng = np.random.default_rng(order)
+    for c in range(8):
+        it = ((c >> 2) & 1, (c >> 1) & 1, c & 1)
+        P, cb, fbox, u, v = _fabs((fam, order, it), (-3, 2, -1), (40, 21, 17), 2, rng)
+        crse = FArrayBox(cb, 2, tensor=torch.from_numpy(u).cuda())
+        fine = FArrayBox(fbox, 2)
+        P.interp(crse, 0, fine, 0, 2, fbox)
+        vb = FArrayBox(fbox, 2, tensor=torch.from_numpy(v).cuda())
+        cbar = FArrayBox(cb, 2)
+        P.interp_adjoint(cbar, 0, vb, 0, 2, fbox)
+        torch.cuda.synchronize()
+        lhs = float((fine.t * vb.t).sum())
+        rhs = float((crse.t * cbar.t).sum())
+        scale = float((fine.t.abs() * vb.t.abs()).sum())
+        assert abs(lhs - rhs) <= 1e-13 * scale, (fam, order, it, lhs, rhs)
+
+
+def _dense_P(op_o, cb, rlo, rhi):
+    """Columns of P from the oracle applied to coarse unit vectors."""
+    ncoarse = int(np.prod(cb.size))
+    fshape = tuple(rhi[d] - rlo[d] + 1 for d in (2, 1, 0))
+    cols = []
+    for n in range(ncoarse):
+        e = np.zeros(ncoarse)
+        e[n] = 1.0
+        ch = oracle.Fab(cb.lo, cb.hi, 1, e.reshape((1,) + tuple(reversed(cb.size))))
+        fh = oracle.Fab(rlo, rhi, 1)
+        oracle.prolongate(op_o, ch, fh, rlo, rhi)
+        cols.append(fh.arr.reshape(-1))
+    return np.stack(cols, axis=1), fshape
+
+
+@pytest.mark.parametrize("fam,order,it", [("natural", 5, (0, 0, 0)), ("natural", 3, (1, 0, 1)),
+                                          ("interpolate", 7, (0, 1, 0)), ("ddf", 5, (1, 1, 1))])
+def test_dense_transpose_small_box(fam, order, it):
+    rng = np.random.default_rng(3)
+    rlo, rhi = (1, 0, -1), (8, 7, 6)   # an 8^3 fine sub-box
+    op_o = oracle.lookup(fam, order, it)
+    P, cb, fbox, u, v = _fabs((fam, order, it), rlo, rhi, 1, rng)
+    A, fshape = _dense_P(op_o, cb, rlo, rhi)
+    expect = (A.T @ v.reshape(-1)).reshape((1,) + tuple(reversed(cb.size)))
+    vb = FArrayBox(fbox, 1, tensor=torch.from_numpy(v).cuda())
+    cbar = FArrayBox(cb, 1, tensor=torch.full((1,) + tuple(reversed(cb.size)), 0.25,
+                                               dtype=torch.float64, device="cuda"))
+    P.interp_adjoint(cbar, 0, vb, 0, 1, fbox)
+    torch.cuda.synchronize()
+    got = cbar.t.cpu().numpy() - 0.25   # the adjoint accumulates
+    assert np.max(np.abs(got - expect)) <= 1e-13 * max(1.0, np.max(np.abs(expect)))
+
+
+@pytest.mark.parametrize("it", [(1, 1, 1), (1, 0, 1), (0, 1, 0)])
+def test_minmod_adjoint_against_dense_jacobian(it):
+    rng = np.random.default_rng(4)
+    rlo, rhi = (0, 1, -2), (7, 6, 4)
+    op_o = oracle.lookup("minmod", 3, it)
+    P, cb, fbox, u, v = _fabs(("minmod", 3, it), rlo, rhi, 1, rng)
+    ncoarse = int(np.prod(cb.size))
+    h = 1e-7
+
+    def fwd(x):
+        ch = oracle.Fab(cb.lo, cb.hi, 1, x.reshape((1,) + tuple(reversed(cb.size))))
+        fh = oracle.Fab(rlo, rhi, 1)
+        oracle.prolongate(op_o, ch, fh, rlo, rhi)
+        return fh.arr.reshape(-1)
+    u0 = u.reshape(-1)
+    J = np.zeros((int(np.prod(fbox.size)), ncoarse))
+    for n in range(ncoarse):
+        e = np.zeros(ncoarse)
+        e[n] = h
+        J[:, n] = (fwd(u0 + e) - fwd(u0 - e)) / (2 * h)
+    expect = (J.T @ v.reshape(-1)).reshape((1,) + tuple(reversed(cb.size)))
+    crse = FArrayBox(cb, 1, tensor=torch.from_numpy(u).cuda())
+    vb = FArrayBox(fbox, 1, tensor=torch.from_numpy(v).cuda())
+    cbar = FArrayBox(cb, 1)
+    P.interp_adjoint(cbar, 0, vb, 0, 1, fbox, crse_in=crse)
+    torch.cuda.synchronize()
+    got = cbar.t.cpu().numpy()
+    assert np.max(np.abs(got - expect)) <= 1e-7 * max(1.0, np.max(np.abs(expect)))
+
+
+def test_minmod_dot_product_with_linearisation():
+    """<J u, v> == <u, J^T v> where J u = P(u) for a positively homogeneous
+    limiter (minmod(a u) = a minmod(u) for a > 0, so P(u) = J(u) u)."""
+    rng = np.random.default_rng(8)
+    rlo, rhi = (-4, 3, 0), (37, 22, 19)
+    P, cb, fbox, u, v = _fabs(("minmod", 5, (1, 1, 1)), rlo, rhi, 2, rng)
+    crse = FArrayBox(cb, 2, tensor=torch.from_numpy(u).cuda())
+    fine = FArrayBox(fbox, 2)
+    P.interp(crse, 0, fine, 0, 2, fbox)
+    vb = FArrayBox(fbox, 2, tensor=torch.from_numpy(v).cuda())
+    cbar = FArrayBox(cb, 2)
+    P.interp_adjoint(cbar, 0, vb, 0, 2, fbox, crse_in=crse)
+    torch.cuda.synchronize()
+    lhs = float((fine.t * vb.t).sum())
+    rhs = float((crse.t * cbar.t).sum())
+    assert abs(lhs - rhs) <= 1e-12 * float((fine.t.abs() * vb.t.abs()).sum())
+
+
+def test_restrict_adjoint_dot_product():
+    from carpetx_amd.interp import restrict, restrict_adjoint
+    rng = np.random.default_rng(2)
+    for it in [(0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 1)]:
+        fbox = Box((-4, -2, 0), (21, 17, 15))
+        creg = Box((-2, -1, 0), tuple((fbox.hi[d] - it[d]) // 2 for d in range(3)))
+        f = FArrayBox(fbox, 1, tensor=torch.from_numpy(rng.uniform(-1, 1, (1,) + tuple(reversed(fbox.size)))).cuda())
+        c = FArrayBox(creg, 1)
+        restrict(f, c, it, crse_region=creg)
+        w = FArrayBox(creg, 1, tensor=torch.from_numpy(rng.uniform(-1, 1, (1,) + tuple(reversed(creg.size)))).cuda())
+        fb = FArrayBox(fbox, 1)
+        restrict_adjoint(w, fb, it, 1, creg)
+        torch.cuda.synchronize()
+        lhs = float((c.t * w.t).sum())
+        rhs = float((f.t * fb.t).sum())
+        assert abs(lhs - rhs) < 1e-12 * float((c.t.abs() * w.t.abs()).sum()), it
