@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(256)
         const int gx = fxa + fx, gy = fya + fy;
         double v = 0.0;
         if (gx >= t.rlo[0] && gx <= t.rhi[0] && gy >= t.rlo[1] && gy <= t.rhi[1])
-          v = fb.p[didx(fb, gx, gy, fz)];
+          v = gld(fb.p, didx(fb, gx, gy, fz));
         Fs[fy * CXP_ADJ_FXS + fx] = v;
       }
       __syncthreads();
@@ -193,8 +193,8 @@ __global__ void __launch_bounds__(256)
         const int cil = p & (CTX - 1);
         const int cjl = p >> CXP_ADJ_LX;
         if (cil < CTXa && cjl < CTYa) {
-          double *dst = cb.p + didx(cb, ci0 + cil, cj0 + cjl, kc);
-          *dst += acc[n][0];
+          const long long di = didx(cb, ci0 + cil, cj0 + cjl, kc);
+          gst(cb.p, di, gld(cb.p, di) + acc[n][0]);
         }
       }
     }
@@ -370,6 +370,10 @@ __global__ void __launch_bounds__(256) k_adj_mm_x(MmTask t, long long comp) {
   double s = 0.0;
   for (int o = sid_wlo(sx); o <= sid_whi(sx); ++o) {
     const int ix = ic - o;
+    // only coarse x owning a fine point of the region is read: its
+    // neighbours ix-1, ix+1 then lie inside CoarseBox(region)
+    if (2 * ix + 1 < t.rlo[0] || 2 * ix > t.rhi[0])
+      continue;
     double vm = 0, v0 = 0, vp = 0;
     if (sx == SID_MINMOD) {
       const double *row = c.p + didx(c, ix, jc, kc);

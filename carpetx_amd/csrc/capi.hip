@@ -108,11 +108,10 @@ struct TileCfg {
 // Tile shapes (fine TX x TY per plane, ring depth R); every one fits the
 // kernel's LDS budget (checked at first use).
 const TileCfg kCfgs[] = {
-    {64, 8, 8, 6, 3, 3},   {64, 4, 8, 6, 2, 3},  {32, 16, 8, 5, 4, 3},
-    {32, 8, 8, 5, 3, 3},   {16, 32, 8, 4, 5, 3}, {16, 16, 8, 4, 4, 3},
-    {8, 64, 8, 3, 6, 3},   {8, 32, 8, 3, 5, 3},  {4, 128, 8, 2, 7, 3},
-    {4, 64, 8, 2, 6, 3},   {64, 4, 16, 6, 2, 4}, {32, 8, 16, 5, 3, 4},
-    {16, 16, 16, 4, 4, 4}, {8, 32, 16, 3, 5, 4}, {4, 64, 16, 2, 6, 4},
+    {64, 4, 8, 6, 2, 3},   {32, 8, 8, 5, 3, 3},   {16, 16, 8, 4, 4, 3},
+    {8, 32, 8, 3, 5, 3},   {4, 64, 8, 2, 6, 3},   {64, 4, 16, 6, 2, 4},
+    {32, 8, 16, 5, 3, 4},  {16, 16, 16, 4, 4, 4}, {8, 32, 16, 3, 5, 4},
+    {4, 64, 16, 2, 6, 4},
 };
 constexpr int kNumTileConfigs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -161,17 +160,17 @@ bool cfg_fits(int cfg, const SepTask &t) {
 int default_cfg(const SepTask &t) {
   const int nx = t.rhi[0] - t.rlo[0] + 1;
   const int ny = t.rhi[1] - t.rlo[1] + 1;
-  if (t.whi[2] - t.wlo[2] + 1 > 8)
-    return nx > 32 ? 10 : nx > 16 ? 11 : nx > 8 ? 12 : nx > 4 ? 13 : 14;
+  (void)ny;
+  const int base = t.whi[2] - t.wlo[2] + 1 > 8 ? 5 : 0;
   if (nx <= 4)
-    return ny > 64 ? 8 : 9;
+    return base + 4;
   if (nx <= 8)
-    return ny > 32 ? 6 : 7;
+    return base + 3;
   if (nx <= 16)
-    return ny > 16 ? 4 : 5;
+    return base + 2;
   if (nx <= 32)
-    return ny > 8 ? 2 : 3;
-  return ny <= 4 ? 1 : 0;
+    return base + 1;
+  return base + 0;
 }
 
 using TuneKey = std::tuple<int, int, int, int>;
@@ -257,8 +256,10 @@ int launch_sep_box(const SepTask &t, int ncomp, hipStream_t s) {
                                ncomp < 1024)
     zchunk = (zchunk + 1) / 2;
   const int nzc = (nz + zchunk - 1) / zchunk;
-  hipLaunchKernelGGL(k_sep_box, dim3(ntx * nty * nzc, ncomp), dim3(CXP_NT), 0,
-                     s, t, c.lx, c.ly, c.lr, ntx, nty, zchunk);
+  const size_t lds =
+      sizeof(double) * geom_lds_doubles(make_geom(c.lx, c.ly, c.lr));
+  hipLaunchKernelGGL(k_sep_box, dim3(ntx * nty * nzc, ncomp), dim3(CXP_NT),
+                     lds, s, t, c.lx, c.ly, c.lr, ntx, nty, zchunk);
   return hip_status(hipGetLastError());
 }
 
@@ -297,6 +298,7 @@ struct cxp_worklist_s {
   size_t cap_tasks = 0, cap_tiles = 0;
   bool uploaded = false;
   int grid = 0;
+  size_t lds = 0;  // dynamic LDS of the batched launch (max over tile shapes)
   int64_t fine_points = 0, algo_bytes = 0;
 };
 
@@ -320,6 +322,7 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
   wl->tasks.clear();
   wl->tiles.clear();
   wl->point_patches.clear();
+  wl->lds = 0;
   for (size_t n = 0; n < wl->patches.size(); ++n) {
     const auto &p = wl->patches[n];
     if (!op_is_separable(p.op->d, false)) {
@@ -339,6 +342,8 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
     const int task = (int)wl->tasks.size();
     wl->tasks.push_back(t);
     const TileCfg &c = kCfgs[cfg];
+    wl->lds = std::max(wl->lds, sizeof(double) * (size_t)geom_lds_doubles(
+                                                     make_geom(c.lx, c.ly, c.lr)));
     for (int comp = 0; comp < p.ncomp; ++comp)
       for (int z0 = p.rlo[2]; z0 <= p.rhi[2]; z0 += zc)
         for (int y0 = p.rlo[1]; y0 <= p.rhi[1]; y0 += c.ty)
@@ -390,8 +395,8 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
   return CXP_OK;
 }
 
-int persistent_grid(int ntiles) {
-  static int cus = 0, per_cu = 0;
+int persistent_grid(int ntiles, size_t lds) {
+  static int cus = 0;
   if (!cus) {
     int dev = 0;
     (void)hipGetDevice(&dev);
@@ -400,14 +405,12 @@ int persistent_grid(int ntiles) {
       cus = prop.multiProcessorCount;
     if (cus <= 0)
       cus = 256;
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sep_batched,
-                                                     CXP_NT, 0) == hipSuccess &&
-        nb > 0)
-      per_cu = nb;
-    else
-      per_cu = 2;
   }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sep_batched,
+                                                   CXP_NT, lds) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 2;
   int g = cus * per_cu;
   g -= g % 8;
   return std::max(1, std::min(ntiles, g));
@@ -415,7 +418,7 @@ int persistent_grid(int ntiles) {
 
 int launch_batched(cxp_worklist_s *wl, hipStream_t s) {
   if (!wl->tiles.empty()) {
-    hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), 0, s,
+    hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), wl->lds, s,
                        wl->d_tasks, wl->d_tiles, (int)wl->tiles.size());
     if (hipGetLastError() != hipSuccess)
       return CXP_ERR_HIP;
@@ -740,7 +743,7 @@ int cxp_worklist_upload(cxp_worklist wl, void *stream) {
   const int e = upload_lists(wl, (hipStream_t)stream);
   if (e)
     return e;
-  wl->grid = persistent_grid((int)wl->tiles.size());
+  wl->grid = persistent_grid((int)wl->tiles.size(), wl->lds);
   wl->uploaded = true;
   return CXP_OK;
 }
@@ -870,12 +873,13 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
           continue;
         if ((status = upload_lists(wl, s)))
           break;
-        wl->grid = persistent_grid((int)wl->tiles.size());
+        wl->grid = persistent_grid((int)wl->tiles.size(), wl->lds);
         float tot = 0;
         for (int r = 0; r <= reps; ++r) {
           (void)hipEventRecord(e0, s);
-          hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), 0, s,
-                             wl->d_tasks, wl->d_tiles, (int)wl->tiles.size());
+          hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT),
+                             wl->lds, s, wl->d_tasks, wl->d_tiles,
+                             (int)wl->tiles.size());
           (void)hipEventRecord(e1, s);
           if (hipEventSynchronize(e1) != hipSuccess) {
             status = CXP_ERR_HIP;

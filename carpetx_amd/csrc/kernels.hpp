@@ -34,6 +34,20 @@ struct DFab {
   long long sz;  // plane stride
 };
 
+// Global-address-space accessors.  Pointers that come out of task structs
+// are generic; dereferenced as such the compiler emits flat instructions,
+// which also count against lgkmcnt, so every LDS barrier would wait for all
+// outstanding HBM traffic.  Casting to address space 1 gives global_load /
+// global_store, tracked by vmcnt only.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) const double gcdouble;
+__device__ __forceinline__ double gld(const double *p, long long i) {
+  return ((gcdouble *)p)[i];
+}
+__device__ __forceinline__ void gst(double *p, long long i, double v) {
+  ((gdouble *)p)[i] = v;
+}
+
 __device__ __forceinline__ long long didx(const DFab &f, int i, int j, int k) {
   return (long long)(i - f.lo[0]) + (long long)(j - f.lo[1]) * f.sy +
          (long long)(k - f.lo[2]) * f.sz;
@@ -88,13 +102,13 @@ __host__ __device__ inline int geom_lds_doubles(const Geom &g) {
   return g.cys * g.cxs + g.cys * (1 << g.lx) + (1 << g.lr) * g.npt;
 }
 
-// LDS budget per workgroup (doubles): sized for 512 fine points per plane
-// with an 8-plane ring, or 256 points with a 16-plane ring.
+// LDS budget per workgroup (doubles): 256 fine points per plane with up to
+// a 16-plane ring.  Launches size their dynamic LDS to the shapes they use.
 constexpr int kLdsDoubles = 5632;
 #define CXP_NT 256
-// Coarse values per thread and plane: the largest coarse tile of any shape
-// in use is (TX/2 + 10) x (TY/2 + 10) <= 12 x 74 = 888 <= 4 * 256.
-#define CXP_PRE 4
+// Coarse values per thread and plane: every shape has TX * TY = 256 and a
+// coarse tile of at most (TX/2 + 9) x (TY/2 + 9) <= 41 x 11 = 451 <= 2 * 256.
+#define CXP_PRE 2
 
 template <int SID>
 __device__ __forceinline__ void sep_xpass(const Geom &g,
@@ -155,7 +169,7 @@ __device__ __forceinline__ void sep_zpass(const Geom &g,
       if (fi < TXa && fj < TYa) {
         const double v = Op1<SID>::apply(
             [&](int o) { return ring[((kc + o) & rmask) * g.npt + p]; }, off);
-        fine.p[didx(fine, x0 + fi, y0 + fj, fz)] = v;
+        gst(fine.p, didx(fine, x0 + fi, y0 + fj, fz), v);
       }
     }
   }
@@ -208,7 +222,7 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     const double *src = crse.p + didx(crse, ci0, cj0, kf);
 #pragma unroll
     for (int r = 0; r < CXP_PRE; ++r)
-      pre[r] = soff[r] >= 0 ? src[loff[r]] : 0.0;
+      pre[r] = soff[r] >= 0 ? gld(src, loff[r]) : 0.0;
   }
 
   for (int kp = kf; kp <= kl; ++kp) {
@@ -222,7 +236,7 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
 #pragma unroll
       for (int r = 0; r < CXP_PRE; ++r)
         if (soff[r] >= 0)
-          pre[r] = src[loff[r]];
+          pre[r] = gld(src, loff[r]);
     }
     __syncthreads();
     switch (sx) {
@@ -254,7 +268,6 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
       }
     }
   }
-  __syncthreads();
 }
 
 // Per-box launch: grid.x = tiles (x fastest, then y, then z-chunks),
@@ -262,7 +275,7 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
 __global__ void __launch_bounds__(CXP_NT)
     k_sep_box(SepTask t, int lx, int ly, int lr, int ntx, int nty,
               int zchunk) {
-  __shared__ double smem[kLdsDoubles];
+  extern __shared__ double smem[];
   const Geom g = make_geom(lx, ly, lr);
   const int b = blockIdx.x;
   const int tx = b % ntx;
@@ -283,7 +296,7 @@ __global__ void __launch_bounds__(CXP_NT)
 __global__ void __launch_bounds__(CXP_NT)
     k_sep_batched(const SepTask *__restrict__ tasks,
                   const SepTile *__restrict__ tiles, int ntiles) {
-  __shared__ double smem[kLdsDoubles];
+  extern __shared__ double smem[];
   const int nblk = gridDim.x;
   const int bid = blockIdx.x;
   int vb = bid;
