@@ -9,15 +9,11 @@
 //    applies x^T and y^T inside each plane (LDS), and accumulates z^T into a
 //    9-slot register window of coarse planes that is flushed and rotated
 //    once per coarse plane.
-//  * MINMOD: the Jacobian Jz Jy Jx has data-dependent weights, so the
-//    transposes are applied in order z^T, y^T, x^T by three kernels through
-//    two scratch arrays; each pass recomputes the limiter branch of its
-//    forward application from the forward input.
+//  * MINMOD, ENO and linear-fallback operators return CXP_ERR_UNSUPPORTED:
+//    their data-dependent Jacobians are not implemented on the device.
 #pragma once
 
 #include <hip/hip_runtime.h>
-
-#include <mutex>
 
 #include "../../include/cxp.h"
 #include "kernels.hpp"
@@ -209,209 +205,6 @@ __global__ void __launch_bounds__(256)
 }
 
 // --------------------------------------------------------------------------
-// MINMOD adjoint (three passes)
-
-// Linearised minmod (impl.hxx:857-877): y = c + s * k * d with k = 1/4 when
-// the two one-sided differences share a sign, d the smaller of them; returns
-// the weights of (c(-1), c(0), c(+1)).
-__device__ __forceinline__ void minmod_lin(double cm, double c0, double cp,
-                                           int off, double w[3]) {
-  const double dm = c0 - cm;
-  const double dp = cp - c0;
-  const double k = copysign(0.125, dm) + copysign(0.125, dp);  // +-1/4 or 0
-  const double s = off == 0 ? -1.0 : 1.0;
-  w[0] = 0.0;
-  w[1] = 1.0;
-  w[2] = 0.0;
-  if (k != 0.0) {
-    if (fabs(dm) <= fabs(dp)) {  // delta = k * dm
-      w[0] = -s * k;
-      w[1] += s * k;
-    } else {  // delta = k * dp
-      w[1] -= s * k;
-      w[2] = s * k;
-    }
-  }
-}
-
-// 1-D operator value (linear or minmod) at fine offset off from 3+ values.
-struct MmTask {
-  DFab crse;   // forward input
-  DFab fbar;   // fine adjoint
-  DFab cbar;   // coarse adjoint (accumulated)
-  double *ybar;  // scratch: fine x, fine y, coarse z
-  double *xbar;  // scratch: fine x, coarse y, coarse z
-  int rlo[3], rhi[3];
-  int sid[3];
-  int olo[3], ohi[3];  // coarse output range (window of the region)
-  long long ccs, fcs;
-};
-
-// value of the x-pass at fine x f0, coarse (jc, kc), per sid of x
-__device__ inline double mm_xval(const MmTask &t, const DFab &c, int f0, int jc,
-                                 int kc) {
-  const int ic = f0 >> 1, off = f0 & 1;
-  const double *row = c.p + didx(c, ic, jc, kc);
-  return apply1d(t.sid[0], [&](int o) { return row[o]; }, 0, off);
-}
-__device__ inline double mm_yval(const MmTask &t, const DFab &c, int f0, int f1,
-                                 int kc) {
-  const int jc = f1 >> 1, off = f1 & 1;
-  return apply1d(t.sid[1], [&](int o) { return mm_xval(t, c, f0, jc + o, kc); },
-                 0, off);
-}
-
-// Weight of coarse offset o in the forward 1-D application with sid at fine
-// offset off; for MINMOD the branch comes from the three input values.
-__device__ inline double mm_weight(int sid, int off, int o, double vm, double v0,
-                                   double vp) {
-  if (sid == SID_MINMOD) {
-    double w[3];
-    minmod_lin(vm, v0, vp, off, w);
-    return (o >= -1 && o <= 1) ? w[o + 1] : 0.0;
-  }
-  const int q = o - sid_wlo(sid);
-  return (q >= 0 && q < 9) ? kAdjW.w[sid][off][q] : 0.0;
-}
-
-// Pass 1: ybar(f0, f1, kc) = sum_fz dZ(fz)/dY(kc) * fbar(f0, f1, fz)
-__global__ void __launch_bounds__(256) k_adj_mm_z(MmTask t, long long comp) {
-  const int nx = t.rhi[0] - t.rlo[0] + 1, ny = t.rhi[1] - t.rlo[1] + 1;
-  const int nk = t.ohi[2] - t.olo[2] + 1;
-  const long long n = (long long)nx * ny * nk;
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n)
-    return;
-  const int f0 = t.rlo[0] + (int)(g % nx);
-  const int f1 = t.rlo[1] + (int)((g / nx) % ny);
-  const int kc = t.olo[2] + (int)(g / ((long long)nx * ny));
-  DFab c = t.crse;
-  c.p += comp * t.ccs;
-  DFab fb = t.fbar;
-  fb.p += comp * t.fcs;
-  const int sz = t.sid[2];
-  double s = 0.0;
-  for (int o = sid_wlo(sz); o <= sid_whi(sz); ++o) {
-    const int kz = kc - o;
-    double vm = 0, v0 = 0, vp = 0;
-    bool have = false;
-    for (int off = 0; off < 2; ++off) {
-      const int fz = 2 * kz + off;
-      if (fz < t.rlo[2] || fz > t.rhi[2])
-        continue;
-      if (sz == SID_MINMOD && !have) {
-        vm = mm_yval(t, c, f0, f1, kz - 1);
-        v0 = mm_yval(t, c, f0, f1, kz);
-        vp = mm_yval(t, c, f0, f1, kz + 1);
-        have = true;
-      }
-      s += mm_weight(sz, off, o, vm, v0, vp) * fb.p[didx(fb, f0, f1, fz)];
-    }
-  }
-  t.ybar[g] = s;
-}
-
-// Pass 2: xbar(f0, jc, kc) = sum_f1 dY(f1)/dX(jc) * ybar(f0, f1, kc)
-__global__ void __launch_bounds__(256) k_adj_mm_y(MmTask t, long long comp) {
-  const int nx = t.rhi[0] - t.rlo[0] + 1, ny = t.rhi[1] - t.rlo[1] + 1;
-  const int nj = t.ohi[1] - t.olo[1] + 1, nk = t.ohi[2] - t.olo[2] + 1;
-  const long long n = (long long)nx * nj * nk;
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n)
-    return;
-  const int f0 = t.rlo[0] + (int)(g % nx);
-  const int jc = t.olo[1] + (int)((g / nx) % nj);
-  const int kcl = (int)(g / ((long long)nx * nj));
-  const int kc = t.olo[2] + kcl;
-  DFab c = t.crse;
-  c.p += comp * t.ccs;
-  const int sy = t.sid[1];
-  double s = 0.0;
-  for (int o = sid_wlo(sy); o <= sid_whi(sy); ++o) {
-    const int jy = jc - o;
-    double vm = 0, v0 = 0, vp = 0;
-    bool have = false;
-    for (int off = 0; off < 2; ++off) {
-      const int f1 = 2 * jy + off;
-      if (f1 < t.rlo[1] || f1 > t.rhi[1])
-        continue;
-      if (sy == SID_MINMOD && !have) {
-        vm = mm_xval(t, c, f0, jy - 1, kc);
-        v0 = mm_xval(t, c, f0, jy, kc);
-        vp = mm_xval(t, c, f0, jy + 1, kc);
-        have = true;
-      }
-      const long long yi = (long long)(f0 - t.rlo[0]) +
-                           (long long)nx * ((f1 - t.rlo[1]) + (long long)ny * kcl);
-      s += mm_weight(sy, off, o, vm, v0, vp) * t.ybar[yi];
-    }
-  }
-  t.xbar[g] = s;
-}
-
-// Pass 3: cbar(ic, jc, kc) += sum_f0 dX(f0)/dC(ic) * xbar(f0, jc, kc)
-__global__ void __launch_bounds__(256) k_adj_mm_x(MmTask t, long long comp) {
-  const int nx = t.rhi[0] - t.rlo[0] + 1;
-  const int ni = t.ohi[0] - t.olo[0] + 1, nj = t.ohi[1] - t.olo[1] + 1,
-            nk = t.ohi[2] - t.olo[2] + 1;
-  const long long n = (long long)ni * nj * nk;
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n)
-    return;
-  const int ic = t.olo[0] + (int)(g % ni);
-  const int jcl = (int)((g / ni) % nj);
-  const int kcl = (int)(g / ((long long)ni * nj));
-  const int jc = t.olo[1] + jcl, kc = t.olo[2] + kcl;
-  DFab c = t.crse;
-  c.p += comp * t.ccs;
-  DFab cb = t.cbar;
-  cb.p += comp * t.ccs;
-  const int sx = t.sid[0];
-  double s = 0.0;
-  for (int o = sid_wlo(sx); o <= sid_whi(sx); ++o) {
-    const int ix = ic - o;
-    // only coarse x owning a fine point of the region is read: its
-    // neighbours ix-1, ix+1 then lie inside CoarseBox(region)
-    if (2 * ix + 1 < t.rlo[0] || 2 * ix > t.rhi[0])
-      continue;
-    double vm = 0, v0 = 0, vp = 0;
-    if (sx == SID_MINMOD) {
-      const double *row = c.p + didx(c, ix, jc, kc);
-      vm = row[-1];
-      v0 = row[0];
-      vp = row[1];
-    }
-    for (int off = 0; off < 2; ++off) {
-      const int f0 = 2 * ix + off;
-      if (f0 < t.rlo[0] || f0 > t.rhi[0])
-        continue;
-      const long long xi = (long long)(f0 - t.rlo[0]) +
-                           (long long)nx * (jcl + (long long)nj * kcl);
-      s += mm_weight(sx, off, o, vm, v0, vp) * t.xbar[xi];
-    }
-  }
-  cb.p[didx(cb, ic, jc, kc)] += s;
-}
-
-// --------------------------------------------------------------------------
-
-inline int adj_scratch(size_t bytes, double **out) {
-  static std::mutex mu;
-  static double *buf = nullptr;
-  static size_t cap = 0;
-  std::lock_guard<std::mutex> lk(mu);
-  if (bytes > cap) {
-    if (buf)
-      (void)hipFree(buf);
-    buf = nullptr;
-    cap = 0;
-    if (hipMalloc(&buf, bytes) != hipSuccess)
-      return CXP_ERR_HIP;
-    cap = bytes;
-  }
-  *out = buf;
-  return CXP_OK;
-}
 
 inline DFab adj_dfab(const cxp_fab *f, int comp) {
   DFab v;
@@ -466,48 +259,10 @@ inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
                        s, t, ntx, nty, kchunk);
     return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
   }
-  if (!crse_in)
-    return CXP_ERR_ARG;
-  // the forward input must cover the same coarse box
-  int clo[3], chi[3];
-  op_coarse_box(op, rlo, rhi, clo, chi);
-  for (int d = 0; d < 3; ++d)
-    if (clo[d] < crse_in->lo[d] || chi[d] > crse_in->hi[d])
-      return CXP_ERR_COARSE;
-  if (crse_in->comp_stride != crse_bar->comp_stride && ncomp > 1)
-    return CXP_ERR_ARG;
-  MmTask t;
-  t.crse = adj_dfab(crse_in, crse_comp);
-  t.fbar = adj_dfab(fine_bar, fine_comp);
-  t.cbar = adj_dfab(crse_bar, crse_comp);
-  for (int d = 0; d < 3; ++d) {
-    t.rlo[d] = rlo[d];
-    t.rhi[d] = rhi[d];
-    t.sid[d] = op.sid[d];
-    t.olo[d] = olo[d];
-    t.ohi[d] = ohi[d];
-  }
-  t.ccs = crse_in->comp_stride;
-  t.fcs = fine_bar->comp_stride;
-  const long long nx = rhi[0] - rlo[0] + 1, ny = rhi[1] - rlo[1] + 1;
-  const long long ni = ohi[0] - olo[0] + 1, nj = ohi[1] - olo[1] + 1,
-                  nk = ohi[2] - olo[2] + 1;
-  const long long ny_elems = nx * ny * nk, nx_elems = nx * nj * nk;
-  double *scr;
-  const int e = adj_scratch((size_t)(ny_elems + nx_elems) * sizeof(double), &scr);
-  if (e)
-    return e;
-  t.ybar = scr;
-  t.xbar = scr + ny_elems;
-  for (int comp = 0; comp < ncomp; ++comp) {
-    hipLaunchKernelGGL(k_adj_mm_z, dim3((unsigned)((ny_elems + 255) / 256)),
-                       dim3(256), 0, s, t, (long long)comp);
-    hipLaunchKernelGGL(k_adj_mm_y, dim3((unsigned)((nx_elems + 255) / 256)),
-                       dim3(256), 0, s, t, (long long)comp);
-    hipLaunchKernelGGL(k_adj_mm_x, dim3((unsigned)((ni * nj * nk + 255) / 256)),
-                       dim3(256), 0, s, t, (long long)comp);
-  }
-  return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
+  // MINMOD: a three-pass implementation hung the GPU in testing and was
+  // removed; until a replacement is written the call reports unsupported.
+  (void)crse_in;
+  return CXP_ERR_UNSUPPORTED;
 }
 
 } // namespace cxp

@@ -85,6 +85,12 @@ def test_dense_transpose_small_box(fam, order, it):
     assert np.max(np.abs(got - expect)) <= 1e-13 * max(1.0, np.max(np.abs(expect)))
 
 
+MINMOD_OFF = ("the MINMOD adjoint is switched off: its first implementation hung the "
+              "GPU twice in testing and was removed; the entry point returns "
+              "CXP_ERR_UNSUPPORTED until it is rewritten")
+
+
+@pytest.mark.skip(reason=MINMOD_OFF)
 @pytest.mark.parametrize("it", [(1, 1, 1), (1, 0, 1), (0, 1, 0)])
 def test_minmod_adjoint_against_dense_jacobian(it):
     rng = np.random.default_rng(4)
@@ -115,6 +121,7 @@ def test_minmod_adjoint_against_dense_jacobian(it):
     assert np.max(np.abs(got - expect)) <= 1e-7 * max(1.0, np.max(np.abs(expect)))
 
 
+@pytest.mark.skip(reason=MINMOD_OFF)
 def test_minmod_dot_product_with_linearisation():
     """<J u, v> == <u, J^T v> where J u = P(u) for a positively homogeneous
     limiter (minmod(a u) = a minmod(u) for a > 0, so P(u) = J(u) u)."""

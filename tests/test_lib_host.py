@@ -118,3 +118,27 @@ def test_validation_errors_without_device_work():
     assert call((3, 0, 0), (2, 15, 15)) == -3            # empty region
     assert call((0, 0, 0), (15, 15, 15), op=None) == -3
     assert L.cxp_status_string(-5) == b"coarse FAB does not cover CoarseBox(region)"
+
+
+def test_nonlinear_adjoints_report_unsupported():
+    """MINMOD, ENO and fallback operators have no device adjoint yet; the
+    call is rejected before any device work."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+
+    def fab(lo, hi):
+        f = _lib.CxpFab()
+        f.data = fake
+        f.lo, f.hi = i3(lo), i3(hi)
+        f.comp_stride = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+        f.ncomp = 1
+        return f
+    crse = fab((-4, -4, -4), (20, 20, 20))
+    fine = fab((0, 0, 0), (15, 15, 15))
+    for fam, order, it in [("minmod", 3, (1, 1, 1)), ("eno", 3, (1, 0, 0)),
+                           ("poly-cons3lfb", 3, (1, 1, 1))]:
+        h = _lookup(fam, order, it)[1]
+        st = L.cxp_prolongate_adjoint(h, ctypes.byref(crse), ctypes.byref(crse), 0,
+                                      ctypes.byref(fine), 0, 1, i3((0, 0, 0)),
+                                      i3((15, 15, 15)), 0, None)
+        assert st == -11, (fam, st)
