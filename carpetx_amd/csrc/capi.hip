@@ -102,31 +102,25 @@ int hip_status(hipError_t e) { return e == hipSuccess ? CXP_OK : CXP_ERR_HIP; }
 // Tile-shape selection
 
 struct TileCfg {
-  int tx, ty, ring;
-  int lx, ly, lr;
+  int tx, ty, lx;
 };
-// Tile shapes (fine TX x TY per plane, ring depth R); every one fits the
-// kernel's LDS budget (checked at first use).
+// Tile shapes: TX x TY fine points per plane, TX * TY = 512 (two fine y per
+// thread).  Every one fits the kernel's LDS budget (checked at first use).
 const TileCfg kCfgs[] = {
-    {64, 4, 8, 6, 2, 3},   {32, 8, 8, 5, 3, 3},   {16, 16, 8, 4, 4, 3},
-    {8, 32, 8, 3, 5, 3},   {4, 64, 8, 2, 6, 3},   {64, 4, 16, 6, 2, 4},
-    {32, 8, 16, 5, 3, 4},  {16, 16, 16, 4, 4, 4}, {8, 32, 16, 3, 5, 4},
-    {4, 64, 16, 2, 6, 4},
+    {128, 4, 7}, {64, 8, 6}, {32, 16, 5}, {16, 32, 4}, {8, 64, 3}, {4, 128, 2},
 };
 constexpr int kNumTileConfigs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-int cfg_code(int cfg) {
-  return kCfgs[cfg].lx | (kCfgs[cfg].ly << 4) | (kCfgs[cfg].lr << 8);
-}
+int cfg_code(int cfg) { return kCfgs[cfg].lx; }
 
 bool cfgs_checked() {
   for (int c = 0; c < kNumTileConfigs; ++c) {
-    const Geom g = make_geom(kCfgs[c].lx, kCfgs[c].ly, kCfgs[c].lr);
+    const Geom g = make_geom(kCfgs[c].lx);
     const int max_coarse = (kCfgs[c].tx / 2 + 9) * (kCfgs[c].ty / 2 + 9);
-    if (geom_lds_doubles(g) > kLdsDoubles || g.npt % CXP_NT != 0 ||
-        max_coarse > CXP_PRE * CXP_NT || max_coarse > g.cxs * g.cys ||
-        (1 << kCfgs[c].lx) != kCfgs[c].tx || (1 << kCfgs[c].ly) != kCfgs[c].ty ||
-        (1 << kCfgs[c].lr) != kCfgs[c].ring)
+    if (geom_lds_doubles(g) > kLdsDoubles || g.tx * g.ty != 2 * CXP_NT ||
+        max_coarse > CXP_PRE * CXP_NT || g.tx != kCfgs[c].tx ||
+        g.ty != kCfgs[c].ty || (kCfgs[c].tx / 2 + 9) > g.cxs ||
+        (kCfgs[c].ty / 2 + 9) > g.cys)
       return false;
   }
   return true;
@@ -150,27 +144,28 @@ int size_class(const int rlo[3], const int rhi[3]) {
 }
 
 bool cfg_fits(int cfg, const SepTask &t) {
-  const TileCfg &c = kCfgs[cfg];
+  if (cfg < 0 || cfg >= kNumTileConfigs)
+    return false;
   for (int d = 0; d < 3; ++d)
-    if (t.whi[d] - t.wlo[d] + 1 > 9)
+    if (t.whi[d] - t.wlo[d] + 1 > CXP_ZW)
       return false;
-  return t.whi[2] - t.wlo[2] + 1 <= c.ring;
+  return true;
 }
 
 int default_cfg(const SepTask &t) {
-  const int nx = t.rhi[0] - t.rlo[0] + 1;
-  const int ny = t.rhi[1] - t.rlo[1] + 1;
-  (void)ny;
-  const int base = t.whi[2] - t.wlo[2] + 1 > 8 ? 5 : 0;
+  const int nx = t.rhi[0] - (t.rlo[0] & ~1) + 1;
+  const int ny = t.rhi[1] - (t.rlo[1] & ~1) + 1;
   if (nx <= 4)
-    return base + 4;
+    return 5;
   if (nx <= 8)
-    return base + 3;
+    return 4;
   if (nx <= 16)
-    return base + 2;
+    return 3;
   if (nx <= 32)
-    return base + 1;
-  return base + 0;
+    return 2;
+  if (ny <= 4 || nx > 64)
+    return 0;
+  return 1;
 }
 
 using TuneKey = std::tuple<int, int, int, int>;
@@ -245,8 +240,8 @@ int launch_sep_box(const SepTask &t, int ncomp, hipStream_t s) {
   int cfg, zc;
   choose_cfg(t, cfg, zc);
   const TileCfg &c = kCfgs[cfg];
-  const int nx = t.rhi[0] - t.rlo[0] + 1;
-  const int ny = t.rhi[1] - t.rlo[1] + 1;
+  const int nx = t.rhi[0] - (t.rlo[0] & ~1) + 1;
+  const int ny = t.rhi[1] - (t.rlo[1] & ~1) + 1;
   const int nz = t.rhi[2] - t.rlo[2] + 1;
   const int ntx = (nx + c.tx - 1) / c.tx;
   const int nty = (ny + c.ty - 1) / c.ty;
@@ -256,10 +251,8 @@ int launch_sep_box(const SepTask &t, int ncomp, hipStream_t s) {
                                ncomp < 1024)
     zchunk = (zchunk + 1) / 2;
   const int nzc = (nz + zchunk - 1) / zchunk;
-  const size_t lds =
-      sizeof(double) * geom_lds_doubles(make_geom(c.lx, c.ly, c.lr));
-  hipLaunchKernelGGL(k_sep_box, dim3(ntx * nty * nzc, ncomp), dim3(CXP_NT),
-                     lds, s, t, c.lx, c.ly, c.lr, ntx, nty, zchunk);
+  hipLaunchKernelGGL(k_sep_box, dim3(ntx * nty * nzc, ncomp), dim3(CXP_NT), 0,
+                     s, t, c.lx, ntx, nty, zchunk);
   return hip_status(hipGetLastError());
 }
 
@@ -298,7 +291,6 @@ struct cxp_worklist_s {
   size_t cap_tasks = 0, cap_tiles = 0;
   bool uploaded = false;
   int grid = 0;
-  size_t lds = 0;  // dynamic LDS of the batched launch (max over tile shapes)
   int64_t fine_points = 0, algo_bytes = 0;
 };
 
@@ -322,7 +314,6 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
   wl->tasks.clear();
   wl->tiles.clear();
   wl->point_patches.clear();
-  wl->lds = 0;
   for (size_t n = 0; n < wl->patches.size(); ++n) {
     const auto &p = wl->patches[n];
     if (!op_is_separable(p.op->d, false)) {
@@ -342,12 +333,10 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
     const int task = (int)wl->tasks.size();
     wl->tasks.push_back(t);
     const TileCfg &c = kCfgs[cfg];
-    wl->lds = std::max(wl->lds, sizeof(double) * (size_t)geom_lds_doubles(
-                                                     make_geom(c.lx, c.ly, c.lr)));
     for (int comp = 0; comp < p.ncomp; ++comp)
       for (int z0 = p.rlo[2]; z0 <= p.rhi[2]; z0 += zc)
-        for (int y0 = p.rlo[1]; y0 <= p.rhi[1]; y0 += c.ty)
-          for (int x0 = p.rlo[0]; x0 <= p.rhi[0]; x0 += c.tx) {
+        for (int y0 = p.rlo[1] & ~1; y0 <= p.rhi[1]; y0 += c.ty)
+          for (int x0 = p.rlo[0] & ~1; x0 <= p.rhi[0]; x0 += c.tx) {
             SepTile tl;
             tl.task = task;
             tl.comp = comp;
@@ -395,7 +384,7 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
   return CXP_OK;
 }
 
-int persistent_grid(int ntiles, size_t lds) {
+int persistent_grid(int ntiles) {
   static int cus = 0;
   if (!cus) {
     int dev = 0;
@@ -408,7 +397,7 @@ int persistent_grid(int ntiles, size_t lds) {
   }
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sep_batched,
-                                                   CXP_NT, lds) != hipSuccess ||
+                                                   CXP_NT, 0) != hipSuccess ||
       per_cu <= 0)
     per_cu = 2;
   int g = cus * per_cu;
@@ -418,7 +407,7 @@ int persistent_grid(int ntiles, size_t lds) {
 
 int launch_batched(cxp_worklist_s *wl, hipStream_t s) {
   if (!wl->tiles.empty()) {
-    hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), wl->lds, s,
+    hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), 0, s,
                        wl->d_tasks, wl->d_tiles, (int)wl->tiles.size());
     if (hipGetLastError() != hipSuccess)
       return CXP_ERR_HIP;
@@ -743,7 +732,7 @@ int cxp_worklist_upload(cxp_worklist wl, void *stream) {
   const int e = upload_lists(wl, (hipStream_t)stream);
   if (e)
     return e;
-  wl->grid = persistent_grid((int)wl->tiles.size(), wl->lds);
+  wl->grid = persistent_grid((int)wl->tiles.size());
   wl->uploaded = true;
   return CXP_OK;
 }
@@ -827,15 +816,15 @@ int cxp_graph_destroy(cxp_graph g) {
 
 int cxp_tile_config_count(void) { return kNumTileConfigs; }
 
-int cxp_tile_config(int cfg, int *tx, int *ty, int *ring) {
+int cxp_tile_config(int cfg, int *tx, int *ty, int *zwindow) {
   if (cfg < 0 || cfg >= kNumTileConfigs)
     return CXP_ERR_ARG;
   if (tx)
     *tx = kCfgs[cfg].tx;
   if (ty)
     *ty = kCfgs[cfg].ty;
-  if (ring)
-    *ring = kCfgs[cfg].ring;
+  if (zwindow)
+    *zwindow = CXP_ZW;
   return CXP_OK;
 }
 
@@ -873,12 +862,12 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
           continue;
         if ((status = upload_lists(wl, s)))
           break;
-        wl->grid = persistent_grid((int)wl->tiles.size(), wl->lds);
+        wl->grid = persistent_grid((int)wl->tiles.size());
         float tot = 0;
         for (int r = 0; r <= reps; ++r) {
           (void)hipEventRecord(e0, s);
-          hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT),
-                             wl->lds, s, wl->d_tasks, wl->d_tiles,
+          hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), 0,
+                             s, wl->d_tasks, wl->d_tiles,
                              (int)wl->tiles.size());
           (void)hipEventRecord(e1, s);
           if (hipEventSynchronize(e1) != hipSuccess) {

@@ -69,7 +69,7 @@ struct SepTile {
   int comp;  // component offset applied with ccs/fcs
   int x0, y0;
   int z0, z1;
-  int cfg;  // tile shape: lx | ly << 4 | lr << 8
+  int cfg;  // tile shape: log2 of the fine tile width
   int pad;
 };
 
@@ -80,118 +80,99 @@ struct SepTile {
   X(SID_CCONS1) X(SID_CCONS2) X(SID_CCONS3) X(SID_CCONS4) X(SID_CCONS5)        \
   X(SID_CCONS6) X(SID_CCONS7) X(SID_MINMOD)
 
-// Runtime tile geometry.  TX and TY are powers of two (x-lanes of a wave stay
-// on one fine row when TX >= 64); R is the ring depth in coarse planes.
+// Runtime tile geometry.  A workgroup of 256 threads owns a TX x TY fine
+// (x, y) tile with TX * TY = 512: thread t owns fine x = x0 + (t % TX) and
+// the fine y pair y0 + 2 (t / TX) + {0, 1}.  Tiles start at even x and y, so
+// every pair shares one coarse index.
 struct Geom {
-  int lx, ly, lr;    // log2 of TX, TY, R
-  int cxs, cys;      // LDS capacity of the coarse tile (row stride, rows)
-  int npt;           // TX * TY
+  int lx;        // log2 TX
+  int tx, ty;    // fine tile extents
+  int cxs, cys;  // LDS capacity of the coarse tile (row stride, rows)
+  int xs_off;    // start of the x-pass buffer (16-byte aligned)
 };
 
-__host__ __device__ inline Geom make_geom(int lx, int ly, int lr) {
+__host__ __device__ inline Geom make_geom(int lx) {
   Geom g;
   g.lx = lx;
-  g.ly = ly;
-  g.lr = lr;
-  g.cxs = (1 << lx) / 2 + 10;  // fine tile / 2 + 1 + 8 halo, padded
-  g.cys = (1 << ly) / 2 + 10;
-  g.npt = 1 << (lx + ly);
+  g.tx = 1 << lx;
+  g.ty = 512 >> lx;
+  g.cxs = g.tx / 2 + 10;  // tile / 2 + 1 + 8 halo, padded
+  g.cys = g.ty / 2 + 10;
+  g.xs_off = (g.cys * g.cxs + 1) & ~1;
   return g;
 }
 __host__ __device__ inline int geom_lds_doubles(const Geom &g) {
-  return g.cys * g.cxs + g.cys * (1 << g.lx) + (1 << g.lr) * g.npt;
+  return g.xs_off + g.cys * g.tx;
 }
 
-// LDS budget per workgroup (doubles): 256 fine points per plane with up to
-// a 16-plane ring.  Launches size their dynamic LDS to the shapes they use.
-constexpr int kLdsDoubles = 5632;
+// LDS budget per workgroup (doubles): the largest shape in use, TX = 128,
+// TY = 4, needs 12 x 74 + 12 x 128 = 2424.
+constexpr int kLdsDoubles = 2432;
 #define CXP_NT 256
-// Coarse values per thread and plane: every shape has TX * TY = 256 and a
-// coarse tile of at most (TX/2 + 9) x (TY/2 + 9) <= 41 x 11 = 451 <= 2 * 256.
-#define CXP_PRE 2
+// Coarse values per thread and plane: at most (TX/2 + 9) x (TY/2 + 9)
+// <= 73 x 11 = 803 <= 4 * 256 for every shape with TX <= 128.
+#define CXP_PRE 4
+// Register window of coarse z-planes: the widest z stencil spans 9 planes.
+#define CXP_ZW 9
 
+// x-pass: X(fine x pair, coarse y row) from the coarse plane tile.  One work
+// item computes both fine offsets of a coarse x and writes them as a pair.
 template <int SID>
 __device__ __forceinline__ void sep_xpass(const Geom &g,
                                           const double *__restrict__ Cs,
                                           double *__restrict__ Xs, int CY,
-                                          int x0, int TXa, int ci0, int tid) {
-  const int TX = 1 << g.lx;
-  for (int idx = tid; idx < (CY << g.lx); idx += CXP_NT) {
-    const int cy = idx >> g.lx;
-    const int fi = idx & (TX - 1);
-    if (fi < TXa) {
-      const int fx = x0 + fi;
-      const int ic = fx >> 1;
-      const int off = fx & 1;
-      const double *row = Cs + cy * g.cxs + (ic - ci0);
-      Xs[idx] = Op1<SID>::apply([&](int o) { return row[o]; }, off);
-    }
+                                          int npx, int x0, int ci0, int tid) {
+  const int ic0 = x0 >> 1;
+  for (int idx = tid; idx < CY * npx; idx += CXP_NT) {
+    const int cy = idx / npx;
+    const int q = idx - cy * npx;
+    const double *row = Cs + cy * g.cxs + (ic0 + q - ci0);
+    const auto c = [&](int o) { return row[o]; };
+    double2 v;
+    v.x = Op1<SID>::apply(c, 0);
+    v.y = Op1<SID>::apply(c, 1);
+    *reinterpret_cast<double2 *>(Xs + cy * g.tx + 2 * q) = v;
   }
 }
 
+// y-pass: both fine offsets of the owned coarse y pair.
 template <int SID>
 __device__ __forceinline__ void sep_ypass(const Geom &g,
                                           const double *__restrict__ Xs,
-                                          double *__restrict__ ring, int kp,
-                                          int y0, int TXa, int TYa, int cj0,
-                                          int tid) {
-  const int TX = 1 << g.lx;
-  double *slot = ring + (kp & ((1 << g.lr) - 1)) * g.npt;
-  for (int p = tid; p < g.npt; p += CXP_NT) {
-    const int fj = p >> g.lx;
-    const int fi = p & (TX - 1);
-    if (fi < TXa && fj < TYa) {
-      const int fy = y0 + fj;
-      const int jc = fy >> 1;
-      const int off = fy & 1;
-      const double *col = Xs + ((jc - cj0) << g.lx) + fi;
-      slot[p] = Op1<SID>::apply([&](int o) { return col[o << g.lx]; }, off);
-    }
-  }
+                                          int row0, int fi, double &y0v,
+                                          double &y1v) {
+  const double *col = Xs + (row0 << g.lx) + fi;
+  const auto c = [&](int o) { return col[o << g.lx]; };
+  y0v = Op1<SID>::apply(c, 0);
+  y1v = Op1<SID>::apply(c, 1);
 }
 
+// z-pass from the register window: w[ZW-1] holds coarse plane kc + whi.
 template <int SID>
-__device__ __forceinline__ void sep_zpass(const Geom &g,
-                                          const double *__restrict__ ring,
-                                          const DFab &fine, int kc, int z0,
-                                          int z1, int x0, int y0, int TXa,
-                                          int TYa, int tid) {
-  const int TX = 1 << g.lx;
-  const int rmask = (1 << g.lr) - 1;
-#pragma unroll
-  for (int off = 0; off < 2; ++off) {
-    const int fz = 2 * kc + off;
-    if (fz < z0 || fz > z1)
-      continue;
-    for (int p = tid; p < g.npt; p += CXP_NT) {
-      const int fj = p >> g.lx;
-      const int fi = p & (TX - 1);
-      if (fi < TXa && fj < TYa) {
-        const double v = Op1<SID>::apply(
-            [&](int o) { return ring[((kc + o) & rmask) * g.npt + p]; }, off);
-        gst(fine.p, didx(fine, x0 + fi, y0 + fj, fz), v);
-      }
-    }
-  }
+__device__ __forceinline__ double sep_zval(const double (&w)[CXP_ZW],
+                                           int off) {
+  constexpr int whi = sid_whi(SID);
+  return Op1<SID>::apply([&](int o) { return w[CXP_ZW - 1 - whi + o]; },
+                         off);
 }
 
 // Process one tile [x0, x0+TX) x [y0, y0+TY) x [z0, z1] of a task, clipped
-// to the task's fine region.  All threads of the block must call it.
+// to the task's fine region (x0, y0 even).  All threads of the block call it.
 __device__ inline void sep_tile(const Geom &g, const SepTask &t,
                                 long long comp, int x0, int y0, int z0, int z1,
                                 double *smem) {
   double *Cs = smem;
-  double *Xs = Cs + g.cys * g.cxs;
-  double *ring = Xs + (g.cys << g.lx);
+  double *Xs = Cs + g.xs_off;
   const int tid = threadIdx.x;
-  const int TXa = min(1 << g.lx, t.rhi[0] - x0 + 1);
-  const int TYa = min(1 << g.ly, t.rhi[1] - y0 + 1);
+  const int xe = min(x0 + g.tx - 1, t.rhi[0]);  // last fine x of the tile
+  const int ye = min(y0 + g.ty - 1, t.rhi[1]);
   const int ci0 = (x0 >> 1) + t.wlo[0];
-  const int ci1 = ((x0 + TXa - 1) >> 1) + t.whi[0];
+  const int ci1 = (xe >> 1) + t.whi[0];
   const int cj0 = (y0 >> 1) + t.wlo[1];
-  const int cj1 = ((y0 + TYa - 1) >> 1) + t.whi[1];
+  const int cj1 = (ye >> 1) + t.whi[1];
   const int CX = ci1 - ci0 + 1;
   const int CY = cj1 - cj0 + 1;
+  const int npx = (xe >> 1) - (x0 >> 1) + 1;  // coarse x of the tile
   const int kf = (z0 >> 1) + t.wlo[2];
   const int kl = (z1 >> 1) + t.whi[2];
   const int kc0 = z0 >> 1;
@@ -205,9 +186,24 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
   const int sx = t.sid[0], sy = t.sid[1], sz = t.sid[2];
   const int ncl = CX * CY;
 
+  // owned fine points: x = x0 + fi, y = y0 + 2 fjp + {0, 1}
+  const int fi = tid & (g.tx - 1);
+  const int fjp = tid >> g.lx;
+  const int fx = x0 + fi;
+  const int fy = y0 + 2 * fjp;
+  const bool xin = fx >= t.rlo[0] && fx <= xe;
+  const bool pair_in = fy <= ye;  // the pair overlaps the tile's rows
+  const bool yin0 = pair_in && fy >= t.rlo[1];
+  const bool yin1 = pair_in && fy + 1 <= ye;
+  const int row0 = (fy >> 1) - cj0;
+  // x/y part of the owned points' offsets in the fine FAB
+  const long long fo0 = (long long)(fx - fine.lo[0]) +
+                        (long long)(fy - fine.lo[1]) * fine.sy;
+  const long long fo1 = fo0 + fine.sy;
+
   // Each thread moves up to CXP_PRE coarse values per plane; the next
   // plane's loads are issued before the current plane is processed so their
-  // latency overlaps the three passes.
+  // latency overlaps the passes.
   int soff[CXP_PRE], loff[CXP_PRE];
 #pragma unroll
   for (int r = 0; r < CXP_PRE; ++r) {
@@ -223,6 +219,12 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
 #pragma unroll
     for (int r = 0; r < CXP_PRE; ++r)
       pre[r] = soff[r] >= 0 ? gld(src, loff[r]) : 0.0;
+  }
+  double w0[CXP_ZW], w1[CXP_ZW];
+#pragma unroll
+  for (int q = 0; q < CXP_ZW; ++q) {
+    w0[q] = 0.0;
+    w1[q] = 0.0;
   }
 
   for (int kp = kf; kp <= kl; ++kp) {
@@ -242,26 +244,47 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     switch (sx) {
 #define CXP_X(S)                                                               \
   case S:                                                                      \
-    sep_xpass<S>(g, Cs, Xs, CY, x0, TXa, ci0, tid);                            \
+    sep_xpass<S>(g, Cs, Xs, CY, npx, x0, ci0, tid);                            \
     break;
       CXP_LINEAR_SIDS(CXP_X)
 #undef CXP_X
     }
     __syncthreads();
-    switch (sy) {
+    // y-pass into the register window (rotated by one coarse plane)
+#pragma unroll
+    for (int q = 0; q < CXP_ZW - 1; ++q) {
+      w0[q] = w0[q + 1];
+      w1[q] = w1[q + 1];
+    }
+    double ya = 0.0, yb = 0.0;
+    if (xin && pair_in) {
+      switch (sy) {
 #define CXP_Y(S)                                                               \
   case S:                                                                      \
-    sep_ypass<S>(g, Xs, ring, kp, y0, TXa, TYa, cj0, tid);                     \
+    sep_ypass<S>(g, Xs, row0, fi, ya, yb);                                     \
     break;
-      CXP_LINEAR_SIDS(CXP_Y)
+        CXP_LINEAR_SIDS(CXP_Y)
 #undef CXP_Y
+      }
     }
+    w0[CXP_ZW - 1] = ya;
+    w1[CXP_ZW - 1] = yb;
+    // z-pass: both fine z of coarse plane kc = kp - whi
     const int kc = kp - t.whi[2];
-    if (kc >= kc0 && kc <= kc1) {
+    if (kc >= kc0 && kc <= kc1 && xin && pair_in) {
       switch (sz) {
 #define CXP_Z(S)                                                               \
   case S:                                                                      \
-    sep_zpass<S>(g, ring, fine, kc, z0, z1, x0, y0, TXa, TYa, tid);            \
+    _Pragma("unroll") for (int off = 0; off < 2; ++off) {                      \
+      const int fz = 2 * kc + off;                                             \
+      if (fz >= z0 && fz <= z1) {                                              \
+        const long long zo = (long long)(fz - fine.lo[2]) * fine.sz;           \
+        if (yin0)                                                              \
+          gst(fine.p, fo0 + zo, sep_zval<S>(w0, off));                         \
+        if (yin1)                                                              \
+          gst(fine.p, fo1 + zo, sep_zval<S>(w1, off));                         \
+      }                                                                        \
+    }                                                                          \
     break;
         CXP_LINEAR_SIDS(CXP_Z)
 #undef CXP_Z
@@ -273,30 +296,29 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
 // Per-box launch: grid.x = tiles (x fastest, then y, then z-chunks),
 // grid.y = component.  The tile shape is uniform over the launch.
 __global__ void __launch_bounds__(CXP_NT)
-    k_sep_box(SepTask t, int lx, int ly, int lr, int ntx, int nty,
-              int zchunk) {
-  extern __shared__ double smem[];
-  const Geom g = make_geom(lx, ly, lr);
+    k_sep_box(SepTask t, int lx, int ntx, int nty, int zchunk) {
+  __shared__ double smem[kLdsDoubles];
+  const Geom g = make_geom(lx);
   const int b = blockIdx.x;
   const int tx = b % ntx;
   const int ty = (b / ntx) % nty;
   const int tz = b / (ntx * nty);
-  const int x0 = t.rlo[0] + (tx << lx);
-  const int y0 = t.rlo[1] + (ty << ly);
+  const int x0 = (t.rlo[0] & ~1) + tx * g.tx;
+  const int y0 = (t.rlo[1] & ~1) + ty * g.ty;
   const int z0 = t.rlo[2] + tz * zchunk;
   const int z1 = min(t.rhi[2], z0 + zchunk - 1);
   sep_tile(g, t, blockIdx.y, x0, y0, z0, z1, smem);
 }
 
 // Batched persistent launch: the grid walks the device-resident tile list;
-// each tile carries its task and tile shape (cfg packs lx | ly<<4 | lr<<8).
-// Workgroups are dispatched round-robin over the 8 XCDs, so the list index
-// is remapped to give each XCD a contiguous run of tiles (neighbouring tiles
-// share coarse halo, which then stays in one L2).
+// each tile carries its task and tile shape (cfg = lx).  Workgroups are
+// dispatched round-robin over the 8 XCDs, so the list index is remapped to
+// give each XCD a contiguous run of tiles (neighbouring tiles share coarse
+// halo, which then stays in one L2).
 __global__ void __launch_bounds__(CXP_NT)
     k_sep_batched(const SepTask *__restrict__ tasks,
                   const SepTile *__restrict__ tiles, int ntiles) {
-  extern __shared__ double smem[];
+  __shared__ double smem[kLdsDoubles];
   const int nblk = gridDim.x;
   const int bid = blockIdx.x;
   int vb = bid;
@@ -305,8 +327,8 @@ __global__ void __launch_bounds__(CXP_NT)
   for (int n = vb; n < ntiles; n += nblk) {
     const SepTile tl = tiles[n];
     const SepTask t = tasks[tl.task];
-    const Geom g = make_geom(tl.cfg & 15, (tl.cfg >> 4) & 15, (tl.cfg >> 8) & 15);
-    sep_tile(g, t, tl.comp, tl.x0, tl.y0, tl.z0, tl.z1, smem);
+    sep_tile(make_geom(tl.cfg), t, tl.comp, tl.x0, tl.y0, tl.z0, tl.z1, smem);
+    __syncthreads();  // the next tile rewrites Cs / Xs
   }
 }
 

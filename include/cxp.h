@@ -153,7 +153,7 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream);
 int cxp_tune_save(const char *path);
 int cxp_tune_load(const char *path);
 int cxp_tile_config_count(void);
-int cxp_tile_config(int cfg, int *tx, int *ty, int *ring);
+int cxp_tile_config(int cfg, int *tx, int *ty, int *zwindow);
 
 #ifdef __cplusplus
 }
