@@ -104,10 +104,10 @@ int hip_status(hipError_t e) { return e == hipSuccess ? CXP_OK : CXP_ERR_HIP; }
 struct TileCfg {
   int tx, ty, lx;
 };
-// Tile shapes: TX x TY fine points per plane, TX * TY = 512 (two fine y per
+// Tile shapes: TX x TY fine points per plane, TX * TY = 256 (two fine y per
 // thread).  Every one fits the kernel's LDS budget (checked at first use).
 const TileCfg kCfgs[] = {
-    {128, 4, 7}, {64, 8, 6}, {32, 16, 5}, {16, 32, 4}, {8, 64, 3}, {4, 128, 2},
+    {64, 4, 6}, {32, 8, 5}, {16, 16, 4}, {8, 32, 3}, {4, 64, 2},
 };
 constexpr int kNumTileConfigs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -154,18 +154,15 @@ bool cfg_fits(int cfg, const SepTask &t) {
 
 int default_cfg(const SepTask &t) {
   const int nx = t.rhi[0] - (t.rlo[0] & ~1) + 1;
-  const int ny = t.rhi[1] - (t.rlo[1] & ~1) + 1;
   if (nx <= 4)
-    return 5;
-  if (nx <= 8)
     return 4;
-  if (nx <= 16)
+  if (nx <= 8)
     return 3;
-  if (nx <= 32)
+  if (nx <= 16)
     return 2;
-  if (ny <= 4 || nx > 64)
-    return 0;
-  return 1;
+  if (nx <= 32)
+    return 1;
+  return 0;
 }
 
 using TuneKey = std::tuple<int, int, int, int>;
@@ -178,7 +175,7 @@ TuneKey tune_key(const SepTask &t) {
 
 void choose_cfg(const SepTask &t, int &cfg, int &zchunk) {
   cfg = default_cfg(t);
-  zchunk = 16;
+  zchunk = 0;  // not tuned: chosen per launch / list
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(tune_key(t));
   if (it != g_tune.end() && cfg_fits(it->second.first, t)) {
@@ -246,9 +243,9 @@ int launch_sep_box(const SepTask &t, int ncomp, hipStream_t s) {
   const int ntx = (nx + c.tx - 1) / c.tx;
   const int nty = (ny + c.ty - 1) / c.ty;
   // per-box: shorten z-chunks until the launch has enough workgroups
-  int zchunk = std::max(2, std::min(nz, zc));
-  while (zchunk > 8 && (long long)ntx * nty * ((nz + zchunk - 1) / zchunk) *
-                               ncomp < 1024)
+  int zchunk = zc > 0 ? std::min(nz, zc) : nz;
+  while (zc <= 0 && zchunk > 8 &&
+         (long long)ntx * nty * ((nz + zchunk - 1) / zchunk) * ncomp < 2048)
     zchunk = (zchunk + 1) / 2;
   const int nzc = (nz + zchunk - 1) / zchunk;
   hipLaunchKernelGGL(k_sep_box, dim3(ntx * nty * nzc, ncomp), dim3(CXP_NT), 0,
@@ -309,11 +306,20 @@ int64_t patch_coarse_points(const cxp_worklist_s::Patch &p) {
          (chi[2] - clo[2] + 1);
 }
 
+int persistent_grid(int ntiles);
+
+// Tiles of the separable patches.  Untuned patches share one z-chunk length,
+// the longest (fewest halo planes) that still gives the persistent grid
+// several tiles per workgroup slot.
 void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
                  const TuneKey *only_key = nullptr) {
   wl->tasks.clear();
   wl->tiles.clear();
   wl->point_patches.clear();
+  struct Plan {
+    int task, cfg, zc, ncomp;
+  };
+  std::vector<Plan> plans;
   for (size_t n = 0; n < wl->patches.size(); ++n) {
     const auto &p = wl->patches[n];
     if (!op_is_separable(p.op->d, false)) {
@@ -330,21 +336,42 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
       cfg = only_class_key_cfg >> 8;
       zc = only_class_key_cfg & 0xff;
     }
-    const int task = (int)wl->tasks.size();
+    plans.push_back({(int)wl->tasks.size(), cfg, zc, p.ncomp});
     wl->tasks.push_back(t);
-    const TileCfg &c = kCfgs[cfg];
-    for (int comp = 0; comp < p.ncomp; ++comp)
-      for (int z0 = p.rlo[2]; z0 <= p.rhi[2]; z0 += zc)
-        for (int y0 = p.rlo[1] & ~1; y0 <= p.rhi[1]; y0 += c.ty)
-          for (int x0 = p.rlo[0] & ~1; x0 <= p.rhi[0]; x0 += c.tx) {
+  }
+  auto count = [&](int zdef) {
+    long long n = 0;
+    for (const Plan &pl : plans) {
+      const SepTask &t = wl->tasks[pl.task];
+      const TileCfg &c = kCfgs[pl.cfg];
+      const int zc = pl.zc > 0 ? pl.zc : zdef;
+      const long long nx = (t.rhi[0] - (t.rlo[0] & ~1) + c.tx) / c.tx;
+      const long long ny = (t.rhi[1] - (t.rlo[1] & ~1) + c.ty) / c.ty;
+      const long long nz = (t.rhi[2] - t.rlo[2] + zc) / zc;
+      n += nx * ny * nz * pl.ncomp;
+    }
+    return n;
+  };
+  const long long slots = 4LL * persistent_grid(1 << 30);
+  int zdef = 64;
+  while (zdef > 8 && count(zdef) < slots)
+    zdef /= 2;
+  for (const Plan &pl : plans) {
+    const SepTask &t = wl->tasks[pl.task];
+    const TileCfg &c = kCfgs[pl.cfg];
+    const int zc = pl.zc > 0 ? pl.zc : zdef;
+    for (int comp = 0; comp < pl.ncomp; ++comp)
+      for (int z0 = t.rlo[2]; z0 <= t.rhi[2]; z0 += zc)
+        for (int y0 = t.rlo[1] & ~1; y0 <= t.rhi[1]; y0 += c.ty)
+          for (int x0 = t.rlo[0] & ~1; x0 <= t.rhi[0]; x0 += c.tx) {
             SepTile tl;
-            tl.task = task;
+            tl.task = pl.task;
             tl.comp = comp;
             tl.x0 = x0;
             tl.y0 = y0;
             tl.z0 = z0;
-            tl.z1 = std::min(p.rhi[2], z0 + zc - 1);
-            tl.cfg = cfg_code(cfg);
+            tl.z1 = std::min(t.rhi[2], z0 + zc - 1);
+            tl.cfg = cfg_code(pl.cfg);
             tl.pad = 0;
             wl->tiles.push_back(tl);
           }
@@ -846,13 +873,13 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return CXP_ERR_HIP;
-  const int zchunks[3] = {8, 16, 32};
+  const int zchunks[4] = {8, 16, 32, 64};
   int status = CXP_OK;
   for (const TuneKey &k : keys) {
     float best = 1e30f;
     int best_cfg = -1, best_zc = 16;
     for (int cfg = 0; cfg < kNumTileConfigs && !status; ++cfg)
-      for (int zi = 0; zi < 3 && !status; ++zi) {
+      for (int zi = 0; zi < 4 && !status; ++zi) {
         const int zc = zchunks[zi];
         build_tiles(wl, (cfg << 8) | zc, &k);
         bool fits = !wl->tasks.empty();

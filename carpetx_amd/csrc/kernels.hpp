@@ -80,8 +80,8 @@ struct SepTile {
   X(SID_CCONS1) X(SID_CCONS2) X(SID_CCONS3) X(SID_CCONS4) X(SID_CCONS5)        \
   X(SID_CCONS6) X(SID_CCONS7) X(SID_MINMOD)
 
-// Runtime tile geometry.  A workgroup of 256 threads owns a TX x TY fine
-// (x, y) tile with TX * TY = 512: thread t owns fine x = x0 + (t % TX) and
+// Runtime tile geometry.  A workgroup of 128 threads owns a TX x TY fine
+// (x, y) tile with TX * TY = 256: thread t owns fine x = x0 + (t % TX) and
 // the fine y pair y0 + 2 (t / TX) + {0, 1}.  Tiles start at even x and y, so
 // every pair shares one coarse index.
 struct Geom {
@@ -95,7 +95,7 @@ __host__ __device__ inline Geom make_geom(int lx) {
   Geom g;
   g.lx = lx;
   g.tx = 1 << lx;
-  g.ty = 512 >> lx;
+  g.ty = 256 >> lx;
   g.cxs = g.tx / 2 + 10;  // tile / 2 + 1 + 8 halo, padded
   g.cys = g.ty / 2 + 10;
   g.xs_off = (g.cys * g.cxs + 1) & ~1;
@@ -105,12 +105,12 @@ __host__ __device__ inline int geom_lds_doubles(const Geom &g) {
   return g.xs_off + g.cys * g.tx;
 }
 
-// LDS budget per workgroup (doubles): the largest shape in use, TX = 128,
-// TY = 4, needs 12 x 74 + 12 x 128 = 2424.
-constexpr int kLdsDoubles = 2432;
-#define CXP_NT 256
+// LDS budget per workgroup (doubles): the largest shape in use, TX = 64,
+// TY = 4, needs 12 x 42 + 12 x 64 = 1272.
+constexpr int kLdsDoubles = 1280;
+#define CXP_NT 128
 // Coarse values per thread and plane: at most (TX/2 + 9) x (TY/2 + 9)
-// <= 73 x 11 = 803 <= 4 * 256 for every shape with TX <= 128.
+// <= 41 x 11 = 451 <= 4 * 128 for every shape with 4 <= TX <= 64.
 #define CXP_PRE 4
 // Register window of coarse z-planes: the widest z stencil spans 9 planes.
 #define CXP_ZW 9
