@@ -56,6 +56,14 @@ bool fab_ok(const cxp_fab *f) {
   return f->ncomp == 1 || f->comp_stride >= vol;
 }
 
+// Bytes of one component of a FAB.  The separable kernel addresses a
+// component with 32-bit buffer offsets, so it must stay below 2 GiB.
+int64_t fab_component_bytes(const cxp_fab *f) {
+  return 8 * int64_t(f->hi[0] - f->lo[0] + 1) * (f->hi[1] - f->lo[1] + 1) *
+         (f->hi[2] - f->lo[2] + 1);
+}
+constexpr int64_t kMaxComponentBytes = (int64_t(1) << 31) - 4096;
+
 bool box_in(const int lo[3], const int hi[3], const int blo[3],
             const int bhi[3]) {
   for (int d = 0; d < 3; ++d)
@@ -93,6 +101,9 @@ int check_prolong(const cxp_op_s *op, const cxp_fab *crse, int crse_comp,
     return CXP_ERR_COARSE;
   if (per_group && ncomp > 16)
     return CXP_ERR_NCOMP;
+  if (fab_component_bytes(crse) > kMaxComponentBytes ||
+      fab_component_bytes(fine) > kMaxComponentBytes)
+    return CXP_ERR_ARG;
   return CXP_OK;
 }
 
@@ -199,6 +210,8 @@ SepTask make_sep_task(const OpDesc &op, const cxp_fab *crse, int crse_comp,
   }
   t.ccs = crse->comp_stride;
   t.fcs = fine->comp_stride;
+  t.cbytes = (int)fab_component_bytes(crse);
+  t.fbytes = (int)fab_component_bytes(fine);
   return t;
 }
 

@@ -48,6 +48,28 @@ __device__ __forceinline__ void gst(double *p, long long i, double v) {
   ((gdouble *)p)[i] = v;
 }
 
+// Raw buffer access.  Out-of-range offsets are dropped by the hardware
+// (stores) or read as 0 (loads), so masked lanes need no branch and every
+// iteration of a loop issues the same number of memory instructions; the
+// compiler can then wait for the prefetch loads alone (vmcnt(N)) instead of
+// also draining the stores issued after them.
+typedef unsigned int cxp_u32x2 __attribute__((ext_vector_type(2)));
+#define CXP_OOB 0x7ffffff0
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const double *p,
+                                                            int nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, nbytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ double bld(__amdgpu_buffer_rsrc_t r, int boff) {
+  return __builtin_bit_cast(double,
+                            __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, 0));
+}
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, int boff,
+                                    double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(cxp_u32x2, v), r,
+                                        boff, 0, 0);
+}
+
 __device__ __forceinline__ long long didx(const DFab &f, int i, int j, int k) {
   return (long long)(i - f.lo[0]) + (long long)(j - f.lo[1]) * f.sy +
          (long long)(k - f.lo[2]) * f.sz;
@@ -61,6 +83,7 @@ struct SepTask {
   int sid[3];
   int wlo[3], whi[3];  // union stencil window per direction
   long long ccs, fcs;  // component strides (per-box launches)
+  int cbytes, fbytes;  // bytes of one component of each FAB (< 2 GiB)
 };
 
 // A work tile of the batched list.
@@ -116,16 +139,18 @@ constexpr int kLdsDoubles = 1280;
 #define CXP_ZW 9
 
 // x-pass: X(fine x pair, coarse y row) from the coarse plane tile.  One work
-// item computes both fine offsets of a coarse x and writes them as a pair.
+// item computes both fine offsets of a coarse x and writes them as a pair;
+// lanes run over the TX/2 coarse x of a row, rows step by TY.
 template <int SID>
 __device__ __forceinline__ void sep_xpass(const Geom &g,
                                           const double *__restrict__ Cs,
                                           double *__restrict__ Xs, int CY,
                                           int npx, int x0, int ci0, int tid) {
+  const int q = tid & ((g.tx >> 1) - 1);
+  if (q >= npx)
+    return;
   const int ic0 = x0 >> 1;
-  for (int idx = tid; idx < CY * npx; idx += CXP_NT) {
-    const int cy = idx / npx;
-    const int q = idx - cy * npx;
+  for (int cy = tid >> (g.lx - 1); cy < CY; cy += g.ty) {
     const double *row = Cs + cy * g.cxs + (ic0 + q - ci0);
     const auto c = [&](int o) { return row[o]; };
     double2 v;
@@ -196,14 +221,12 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
   const bool yin0 = pair_in && fy >= t.rlo[1];
   const bool yin1 = pair_in && fy + 1 <= ye;
   const int row0 = (fy >> 1) - cj0;
-  // x/y part of the owned points' offsets in the fine FAB
-  const long long fo0 = (long long)(fx - fine.lo[0]) +
-                        (long long)(fy - fine.lo[1]) * fine.sy;
-  const long long fo1 = fo0 + fine.sy;
 
   // Each thread moves up to CXP_PRE coarse values per plane; the next
   // plane's loads are issued before the current plane is processed so their
   // latency overlaps the passes.
+  const __amdgpu_buffer_rsrc_t rc = make_rsrc(crse.p, t.cbytes);
+  const __amdgpu_buffer_rsrc_t rf = make_rsrc(fine.p, t.fbytes);
   int soff[CXP_PRE], loff[CXP_PRE];
 #pragma unroll
   for (int r = 0; r < CXP_PRE; ++r) {
@@ -211,35 +234,43 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     const int cy = idx / CX;
     const int cx = idx - cy * CX;
     soff[r] = idx < ncl ? cy * g.cxs + cx : -1;
-    loff[r] = cy * crse.sy + cx;
+    loff[r] = 8 * (cy * crse.sy + cx);
   }
+  const int pstride = 8 * (int)crse.sz;  // bytes per coarse plane
+  int pbase = 8 * (int)didx(crse, ci0, cj0, kf);
   double pre[CXP_PRE];
-  {
-    const double *src = crse.p + didx(crse, ci0, cj0, kf);
 #pragma unroll
-    for (int r = 0; r < CXP_PRE; ++r)
-      pre[r] = soff[r] >= 0 ? gld(src, loff[r]) : 0.0;
-  }
+  for (int r = 0; r < CXP_PRE; ++r)
+    pre[r] = bld(rc, soff[r] >= 0 ? pbase + loff[r] : CXP_OOB);
+  // dummy stores: every path into the plane loop then has the same pending
+  // loads-then-stores pattern
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    bst(rf, CXP_OOB, 0.0);
+
   double w0[CXP_ZW], w1[CXP_ZW];
 #pragma unroll
   for (int q = 0; q < CXP_ZW; ++q) {
     w0[q] = 0.0;
     w1[q] = 0.0;
   }
+  // byte offsets of the owned points (x/y part) in the fine component
+  const int fo0 = 8 * ((fx - fine.lo[0]) + (fy - fine.lo[1]) * fine.sy);
+  const int fo1 = fo0 + 8 * fine.sy;
+  const bool own = xin && pair_in;
 
   for (int kp = kf; kp <= kl; ++kp) {
-    // coarse plane tile -> LDS, then prefetch the next plane
+    // coarse plane tile -> LDS, then prefetch the next plane (out of range
+    // after the last plane)
 #pragma unroll
     for (int r = 0; r < CXP_PRE; ++r)
       if (soff[r] >= 0)
         Cs[soff[r]] = pre[r];
-    if (kp < kl) {
-      const double *src = crse.p + didx(crse, ci0, cj0, kp + 1);
+    pbase += pstride;
+    const bool more = kp < kl;
 #pragma unroll
-      for (int r = 0; r < CXP_PRE; ++r)
-        if (soff[r] >= 0)
-          pre[r] = gld(src, loff[r]);
-    }
+    for (int r = 0; r < CXP_PRE; ++r)
+      pre[r] = bld(rc, (more && soff[r] >= 0) ? pbase + loff[r] : CXP_OOB);
     __syncthreads();
     switch (sx) {
 #define CXP_X(S)                                                               \
@@ -257,7 +288,7 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
       w1[q] = w1[q + 1];
     }
     double ya = 0.0, yb = 0.0;
-    if (xin && pair_in) {
+    if (own) {
       switch (sy) {
 #define CXP_Y(S)                                                               \
   case S:                                                                      \
@@ -269,27 +300,33 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     }
     w0[CXP_ZW - 1] = ya;
     w1[CXP_ZW - 1] = yb;
-    // z-pass: both fine z of coarse plane kc = kp - whi
+    // z-pass: both fine z of coarse plane kc = kp - whi; always four stores,
+    // the ones outside the tile go out of range
     const int kc = kp - t.whi[2];
-    if (kc >= kc0 && kc <= kc1 && xin && pair_in) {
+    const bool zact = kc >= kc0 && kc <= kc1;
+    double v00 = 0.0, v01 = 0.0, v10 = 0.0, v11 = 0.0;
+    if (zact) {
       switch (sz) {
 #define CXP_Z(S)                                                               \
   case S:                                                                      \
-    _Pragma("unroll") for (int off = 0; off < 2; ++off) {                      \
-      const int fz = 2 * kc + off;                                             \
-      if (fz >= z0 && fz <= z1) {                                              \
-        const long long zo = (long long)(fz - fine.lo[2]) * fine.sz;           \
-        if (yin0)                                                              \
-          gst(fine.p, fo0 + zo, sep_zval<S>(w0, off));                         \
-        if (yin1)                                                              \
-          gst(fine.p, fo1 + zo, sep_zval<S>(w1, off));                         \
-      }                                                                        \
-    }                                                                          \
+    v00 = sep_zval<S>(w0, 0);                                                  \
+    v01 = sep_zval<S>(w0, 1);                                                  \
+    v10 = sep_zval<S>(w1, 0);                                                  \
+    v11 = sep_zval<S>(w1, 1);                                                  \
     break;
         CXP_LINEAR_SIDS(CXP_Z)
 #undef CXP_Z
       }
     }
+    const int fz0 = 2 * kc, fz1 = 2 * kc + 1;
+    const bool z0ok = zact && fz0 >= z0 && fz0 <= z1;
+    const bool z1ok = zact && fz1 >= z0 && fz1 <= z1;
+    const int zb0 = 8 * (int)((long long)(fz0 - fine.lo[2]) * fine.sz);
+    const int zb1 = zb0 + 8 * (int)fine.sz;
+    bst(rf, (own && yin0 && z0ok) ? fo0 + zb0 : CXP_OOB, v00);
+    bst(rf, (own && yin0 && z1ok) ? fo0 + zb1 : CXP_OOB, v01);
+    bst(rf, (own && yin1 && z0ok) ? fo1 + zb0 : CXP_OOB, v10);
+    bst(rf, (own && yin1 && z1ok) ? fo1 + zb1 : CXP_OOB, v11);
   }
 }
 
