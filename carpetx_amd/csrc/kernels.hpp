@@ -124,14 +124,13 @@ __host__ __device__ inline Geom make_geom(int lx) {
   g.xs_off = (g.cys * g.cxs + 1) & ~1;
   return g;
 }
-// Two coarse planes and two x-pass planes, plus one dummy slot.
 __host__ __device__ inline int geom_lds_doubles(const Geom &g) {
-  return ((2 * g.cys * g.cxs + 1) & ~1) + 2 * g.cys * g.tx + 1;
+  return g.xs_off + g.cys * g.tx;
 }
 
 // LDS budget per workgroup (doubles): the largest shape in use, TX = 64,
-// TY = 4, needs 2 x 12 x 42 + 2 x 12 x 64 + 1 = 2545.
-constexpr int kLdsDoubles = 2560;
+// TY = 4, needs 12 x 42 + 12 x 64 = 1272.
+constexpr int kLdsDoubles = 1280;
 #define CXP_NT 128
 // Coarse values per thread and plane: at most (TX/2 + 9) x (TY/2 + 9)
 // <= 41 x 11 = 451 <= 4 * 128 for every shape with 4 <= TX <= 64.
@@ -184,16 +183,11 @@ __device__ __forceinline__ double sep_zval(const double (&w)[CXP_ZW],
 
 // Process one tile [x0, x0+TX) x [y0, y0+TY) x [z0, z1] of a task, clipped
 // to the task's fine region (x0, y0 even).  All threads of the block call it.
-// Each loop iteration handles two coarse planes: one LDS round (two
-// barriers), one stencil dispatch per pass, eight fine stores.
 __device__ inline void sep_tile(const Geom &g, const SepTask &t,
                                 long long comp, int x0, int y0, int z0, int z1,
                                 double *smem) {
-  const int cplane = g.cys * g.cxs;          // doubles per coarse plane tile
-  const int xplane = g.cys * g.tx;           // doubles per x-pass plane
-  double *Cs = smem;                         // [2][cys][cxs]
-  double *Xs = smem + ((2 * cplane + 1) & ~1);  // [2][cys][tx], 16-B aligned
-  double *dummy = smem + kLdsDoubles - 1;    // sink for masked LDS writes
+  double *Cs = smem;
+  double *Xs = Cs + g.xs_off;
   const int tid = threadIdx.x;
   const int xe = min(x0 + g.tx - 1, t.rhi[0]);  // last fine x of the tile
   const int ye = min(y0 + g.ty - 1, t.rhi[1]);
@@ -227,10 +221,10 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
   const bool yin0 = pair_in && fy >= t.rlo[1];
   const bool yin1 = pair_in && fy + 1 <= ye;
   const int row0 = (fy >> 1) - cj0;
-  const bool own = xin && pair_in;
 
-  // Coarse values each thread moves per plane; loads for the next two
-  // planes are issued before the current two are processed.
+  // Each thread moves up to CXP_PRE coarse values per plane; the next
+  // plane's loads are issued before the current plane is processed so their
+  // latency overlaps the passes.
   const __amdgpu_buffer_rsrc_t rc = make_rsrc(crse.p, t.cbytes);
   const __amdgpu_buffer_rsrc_t rf = make_rsrc(fine.p, t.fbytes);
   int soff[CXP_PRE], loff[CXP_PRE];
@@ -244,18 +238,14 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
   }
   const int pstride = 8 * (int)crse.sz;  // bytes per coarse plane
   int pbase = 8 * (int)didx(crse, ci0, cj0, kf);
-  double pre[2][CXP_PRE];
+  double pre[CXP_PRE];
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
-#pragma unroll
-    for (int r = 0; r < CXP_PRE; ++r)
-      pre[h][r] = bld(rc, (soff[r] >= 0 && kf + h <= kl)
-                              ? pbase + h * pstride + loff[r]
-                              : CXP_OOB);
+  for (int r = 0; r < CXP_PRE; ++r)
+    pre[r] = bld(rc, soff[r] >= 0 ? pbase + loff[r] : CXP_OOB);
   // dummy stores: every path into the plane loop then has the same pending
   // loads-then-stores pattern
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < 4; ++r)
     bst(rf, CXP_OOB, 0.0);
 
   double w0[CXP_ZW], w1[CXP_ZW];
@@ -267,94 +257,76 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
   // byte offsets of the owned points (x/y part) in the fine component
   const int fo0 = 8 * ((fx - fine.lo[0]) + (fy - fine.lo[1]) * fine.sy);
   const int fo1 = fo0 + 8 * fine.sy;
-  const int zstride = 8 * (int)fine.sz;
+  const bool own = xin && pair_in;
 
-  for (int kp = kf; kp <= kl; kp += 2) {
-    // two coarse plane tiles -> LDS (masked lanes write a dummy slot), then
-    // prefetch the next two (out of range past the last plane)
+  for (int kp = kf; kp <= kl; ++kp) {
+    // coarse plane tile -> LDS, then prefetch the next plane (out of range
+    // after the last plane)
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int r = 0; r < CXP_PRE; ++r)
+      if (soff[r] >= 0)
+        Cs[soff[r]] = pre[r];
+    pbase += pstride;
+    const bool more = kp < kl;
 #pragma unroll
-      for (int r = 0; r < CXP_PRE; ++r) {
-        double *dst = soff[r] >= 0 ? Cs + h * cplane + soff[r] : dummy;
-        *dst = pre[h][r];
-      }
-    pbase += 2 * pstride;
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int r = 0; r < CXP_PRE; ++r)
-        pre[h][r] = bld(rc, (soff[r] >= 0 && kp + 2 + h <= kl)
-                                ? pbase + h * pstride + loff[r]
-                                : CXP_OOB);
+    for (int r = 0; r < CXP_PRE; ++r)
+      pre[r] = bld(rc, (more && soff[r] >= 0) ? pbase + loff[r] : CXP_OOB);
     __syncthreads();
     switch (sx) {
 #define CXP_X(S)                                                               \
   case S:                                                                      \
     sep_xpass<S>(g, Cs, Xs, CY, npx, x0, ci0, tid);                            \
-    sep_xpass<S>(g, Cs + cplane, Xs + xplane, CY, npx, x0, ci0, tid);          \
     break;
       CXP_LINEAR_SIDS(CXP_X)
 #undef CXP_X
     }
     __syncthreads();
-    // y-pass of both planes (fi, row0 stay inside the tile for every lane)
-    double ya0 = 0.0, yb0 = 0.0, ya1 = 0.0, yb1 = 0.0;
+    // y-pass into the register window (rotated by one coarse plane)
+#pragma unroll
+    for (int q = 0; q < CXP_ZW - 1; ++q) {
+      w0[q] = w0[q + 1];
+      w1[q] = w1[q + 1];
+    }
+    double ya = 0.0, yb = 0.0;
     if (own) {
       switch (sy) {
 #define CXP_Y(S)                                                               \
   case S:                                                                      \
-    sep_ypass<S>(g, Xs, row0, fi, ya0, yb0);                                   \
-    sep_ypass<S>(g, Xs + xplane, row0, fi, ya1, yb1);                          \
+    sep_ypass<S>(g, Xs, row0, fi, ya, yb);                                     \
     break;
         CXP_LINEAR_SIDS(CXP_Y)
 #undef CXP_Y
       }
     }
-    // push plane kp, z-pass of kc = kp - whi; push kp + 1, z-pass of kc + 1
-    double v[2][4];
-    switch (sz) {
+    w0[CXP_ZW - 1] = ya;
+    w1[CXP_ZW - 1] = yb;
+    // z-pass: both fine z of coarse plane kc = kp - whi; always four stores,
+    // the ones outside the tile go out of range
+    const int kc = kp - t.whi[2];
+    const bool zact = kc >= kc0 && kc <= kc1;
+    double v00 = 0.0, v01 = 0.0, v10 = 0.0, v11 = 0.0;
+    if (zact) {
+      switch (sz) {
 #define CXP_Z(S)                                                               \
   case S:                                                                      \
-    _Pragma("unroll") for (int q = 0; q < CXP_ZW - 1; ++q) {                   \
-      w0[q] = w0[q + 1];                                                       \
-      w1[q] = w1[q + 1];                                                       \
-    }                                                                          \
-    w0[CXP_ZW - 1] = ya0;                                                      \
-    w1[CXP_ZW - 1] = yb0;                                                      \
-    v[0][0] = sep_zval<S>(w0, 0);                                              \
-    v[0][1] = sep_zval<S>(w0, 1);                                              \
-    v[0][2] = sep_zval<S>(w1, 0);                                              \
-    v[0][3] = sep_zval<S>(w1, 1);                                              \
-    _Pragma("unroll") for (int q = 0; q < CXP_ZW - 1; ++q) {                   \
-      w0[q] = w0[q + 1];                                                       \
-      w1[q] = w1[q + 1];                                                       \
-    }                                                                          \
-    w0[CXP_ZW - 1] = ya1;                                                      \
-    w1[CXP_ZW - 1] = yb1;                                                      \
-    v[1][0] = sep_zval<S>(w0, 0);                                              \
-    v[1][1] = sep_zval<S>(w0, 1);                                              \
-    v[1][2] = sep_zval<S>(w1, 0);                                              \
-    v[1][3] = sep_zval<S>(w1, 1);                                              \
+    v00 = sep_zval<S>(w0, 0);                                                  \
+    v01 = sep_zval<S>(w0, 1);                                                  \
+    v10 = sep_zval<S>(w1, 0);                                                  \
+    v11 = sep_zval<S>(w1, 1);                                                  \
     break;
-      CXP_LINEAR_SIDS(CXP_Z)
+        CXP_LINEAR_SIDS(CXP_Z)
 #undef CXP_Z
+      }
     }
-    // eight stores, the ones outside the tile go out of range
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int kc = kp + h - t.whi[2];
-      const bool zact = kc >= kc0 && kc <= kc1 && kp + h <= kl;
-      const int fz0 = 2 * kc;
-      const bool z0ok = zact && fz0 >= z0 && fz0 <= z1;
-      const bool z1ok = zact && fz0 + 1 >= z0 && fz0 + 1 <= z1;
-      const int zb0 = (fz0 - fine.lo[2]) * zstride;
-      const int zb1 = zb0 + zstride;
-      bst(rf, (own && yin0 && z0ok) ? fo0 + zb0 : CXP_OOB, v[h][0]);
-      bst(rf, (own && yin0 && z1ok) ? fo0 + zb1 : CXP_OOB, v[h][1]);
-      bst(rf, (own && yin1 && z0ok) ? fo1 + zb0 : CXP_OOB, v[h][2]);
-      bst(rf, (own && yin1 && z1ok) ? fo1 + zb1 : CXP_OOB, v[h][3]);
-    }
+    const int fz0 = 2 * kc, fz1 = 2 * kc + 1;
+    const bool z0ok = zact && fz0 >= z0 && fz0 <= z1;
+    const bool z1ok = zact && fz1 >= z0 && fz1 <= z1;
+    const int zb0 = 8 * (int)((long long)(fz0 - fine.lo[2]) * fine.sz);
+    const int zb1 = zb0 + 8 * (int)fine.sz;
+    bst(rf, (own && yin0 && z0ok) ? fo0 + zb0 : CXP_OOB, v00);
+    bst(rf, (own && yin0 && z1ok) ? fo0 + zb1 : CXP_OOB, v01);
+    bst(rf, (own && yin1 && z0ok) ? fo1 + zb0 : CXP_OOB, v10);
+    bst(rf, (own && yin1 && z1ok) ? fo1 + zb1 : CXP_OOB, v11);
   }
 }
 
