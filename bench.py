@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--tune", action="store_true", help="run the autotuner first")
     ap.add_argument("--cpu-boxes", type=int, default=48)
+    ap.add_argument("--no-wave", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="finish with three 512 MiB device copies (PMC byte-counter calibration)")
     return ap.parse_args()
 
 
@@ -119,6 +122,85 @@ def bench_level64(args, stream, log):
             batched_hbm_frac=round(st["algo_bytes"] / (mb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
         log("level_64 %s: batched %.4f ms, per-box %.4f ms" % (name, mb, mp))
     return out
+
+
+def bench_adjoint(stream, log):
+    """96^3 -> 192^3 VC 5th-order prolongation: forward vs adjoint (per-box
+    launches on one stream), plus the CCC minmod forward."""
+    import torch
+    from carpetx_amd.interp import Box, FArrayBox, get_interpolator
+    out = {}
+    region = Box((0, 0, 0), (191, 191, 191))
+    for name, typ, order, it in [("poly5_vvv", "interpolate", 5, (0, 0, 0)),
+                                 ("minmod_ccc", "minmod", 1, (1, 1, 1))]:
+        P = get_interpolator(typ, order, it)
+        cb = P.CoarseBox(region)
+        g = torch.Generator(device="cuda").manual_seed(4)
+        crse = FArrayBox(cb, 1, tensor=torch.rand((1,) + tuple(reversed(cb.size)),
+                                                   dtype=torch.float64, device="cuda",
+                                                   generator=g) * 2 - 1)
+        fine = FArrayBox(region, 1)
+        cbar = FArrayBox(cb, 1)
+        with torch.cuda.stream(stream):
+            for _ in range(3):
+                P.interp(crse, 0, fine, 0, 1, region, stream=stream)
+            tf = timed_events(lambda: P.interp(crse, 0, fine, 0, 1, region, stream=stream),
+                              20, stream)
+        algo = 8 * (region.numPts + cb.numPts)
+        ms_f = sum(tf) / len(tf)
+        rec = dict(fine_pts=region.numPts, algo_MB=round(algo / 1e6, 2),
+                   fwd_ms=round(ms_f, 4), fwd_hbm_frac=round(algo / (ms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        try:
+            with torch.cuda.stream(stream):
+                P.interp_adjoint(cbar, 0, fine, 0, 1, region, crse_in=crse, stream=stream)
+                ta = timed_events(lambda: P.interp_adjoint(cbar, 0, fine, 0, 1, region,
+                                                           crse_in=crse, stream=stream),
+                                  20, stream)
+            ms_a = sum(ta) / len(ta)
+            rec.update(adj_ms=round(ms_a, 4), adj_over_fwd=round(ms_a / ms_f, 3),
+                       adj_hbm_frac=round(algo / (ms_a * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        except Exception as e:  # operators without a device adjoint
+            rec.update(adj_ms=None, adj_note=str(e).split(":")[0][:80])
+        out[name] = rec
+        log("adjoint %s: %s" % (name, rec))
+    return out
+
+
+def bench_wave(log, n=128, steps=16, ck=4):
+    """Two-level subcycled wave evolution (coarse n^3, fine covering the
+    central half), RK4, ghost shells prolongated at every stage; forward and
+    backward (gradient w.r.t. the initial data, checkpoint every ck steps)."""
+    import torch
+    from carpetx_amd.wave import TwoLevelWave, loss
+    w = TwoLevelWave(n=n)
+    sc0, sf0 = w.gaussian(width=0.05)
+    xc = sc0.clone().requires_grad_(True)
+    xf = sf0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a, b = w.evolve(xc, xf, steps, checkpoint_every=ck)
+    L = loss(a, b)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    rec = dict(n=n, steps=steps, checkpoint_every=ck,
+               fwd_ms_per_step=round(1e3 * (t1 - t0) / steps, 3),
+               bwd_ms_per_step=round(1e3 * (t2 - t1) / steps, 3),
+               grad_norm=float(xc.grad.norm()), loss=float(L))
+    log("wave: %s" % rec)
+    return rec
+
+
+def load_traffic(name):
+    """HBM traffic per launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(name)
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline(wlz, nboxes, log):
@@ -211,7 +293,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": (load_traffic("uneven_512") or {}).get("bytes_per_launch"),
         },
         "extra": {
             "per_box_ms": round(res["per_box_ms"], 4),
@@ -222,10 +305,19 @@ def main():
     }
     if not args.no_extra:
         line["extra"]["level_64"] = bench_level64(args, stream, log)
+        line["extra"]["adjoint_96_192"] = bench_adjoint(stream, log)
+        if not args.no_wave:
+            line["extra"]["wave_two_level"] = bench_wave(log)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wlz, args.cpu_boxes, log)
     else:
         line["cpu_baseline"] = None
+    if args.copy_calibration:
+        src = torch.rand(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     print(json.dumps(line), flush=True)
 
 

@@ -1,0 +1,101 @@
+"""torch.autograd wrappers around the rf=2 transfer operators.
+
+Forward and backward both run in libcxp.so: the forward is cxp_prolongate /
+cxp_restrict, the backward is the hand-written adjoint kernel
+(cxp_prolongate_adjoint / cxp_restrict_adjoint).  Tensors are FABs laid out
+(ncomp, nz, ny, nx) over an index box, x fastest.
+
+    fine = prolongate(P, crse, crse_box, fine_region)
+    crse = restrict(fine, fine_box, crse_region, indextype)
+"""
+import torch
+
+from .interp import Box, FArrayBox, restrict as _restrict, \
+    restrict_adjoint as _restrict_adjoint
+
+
+def _fab(t, box):
+    return FArrayBox(box, t.shape[0], tensor=t)
+
+
+class _Prolongate(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, crse, P, crse_box, fine_region):
+        crse = crse.contiguous()
+        ncomp = crse.shape[0]
+        fine = torch.empty((ncomp,) + tuple(reversed(fine_region.size)),
+                           dtype=torch.float64, device=crse.device)
+        P.interp(_fab(crse, crse_box), 0, _fab(fine, fine_region), 0, ncomp,
+                 fine_region)
+        ctx.P, ctx.crse_box, ctx.fine_region = P, crse_box, fine_region
+        ctx.save_for_backward(crse)
+        return fine
+
+    @staticmethod
+    def backward(ctx, gfine):
+        (crse,) = ctx.saved_tensors
+        gfine = gfine.contiguous()
+        gcrse = torch.zeros_like(crse)
+        ctx.P.interp_adjoint(_fab(gcrse, ctx.crse_box), 0,
+                             _fab(gfine, ctx.fine_region), 0, crse.shape[0],
+                             ctx.fine_region, crse_in=_fab(crse, ctx.crse_box))
+        return gcrse, None, None, None
+
+
+def prolongate(P, crse, crse_box, fine_region):
+    """Differentiable prolongation of ``crse`` (covering ``crse_box``) onto
+    ``fine_region``; returns a tensor shaped like ``fine_region``."""
+    return _Prolongate.apply(crse, P, crse_box, fine_region)
+
+
+class _Restrict(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, fine, fine_box, crse_region, indextype):
+        fine = fine.contiguous()
+        ncomp = fine.shape[0]
+        crse = torch.empty((ncomp,) + tuple(reversed(crse_region.size)),
+                           dtype=torch.float64, device=fine.device)
+        _restrict(_fab(fine, fine_box), _fab(crse, crse_region), indextype,
+                  ncomp=ncomp, crse_region=crse_region)
+        ctx.fine_box, ctx.crse_region, ctx.indextype = fine_box, crse_region, indextype
+        ctx.shape = fine.shape
+        return crse
+
+    @staticmethod
+    def backward(ctx, gcrse):
+        gcrse = gcrse.contiguous()
+        gfine = torch.zeros(ctx.shape, dtype=torch.float64, device=gcrse.device)
+        _restrict_adjoint(_fab(gcrse, ctx.crse_region), _fab(gfine, ctx.fine_box),
+                          ctx.indextype, gcrse.shape[0], ctx.crse_region)
+        return gfine, None, None, None
+
+
+def restrict(fine, fine_box, crse_region, indextype):
+    """Differentiable restriction of ``fine`` (covering ``fine_box``) onto
+    ``crse_region``."""
+    return _Restrict.apply(fine, fine_box, crse_region, tuple(indextype))
+
+
+def region_view(t, box, region):
+    """Slice of a FAB tensor ``t`` over ``box`` restricted to ``region``."""
+    sl = tuple(slice(region.lo[d] - box.lo[d], region.hi[d] - box.lo[d] + 1)
+               for d in (2, 1, 0))
+    return t[(slice(None),) + sl]
+
+
+def fill_ghosts(P, crse, crse_box, fine, fine_box, interior):
+    """Differentiable ghost-shell fill: returns ``fine`` with every point of
+    ``fine_box`` outside ``interior`` replaced by the prolongation of
+    ``crse``.  The shell is split into the six slabs of
+    cxp_worklist_append_shell."""
+    from .workloads import shell_slabs
+    out = fine.clone()
+    for slab in shell_slabs(fine_box, interior):
+        vals = prolongate(P, crse, crse_box, slab)
+        sl = tuple(slice(slab.lo[d] - fine_box.lo[d], slab.hi[d] - fine_box.lo[d] + 1)
+                   for d in (2, 1, 0))
+        out[(slice(None),) + sl] = vals
+    return out
+
+
+__all__ = ["prolongate", "restrict", "fill_ghosts", "region_view", "Box"]

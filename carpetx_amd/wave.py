@@ -1,0 +1,171 @@
+"""Differentiable two-level scalar-wave evolution (box-in-box, subcycled).
+
+The scalar wave equation u_t = rho, rho_t = Laplace(u) is evolved on
+
+* a coarse periodic vertex-centred level of n^3 points, spacing h = 1/n;
+* a fine level of spacing h/2 covering the central half, n+1 points per
+  direction plus 2 ghost points, whose ghost shell is prolongated from the
+  coarse level (natural family, order 5, vertex-centred) at every RK4 stage.
+
+One coarse step is an RK4 step on the coarse level followed by two RK4 steps
+on the fine level (subcycling, dt_f = dt/2); the fine ghost values at a
+stage time are prolongated from the coarse state linearly interpolated in
+time between t_n and t_{n+1}.  The coarse points under the fine level are
+then replaced by injection of the fine solution (VC restriction).  The
+Laplacian is the textbook 4th-order centred one,
+(-u[i-2] + 16 u[i-1] - 30 u[i] + 16 u[i+1] - u[i+2]) / (12 h^2) per
+direction.  (WaveToyX's fd_order=4 stencil, WaveToyX/src/wavetoyx.cxx:148-153,
+uses -1/2 for the +2 neighbour; this module does not reproduce that.)
+
+Orchestration mirrors an RK4 ODESolvers step with a SYNC after every
+substep (ODESolvers/src/solve.cxx:888-916); subcycling itself is our own
+(the reference's is disabled, CarpetX/param.ccl:460-462).
+
+Every prolongation and restriction runs in libcxp.so and is differentiable
+through the hand-written adjoint kernels, so the gradient of a loss with
+respect to the initial data flows back through the whole evolution;
+``checkpoint_every`` coarse steps are recomputed in the backward pass
+instead of being stored.
+"""
+import torch
+from torch.utils.checkpoint import checkpoint
+
+from .autograd import fill_ghosts, region_view, restrict
+from .interp import Box, get_interpolator
+
+GHOSTS = 2
+
+
+def laplacian4(u, h):
+    """4th-order Laplacian of the interior of ``u`` (2 ghost points per side
+    consumed); u shaped (ncomp, nz, ny, nx)."""
+    c = 1.0 / (12.0 * h * h)
+    inner = u[:, 2:-2, 2:-2, 2:-2]
+    out = -90.0 * inner  # -30 per direction
+    for axis in (1, 2, 3):
+        def sh(k):
+            sl = [slice(None), slice(2, -2), slice(2, -2), slice(2, -2)]
+            sl[axis] = slice(2 + k, u.shape[axis] - 2 + k)
+            return u[tuple(sl)]
+        out = out + 16.0 * (sh(-1) + sh(1)) - (sh(-2) + sh(2))
+    return c * out
+
+
+def periodic_pad(u, g):
+    return torch.nn.functional.pad(u.unsqueeze(0), (g, g, g, g, g, g),
+                                   mode="circular").squeeze(0)
+
+
+class TwoLevelWave:
+    def __init__(self, n=128, cfl=0.25, prolongation_order=5, device="cuda"):
+        assert n % 4 == 0
+        self.n = n
+        self.h = 1.0 / n
+        self.dt = cfl * self.h
+        self.device = device
+        self.P = get_interpolator("natural", prolongation_order, (0, 0, 0))
+        q = n // 4
+        # fine interior: coarse [q, 3q] -> fine [2q, 6q] (VC points)
+        self.fine_int = Box((2 * q,) * 3, (6 * q,) * 3)
+        self.fine_box = self.fine_int.grow(GHOSTS)
+        shell_cb = self.P.CoarseBox(self.fine_box)
+        # coarse sub-box read by the ghost fill (inside the periodic domain)
+        self.crse_sub = shell_cb
+        assert all(0 <= shell_cb.lo[d] and shell_cb.hi[d] < n for d in range(3))
+        self.crse_under = Box((q,) * 3, (3 * q,) * 3)  # coarse points under fine
+
+    # --- initial data -----------------------------------------------------
+    def gaussian(self, width=0.05, amp=1.0):
+        n, h, dev = self.n, self.h, self.device
+        xc = torch.arange(n, dtype=torch.float64, device=dev) * h - 0.5
+        uc = amp * torch.exp(-(xc[:, None, None] ** 2 + xc[None, :, None] ** 2 +
+                               xc[None, None, :] ** 2) / (2 * width ** 2))
+        fb = self.fine_box
+        xf = torch.arange(fb.lo[0], fb.hi[0] + 1, dtype=torch.float64, device=dev) * (h / 2) - 0.5
+        uf = amp * torch.exp(-(xf[:, None, None] ** 2 + xf[None, :, None] ** 2 +
+                               xf[None, None, :] ** 2) / (2 * width ** 2))
+        zc = torch.zeros_like(uc)
+        zf = torch.zeros_like(uf)
+        return torch.stack([uc, zc]), torch.stack([uf, zf])
+
+    # --- right-hand sides ---------------------------------------------------
+    def rhs_coarse(self, s):
+        sp = periodic_pad(s, 2)
+        lap = laplacian4(sp[0:1], self.h)[0]
+        return torch.stack([s[1], lap])
+
+    def rhs_fine(self, s):
+        """RHS on the fine interior (ghosts already filled)."""
+        lap = laplacian4(s[0:1], self.h / 2)[0]
+        inner = s[:, 2:-2, 2:-2, 2:-2]
+        return torch.stack([inner[1], lap])
+
+    def fill(self, sf, crse_state):
+        sub = region_view(crse_state, Box((0, 0, 0), (self.n - 1,) * 3), self.crse_sub)
+        return fill_ghosts(self.P, sub, self.crse_sub, sf, self.fine_box, self.fine_int)
+
+    # --- time stepping ------------------------------------------------------
+    def rk4_coarse(self, s):
+        dt = self.dt
+        k1 = self.rhs_coarse(s)
+        k2 = self.rhs_coarse(s + 0.5 * dt * k1)
+        k3 = self.rhs_coarse(s + 0.5 * dt * k2)
+        k4 = self.rhs_coarse(s + dt * k3)
+        return s + (dt / 6.0) * (k1 + 2 * k2 + 2 * k3 + k4)
+
+    def rk4_fine(self, sf, c0, c1, theta0, dtf):
+        """One fine RK4 step from time theta0 (in units of the coarse step);
+        ghost values at each stage from the coarse state interpolated
+        linearly between c0 (t_n) and c1 (t_{n+1})."""
+        def at(th):
+            return (1.0 - th) * c0 + th * c1
+
+        def pad_interior(inner, like):
+            out = like.clone()
+            out[:, 2:-2, 2:-2, 2:-2] = inner
+            return out
+        half = 0.5 * dtf / self.dt
+        y1 = self.fill(sf, at(theta0))
+        k1 = self.rhs_fine(y1)
+        y2 = self.fill(pad_interior(y1[:, 2:-2, 2:-2, 2:-2] + 0.5 * dtf * k1, y1), at(theta0 + half))
+        k2 = self.rhs_fine(y2)
+        y3 = self.fill(pad_interior(y1[:, 2:-2, 2:-2, 2:-2] + 0.5 * dtf * k2, y1), at(theta0 + half))
+        k3 = self.rhs_fine(y3)
+        y4 = self.fill(pad_interior(y1[:, 2:-2, 2:-2, 2:-2] + dtf * k3, y1), at(theta0 + 2 * half))
+        k4 = self.rhs_fine(y4)
+        new_inner = y1[:, 2:-2, 2:-2, 2:-2] + (dtf / 6.0) * (k1 + 2 * k2 + 2 * k3 + k4)
+        return self.fill(pad_interior(new_inner, y1), at(theta0 + 2 * half))
+
+    def step(self, sc, sf):
+        c1 = self.rk4_coarse(sc)
+        dtf = 0.5 * self.dt
+        sf = self.rk4_fine(sf, sc, c1, 0.0, dtf)
+        sf = self.rk4_fine(sf, sc, c1, 0.5, dtf)
+        # restriction: coarse points under the fine level from fine injection
+        inj = restrict(sf, self.fine_box, self.crse_under, (0, 0, 0))
+        cu = self.crse_under
+        c1 = c1.clone()
+        c1[:, cu.lo[2]:cu.hi[2] + 1, cu.lo[1]:cu.hi[1] + 1, cu.lo[0]:cu.hi[0] + 1] = inj
+        return c1, sf
+
+    def evolve(self, sc, sf, nsteps, checkpoint_every=0):
+        if checkpoint_every <= 0:
+            for _ in range(nsteps):
+                sc, sf = self.step(sc, sf)
+            return sc, sf
+
+        def block(sc, sf, k):
+            for _ in range(k):
+                sc, sf = self.step(sc, sf)
+            return sc, sf
+        done = 0
+        while done < nsteps:
+            k = min(checkpoint_every, nsteps - done)
+            sc, sf = checkpoint(block, sc, sf, k, use_reentrant=False)
+            done += k
+        return sc, sf
+
+
+def loss(sc, sf):
+    """Energy-like quadratic functional of the final state."""
+    return 0.5 * (sc[0] ** 2).sum() + 0.5 * (sf[0, 2:-2, 2:-2, 2:-2] ** 2).sum() / 8.0
