@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/cxp.h"
 #include "kernels.hpp"
 #include "ops.hpp"
@@ -73,6 +75,79 @@ __constant__ AdjW kAdjW = make_adjw();
 
 // --------------------------------------------------------------------------
 // Linear adjoint
+//
+// Workgroup = 256 threads owning a 16 x 16 coarse (x, y) output tile and a
+// chunk of coarse z-planes.  It marches through the fine z-planes that feed
+// the chunk.  Per fine plane:
+//   fine window (2(16+span_x) x 2(16+span_y)) -> LDS, split by x parity,
+//       loaded one plane ahead into registers with raw buffer loads (points
+//       outside the fine region read as 0, so no branches);
+//   x^T: fine x -> coarse x for every window row      -> LDS
+//   y^T: fine y -> coarse y, one output per thread   -> register
+//   z^T: accumulate into a register window of coarse planes, one plane
+//        completed (and added to crse_bar) per pair of fine planes.
+// Each transpose is compiled per stencil with its nonzero weights only.
+
+constexpr AdjW kAdjWc = make_adjw();
+
+template <int S> constexpr int adj_span() { return sid_whi(S) - sid_wlo(S); }
+
+// sum_q w[S][OFF][q] * src[(b - q) * stride], zero weights skipped
+template <int S, int OFF, int Q> struct AdjTaps {
+  __device__ static __forceinline__ void add(double &s, const double *src,
+                                             int b, int stride) {
+    constexpr double w = kAdjWc.w[S][OFF][Q];
+    if constexpr (w != 0.0)
+      s += w * src[(b - Q) * stride];
+    AdjTaps<S, OFF, Q + 1>::add(s, src, b, stride);
+  }
+};
+template <int S, int OFF> struct AdjTaps<S, OFF, 9> {
+  __device__ static __forceinline__ void add(double &, const double *, int,
+                                             int) {}
+};
+
+// x^T of one window row: Fe/Fo hold the row's even/odd fine points; output
+// coarse local cx = sum_{off,q} w[off][q] * F_off[cx + span - q].
+template <int S>
+__device__ __forceinline__ double adj_x(const double *Fe, const double *Fo,
+                                        int cx) {
+  constexpr int span = adj_span<S>();
+  double s = 0.0;
+  AdjTaps<S, 0, 0>::add(s, Fe, cx + span, 1);
+  AdjTaps<S, 1, 0>::add(s, Fo, cx + span, 1);
+  return s;
+}
+
+// y^T: Q rows are fine window rows (stride 16); row of (cy, q, off) is
+// 2 (cy + span - q) + off.
+template <int S>
+__device__ __forceinline__ double adj_y(const double *Q, int cx, int cy) {
+  constexpr int span = adj_span<S>();
+  double s = 0.0;
+  AdjTaps<S, 0, 0>::add(s, Q + cx, cy + span, 32);
+  AdjTaps<S, 1, 0>::add(s, Q + cx + 16, cy + span, 32);
+  return s;
+}
+
+template <int S, int OFF, int Q> struct AdjZ {
+  __device__ static __forceinline__ void add(double (&acc)[9], double v) {
+    constexpr double w = kAdjWc.w[S][OFF][Q];
+    if constexpr (w != 0.0)
+      acc[Q] += w * v;
+    AdjZ<S, OFF, Q + 1>::add(acc, v);
+  }
+};
+template <int S, int OFF> struct AdjZ<S, OFF, 9> {
+  __device__ static __forceinline__ void add(double (&)[9], double) {}
+};
+
+#define CXP_ADJ_LINEAR_SIDS(X)                                                 \
+  X(SID_VPOLY1) X(SID_VPOLY3) X(SID_VPOLY5) X(SID_VPOLY7) X(SID_CPOLY0)        \
+  X(SID_CPOLY1) X(SID_CPOLY2) X(SID_CPOLY3) X(SID_CPOLY4) X(SID_CPOLY5)        \
+  X(SID_CPOLY6) X(SID_CPOLY7) X(SID_VCONS0) X(SID_VCONS1) X(SID_CCONS0)        \
+  X(SID_CCONS1) X(SID_CCONS2) X(SID_CCONS3) X(SID_CCONS4) X(SID_CCONS5)        \
+  X(SID_CCONS6) X(SID_CCONS7)
 
 struct AdjTask {
   DFab fbar;  // fine adjoint (input)
@@ -82,125 +157,140 @@ struct AdjTask {
   int wlo[3], whi[3];
   int olo[3], ohi[3];  // coarse output range
   long long fcs, ccs;
+  int fbytes, cbytes;  // bytes of one component of each FAB
 };
 
-#define CXP_ADJ_LX 5   // coarse tile x = 32
-#define CXP_ADJ_LY 4   // coarse tile y = 16
-#define CXP_ADJ_FXS 84 // fine window row capacity: 2*32 + 2*9 + 2
-#define CXP_ADJ_FYS 52 // fine window rows: 2*16 + 2*9 + 2
+#define CXP_ADJ_T 16                 // coarse tile edge (x and y)
+#define CXP_ADJ_WMAX 48              // 2 (16 + max span 8)
+#define CXP_ADJ_R 9                  // ceil(48 * 48 / 256) window loads
+#define CXP_ADJ_HALF (CXP_ADJ_WMAX / 2)
 
 __global__ void __launch_bounds__(256)
     k_adj_linear(AdjTask t, int ntx, int nty, int kchunk) {
-  constexpr int CTX = 1 << CXP_ADJ_LX;
-  constexpr int CTY = 1 << CXP_ADJ_LY;
-  constexpr int NP = CTX * CTY / 256;
-  __shared__ double Fs[CXP_ADJ_FYS * CXP_ADJ_FXS];
-  __shared__ double Qs[CXP_ADJ_FYS * CTX];
+  __shared__ double Fs[2 * CXP_ADJ_WMAX * CXP_ADJ_HALF]; // [parity][row][x/2]
+  __shared__ double Qs[CXP_ADJ_WMAX * CXP_ADJ_T];        // [row][cx]
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
   const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
   const long long comp = blockIdx.y;
-  const int ci0 = t.olo[0] + tx * CTX;
-  const int cj0 = t.olo[1] + ty * CTY;
+  const int ci0 = t.olo[0] + tx * CXP_ADJ_T;
+  const int cj0 = t.olo[1] + ty * CXP_ADJ_T;
   const int kca = t.olo[2] + tz * kchunk;
   const int kcb = min(t.ohi[2], kca + kchunk - 1);
-  const int CTXa = min(CTX, t.ohi[0] - ci0 + 1);
-  const int CTYa = min(CTY, t.ohi[1] - cj0 + 1);
+  const int spx = t.whi[0] - t.wlo[0], spy = t.whi[1] - t.wlo[1];
+  const int W = 2 * (CXP_ADJ_T + spx), H = 2 * (CXP_ADJ_T + spy);
+  const int hw = W >> 1;
   const int fxa = 2 * (ci0 - t.whi[0]);
-  const int fxw = 2 * (ci0 + CTXa - 1 - t.wlo[0]) + 1 - fxa + 1;
   const int fya = 2 * (cj0 - t.whi[1]);
-  const int fyw = 2 * (cj0 + CTYa - 1 - t.wlo[1]) + 1 - fya + 1;
-  const int nqx = t.whi[0] - t.wlo[0] + 1;
-  const int nqy = t.whi[1] - t.wlo[1] + 1;
-  DFab fb = t.fbar;
-  fb.p += comp * t.fcs;
-  DFab cb = t.cbar;
-  cb.p += comp * t.ccs;
-  const double(*wx)[9] = kAdjW.w[t.sid[0]];
-  const double(*wy)[9] = kAdjW.w[t.sid[1]];
-  const double(*wz)[9] = kAdjW.w[t.sid[2]];
+  const __amdgpu_buffer_rsrc_t rf =
+      make_rsrc(t.fbar.p + comp * t.fcs, t.fbytes);
+  const __amdgpu_buffer_rsrc_t rc =
+      make_rsrc(t.cbar.p + comp * t.ccs, t.cbytes);
 
-  double acc[NP][9];
+  // Per-thread window slots: byte offset inside a fine plane (or -1 when the
+  // point lies outside the window or the fine region) and LDS slot.
+  int soff[CXP_ADJ_R], lslot[CXP_ADJ_R];
 #pragma unroll
-  for (int n = 0; n < NP; ++n)
-#pragma unroll
-    for (int q = 0; q < 9; ++q)
-      acc[n][q] = 0.0;
+  for (int r = 0; r < CXP_ADJ_R; ++r) {
+    const int idx = tid + 256 * r;
+    const int fy = idx / W;
+    const int fx = idx - fy * W;
+    const int gx = fxa + fx, gy = fya + fy;
+    const bool in = idx < W * H && gx >= t.rlo[0] && gx <= t.rhi[0] &&
+                    gy >= t.rlo[1] && gy <= t.rhi[1];
+    soff[r] = in ? 8 * (int)((gx - t.fbar.lo[0]) +
+                             (long long)(gy - t.fbar.lo[1]) * t.fbar.sy)
+                 : -1;
+    lslot[r] = idx < W * H
+                   ? (fx & 1) * (CXP_ADJ_WMAX * CXP_ADJ_HALF) + fy * hw + (fx >> 1)
+                   : -1;
+  }
+  const int cx = tid & 15, cy = tid >> 4;
+  const int sidx = t.sid[0], sidy = t.sid[1], sidz = t.sid[2];
 
-  for (int m = kca - t.whi[2]; m <= kcb - t.wlo[2]; ++m) {
+  const int mA = kca - t.whi[2], mB = kcb - t.wlo[2];
+  // fine plane fz -> byte base, or -1 outside the region
+  auto plane = [&](int fz) -> int {
+    return (fz >= t.rlo[2] && fz <= t.rhi[2])
+               ? 8 * (int)((long long)(fz - t.fbar.lo[2]) * t.fbar.sz)
+               : -1;
+  };
+  double pre[CXP_ADJ_R];
+  {
+    const int pb = plane(2 * mA);
 #pragma unroll
-    for (int off = 0; off < 2; ++off) {
-      const int fz = 2 * m + off;
-      if (fz < t.rlo[2] || fz > t.rhi[2])
-        continue;
-      // fine plane window -> LDS, zero outside the region
-      for (int idx = tid; idx < fxw * fyw; idx += 256) {
-        const int fy = idx / fxw;
-        const int fx = idx - fy * fxw;
-        const int gx = fxa + fx, gy = fya + fy;
+    for (int r = 0; r < CXP_ADJ_R; ++r)
+      pre[r] = bld(rf, (pb >= 0 && soff[r] >= 0) ? pb + soff[r] : CXP_OOB);
+  }
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q)
+    acc[q] = 0.0;
+
+  for (int m = mA; m <= mB; ++m) {
+    auto fine_plane = [&](auto offc) {
+      constexpr int off = decltype(offc)::value;
+      // this plane's window -> LDS
+#pragma unroll
+      for (int r = 0; r < CXP_ADJ_R; ++r)
+        if (lslot[r] >= 0)
+          Fs[lslot[r]] = pre[r];
+      // next plane -> registers
+      {
+        const int nz = 2 * m + off + 1;
+        const int pb = nz <= 2 * mB + 1 ? plane(nz) : -1;
+#pragma unroll
+        for (int r = 0; r < CXP_ADJ_R; ++r)
+          pre[r] = bld(rf, (pb >= 0 && soff[r] >= 0) ? pb + soff[r] : CXP_OOB);
+      }
+      __syncthreads();
+      // x^T over every window row
+      for (int idx = tid; idx < H * CXP_ADJ_T; idx += 256) {
+        const int row = idx >> 4, qx = idx & 15;
+        const double *Fe = Fs + row * hw;
+        const double *Fo = Fe + CXP_ADJ_WMAX * CXP_ADJ_HALF;
         double v = 0.0;
-        if (gx >= t.rlo[0] && gx <= t.rhi[0] && gy >= t.rlo[1] && gy <= t.rhi[1])
-          v = gld(fb.p, didx(fb, gx, gy, fz));
-        Fs[fy * CXP_ADJ_FXS + fx] = v;
+        switch (sidx) {
+#define CXP_AX(S)                                                              \
+  case S:                                                                      \
+    v = adj_x<S>(Fe, Fo, qx);                                                  \
+    break;
+          CXP_ADJ_LINEAR_SIDS(CXP_AX)
+#undef CXP_AX
+        }
+        Qs[idx] = v;
       }
       __syncthreads();
-      // x^T: fine x -> coarse x, every window row
-      for (int idx = tid; idx < (fyw << CXP_ADJ_LX); idx += 256) {
-        const int fy = idx >> CXP_ADJ_LX;
-        const int cil = idx & (CTX - 1);
-        if (cil < CTXa) {
-          const int ci = ci0 + cil;
-          const double *row = Fs + fy * CXP_ADJ_FXS;
-          double s = 0.0;
-          for (int q = 0; q < nqx; ++q) {
-            const int base = 2 * (ci - (q + t.wlo[0])) - fxa;
-            s += wx[0][q] * row[base] + wx[1][q] * row[base + 1];
-          }
-          Qs[idx] = s;
-        }
+      double s = 0.0;
+      switch (sidy) {
+#define CXP_AY(S)                                                              \
+  case S:                                                                      \
+    s = adj_y<S>(Qs, cx, cy);                                                  \
+    break;
+        CXP_ADJ_LINEAR_SIDS(CXP_AY)
+#undef CXP_AY
       }
-      __syncthreads();
-      // y^T, then z^T into the coarse-plane window
-#pragma unroll
-      for (int n = 0; n < NP; ++n) {
-        const int p = tid + n * 256;
-        const int cil = p & (CTX - 1);
-        const int cjl = p >> CXP_ADJ_LX;
-        if (cil < CTXa && cjl < CTYa) {
-          const int cj = cj0 + cjl;
-          double s = 0.0;
-          for (int q = 0; q < nqy; ++q) {
-            const int base = 2 * (cj - (q + t.wlo[1])) - fya;
-            s += wy[0][q] * Qs[(base << CXP_ADJ_LX) + cil] +
-                 wy[1][q] * Qs[((base + 1) << CXP_ADJ_LX) + cil];
-          }
-#pragma unroll
-          for (int q = 0; q < 9; ++q)
-            acc[n][q] += wz[off][q] * s;
-        }
+      switch (sidz) {
+#define CXP_AZ(S)                                                              \
+  case S:                                                                      \
+    AdjZ<S, off, 0>::add(acc, s);                                              \
+    break;
+        CXP_ADJ_LINEAR_SIDS(CXP_AZ)
+#undef CXP_AZ
       }
-    }
+    };
+    fine_plane(std::integral_constant<int, 0>{});
+    fine_plane(std::integral_constant<int, 1>{});
     // coarse plane m + wlo is complete
     const int kc = m + t.wlo[2];
-    if (kc >= kca && kc <= kcb) {
+    const int ci = ci0 + cx, cj = cj0 + cy;
+    const bool own = kc >= kca && ci <= t.ohi[0] && cj <= t.ohi[1];
+    const int co = own ? 8 * (int)didx(t.cbar, ci, cj, kc) : CXP_OOB;
+    bst(rc, co, bld(rc, co) + acc[0]);
 #pragma unroll
-      for (int n = 0; n < NP; ++n) {
-        const int p = tid + n * 256;
-        const int cil = p & (CTX - 1);
-        const int cjl = p >> CXP_ADJ_LX;
-        if (cil < CTXa && cjl < CTYa) {
-          const long long di = didx(cb, ci0 + cil, cj0 + cjl, kc);
-          gst(cb.p, di, gld(cb.p, di) + acc[n][0]);
-        }
-      }
-    }
-#pragma unroll
-    for (int n = 0; n < NP; ++n) {
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        acc[n][q] = acc[n][q + 1];
-      acc[n][8] = 0.0;
-    }
+    for (int q = 0; q < 8; ++q)
+      acc[q] = acc[q + 1];
+    acc[8] = 0.0;
   }
 }
 
@@ -214,6 +304,11 @@ inline DFab adj_dfab(const cxp_fab *f, int comp) {
   v.sy = f->hi[0] - f->lo[0] + 1;
   v.sz = (long long)v.sy * (f->hi[1] - f->lo[1] + 1);
   return v;
+}
+
+inline long long adj_component_bytes(const cxp_fab *f) {
+  return 8LL * (f->hi[0] - f->lo[0] + 1) * (f->hi[1] - f->lo[1] + 1) *
+         (f->hi[2] - f->lo[2] + 1);
 }
 
 inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
@@ -247,13 +342,18 @@ inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
     }
     t.fcs = fine_bar->comp_stride;
     t.ccs = crse_bar->comp_stride;
-    const int ntx = (ohi[0] - olo[0] + 1 + 31) / 32;
-    const int nty = (ohi[1] - olo[1] + 1 + 15) / 16;
+    // component volumes; the caller has checked both stay below 2 GiB
+    t.fbytes = (int)adj_component_bytes(fine_bar);
+    t.cbytes = (int)adj_component_bytes(crse_bar);
+    const int ntx = (ohi[0] - olo[0] + CXP_ADJ_T) / CXP_ADJ_T;
+    const int nty = (ohi[1] - olo[1] + CXP_ADJ_T) / CXP_ADJ_T;
     const int nk = ohi[2] - olo[2] + 1;
-    int kchunk = nk;
-    while (kchunk > 4 && (long long)ntx * nty * ((nk + kchunk - 1) / kchunk) *
-                                 ncomp < 1024)
-      kchunk = (kchunk + 1) / 2;
+    // z-chunks: about 1024 workgroups, but at least 8 coarse planes each
+    // (a chunk re-reads span_z extra pairs of fine planes)
+    const long long cols = (long long)ntx * nty * ncomp;
+    int kchunk = (int)((nk * cols + 1023) / 1024);
+    kchunk = kchunk < 8 ? 8 : kchunk;
+    kchunk = kchunk > nk ? nk : kchunk;
     const int nkc = (nk + kchunk - 1) / kchunk;
     hipLaunchKernelGGL(k_adj_linear, dim3(ntx * nty * nkc, ncomp), dim3(256), 0,
                        s, t, ntx, nty, kchunk);
