@@ -163,17 +163,24 @@ bool cfg_fits(int cfg, const SepTask &t) {
   return true;
 }
 
+// Untuned tile shape: the one covering the patch's (x, y) extent with the
+// fewest fine points (idle lanes), the wider one on ties (longer coalesced
+// rows).  Tiles start at even x and y.
 int default_cfg(const SepTask &t) {
   const int nx = t.rhi[0] - (t.rlo[0] & ~1) + 1;
-  if (nx <= 4)
-    return 4;
-  if (nx <= 8)
-    return 3;
-  if (nx <= 16)
-    return 2;
-  if (nx <= 32)
-    return 1;
-  return 0;
+  const int ny = t.rhi[1] - (t.rlo[1] & ~1) + 1;
+  int best = 0;
+  long long best_area = -1;
+  for (int c = 0; c < kNumTileConfigs; ++c) {
+    const long long area = (long long)((nx + kCfgs[c].tx - 1) / kCfgs[c].tx) *
+                           kCfgs[c].tx *
+                           ((ny + kCfgs[c].ty - 1) / kCfgs[c].ty) * kCfgs[c].ty;
+    if (best_area < 0 || area < best_area) {
+      best = c;
+      best_area = area;
+    }
+  }
+  return best;
 }
 
 using TuneKey = std::tuple<int, int, int, int>;
