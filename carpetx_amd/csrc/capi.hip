@@ -8,10 +8,11 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <queue>
 #include <vector>
 
 #include "../../include/cxp.h"
-#include "kernels.hpp"
+#include "batched.hpp"
 #include "ops.hpp"
 #include "adjoint.hpp"
 
@@ -253,7 +254,20 @@ int launch_point(const PointTask &t, int ncomp, int per_group,
   return hip_status(hipGetLastError());
 }
 
+int launch_thin_box(const SepTask &t, int ncomp, hipStream_t s) {
+  const int nx = t.rhi[0] - (t.rlo[0] & ~1) + 1;
+  const int ny = t.rhi[1] - (t.rlo[1] & ~1) + 1;
+  const int ntx = (nx + 15) / 16, nty = (ny + 15) / 16;
+  const size_t lds = 8 * (size_t)thin_lds_doubles(t.whi[1] - t.wlo[1],
+                                                  t.whi[2] - t.wlo[2]);
+  hipLaunchKernelGGL(k_thin_box, dim3(ntx * nty, ncomp), dim3(CXP_NT), lds, s,
+                     t, ntx);
+  return hip_status(hipGetLastError());
+}
+
 int launch_sep_box(const SepTask &t, int ncomp, hipStream_t s) {
+  if (thin_eligible(t))
+    return launch_thin_box(t, ncomp, s);
   int cfg, zc;
   choose_cfg(t, cfg, zc);
   const TileCfg &c = kCfgs[cfg];
@@ -305,9 +319,13 @@ struct cxp_worklist_s {
   std::vector<int> point_patches;  // indices of non-separable patches
   SepTask *d_tasks = nullptr;
   SepTile *d_tiles = nullptr;
+  std::vector<int> wg_start;  // per-workgroup tile ranges of k_fill_batched
+  int *d_wg_start = nullptr;
+  size_t cap_wg = 0;
   size_t cap_tasks = 0, cap_tiles = 0;
   bool uploaded = false;
   int grid = 0;
+  int lds = 0;  // bytes of dynamic LDS of k_fill_batched
   int64_t fine_points = 0, algo_bytes = 0;
 };
 
@@ -326,7 +344,102 @@ int64_t patch_coarse_points(const cxp_worklist_s::Patch &p) {
          (chi[2] - clo[2] + 1);
 }
 
-int persistent_grid(int ntiles);
+int persistent_grid(int ntiles, int lds);
+
+// Orders the tile list into one contiguous range per workgroup of the
+// persistent grid (see k_fill_batched).  Cost estimates, in units of one
+// plane iteration of the z-marching routine: the coarse planes a separable
+// tile marches through plus a fixed per-tile overhead; a thin tile costs
+// about two plane iterations (measured on the uneven level).
+void schedule_tiles(cxp_worklist_s *wl, int grid) {
+  struct Group {
+    size_t first, count;
+    long long cost;
+  };
+  const size_t nt = wl->tiles.size();
+  std::vector<long long> cost(nt);
+  std::vector<Group> groups;
+  for (size_t i = 0; i < nt; ++i) {
+    const SepTile &tl = wl->tiles[i];
+    const SepTask &t = wl->tasks[tl.task];
+    cost[i] = tl.cfg < 0 ? 2
+                         : (long long)((tl.z1 >> 1) + t.whi[2] - (tl.z0 >> 1) -
+                                       t.wlo[2] + 1) + 4;
+    if (groups.empty() || wl->tiles[groups.back().first].task != tl.task ||
+        wl->tiles[groups.back().first].comp != tl.comp)
+      groups.push_back({i, 0, 0});
+    ++groups.back().count;
+    groups.back().cost += cost[i];
+  }
+  const int nx = grid >= 8 ? 8 : 1;  // XCD classes of workgroups
+  // groups -> XCD, the most expensive first to the least loaded
+  std::vector<size_t> order(groups.size());
+  for (size_t i = 0; i < order.size(); ++i)
+    order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return groups[a].cost > groups[b].cost;
+  });
+  std::vector<std::vector<size_t>> xcd_tiles(nx);
+  std::vector<long long> xload(nx, 0);
+  for (size_t gi : order) {
+    int best = 0;
+    for (int x = 1; x < nx; ++x)
+      if (xload[x] < xload[best])
+        best = x;
+    for (size_t k = 0; k < groups[gi].count; ++k)
+      xcd_tiles[best].push_back(groups[gi].first + k);
+    xload[best] += groups[gi].cost;
+  }
+  // within an XCD: longest tile first to the least loaded of its workgroups
+  std::vector<std::vector<size_t>> wg_tiles(grid);
+  for (int x = 0; x < nx; ++x) {
+    std::vector<size_t> &ts = xcd_tiles[x];
+    std::stable_sort(ts.begin(), ts.end(),
+                     [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    using Slot = std::pair<long long, int>;  // (load, workgroup)
+    std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+    for (int b = x; b < grid; b += nx)
+      heap.push({0, b});
+    for (size_t i : ts) {
+      Slot sl = heap.top();
+      heap.pop();
+      wg_tiles[sl.second].push_back(i);
+      sl.first += cost[i];
+      heap.push(sl);
+    }
+  }
+  // per workgroup: interleave separable and thin tiles by cost progress
+  std::vector<SepTile> out;
+  out.reserve(nt);
+  wl->wg_start.assign(grid + 1, 0);
+  for (int b = 0; b < grid; ++b) {
+    wl->wg_start[b] = (int)out.size();
+    std::vector<size_t> sep, thin;
+    long long csep = 0, cthin = 0;
+    for (size_t i : wg_tiles[b]) {
+      if (wl->tiles[i].cfg < 0) {
+        thin.push_back(i);
+        cthin += cost[i];
+      } else {
+        sep.push_back(i);
+        csep += cost[i];
+      }
+    }
+    size_t a = 0, c = 0;
+    long long dsep = 0, dthin = 0;
+    while (a < sep.size() || c < thin.size()) {
+      // the kind whose done fraction is smaller goes next
+      const bool take_sep =
+          c >= thin.size() ||
+          (a < sep.size() && (double)dsep * cthin <= (double)dthin * csep);
+      const size_t i = take_sep ? sep[a++] : thin[c++];
+      (take_sep ? dsep : dthin) += cost[i];
+      out.push_back(wl->tiles[i]);
+    }
+  }
+  wl->wg_start[grid] = (int)out.size();
+  wl->tiles.swap(out);
+}
 
 // Tiles of the separable patches.  Untuned patches share one z-chunk length,
 // the longest (fewest halo planes) that still gives the persistent grid
@@ -335,6 +448,7 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
                  const TuneKey *only_key = nullptr) {
   wl->tasks.clear();
   wl->tiles.clear();
+  wl->lds = 8 * kLdsDoubles;
   wl->point_patches.clear();
   struct Plan {
     int task, cfg, zc, ncomp;
@@ -348,6 +462,31 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
     }
     SepTask t = make_sep_task(p.op->d, &p.crse, p.crse_comp, &p.fine,
                               p.fine_comp, p.rlo, p.rhi);
+    if (thin_eligible(t)) {
+      // whole-z 16 x 16 tiles; no tile shape to tune
+      if (only_key)
+        continue;
+      const int task = (int)wl->tasks.size();
+      wl->tasks.push_back(t);
+      const int lds = 8 * thin_lds_doubles(t.whi[1] - t.wlo[1],
+                                           t.whi[2] - t.wlo[2]);
+      wl->lds = std::max(wl->lds, lds);
+      for (int comp = 0; comp < p.ncomp; ++comp)
+        for (int y0 = t.rlo[1] & ~1; y0 <= t.rhi[1]; y0 += 16)
+          for (int x0 = t.rlo[0] & ~1; x0 <= t.rhi[0]; x0 += 16) {
+            SepTile tl;
+            tl.task = task;
+            tl.comp = comp;
+            tl.x0 = x0;
+            tl.y0 = y0;
+            tl.z0 = t.rlo[2];
+            tl.z1 = t.rhi[2];
+            tl.cfg = -1;
+            tl.pad = 0;
+            wl->tiles.push_back(tl);
+          }
+      continue;
+    }
     if (only_key && tune_key(t) != *only_key)
       continue;
     int cfg, zc;
@@ -372,7 +511,7 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
     }
     return n;
   };
-  const long long slots = 4LL * persistent_grid(1 << 30);
+  const long long slots = 4LL * persistent_grid(1 << 30, wl->lds);
   int zdef = 64;
   while (zdef > 8 && count(zdef) < slots)
     zdef /= 2;
@@ -396,6 +535,8 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
             wl->tiles.push_back(tl);
           }
   }
+  wl->grid = persistent_grid((int)wl->tiles.size(), wl->lds);
+  schedule_tiles(wl, wl->grid);
 }
 
 int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
@@ -415,6 +556,19 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
     if (hipMalloc(&wl->d_tiles, wl->cap_tiles * sizeof(SepTile)) != hipSuccess)
       return CXP_ERR_HIP;
   }
+  if (wl->wg_start.size() > wl->cap_wg) {
+    if (wl->d_wg_start)
+      (void)hipFree(wl->d_wg_start);
+    wl->d_wg_start = nullptr;
+    wl->cap_wg = wl->wg_start.size() * 2;
+    if (hipMalloc(&wl->d_wg_start, wl->cap_wg * sizeof(int)) != hipSuccess)
+      return CXP_ERR_HIP;
+  }
+  if (!wl->wg_start.empty() &&
+      hipMemcpyAsync(wl->d_wg_start, wl->wg_start.data(),
+                     wl->wg_start.size() * sizeof(int), hipMemcpyHostToDevice,
+                     s) != hipSuccess)
+    return CXP_ERR_HIP;
   if (!wl->tasks.empty() &&
       hipMemcpyAsync(wl->d_tasks, wl->tasks.data(),
                      wl->tasks.size() * sizeof(SepTask),
@@ -431,7 +585,7 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
   return CXP_OK;
 }
 
-int persistent_grid(int ntiles) {
+int persistent_grid(int ntiles, int lds) {
   static int cus = 0;
   if (!cus) {
     int dev = 0;
@@ -443,8 +597,8 @@ int persistent_grid(int ntiles) {
       cus = 256;
   }
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sep_batched,
-                                                   CXP_NT, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_batched,
+                                                   CXP_NT, lds) != hipSuccess ||
       per_cu <= 0)
     per_cu = 2;
   int g = cus * per_cu;
@@ -452,13 +606,20 @@ int persistent_grid(int ntiles) {
   return std::max(1, std::min(ntiles, g));
 }
 
+// The separable tiles of a list: one persistent launch.
+int launch_sep_list(cxp_worklist_s *wl, hipStream_t s) {
+  if (wl->tiles.empty())
+    return CXP_OK;
+  hipLaunchKernelGGL(k_fill_batched, dim3(wl->grid), dim3(CXP_NT),
+                     (size_t)wl->lds, s, wl->d_tasks, wl->d_tiles,
+                     wl->d_wg_start);
+  return hip_status(hipGetLastError());
+}
+
 int launch_batched(cxp_worklist_s *wl, hipStream_t s) {
-  if (!wl->tiles.empty()) {
-    hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), 0, s,
-                       wl->d_tasks, wl->d_tiles, (int)wl->tiles.size());
-    if (hipGetLastError() != hipSuccess)
-      return CXP_ERR_HIP;
-  }
+  if (const int e = launch_sep_list(wl, s))
+    return e;
+
   for (int n : wl->point_patches) {
     const auto &p = wl->patches[n];
     const PointTask t = make_point_task(p.op->d, &p.crse, p.crse_comp,
@@ -659,6 +820,8 @@ int cxp_worklist_destroy(cxp_worklist wl) {
     (void)hipFree(wl->d_tasks);
   if (wl->d_tiles)
     (void)hipFree(wl->d_tiles);
+  if (wl->d_wg_start)
+    (void)hipFree(wl->d_wg_start);
   delete wl;
   return CXP_OK;
 }
@@ -779,7 +942,6 @@ int cxp_worklist_upload(cxp_worklist wl, void *stream) {
   const int e = upload_lists(wl, (hipStream_t)stream);
   if (e)
     return e;
-  wl->grid = persistent_grid((int)wl->tiles.size());
   wl->uploaded = true;
   return CXP_OK;
 }
@@ -909,13 +1071,11 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
           continue;
         if ((status = upload_lists(wl, s)))
           break;
-        wl->grid = persistent_grid((int)wl->tiles.size());
         float tot = 0;
         for (int r = 0; r <= reps; ++r) {
           (void)hipEventRecord(e0, s);
-          hipLaunchKernelGGL(k_sep_batched, dim3(wl->grid), dim3(CXP_NT), 0,
-                             s, wl->d_tasks, wl->d_tiles,
-                             (int)wl->tiles.size());
+          if ((status = launch_sep_list(wl, s)))
+            break;
           (void)hipEventRecord(e1, s);
           if (hipEventSynchronize(e1) != hipSuccess) {
             status = CXP_ERR_HIP;

@@ -92,7 +92,7 @@ struct SepTile {
   int comp;  // component offset applied with ccs/fcs
   int x0, y0;
   int z0, z1;
-  int cfg;  // tile shape: log2 of the fine tile width
+  int cfg;  // tile shape: log2 of the fine tile width; -1: thin tile
   int pad;
 };
 
@@ -345,28 +345,6 @@ __global__ void __launch_bounds__(CXP_NT)
   const int z0 = t.rlo[2] + tz * zchunk;
   const int z1 = min(t.rhi[2], z0 + zchunk - 1);
   sep_tile(g, t, blockIdx.y, x0, y0, z0, z1, smem);
-}
-
-// Batched persistent launch: the grid walks the device-resident tile list;
-// each tile carries its task and tile shape (cfg = lx).  Workgroups are
-// dispatched round-robin over the 8 XCDs, so the list index is remapped to
-// give each XCD a contiguous run of tiles (neighbouring tiles share coarse
-// halo, which then stays in one L2).
-__global__ void __launch_bounds__(CXP_NT)
-    k_sep_batched(const SepTask *__restrict__ tasks,
-                  const SepTile *__restrict__ tiles, int ntiles) {
-  __shared__ double smem[kLdsDoubles];
-  const int nblk = gridDim.x;
-  const int bid = blockIdx.x;
-  int vb = bid;
-  if (nblk >= 8 && nblk % 8 == 0)
-    vb = (bid % 8) * (nblk / 8) + bid / 8;
-  for (int n = vb; n < ntiles; n += nblk) {
-    const SepTile tl = tiles[n];
-    const SepTask t = tasks[tl.task];
-    sep_tile(make_geom(tl.cfg), t, tl.comp, tl.x0, tl.y0, tl.z0, tl.z1, smem);
-    __syncthreads();  // the next tile rewrites Cs / Xs
-  }
 }
 
 // ---------------------------------------------------------------------------
