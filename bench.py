@@ -85,6 +85,15 @@ def bench_uneven(args, stream, log):
     ms = sum(times) / len(times)
     res = dict(ms=ms, wall_ms=1e3 * wall / args.steps, stats=st)
 
+    # the list's write pattern alone: CarpetX's poisoning of the same points
+    # (one store per point, no reads)
+    with torch.cuda.stream(stream):
+        wl.poison(stream=stream)
+        tpz = timed_events(lambda: wl.poison(stream=stream), 10, stream)
+        graph.launch(stream)  # the shells hold the fill again
+        torch.cuda.synchronize()
+    res["store_floor_ms"] = sum(tpz) / len(tpz)
+
     if args.no_per_box:
         res["per_box_ms"] = None
         return wlz, wl, res
@@ -300,7 +309,11 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (load_traffic("uneven_512") or {}).get("bytes_per_launch"),
         },
-        "extra": {"wall_ms_per_step": round(res["wall_ms"], 4)},
+        "extra": {
+            "wall_ms_per_step": round(res["wall_ms"], 4),
+            "store_floor_ms": round(res["store_floor_ms"], 4),
+            "fill_over_store_floor": round(ms / res["store_floor_ms"], 3),
+        },
     }
     if res["per_box_ms"] is not None:
         line["extra"].update(

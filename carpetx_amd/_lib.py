@@ -74,6 +74,7 @@ _SIGS = {
     "cxp_worklist_stats": (c_int, [c_void_p, P_INT64, P_INT64, P_INT64, P_INT64]),
     "cxp_prolongate_batched": (c_int, [c_void_p, c_void_p]),
     "cxp_prolongate_per_patch": (c_int, [c_void_p, c_void_p]),
+    "cxp_worklist_poison": (c_int, [c_void_p, ctypes.c_double, c_void_p]),
     "cxp_graph_capture_begin": (c_int, [c_void_p]),
     "cxp_graph_capture_end": (c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
     "cxp_graph_launch": (c_int, [c_void_p, c_void_p]),

@@ -5,12 +5,18 @@ fine patches (CarpetX/src/fillpatch.cxx:116-119,179-180,213-216) by one
 persistent kernel over every (patch, component, tile) of a level.
 """
 import ctypes
+import struct
 
 import torch
 
 from . import _lib
 from ._lib import check, i3
 from .interp import _stream
+
+
+# CarpetX's FP64 poison pattern (CarpetX/src/valid.hxx:249-256), a NaN.
+POISON_BITS = 0xfff80000deadbeef
+POISON = struct.unpack("<d", POISON_BITS.to_bytes(8, "little"))[0]
 
 
 class WorkList:
@@ -62,6 +68,12 @@ class WorkList:
     def run_per_patch(self, stream=None):
         check("cxp_prolongate_per_patch", _lib.lib().cxp_prolongate_per_patch(
             self.h, _stream(stream)))
+
+    def poison(self, value=POISON, stream=None):
+        """Write ``value`` into every fine point of every patch (requires an
+        uploaded list)."""
+        check("cxp_worklist_poison", _lib.lib().cxp_worklist_poison(
+            self.h, float(value), _stream(stream)))
 
     def autotune(self, reps=3, stream=None):
         check("cxp_autotune", _lib.lib().cxp_autotune(self.h, reps, _stream(stream)))

@@ -323,6 +323,11 @@ struct cxp_worklist_s {
   int *d_wg_start = nullptr;
   size_t cap_wg = 0;
   size_t cap_tasks = 0, cap_tiles = 0;
+  std::vector<PoisonRun> poison_runs;  // one per (patch, component)
+  std::vector<PoisonChunk> poison_chunks;
+  PoisonRun *d_poison_runs = nullptr;
+  PoisonChunk *d_poison_chunks = nullptr;
+  size_t cap_poison_runs = 0, cap_poison_chunks = 0;
   bool uploaded = false;
   int grid = 0;
   int lds = 0;  // bytes of dynamic LDS of k_fill_batched
@@ -539,7 +544,57 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
   schedule_tiles(wl, wl->grid);
 }
 
+// Grows a device array to hold v and copies v into it (asynchronously).
+template <typename T>
+int upload_vec(const std::vector<T> &v, T *&d, size_t &cap, hipStream_t s) {
+  if (v.size() > cap) {
+    if (d)
+      (void)hipFree(d);
+    d = nullptr;
+    cap = v.size() * 2;
+    if (hipMalloc(&d, cap * sizeof(T)) != hipSuccess) {
+      cap = 0;
+      return CXP_ERR_HIP;
+    }
+  }
+  if (!v.empty() && hipMemcpyAsync(d, v.data(), v.size() * sizeof(T),
+                                   hipMemcpyHostToDevice, s) != hipSuccess)
+    return CXP_ERR_HIP;
+  return CXP_OK;
+}
+
+// Runs and chunks of k_poison for every patch of the list.
+void build_poison(cxp_worklist_s *wl) {
+  wl->poison_runs.clear();
+  wl->poison_chunks.clear();
+  for (const auto &p : wl->patches) {
+    const int nx = p.rhi[0] - p.rlo[0] + 1, ny = p.rhi[1] - p.rlo[1] + 1;
+    const int nrows = ny * (p.rhi[2] - p.rlo[2] + 1);
+    const int per = std::max(1, 2048 / nx);
+    for (int c = 0; c < p.ncomp; ++c) {
+      const DFab f = dfab(&p.fine, p.fine_comp + c);
+      PoisonRun r;
+      r.p = f.p + didx(f, p.rlo[0], p.rlo[1], p.rlo[2]);
+      r.sy = f.sy;
+      r.sz = f.sz;
+      r.nx = nx;
+      r.ny = ny;
+      const int run = (int)wl->poison_runs.size();
+      wl->poison_runs.push_back(r);
+      for (int row = 0; row < nrows; row += per)
+        wl->poison_chunks.push_back({run, row, std::min(per, nrows - row)});
+    }
+  }
+}
+
 int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
+  build_poison(wl);
+  if (int e = upload_vec(wl->poison_runs, wl->d_poison_runs,
+                         wl->cap_poison_runs, s))
+    return e;
+  if (int e = upload_vec(wl->poison_chunks, wl->d_poison_chunks,
+                         wl->cap_poison_chunks, s))
+    return e;
   if (wl->tasks.size() > wl->cap_tasks) {
     if (wl->d_tasks)
       (void)hipFree(wl->d_tasks);
@@ -822,6 +877,10 @@ int cxp_worklist_destroy(cxp_worklist wl) {
     (void)hipFree(wl->d_tiles);
   if (wl->d_wg_start)
     (void)hipFree(wl->d_wg_start);
+  if (wl->d_poison_runs)
+    (void)hipFree(wl->d_poison_runs);
+  if (wl->d_poison_chunks)
+    (void)hipFree(wl->d_poison_chunks);
   delete wl;
   return CXP_OK;
 }
@@ -959,6 +1018,19 @@ int cxp_worklist_stats(cxp_worklist wl, int64_t *npatches, int64_t *ntiles,
   if (algo_bytes)
     *algo_bytes = wl->algo_bytes;
   return CXP_OK;
+}
+
+int cxp_worklist_poison(cxp_worklist wl, double value, void *stream) {
+  if (!wl)
+    return CXP_ERR_ARG;
+  if (!wl->uploaded)
+    return CXP_ERR_STATE;
+  if (wl->poison_chunks.empty())
+    return CXP_OK;
+  hipLaunchKernelGGL(k_poison, dim3((unsigned)wl->poison_chunks.size()),
+                     dim3(256), 0, (hipStream_t)stream, wl->d_poison_runs,
+                     wl->d_poison_chunks, value);
+  return hip_status(hipGetLastError());
 }
 
 int cxp_prolongate_batched(cxp_worklist wl, void *stream) {

@@ -70,7 +70,8 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, int boff,
                                         boff, 0, 0);
 }
 
-__device__ __forceinline__ long long didx(const DFab &f, int i, int j, int k) {
+__host__ __device__ __forceinline__ long long didx(const DFab &f, int i, int j,
+                                                  int k) {
   return (long long)(i - f.lo[0]) + (long long)(j - f.lo[1]) * f.sy +
          (long long)(k - f.lo[2]) * f.sz;
 }
@@ -345,6 +346,41 @@ __global__ void __launch_bounds__(CXP_NT)
   const int z0 = t.rlo[2] + tz * zchunk;
   const int z1 = min(t.rhi[2], z0 + zchunk - 1);
   sep_tile(g, t, blockIdx.y, x0, y0, z0, z1, smem);
+}
+
+// ---------------------------------------------------------------------------
+// Poisoning: one value into every fine point of a list's patches, as
+// CarpetX's poison_invalid_gf writes the poison pattern into invalid ghost
+// points before they are filled (CarpetX/src/valid.cxx:157-215).  A run is
+// one (patch, component) region, rows of nx points; a chunk is a run of
+// rows one workgroup writes, about 2048 points.  Stores only, one per point,
+// x fastest, so its time is the floor of the list's write pattern.
+
+struct PoisonRun {
+  double *p;     // the region's first point (lo corner) in its component
+  int sy;        // x-row stride
+  long long sz;  // plane stride
+  int nx, ny;    // region extents in x and y
+};
+struct PoisonChunk {
+  int run;
+  int row0, nrows;  // rows (y fastest, then z) of the region
+};
+
+__global__ void __launch_bounds__(256)
+    k_poison(const PoisonRun *__restrict__ runs,
+             const PoisonChunk *__restrict__ chunks, double value) {
+  const PoisonChunk ch = chunks[blockIdx.x];
+  const PoisonRun r = runs[ch.run];
+  const int npts = ch.nrows * r.nx;
+  for (int i = threadIdx.x; i < npts; i += 256) {
+    const int rr = i / r.nx;
+    const int x = i - rr * r.nx;
+    const int row = ch.row0 + rr;
+    const int z = row / r.ny;
+    const int y = row - z * r.ny;
+    gst(r.p, x + (long long)y * r.sy + z * r.sz, value);
+  }
 }
 
 // ---------------------------------------------------------------------------

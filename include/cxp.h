@@ -138,6 +138,13 @@ int cxp_worklist_stats(cxp_worklist wl, int64_t *npatches, int64_t *ntiles,
 int cxp_prolongate_batched(cxp_worklist wl, void *stream);
 /* The plain path over the same list: one cxp_prolongate launch per patch. */
 int cxp_prolongate_per_patch(cxp_worklist wl, void *stream);
+/* Writes `value` into every fine point of every patch of an uploaded list:
+ * poison_invalid_gf (CarpetX/src/valid.cxx:157-215) for the regions a fill
+ * will write, with CarpetX's FP64 poison pattern CXP_POISON_BITS
+ * (valid.hxx:249-256) or any other value.  One store per point, no reads:
+ * its run time is the floor of the list's write pattern. */
+#define CXP_POISON_BITS 0xfff80000deadbeefULL
+int cxp_worklist_poison(cxp_worklist wl, double value, void *stream);
 
 /* ---- hipGraph capture of a fill sequence. */
 typedef struct cxp_graph_s *cxp_graph;

@@ -7,7 +7,7 @@ import torch
 
 import oracle
 from carpetx_amd import workloads
-from carpetx_amd.batched import Graph, WorkList
+from carpetx_amd.batched import POISON_BITS, Graph, WorkList
 from carpetx_amd.interp import Box, FArrayBox
 
 pytestmark = pytest.mark.gpu
@@ -60,6 +60,28 @@ def test_batched_shell_fill_matches_oracle_and_per_patch():
     torch.cuda.synchronize()
     for b, e in zip(w["boxes"], expect):
         assert np.array_equal(b["fine"].t.cpu().numpy(), e)
+
+
+def test_poison_writes_exactly_the_shell():
+    """cxp_worklist_poison writes CarpetX's poison bit pattern into every
+    shell point and nothing else; a fill afterwards overwrites all of it."""
+    w = small_uneven(nboxes=12, seed=9)
+    wl = WorkList()
+    for b in w["boxes"]:
+        b["fine"].t.fill_(7.0)
+        wl.append_shell(b["op"], b["crse"], 0, b["fine"], 0, w["ncomp"], b["interior"])
+    wl.upload()
+    wl.poison()
+    torch.cuda.synchronize()
+    for b in w["boxes"]:
+        bits = b["fine"].t.cpu().numpy().view(np.uint64)
+        m = _shell_mask(b).cpu().numpy()
+        assert (bits[m] == np.uint64(POISON_BITS)).all()
+        assert (b["fine"].t.cpu().numpy()[~m] == 7.0).all()
+    wl.run()
+    torch.cuda.synchronize()
+    for b in w["boxes"]:
+        assert torch.isfinite(b["fine"].t).all()
 
 
 def test_graph_replays_match_eager():
