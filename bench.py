@@ -208,7 +208,7 @@ def bench_wave(log, n=128, steps=16, ck=4):
 
 def load_traffic(name):
     """HBM traffic per launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "k_sep_batched_uneven_512_pmc.json")
+    path = os.path.join(ROOT, "profiles", "k_fill_batched_uneven_512_pmc.json")
     try:
         with open(path) as f:
             return json.load(f).get(name)
