@@ -1,4 +1,5 @@
-"""Build recipe for libcxp.so (hipcc, gfx950) and the CPU oracle.
+"""Build recipe for libcxp.so (hipcc, gfx950), the C++ host check program
+and the CPU oracle.
 
     python -m carpetx_amd.build          # builds carpetx_amd/libcxp.so
 """
@@ -52,6 +53,20 @@ def build_lib():
         os.remove(o)
 
 
+HOST_CHECK = os.path.join(HERE, "host", "prolongate_check")
+
+
+def build_host_check():
+    """The C++ host API's check program, linked against libcxp.so."""
+    tmp = HOST_CHECK + ".tmp"
+    run([HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950",
+         "-Wno-unused-command-line-argument",
+         "-I" + os.path.join(ROOT, "include"),
+         os.path.join(HERE, "host", "prolongate_check.cpp"),
+         "-L" + HERE, "-lcxp", "-Wl,-rpath,$ORIGIN/..", "-o", tmp])
+    os.replace(tmp, HOST_CHECK)
+
+
 def build_oracle():
     run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"])
 
@@ -59,6 +74,7 @@ def build_oracle():
 def main():
     t0 = time.time()
     build_lib()
+    build_host_check()
     print("built %s in %.1fs" % (LIB, time.time() - t0))
 
 

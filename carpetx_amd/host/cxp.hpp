@@ -1,0 +1,253 @@
+// cxp.hpp -- C++17 host API over libcxp.so's C ABI (include/cxp.h).
+//
+// Mirrors the reference's operator interface for the rf=2 transfer path so
+// C++ callers read like CarpetX:
+//   * get_interpolator(type, order, indextype)   CarpetX/src/driver.cxx:593-642
+//   * Prolongate3DRF2::CoarseBox / interp / interp_face
+//                                  CarpetX/src/prolongate_3d_rf2.hxx:76-120
+//   * restrict_fab                 Restrict(), CarpetX/src/schedule.cxx:2791-2816
+//   * WorkList / Graph             the FillPatchInterp loop over fine patches
+//                                  (CarpetX/src/fillpatch.cxx:116-119)
+// FABs are views of device memory in AMReX FArrayBox layout.  Where the
+// reference aborts (assert / CCTK_VERROR), these throw cxp::Error carrying
+// the library status.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+
+#include "../../include/cxp.h"
+
+namespace cxp {
+
+class Error : public std::runtime_error {
+public:
+  Error(const std::string &what, int status)
+      : std::runtime_error(what + ": " + cxp_status_string(status)),
+        status_(status) {}
+  int status() const { return status_; }
+
+private:
+  int status_;
+};
+
+inline void check(int status, const char *what) {
+  if (status != CXP_OK)
+    throw Error(what, status);
+}
+
+using IntVect = std::array<int, 3>;
+
+// Inclusive index box [lo, hi] of a level.
+struct Box {
+  IntVect lo{0, 0, 0}, hi{-1, -1, -1};
+  Box() = default;
+  Box(IntVect l, IntVect h) : lo(l), hi(h) {}
+  bool ok() const { return lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2]; }
+  int64_t numPts() const {
+    return ok() ? int64_t(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) *
+                      (hi[2] - lo[2] + 1)
+                : 0;
+  }
+  Box grow(int g) const {
+    return Box({lo[0] - g, lo[1] - g, lo[2] - g}, {hi[0] + g, hi[1] + g, hi[2] + g});
+  }
+  bool contains(const Box &b) const {
+    for (int d = 0; d < 3; ++d)
+      if (b.lo[d] < lo[d] || b.hi[d] > hi[d])
+        return false;
+    return true;
+  }
+  bool operator==(const Box &o) const { return lo == o.lo && hi == o.hi; }
+};
+
+// Device-memory FAB view: component c of point (i,j,k) at
+// data[c*comp_stride + (i-lo0) + (j-lo1)*nx + (k-lo2)*nx*ny].
+struct Fab {
+  double *data = nullptr;
+  Box box;
+  int ncomp = 1;
+  int64_t comp_stride = 0;  // 0: the box volume
+  cxp_fab c() const {
+    cxp_fab f;
+    f.data = data;
+    for (int d = 0; d < 3; ++d) {
+      f.lo[d] = box.lo[d];
+      f.hi[d] = box.hi[d];
+    }
+    f.comp_stride = comp_stride ? comp_stride : box.numPts();
+    f.ncomp = ncomp;
+    return f;
+  }
+};
+
+// One prolongation operator (an immutable, never-freed library handle, like
+// the reference's static operator singletons).
+class Prolongate3DRF2 {
+public:
+  explicit Prolongate3DRF2(cxp_op op) : op_(op) {}
+  cxp_op handle() const { return op_; }
+
+  Box CoarseBox(const Box &fine, int ratio = 2) const {
+    if (ratio != 2)
+      throw Error("CoarseBox", CXP_ERR_RATIO);
+    Box c;
+    check(cxp_coarse_box(op_, fine.lo.data(), fine.hi.data(), c.lo.data(),
+                         c.hi.data()),
+          "cxp_coarse_box");
+    return c;
+  }
+
+  void interp(const Fab &crse, int crse_comp, const Fab &fine, int fine_comp,
+              int ncomp, const Box &fine_region, int ratio = 2,
+              bool per_group = false, void *stream = nullptr) const {
+    if (ratio != 2)
+      throw Error("interp", CXP_ERR_RATIO);
+    const cxp_fab c = crse.c(), f = fine.c();
+    check(cxp_prolongate(op_, &c, crse_comp, &f, fine_comp, ncomp,
+                         fine_region.lo.data(), fine_region.hi.data(),
+                         per_group ? 1 : 0, stream),
+          "cxp_prolongate");
+  }
+
+  // The reference ignores solve_mask and forwards to interp
+  // (prolongate_3d_rf2_impl.hxx:2204-2207).
+  void interp_face(const Fab &crse, int crse_comp, const Fab &fine,
+                   int fine_comp, int ncomp, const Box &fine_region,
+                   int ratio = 2, bool per_group = false,
+                   void *stream = nullptr) const {
+    interp(crse, crse_comp, fine, fine_comp, ncomp, fine_region, ratio,
+           per_group, stream);
+  }
+
+  // crse_bar += P^T fine_bar over CoarseBox(fine_region).
+  void interp_adjoint(const Fab &crse_in, const Fab &crse_bar, int crse_comp,
+                      const Fab &fine_bar, int fine_comp, int ncomp,
+                      const Box &fine_region, void *stream = nullptr) const {
+    const cxp_fab ci = crse_in.c(), cb = crse_bar.c(), fb = fine_bar.c();
+    check(cxp_prolongate_adjoint(op_, &ci, &cb, crse_comp, &fb, fine_comp,
+                                 ncomp, fine_region.lo.data(),
+                                 fine_region.hi.data(), 0, stream),
+          "cxp_prolongate_adjoint");
+  }
+
+  // Per-direction (centring, interpolation, order, ghosts) and fallback.
+  void describe(IntVect &cent, IntVect &intp, IntVect &order, IntVect &ghosts,
+                int &fb) const {
+    check(cxp_op_describe(op_, cent.data(), intp.data(), order.data(),
+                          ghosts.data(), &fb),
+          "cxp_op_describe");
+  }
+
+private:
+  cxp_op op_;
+};
+
+// get_interpolator(prolongation_type, prolongation_order, indextype).
+inline const Prolongate3DRF2 &get_interpolator(const std::string &type,
+                                               int order,
+                                               const IntVect &indextype) {
+  static std::mutex mu;
+  static std::map<std::tuple<std::string, int, int, int, int>,
+                  std::unique_ptr<Prolongate3DRF2>>
+      cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto &slot = cache[std::make_tuple(type, order, indextype[0], indextype[1],
+                                     indextype[2])];
+  if (!slot) {
+    cxp_op op = nullptr;
+    check(cxp_op_lookup(type.c_str(), order, indextype.data(), &op),
+          "cxp_op_lookup");
+    slot = std::make_unique<Prolongate3DRF2>(op);
+  }
+  return *slot;
+}
+
+// Restriction of `fine` onto coarse region `crse_region` of `crse`:
+// injection in vertex-centred directions, mean of the two children in
+// cell-centred ones.
+inline void restrict_fab(const IntVect &indextype, const Fab &fine,
+                         int fine_comp, const Fab &crse, int crse_comp,
+                         int ncomp, const Box &crse_region,
+                         void *stream = nullptr) {
+  const cxp_fab f = fine.c(), c = crse.c();
+  check(cxp_restrict(indextype.data(), &f, fine_comp, &c, crse_comp, ncomp,
+                     crse_region.lo.data(), crse_region.hi.data(), stream),
+        "cxp_restrict");
+}
+
+// A device-resident list of interp() calls, run by one persistent kernel.
+class WorkList {
+public:
+  WorkList() { check(cxp_worklist_create(&wl_), "cxp_worklist_create"); }
+  ~WorkList() {
+    if (wl_)
+      cxp_worklist_destroy(wl_);
+  }
+  WorkList(const WorkList &) = delete;
+  WorkList &operator=(const WorkList &) = delete;
+
+  void append(const Prolongate3DRF2 &op, const Fab &crse, int crse_comp,
+              const Fab &fine, int fine_comp, int ncomp, const Box &region) {
+    const cxp_fab c = crse.c(), f = fine.c();
+    check(cxp_worklist_append(wl_, op.handle(), &c, crse_comp, &f, fine_comp,
+                              ncomp, region.lo.data(), region.hi.data()),
+          "cxp_worklist_append");
+  }
+  // The ghost shell of `fine`'s box around `interior`.
+  void append_shell(const Prolongate3DRF2 &op, const Fab &crse, int crse_comp,
+                    const Fab &fine, int fine_comp, int ncomp,
+                    const Box &interior) {
+    const cxp_fab c = crse.c(), f = fine.c();
+    check(cxp_worklist_append_shell(wl_, op.handle(), &c, crse_comp, &f,
+                                    fine_comp, ncomp, interior.lo.data(),
+                                    interior.hi.data()),
+          "cxp_worklist_append_shell");
+  }
+  void upload(void *stream = nullptr) {
+    check(cxp_worklist_upload(wl_, stream), "cxp_worklist_upload");
+  }
+  void run(void *stream = nullptr) {
+    check(cxp_prolongate_batched(wl_, stream), "cxp_prolongate_batched");
+  }
+  void run_per_patch(void *stream = nullptr) {
+    check(cxp_prolongate_per_patch(wl_, stream), "cxp_prolongate_per_patch");
+  }
+  void poison(double value, void *stream = nullptr) {
+    check(cxp_worklist_poison(wl_, value, stream), "cxp_worklist_poison");
+  }
+  cxp_worklist handle() const { return wl_; }
+
+private:
+  cxp_worklist wl_ = nullptr;
+};
+
+// A captured fill sequence (hipGraph).
+class Graph {
+public:
+  template <typename F> Graph(F &&record, void *stream) {
+    check(cxp_graph_capture_begin(stream), "cxp_graph_capture_begin");
+    record();
+    check(cxp_graph_capture_end(stream, &g_), "cxp_graph_capture_end");
+  }
+  ~Graph() {
+    if (g_)
+      cxp_graph_destroy(g_);
+  }
+  Graph(const Graph &) = delete;
+  Graph &operator=(const Graph &) = delete;
+  void launch(void *stream = nullptr) {
+    check(cxp_graph_launch(g_, stream), "cxp_graph_launch");
+  }
+
+private:
+  cxp_graph g_ = nullptr;
+};
+
+} // namespace cxp

@@ -142,3 +142,15 @@ def test_nonlinear_adjoints_report_unsupported():
                                       ctypes.byref(fine), 0, 1, i3((0, 0, 0)),
                                       i3((15, 15, 15)), 0, None)
         assert st == -11, (fam, st)
+
+
+def test_cpp_host_header_compiles():
+    """The C++ host API (carpetx_amd/host/cxp.hpp) is self-contained C++17
+    over the C header."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = os.path.join(root, "carpetx_amd", "host", "cxp.hpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra",
+                        "-x", "c++", hdr], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
