@@ -9,8 +9,15 @@
 //    applies x^T and y^T inside each plane (LDS), and accumulates z^T into a
 //    9-slot register window of coarse planes that is flushed and rotated
 //    once per coarse plane.
-//  * MINMOD, ENO and linear-fallback operators return CXP_ERR_UNSUPPORTED:
-//    their data-dependent Jacobians are not implemented on the device.
+//  * ENO and linear-fallback operators (prolongate_3d_rf2_impl.hxx
+//    :1417-1667, :1836-2148): each fine point's value is the tensor stencil
+//    of one branch (an ENO shift triple, and the high-order or the linear
+//    stencil), chosen from the data and constant between decision surfaces,
+//    so the Jacobian at crse_in is that branch's stencil.  One pass records
+//    every fine point's branch (the forward's own decision code); a second,
+//    one thread per coarse point, gathers over the fine points whose
+//    stencils can reach it.
+//  * MINMOD operators return CXP_ERR_UNSUPPORTED.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -80,8 +87,8 @@ __constant__ AdjW kAdjW = make_adjw();
 // chunk of coarse z-planes.  It marches through the fine z-planes that feed
 // the chunk.  Per fine plane:
 //   fine window (2(16+span_x) x 2(16+span_y)) -> LDS, split by x parity,
-//       loaded one plane ahead into registers with raw buffer loads (points
-//       outside the fine region read as 0, so no branches);
+//       loaded two fine planes ahead into registers with raw buffer loads
+//       (points outside the fine region read as 0, so no branches);
 //   x^T: fine x -> coarse x for every window row      -> LDS
 //   y^T: fine y -> coarse y, one output per thread   -> register
 //   z^T: accumulate into a register window of coarse planes, one plane
@@ -215,33 +222,45 @@ __global__ void __launch_bounds__(256)
                ? 8 * (int)((long long)(fz - t.fbar.lo[2]) * t.fbar.sz)
                : -1;
   };
-  double pre[CXP_ADJ_R];
-  {
-    const int pb = plane(2 * mA);
+  // pre[0]: even fine planes 2m, pre[1]: odd ones 2m+1, each loaded one
+  // coarse step (two fine planes) ahead
+  double pre[2][CXP_ADJ_R];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int pb = plane(2 * mA + h);
 #pragma unroll
     for (int r = 0; r < CXP_ADJ_R; ++r)
-      pre[r] = bld(rf, (pb >= 0 && soff[r] >= 0) ? pb + soff[r] : CXP_OOB);
+      pre[h][r] = bld(rf, (pb >= 0 && soff[r] >= 0) ? pb + soff[r] : CXP_OOB);
   }
   double acc[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q)
     acc[q] = 0.0;
 
+  const int ci = ci0 + cx, cj = cj0 + cy;
   for (int m = mA; m <= mB; ++m) {
+    // coarse plane m + wlo is completed by this step; its crse_bar values
+    // are loaded first, so waiting for them later does not wait for the
+    // prefetches issued below (the vector memory counter is in order)
+    const int kc = m + t.wlo[2];
+    const bool own = kc >= kca && ci <= t.ohi[0] && cj <= t.ohi[1];
+    const int co = own ? 8 * (int)didx(t.cbar, ci, cj, kc) : CXP_OOB;
+    const double cold = bld(rc, co);
     auto fine_plane = [&](auto offc) {
       constexpr int off = decltype(offc)::value;
       // this plane's window -> LDS
 #pragma unroll
       for (int r = 0; r < CXP_ADJ_R; ++r)
         if (lslot[r] >= 0)
-          Fs[lslot[r]] = pre[r];
-      // next plane -> registers
+          Fs[lslot[r]] = pre[off][r];
+      // the plane of this parity one coarse step ahead -> registers
       {
-        const int nz = 2 * m + off + 1;
+        const int nz = 2 * (m + 1) + off;
         const int pb = nz <= 2 * mB + 1 ? plane(nz) : -1;
 #pragma unroll
         for (int r = 0; r < CXP_ADJ_R; ++r)
-          pre[r] = bld(rf, (pb >= 0 && soff[r] >= 0) ? pb + soff[r] : CXP_OOB);
+          pre[off][r] =
+              bld(rf, (pb >= 0 && soff[r] >= 0) ? pb + soff[r] : CXP_OOB);
       }
       __syncthreads();
       // x^T over every window row
@@ -282,16 +301,128 @@ __global__ void __launch_bounds__(256)
     fine_plane(std::integral_constant<int, 0>{});
     fine_plane(std::integral_constant<int, 1>{});
     // coarse plane m + wlo is complete
-    const int kc = m + t.wlo[2];
-    const int ci = ci0 + cx, cj = cj0 + cy;
-    const bool own = kc >= kca && ci <= t.ohi[0] && cj <= t.ohi[1];
-    const int co = own ? 8 * (int)didx(t.cbar, ci, cj, kc) : CXP_OOB;
-    bst(rc, co, bld(rc, co) + acc[0]);
+    bst(rc, co, cold + acc[0]);
 #pragma unroll
     for (int q = 0; q < 8; ++q)
       acc[q] = acc[q + 1];
     acc[8] = 0.0;
   }
+}
+
+// --------------------------------------------------------------------------
+// ENO / linear-fallback adjoint
+
+// Branch code of a fine point: (sx+1) + 3 (sy+1) + 9 (sz+1) + 27 need.
+__device__ __forceinline__ unsigned char adj_code(const int shift[3],
+                                                  bool need) {
+  return (unsigned char)((shift[0] + 1) + 3 * (shift[1] + 1) +
+                         9 * (shift[2] + 1) + (need ? 27 : 0));
+}
+
+// Records the branch of every fine point of the region (per component in
+// per-var mode, shared in per-group mode).
+__global__ void __launch_bounds__(256)
+    k_adj_decide(PointTask t, unsigned char *__restrict__ codes,
+                 int per_group) {
+  const long long npts = pt_npts(t);
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= npts)
+    return;
+  int f[3], ic[3], off[3], shift[3];
+  pt_point(t, gid, f, ic, off);
+  bool need;
+  if (per_group) {
+    double vals[16];
+    pt_decide_group(t, off, ic, shift, vals, need);
+    codes[gid] = adj_code(shift, need);
+  } else {
+    const long long comp = blockIdx.y;
+    double val;
+    pt_decide_var(t, comp, off, ic, shift, val, need);
+    codes[comp * npts + gid] = adj_code(shift, need);
+  }
+}
+
+struct AdjPointTask {
+  PointTask pt;        // region and stencils (crse = crse_in)
+  DFab cbar;           // coarse adjoint (accumulated), first component
+  DFab fbar;           // fine adjoint (input), first component
+  long long ccs, fcs;  // component strides of cbar / fbar
+  int sl[3];           // stencils of the fallback branch
+  int olo[3], ohi[3];  // coarse output range
+  int rlo_reach[3], rhi_reach[3];  // coarse offsets any branch reaches
+  int per_group;
+};
+
+// w[d][branch][shift+1][off][o+4]
+#define CXP_ADJ_PT_W (3 * 2 * 3 * 2 * 9)
+
+// One thread per coarse output point and component: sum over the fine
+// points whose recorded branch reads that coarse point.
+__global__ void __launch_bounds__(256)
+    k_adj_gather(AdjPointTask t, const unsigned char *__restrict__ codes) {
+  __shared__ double w[CXP_ADJ_PT_W];
+  for (int e = threadIdx.x; e < CXP_ADJ_PT_W; e += blockDim.x) {
+    const int o = e % 9 - 4;
+    const int off = (e / 9) % 2;
+    const int sh = (e / 18) % 3 - 1;
+    const int br = (e / 54) % 2;
+    const int d = e / 108;
+    const int sid = br ? t.sl[d] : t.pt.sid[d];
+    const int shift = (sid == SID_ENO2 && !br) ? sh : 0;
+    int lo, hi;
+    stencil_radius(sid, shift, off, lo, hi);
+    w[e] = (o >= lo && o <= hi && !(br && sh != 0))
+               ? apply1d(sid, DeltaAcc{o}, shift, off)
+               : 0.0;
+  }
+  __syncthreads();
+  const int nx = t.ohi[0] - t.olo[0] + 1, ny = t.ohi[1] - t.olo[1] + 1;
+  const long long nout = (long long)nx * ny * (t.ohi[2] - t.olo[2] + 1);
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nout)
+    return;
+  const long long comp = blockIdx.y;
+  const int c[3] = {t.olo[0] + (int)(gid % nx),
+                    t.olo[1] + (int)((gid / nx) % ny),
+                    t.olo[2] + (int)(gid / ((long long)nx * ny))};
+  // fine points whose stencil may read c: coarse index ic with
+  // c - ic in [lo_reach, hi_reach]
+  int flo[3], fhi[3];
+  for (int d = 0; d < 3; ++d) {
+    flo[d] = max(t.pt.rlo[d], 2 * (c[d] - t.rhi_reach[d]));
+    fhi[d] = min(t.pt.rhi[d], 2 * (c[d] - t.rlo_reach[d]) + 1);
+  }
+  const long long npts = pt_npts(t.pt);
+  const int rnx = t.pt.rhi[0] - t.pt.rlo[0] + 1;
+  const int rny = t.pt.rhi[1] - t.pt.rlo[1] + 1;
+  const unsigned char *cd = codes + (t.per_group ? 0 : comp * npts);
+  DFab fb = t.fbar;
+  fb.p += comp * t.fcs;
+  double sum = 0.0;
+  for (int fz = flo[2]; fz <= fhi[2]; ++fz)
+    for (int fy = flo[1]; fy <= fhi[1]; ++fy)
+      for (int fx = flo[0]; fx <= fhi[0]; ++fx) {
+        const int code = cd[(fx - t.pt.rlo[0]) +
+                            (long long)(fy - t.pt.rlo[1]) * rnx +
+                            (long long)(fz - t.pt.rlo[2]) * rnx * rny];
+        const int br = code >= 27 ? 1 : 0;
+        const int sh[3] = {code % 3, (code / 3) % 3, (code / 9) % 3};
+        const int fv[3] = {fx, fy, fz};
+        double wt = 1.0;
+        for (int d = 0; d < 3; ++d) {
+          const int o = c[d] - (fv[d] >> 1);
+          wt *= w[(((d * 2 + br) * 3 + (br ? 1 : sh[d])) * 2 + (fv[d] & 1)) *
+                      9 +
+                  o + 4];
+        }
+        if (wt != 0.0)
+          sum += wt * gld(fb.p, didx(fb, fx, fy, fz));
+      }
+  DFab cb = t.cbar;
+  cb.p += comp * t.ccs;
+  const long long ci = didx(cb, c[0], c[1], c[2]);
+  gst(cb.p, ci, gld(cb.p, ci) + sum);
 }
 
 // --------------------------------------------------------------------------
@@ -317,6 +448,7 @@ inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
                           const int rlo[3], const int rhi[3], int per_group,
                           hipStream_t s) {
   (void)per_group;
+  // ENO and fallback operators are launched by adjoint_point_launch
   if (op_has_eno(op) || op_fallback_per_var(op) || op_fallback_per_group(op))
     return CXP_ERR_UNSUPPORTED;
   bool minmod = false;

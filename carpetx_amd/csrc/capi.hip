@@ -302,6 +302,75 @@ int prolongate_impl(const cxp_op_s *op, const cxp_fab *crse, int crse_comp,
   return launch_point(t, ncomp, per_group, s);
 }
 
+// Adjoint of an ENO or linear-fallback operator, linearised at crse_in:
+// record every fine point's branch, then gather per coarse point.
+int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
+                         const cxp_fab *crse_bar, int crse_comp,
+                         const cxp_fab *fine_bar, int fine_comp, int ncomp,
+                         const int rlo[3], const int rhi[3], int per_group,
+                         hipStream_t s) {
+  if (!crse_in)
+    return CXP_ERR_ARG;
+  if (const int e = check_prolong(op, crse_in, crse_comp, fine_bar, fine_comp,
+                                  ncomp, rlo, rhi, per_group))
+    return e;
+  const OpDesc &od = op->d;
+  AdjPointTask t;
+  t.pt = make_point_task(od, crse_in, crse_comp, fine_bar, fine_comp, ncomp,
+                         rlo, rhi);
+  t.cbar = dfab(crse_bar, crse_comp);
+  t.fbar = dfab(fine_bar, fine_comp);
+  t.ccs = crse_bar->comp_stride;
+  t.fcs = fine_bar->comp_stride;
+  t.per_group = per_group ? 1 : 0;
+  int clo[3], chi[3];
+  op_coarse_box(od, rlo, rhi, clo, chi);
+  for (int d = 0; d < 3; ++d) {
+    const int sid = od.sid[d];
+    t.sl[d] = per_group ? sid_linear_per_group(sid) : sid_linear_per_var(sid);
+    int lo = 1 << 20, hi = -(1 << 20);
+    const int ms = sid == SID_ENO2 ? 1 : 0;
+    for (int off = 0; off < 2; ++off) {
+      for (int sh = -ms; sh <= ms; ++sh) {
+        int a, b;
+        stencil_radius(sid, sh, off, a, b);
+        lo = std::min(lo, a);
+        hi = std::max(hi, b);
+      }
+      int a, b;
+      stencil_radius(t.sl[d], 0, off, a, b);
+      lo = std::min(lo, a);
+      hi = std::max(hi, b);
+    }
+    t.rlo_reach[d] = lo;
+    t.rhi_reach[d] = hi;
+    t.olo[d] = std::max(clo[d], (rlo[d] >> 1) + lo);
+    t.ohi[d] = std::min(chi[d], (rhi[d] >> 1) + hi);
+  }
+  long long npts = 1, nout = 1;
+  for (int d = 0; d < 3; ++d) {
+    npts *= rhi[d] - rlo[d] + 1;
+    nout *= t.ohi[d] - t.olo[d] + 1;
+  }
+  unsigned char *codes = nullptr;
+  const size_t nbytes = (size_t)npts * (per_group ? 1 : ncomp);
+  if (hipMallocAsync((void **)&codes, nbytes, s) != hipSuccess)
+    return CXP_ERR_HIP;
+  hipLaunchKernelGGL(k_adj_decide,
+                     dim3((unsigned)((npts + 255) / 256), per_group ? 1 : ncomp),
+                     dim3(256), 0, s, t.pt, codes, t.per_group);
+  int st = hip_status(hipGetLastError());
+  if (st == CXP_OK) {
+    hipLaunchKernelGGL(k_adj_gather,
+                       dim3((unsigned)((nout + 255) / 256), ncomp), dim3(256),
+                       0, s, t, (const unsigned char *)codes);
+    st = hip_status(hipGetLastError());
+  }
+  if (hipFreeAsync(codes, s) != hipSuccess && st == CXP_OK)
+    st = CXP_ERR_HIP;
+  return st;
+}
+
 } // namespace
 
 // ===========================================================================
@@ -854,7 +923,16 @@ int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
                               ncomp, rlo, rhi, per_group);
   if (e)
     return e;
-  return adjoint_launch(op->d, crse_in, crse_bar, crse_comp, fine_bar,
+  const OpDesc &od = op->d;
+  bool minmod = false;
+  for (int d = 0; d < 3; ++d)
+    minmod = minmod || od.sid[d] == SID_MINMOD;
+  if (!minmod && (op_has_eno(od) || op_fallback_per_var(od) ||
+                  op_fallback_per_group(od)))
+    return adjoint_point_launch(op, crse_in, crse_bar, crse_comp, fine_bar,
+                                fine_comp, ncomp, rlo, rhi, per_group,
+                                (hipStream_t)stream);
+  return adjoint_launch(od, crse_in, crse_bar, crse_comp, fine_bar,
                         fine_comp, ncomp, rlo, rhi, per_group,
                         (hipStream_t)stream);
 }

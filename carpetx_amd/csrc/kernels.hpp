@@ -532,85 +532,55 @@ __device__ inline bool pt_slope_change(const DFab &c, int d, const int sr[3][2],
   return need;
 }
 
-__global__ void __launch_bounds__(256) k_point_per_var(PointTask t) {
-  const int nx = t.rhi[0] - t.rlo[0] + 1;
-  const int ny = t.rhi[1] - t.rlo[1] + 1;
-  const int nz = t.rhi[2] - t.rlo[2] + 1;
-  const long long npts = (long long)nx * ny * nz;
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= npts)
-    return;
-  const long long comp = blockIdx.y;
-  const int i = t.rlo[0] + (int)(gid % nx);
-  const int j = t.rlo[1] + (int)((gid / nx) % ny);
-  const int k = t.rlo[2] + (int)(gid / ((long long)nx * ny));
-  const int f[3] = {i, j, k};
-  int ic[3], off[3], shift[3] = {0, 0, 0};
-  for (int d = 0; d < 3; ++d) {
-    ic[d] = f[d] >> 1;
-    off[d] = f[d] & 1;
-  }
+// Branch of a per-var operator at one fine point and component: the ENO
+// shift and, for fallback operators, whether the linear value replaces the
+// high-order one (prolongate_3d_rf2_impl.hxx:1417-1667).  val is the
+// high-order value.
+__device__ inline void pt_decide_var(const PointTask &t, long long comp,
+                                     const int off[3], const int ic[3],
+                                     int shift[3], double &val, bool &need) {
   DFab c = t.crse;
   c.p += comp * t.ccs;
+  shift[0] = shift[1] = shift[2] = 0;
   const bool any_shift =
       t.sid[0] == SID_ENO2 || t.sid[1] == SID_ENO2 || t.sid[2] == SID_ENO2;
   if (any_shift)
     pt_eno_search(t, comp, 1, off, ic, shift);
-  const double val = pt_tensor(c, t.sid, shift, off, ic);
-  double res = val;
+  val = pt_tensor(c, t.sid, shift, off, ic);
+  need = false;
   if (t.fb_per_var) {
-    int sl[3];
-    for (int d = 0; d < 3; ++d)
-      sl[d] = sid_linear_per_var(t.sid[d]);
-    const int zero[3] = {0, 0, 0};
-    const double val_lin = pt_tensor(c, sl, zero, off, ic);
     int sr[3][2];
     for (int d = 0; d < 3; ++d)
       stencil_radius(t.sid[d], shift[d], off[d], sr[d][0], sr[d][1]);
     double mn = __builtin_inf(), mx = -__builtin_inf();
     pt_minmax(c, sr, ic, mn, mx);
-    bool need = val < mn || val > mx;
+    need = val < mn || val > mx;
     for (int d = 0; d < 3; ++d) {
       const SidInfo s = sid_info(t.sid[d]);
       if (s.intp == INTP_CONS || s.intp == INTP_ENO)
         need |= pt_slope_change(c, d, sr, ic);
     }
-    res = need ? val_lin : val;
   }
-  DFab fo = t.fine;
-  fo.p += comp * t.fcs;
-  fo.p[didx(fo, i, j, k)] = res;
 }
 
-__global__ void __launch_bounds__(256) k_point_per_group(PointTask t) {
-  const int nx = t.rhi[0] - t.rlo[0] + 1;
-  const int ny = t.rhi[1] - t.rlo[1] + 1;
-  const int nz = t.rhi[2] - t.rlo[2] + 1;
-  const long long npts = (long long)nx * ny * nz;
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= npts)
-    return;
-  const int i = t.rlo[0] + (int)(gid % nx);
-  const int j = t.rlo[1] + (int)((gid / nx) % ny);
-  const int k = t.rlo[2] + (int)(gid / ((long long)nx * ny));
-  const int f[3] = {i, j, k};
-  int ic[3], off[3], shift[3] = {0, 0, 0};
-  for (int d = 0; d < 3; ++d) {
-    ic[d] = f[d] >> 1;
-    off[d] = f[d] & 1;
-  }
+// Branch of a per-group operator at one fine point, shared by all ncomp
+// components (prolongate_3d_rf2_impl.hxx:1836-2148); vals[c] are the
+// high-order values.
+__device__ inline void pt_decide_group(const PointTask &t, const int off[3],
+                                       const int ic[3], int shift[3],
+                                       double (&vals)[16], bool &need) {
   const int nc = t.ncomp;
+  shift[0] = shift[1] = shift[2] = 0;
   const bool any_shift =
       t.sid[0] == SID_ENO2 || t.sid[1] == SID_ENO2 || t.sid[2] == SID_ENO2;
   if (any_shift)
     pt_eno_search(t, 0, nc, off, ic, shift);
-  double vals[16];
   for (int cc = 0; cc < nc; ++cc) {
     DFab c = t.crse;
     c.p += cc * t.ccs;
     vals[cc] = pt_tensor(c, t.sid, shift, off, ic);
   }
-  bool need = false;
+  need = false;
   if (t.fb_per_group) {
     int sr[3][2];
     for (int d = 0; d < 3; ++d)
@@ -630,11 +600,66 @@ __global__ void __launch_bounds__(256) k_point_per_group(PointTask t) {
           need |= pt_slope_change(c, d, sr, ic);
         }
   }
+}
+
+// Fine point gid of a region in x-fastest order.
+__device__ __forceinline__ void pt_point(const PointTask &t, long long gid,
+                                         int f[3], int ic[3], int off[3]) {
+  const int nx = t.rhi[0] - t.rlo[0] + 1;
+  const int ny = t.rhi[1] - t.rlo[1] + 1;
+  f[0] = t.rlo[0] + (int)(gid % nx);
+  f[1] = t.rlo[1] + (int)((gid / nx) % ny);
+  f[2] = t.rlo[2] + (int)(gid / ((long long)nx * ny));
+  for (int d = 0; d < 3; ++d) {
+    ic[d] = f[d] >> 1;
+    off[d] = f[d] & 1;
+  }
+}
+
+__device__ __forceinline__ long long pt_npts(const PointTask &t) {
+  return (long long)(t.rhi[0] - t.rlo[0] + 1) * (t.rhi[1] - t.rlo[1] + 1) *
+         (t.rhi[2] - t.rlo[2] + 1);
+}
+
+__global__ void __launch_bounds__(256) k_point_per_var(PointTask t) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= pt_npts(t))
+    return;
+  const long long comp = blockIdx.y;
+  int f[3], ic[3], off[3], shift[3];
+  pt_point(t, gid, f, ic, off);
+  double val;
+  bool need;
+  pt_decide_var(t, comp, off, ic, shift, val, need);
+  double res = val;
+  if (need) {
+    DFab c = t.crse;
+    c.p += comp * t.ccs;
+    int sl[3];
+    for (int d = 0; d < 3; ++d)
+      sl[d] = sid_linear_per_var(t.sid[d]);
+    const int zero[3] = {0, 0, 0};
+    res = pt_tensor(c, sl, zero, off, ic);
+  }
+  DFab fo = t.fine;
+  fo.p += comp * t.fcs;
+  fo.p[didx(fo, f[0], f[1], f[2])] = res;
+}
+
+__global__ void __launch_bounds__(256) k_point_per_group(PointTask t) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= pt_npts(t))
+    return;
+  int f[3], ic[3], off[3], shift[3];
+  pt_point(t, gid, f, ic, off);
+  double vals[16];
+  bool need;
+  pt_decide_group(t, off, ic, shift, vals, need);
   int sl[3];
   for (int d = 0; d < 3; ++d)
     sl[d] = sid_linear_per_group(t.sid[d]);
   const int zero[3] = {0, 0, 0};
-  for (int cc = 0; cc < nc; ++cc) {
+  for (int cc = 0; cc < t.ncomp; ++cc) {
     double v = vals[cc];
     if (need) {
       DFab c = t.crse;
@@ -643,7 +668,7 @@ __global__ void __launch_bounds__(256) k_point_per_group(PointTask t) {
     }
     DFab fo = t.fine;
     fo.p += cc * t.fcs;
-    fo.p[didx(fo, i, j, k)] = v;
+    fo.p[didx(fo, f[0], f[1], f[2])] = v;
   }
 }
 

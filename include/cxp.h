@@ -94,9 +94,13 @@ int cxp_restrict(const int indextype[3], const cxp_fab *fine, int fine_comp,
 
 /* Adjoint of cxp_prolongate (no reference counterpart; the reference has no
  * adjoints): crse_bar += P^T fine_bar over CoarseBox(region), in gather form
- * over coarse points, no atomics.  crse_in is the forward input, required
- * for operators whose linearisation depends on the data (MINMOD, ENO,
- * fallback), ignored otherwise. */
+ * over coarse points, no atomics.  Linear operators (POLY, HERMITE, CONS)
+ * ignore crse_in.  ENO and linear-fallback operators are linearised at the
+ * forward input crse_in (required; CXP_ERR_ARG without it): each fine
+ * point's branch (ENO shift, fallback decision) is recomputed from crse_in
+ * with the forward's decision code, and the transpose of that branch's
+ * stencil is applied; per_group selects the per-group decisions.  MINMOD
+ * operators return CXP_ERR_UNSUPPORTED. */
 int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
                            const cxp_fab *crse_bar, int crse_comp,
                            const cxp_fab *fine_bar, int fine_comp, int ncomp,

@@ -6,8 +6,12 @@ forward operator):
   linear operator family and centring;
 * an explicit dense P^T on a small sub-box: columns of P are the oracle's
   forward applied to coarse unit vectors;
-* MINMOD (piecewise linear): the dense Jacobian of the oracle at u, taken by
-  central differences (exact inside one limiter branch), transposed.
+* ENO and linear-fallback operators, per-var and per-group: the dense
+  Jacobian of the oracle at u, taken by central differences (exact inside
+  one branch: every fine point's value is linear in the coarse data for a
+  fixed ENO shift and fallback decision), transposed; and the dot-product
+  identity with the GPU forward's directional derivative;
+* MINMOD: switched off (see MINMOD_OFF).
 """
 import numpy as np
 import pytest
@@ -83,6 +87,72 @@ def test_dense_transpose_small_box(fam, order, it):
     torch.cuda.synchronize()
     got = cbar.t.cpu().numpy() - 0.25   # the adjoint accumulates
     assert np.max(np.abs(got - expect)) <= 1e-13 * max(1.0, np.max(np.abs(expect)))
+
+
+NONLINEAR = [("eno", 3, (1, 1, 1), False), ("eno", 3, (1, 0, 1), True),
+             ("poly-eno3lfb", 3, (0, 1, 1), False), ("poly-eno3lfb", 3, (1, 1, 0), True),
+             ("poly-cons3lfb", 3, (1, 1, 1), False), ("poly-cons3lfb", 5, (1, 0, 0), True),
+             ("poly-cons3lfb", 1, (0, 1, 1), False)]
+
+
+@pytest.mark.parametrize("fam,order,it,pg", NONLINEAR)
+def test_nonlinear_adjoint_against_dense_jacobian(fam, order, it, pg):
+    """Jacobian of the oracle at u by central differences (h = 1e-7),
+    transposed, against the device adjoint linearised at u."""
+    rng = np.random.default_rng(order * 10 + it[0] + 2 * it[1] + 4 * it[2])
+    rlo, rhi = (0, 1, -2), (7, 6, 4)
+    nc = 2
+    op_o = oracle.lookup(fam, order, it)
+    P, cb, fbox, u, v = _fabs((fam, order, it), rlo, rhi, nc, rng)
+    csz = int(np.prod(cb.size))
+    h = 1e-7
+
+    def fwd(x):
+        ch = oracle.Fab(cb.lo, cb.hi, nc, x.reshape((nc,) + tuple(reversed(cb.size))))
+        fh = oracle.Fab(rlo, rhi, nc)
+        oracle.prolongate(op_o, ch, fh, rlo, rhi, ncomp=nc, per_group=pg)
+        return fh.arr.reshape(-1)
+    u0 = u.reshape(-1)
+    J = np.zeros((nc * int(np.prod(fbox.size)), nc * csz))
+    for n in range(nc * csz):
+        e = np.zeros(nc * csz)
+        e[n] = h
+        J[:, n] = (fwd(u0 + e) - fwd(u0 - e)) / (2 * h)
+    expect = (J.T @ v.reshape(-1)).reshape((nc,) + tuple(reversed(cb.size)))
+    crse = FArrayBox(cb, nc, tensor=torch.from_numpy(u).cuda())
+    vb = FArrayBox(fbox, nc, tensor=torch.from_numpy(v).cuda())
+    cbar = FArrayBox(cb, nc)
+    P.interp_adjoint(cbar, 0, vb, 0, nc, fbox, crse_in=crse, per_group=pg)
+    torch.cuda.synchronize()
+    got = cbar.t.cpu().numpy()
+    assert np.max(np.abs(got - expect)) <= 1e-6 * max(1.0, np.max(np.abs(expect)))
+
+
+@pytest.mark.parametrize("fam,order,it,pg", NONLINEAR[:4])
+def test_nonlinear_adjoint_dot_product(fam, order, it, pg):
+    """<J du, v> == <du, J^T v> on a larger region, J du from the GPU
+    forward by central differences."""
+    rng = np.random.default_rng(21)
+    rlo, rhi = (-4, 3, 0), (37, 22, 13)
+    P, cb, fbox, u, v = _fabs((fam, order, it), rlo, rhi, 2, rng)
+    du = rng.uniform(-1, 1, size=u.shape)
+    eps = 1e-7
+
+    def fwd(x):
+        crse = FArrayBox(cb, 2, tensor=torch.from_numpy(x).cuda())
+        fine = FArrayBox(fbox, 2)
+        P.interp(crse, 0, fine, 0, 2, fbox, per_group=pg)
+        return fine.t
+    jdu = (fwd(u + eps * du) - fwd(u - eps * du)) / (2 * eps)
+    vb = FArrayBox(fbox, 2, tensor=torch.from_numpy(v).cuda())
+    cbar = FArrayBox(cb, 2)
+    crse = FArrayBox(cb, 2, tensor=torch.from_numpy(u).cuda())
+    P.interp_adjoint(cbar, 0, vb, 0, 2, fbox, crse_in=crse, per_group=pg)
+    torch.cuda.synchronize()
+    lhs = float((jdu * vb.t).sum())
+    rhs = float((torch.from_numpy(du).cuda() * cbar.t).sum())
+    scale = float((jdu.abs() * vb.t.abs()).sum())
+    assert abs(lhs - rhs) <= 1e-6 * scale, (lhs, rhs)
 
 
 MINMOD_OFF = ("the MINMOD adjoint is switched off: its first implementation hung the "

@@ -120,9 +120,10 @@ def test_validation_errors_without_device_work():
     assert L.cxp_status_string(-5) == b"coarse FAB does not cover CoarseBox(region)"
 
 
-def test_nonlinear_adjoints_report_unsupported():
-    """MINMOD, ENO and fallback operators have no device adjoint yet; the
-    call is rejected before any device work."""
+def test_adjoint_argument_checks_before_device_work():
+    """MINMOD operators have no device adjoint (CXP_ERR_UNSUPPORTED); ENO
+    and fallback operators linearise at crse_in, so a call without it is
+    rejected (CXP_ERR_ARG).  Both before any device work."""
     L = _lib.lib()
     fake = ctypes.c_void_p(0x1000)
 
@@ -135,13 +136,13 @@ def test_nonlinear_adjoints_report_unsupported():
         return f
     crse = fab((-4, -4, -4), (20, 20, 20))
     fine = fab((0, 0, 0), (15, 15, 15))
-    for fam, order, it in [("minmod", 3, (1, 1, 1)), ("eno", 3, (1, 0, 0)),
-                           ("poly-cons3lfb", 3, (1, 1, 1))]:
+    for fam, order, it, want in [("minmod", 3, (1, 1, 1), -11), ("eno", 3, (1, 0, 0), -3),
+                                 ("poly-cons3lfb", 3, (1, 1, 1), -3)]:
         h = _lookup(fam, order, it)[1]
-        st = L.cxp_prolongate_adjoint(h, ctypes.byref(crse), ctypes.byref(crse), 0,
+        st = L.cxp_prolongate_adjoint(h, None, ctypes.byref(crse), 0,
                                       ctypes.byref(fine), 0, 1, i3((0, 0, 0)),
                                       i3((15, 15, 15)), 0, None)
-        assert st == -11, (fam, st)
+        assert st == want, (fam, st)
 
 
 def test_cpp_host_header_compiles():
