@@ -139,7 +139,9 @@ def bench_level64(args, stream, log):
 
 def bench_adjoint(stream, log):
     """96^3 -> 192^3 VC 5th-order prolongation: forward vs adjoint (per-box
-    launches on one stream), plus the CCC minmod forward."""
+    launches on one stream), plus the CCC minmod forward (its adjoint is not
+    available) and the CCC ENO forward and adjoint (linearised at the
+    forward input)."""
     import torch
     from carpetx_amd.interp import Box, FArrayBox, get_interpolator
     out = {}
