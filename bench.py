@@ -253,7 +253,7 @@ def cpu_baseline(wlz, nboxes, log):
     oracle.prolongate_many(items, kind="fast")  # warm-up
     reps = []
     t_total = time.perf_counter()
-    while len(reps) < 3 or (time.perf_counter() - t_total < 12 and len(reps) < 10):
+    while len(reps) < 3 or (time.perf_counter() - t_total < 15 and len(reps) < 25):
         t0 = time.perf_counter()
         oracle.prolongate_many(items, kind="fast")
         reps.append(time.perf_counter() - t0)
