@@ -239,6 +239,7 @@ PointTask make_point_task(const OpDesc &op, const cxp_fab *crse, int crse_comp,
   t.ncomp = ncomp;
   t.ccs = crse->comp_stride;
   t.fcs = fine->comp_stride;
+  t.d2 = nullptr;
   return t;
 }
 
@@ -246,6 +247,24 @@ int launch_point(const PointTask &t, int ncomp, int per_group,
                  hipStream_t s) {
   const long long n = (long long)(t.rhi[0] - t.rlo[0] + 1) *
                       (t.rhi[1] - t.rlo[1] + 1) * (t.rhi[2] - t.rlo[2] + 1);
+  // LDS-tiled kernels when a tile's coarse block fits in 48 KB (always in
+  // per-var mode; per-group mode with few components), else one thread per
+  // point reading global memory
+  const size_t lds =
+      8 * (size_t)pt_tile_block(t.sid) * (size_t)((per_group ? ncomp : 1) + 3);
+  if (lds <= 48 * 1024) {
+    int nt[3];
+    for (int d = 0; d < 3; ++d)
+      nt[d] = (t.rhi[d] - (t.rlo[d] & ~1) + CXP_PT_TILE) / CXP_PT_TILE;
+    const dim3 grid((unsigned)(nt[0] * nt[1] * nt[2]), per_group ? 1 : ncomp);
+    if (per_group)
+      hipLaunchKernelGGL(k_point_tiled<true>, grid, dim3(256), lds, s, t,
+                         nt[0], nt[1]);
+    else
+      hipLaunchKernelGGL(k_point_tiled<false>, grid, dim3(256), lds, s, t,
+                         nt[0], nt[1]);
+    return hip_status(hipGetLastError());
+  }
   const unsigned nb = (unsigned)((n + 255) / 256);
   if (per_group)
     hipLaunchKernelGGL(k_point_per_group, dim3(nb), dim3(256), 0, s, t);

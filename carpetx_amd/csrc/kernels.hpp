@@ -394,6 +394,10 @@ struct PointTask {
   int fb_per_var, fb_per_group;
   int ncomp;
   long long ccs, fcs;
+  // |undivided differences| of the coarse data along x, y, z (3 arrays of
+  // ccs values laid out like crse, already the max over the components the
+  // search covers), or null: computed from crse by the search itself
+  const double *d2;
 };
 
 __device__ __forceinline__ double crse_at(const DFab &c, int i, int j, int k) {
@@ -452,7 +456,26 @@ __device__ inline void pt_eno_search(const PointTask &t, long long comp0,
           }
         }
         double ddx = 0, ddy = 0, ddz = 0;
-        for (int cc = 0; cc < ncomp; ++cc) {
+        if (t.d2) {
+          const DFab &c = t.crse;
+          const double *dx2 = t.d2, *dy2 = t.d2 + t.ccs, *dz2 = t.d2 + 2 * t.ccs;
+          if (use[0])
+            for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
+              for (int dj = r[1][0]; dj <= r[1][1]; ++dj)
+                ddx = fmax(ddx, dx2[didx(c, ic[0] + si, ic[1] + sj + dj,
+                                         ic[2] + sk + dk)]);
+          if (use[1])
+            for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
+              for (int di = r[0][0]; di <= r[0][1]; ++di)
+                ddy = fmax(ddy, dy2[didx(c, ic[0] + si + di, ic[1] + sj,
+                                         ic[2] + sk + dk)]);
+          if (use[2])
+            for (int dj = r[1][0]; dj <= r[1][1]; ++dj)
+              for (int di = r[0][0]; di <= r[0][1]; ++di)
+                ddz = fmax(ddz, dz2[didx(c, ic[0] + si + di, ic[1] + sj + dj,
+                                         ic[2] + sk)]);
+        }
+        for (int cc = 0; cc < (t.d2 ? 0 : ncomp); ++cc) {
           DFab c = t.crse;
           c.p += (comp0 + cc) * t.ccs;
           if (use[0])
@@ -669,6 +692,134 @@ __global__ void __launch_bounds__(256) k_point_per_group(PointTask t) {
     DFab fo = t.fine;
     fo.p += cc * t.fcs;
     fo.p[didx(fo, f[0], f[1], f[2])] = v;
+  }
+}
+
+// LDS-tiled point kernels: a workgroup owns an 8 x 8 x 8 fine tile and first
+// copies CoarseBox(tile) of the coarse FAB into LDS (every read of the
+// decision and value code for the tile's points lies inside it, as for the
+// whole region, prolongate_3d_rf2_impl.hxx:1226-1254), then runs the same
+// per-point code reading the LDS copy through a DFab view.  Same operations
+// in the same order, so the results equal k_point_per_var/_per_group bit for
+// bit; the ENO search's undivided differences come from LDS instead of
+// global memory.
+#define CXP_PT_TILE 8
+
+// CoarseBox of fine [f0, f1] in direction d for stencil sid.
+__host__ __device__ inline void pt_coarse_range(int sid, int f0, int f1,
+                                                int &c0, int &c1) {
+  const SidInfo s = sid_info(sid);
+  c0 = (f0 >> 1) - s.ghosts;
+  c1 = (f1 >> 1) + ((s.cent == CENT_VC && (f1 & 1)) ? 1 : 0) + s.ghosts;
+}
+
+// Largest coarse block (doubles per component) of a tile for these stencils;
+// the tiled kernels hold ncomp such blocks plus three of undivided
+// differences.
+__host__ __device__ inline int pt_tile_block(const int sid[3]) {
+  int v = 1;
+  for (int d = 0; d < 3; ++d)
+    v *= CXP_PT_TILE / 2 + 1 + 2 * sid_info(sid[d]).ghosts;
+  return v;
+}
+
+template <bool PG>
+__global__ void __launch_bounds__(256)
+    k_point_tiled(PointTask t, int ntx, int nty) {
+  extern __shared__ double ptile[];
+  const int b = blockIdx.x;
+  const int tv[3] = {b % ntx, (b / ntx) % nty, b / (ntx * nty)};
+  int f0[3], f1[3], c0[3], c1[3], n[3];
+  for (int d = 0; d < 3; ++d) {
+    f0[d] = (t.rlo[d] & ~1) + CXP_PT_TILE * tv[d];
+    f1[d] = min(f0[d] + CXP_PT_TILE - 1, t.rhi[d]);
+    f0[d] = max(f0[d], t.rlo[d]);
+    pt_coarse_range(t.sid[d], f0[d], f1[d], c0[d], c1[d]);
+    n[d] = c1[d] - c0[d] + 1;
+  }
+  const int vol = n[0] * n[1] * n[2];
+  const int ncb = PG ? t.ncomp : 1;
+  const long long comp0 = PG ? 0 : blockIdx.y;
+  for (int idx = threadIdx.x; idx < vol * ncb; idx += blockDim.x) {
+    const int cc = idx / vol;
+    const int r = idx - cc * vol;
+    const int i = r % n[0], j = (r / n[0]) % n[1], k = r / (n[0] * n[1]);
+    DFab c = t.crse;
+    c.p += (comp0 + cc) * t.ccs;
+    ptile[idx] = crse_at(c, c0[0] + i, c0[1] + j, c0[2] + k);
+  }
+  __syncthreads();
+  // |undivided differences| along each ENO direction, max over the block's
+  // components (the search's max over lines and components is taken with
+  // fmax, which does not depend on the order)
+  double *d2 = ptile + vol * ncb;
+  const int stride[3] = {1, n[0], n[0] * n[1]};
+  for (int d = 0; d < 3; ++d) {
+    if (t.sid[d] != SID_ENO2)
+      continue;
+    for (int r = threadIdx.x; r < vol; r += blockDim.x) {
+      const int id[3] = {r % n[0], (r / n[0]) % n[1], r / (n[0] * n[1])};
+      double m = 0.0;
+      if (id[d] > 0 && id[d] < n[d] - 1)
+        for (int cc = 0; cc < ncb; ++cc) {
+          const double *q = ptile + cc * vol + r;
+          m = fmax(m, fabs(undiv2(q[-stride[d]], q[0], q[stride[d]])));
+        }
+      d2[d * vol + r] = m;
+    }
+  }
+  __syncthreads();
+  PointTask tl = t;
+  tl.crse.p = ptile;
+  for (int d = 0; d < 3; ++d)
+    tl.crse.lo[d] = c0[d];
+  tl.crse.sy = n[0];
+  tl.crse.sz = (long long)n[0] * n[1];
+  tl.ccs = vol;
+  tl.d2 = d2;
+  const int m[3] = {f1[0] - f0[0] + 1, f1[1] - f0[1] + 1, f1[2] - f0[2] + 1};
+  for (int p = threadIdx.x; p < m[0] * m[1] * m[2]; p += blockDim.x) {
+    const int f[3] = {f0[0] + p % m[0], f0[1] + (p / m[0]) % m[1],
+                      f0[2] + p / (m[0] * m[1])};
+    int ic[3], off[3], shift[3];
+    for (int d = 0; d < 3; ++d) {
+      ic[d] = f[d] >> 1;
+      off[d] = f[d] & 1;
+    }
+    bool need;
+    if (!PG) {
+      double val;
+      pt_decide_var(tl, 0, off, ic, shift, val, need);
+      double res = val;
+      if (need) {
+        int sl[3];
+        for (int d = 0; d < 3; ++d)
+          sl[d] = sid_linear_per_var(t.sid[d]);
+        const int zero[3] = {0, 0, 0};
+        res = pt_tensor(tl.crse, sl, zero, off, ic);
+      }
+      DFab fo = t.fine;
+      fo.p += (long long)blockIdx.y * t.fcs;
+      fo.p[didx(fo, f[0], f[1], f[2])] = res;
+    } else {
+      double vals[16];
+      pt_decide_group(tl, off, ic, shift, vals, need);
+      int sl[3];
+      for (int d = 0; d < 3; ++d)
+        sl[d] = sid_linear_per_group(t.sid[d]);
+      const int zero[3] = {0, 0, 0};
+      for (int cc = 0; cc < t.ncomp; ++cc) {
+        double v = vals[cc];
+        if (need) {
+          DFab c = tl.crse;
+          c.p += cc * tl.ccs;
+          v = pt_tensor(c, sl, zero, off, ic);
+        }
+        DFab fo = t.fine;
+        fo.p += cc * t.fcs;
+        fo.p[didx(fo, f[0], f[1], f[2])] = v;
+      }
+    }
   }
 }
 

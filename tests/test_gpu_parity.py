@@ -115,6 +115,17 @@ def test_point_kernel_ops_larger(fam, order, it):
         assert_parity(got, ref, "%s o%d %s pg=%d" % (fam, order, it, pg))
 
 
+@pytest.mark.parametrize("fam,order,it", [
+    ("eno", 5, (1, 1, 1)), ("poly-cons3lfb", 5, (1, 0, 1)), ("poly-eno3lfb", 5, (0, 1, 1)),
+])
+def test_point_kernel_per_group_many_components(fam, order, it):
+    """Per-group mode with 12 components: the coarse block of a tile no
+    longer fits in LDS, so the one-thread-per-point kernel runs."""
+    got, ref = run_case(fam, order, it, (-5, 0, 1), (20, 13, 11), ncomp=12,
+                        per_group=True, seed=13)
+    assert_parity(got, ref, "%s o%d %s pg=1 ncomp=12" % (fam, order, it))
+
+
 def test_component_offsets():
     """crse_comp / fine_comp select components inside multi-component FABs."""
     got, ref = run_case("natural", 3, (0, 1, 0), (0, 0, 0), (17, 12, 9), ncomp=2,
