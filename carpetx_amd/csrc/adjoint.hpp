@@ -172,6 +172,9 @@ struct AdjTask {
 #define CXP_ADJ_R 9                  // ceil(48 * 48 / 256) window loads
 #define CXP_ADJ_HALF (CXP_ADJ_WMAX / 2)
 
+// SU >= 0: every direction uses stencil SU (VVV / CCC operators), compiled
+// without runtime dispatch; SU < 0: per-direction runtime dispatch.
+template <int SU>
 __global__ void __launch_bounds__(256)
     k_adj_linear(AdjTask t, int ntx, int nty, int kchunk) {
   __shared__ double Fs[2 * CXP_ADJ_WMAX * CXP_ADJ_HALF]; // [parity][row][x/2]
@@ -264,38 +267,48 @@ __global__ void __launch_bounds__(256)
       }
       __syncthreads();
       // x^T over every window row
-      for (int idx = tid; idx < H * CXP_ADJ_T; idx += 256) {
-        const int row = idx >> 4, qx = idx & 15;
-        const double *Fe = Fs + row * hw;
-        const double *Fo = Fe + CXP_ADJ_WMAX * CXP_ADJ_HALF;
-        double v = 0.0;
+      auto xrows = [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        for (int idx = tid; idx < H * CXP_ADJ_T; idx += 256) {
+          const int row = idx >> 4, qx = idx & 15;
+          const double *Fe = Fs + row * hw;
+          Qs[idx] = adj_x<S>(Fe, Fe + CXP_ADJ_WMAX * CXP_ADJ_HALF, qx);
+        }
+      };
+      if constexpr (SU >= 0) {
+        xrows(std::integral_constant<int, SU>{});
+      } else {
         switch (sidx) {
 #define CXP_AX(S)                                                              \
   case S:                                                                      \
-    v = adj_x<S>(Fe, Fo, qx);                                                  \
+    xrows(std::integral_constant<int, S>{});                                   \
     break;
           CXP_ADJ_LINEAR_SIDS(CXP_AX)
 #undef CXP_AX
         }
-        Qs[idx] = v;
       }
       __syncthreads();
       double s = 0.0;
-      switch (sidy) {
+      if constexpr (SU >= 0) {
+        s = adj_y<SU>(Qs, cx, cy);
+        AdjZ<SU, off, 0>::add(acc, s);
+      } else {
+        switch (sidy) {
 #define CXP_AY(S)                                                              \
   case S:                                                                      \
     s = adj_y<S>(Qs, cx, cy);                                                  \
     break;
-        CXP_ADJ_LINEAR_SIDS(CXP_AY)
+          CXP_ADJ_LINEAR_SIDS(CXP_AY)
 #undef CXP_AY
-      }
-      switch (sidz) {
+        }
+        switch (sidz) {
 #define CXP_AZ(S)                                                              \
   case S:                                                                      \
     AdjZ<S, off, 0>::add(acc, s);                                              \
     break;
-        CXP_ADJ_LINEAR_SIDS(CXP_AZ)
+          CXP_ADJ_LINEAR_SIDS(CXP_AZ)
 #undef CXP_AZ
+        }
       }
     };
     fine_plane(std::integral_constant<int, 0>{});
@@ -487,8 +500,25 @@ inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
     kchunk = kchunk < 8 ? 8 : kchunk;
     kchunk = kchunk > nk ? nk : kchunk;
     const int nkc = (nk + kchunk - 1) / kchunk;
-    hipLaunchKernelGGL(k_adj_linear, dim3(ntx * nty * nkc, ncomp), dim3(256), 0,
-                       s, t, ntx, nty, kchunk);
+    const dim3 grid(ntx * nty * nkc, ncomp);
+    bool done = false;
+    if (op.sid[0] == op.sid[1] && op.sid[1] == op.sid[2]) {
+      done = true;
+      switch (op.sid[0]) {
+#define CXP_AL(S)                                                              \
+  case S:                                                                      \
+    hipLaunchKernelGGL(k_adj_linear<S>, grid, dim3(256), 0, s, t, ntx, nty,     \
+                       kchunk);                                                \
+    break;
+        CXP_ADJ_LINEAR_SIDS(CXP_AL)
+#undef CXP_AL
+      default:
+        done = false;
+      }
+    }
+    if (!done)
+      hipLaunchKernelGGL(k_adj_linear<-1>, grid, dim3(256), 0, s, t, ntx, nty,
+                         kchunk);
     return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
   }
   // MINMOD: a three-pass implementation hung the GPU in testing and was
