@@ -20,6 +20,10 @@ namespace cxp {
 //    proportion to their cost, so both kinds are in flight at all times.
 // Dynamic LDS: the larger of the separable routine's buffer and the largest
 // thin block of the list.
+// SU >= 0: every separable task of the list uses stencil SU in all three
+// directions (a list of one VVV or CCC operator), and the marching routine
+// is compiled for it without dispatch.
+template <int SU>
 __global__ void __launch_bounds__(CXP_NT, 4)
     k_fill_batched(const SepTask *__restrict__ tasks,
                    const SepTile *__restrict__ tiles,
@@ -32,10 +36,23 @@ __global__ void __launch_bounds__(CXP_NT, 4)
     if (tl.cfg < 0)
       thin_tile(t, tl.comp, tl.x0, tl.y0, fill_smem);
     else
-      sep_tile(make_geom(tl.cfg), t, tl.comp, tl.x0, tl.y0, tl.z0, tl.z1,
-               fill_smem);
+      sep_tile<SU>(make_geom(tl.cfg), t, tl.comp, tl.x0, tl.y0, tl.z0, tl.z1,
+                   fill_smem);
     __syncthreads();  // the next tile rewrites the LDS buffers
   }
+}
+
+// Calls f(kernel) with the k_fill_batched instance for uniform stencil su
+// (or the dispatching one for su < 0).
+template <class F> inline auto with_fill_kernel(int su, F &&f) {
+  switch (su) {
+#define CXP_FK(S)                                                              \
+  case S:                                                                      \
+    return f(k_fill_batched<S>);
+    CXP_LINEAR_SIDS(CXP_FK)
+#undef CXP_FK
+  }
+  return f(k_fill_batched<-1>);
 }
 
 } // namespace cxp

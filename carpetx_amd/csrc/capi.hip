@@ -301,8 +301,21 @@ int launch_sep_box(const SepTask &t, int ncomp, hipStream_t s) {
          (long long)ntx * nty * ((nz + zchunk - 1) / zchunk) * ncomp < 2048)
     zchunk = (zchunk + 1) / 2;
   const int nzc = (nz + zchunk - 1) / zchunk;
-  hipLaunchKernelGGL(k_sep_box, dim3(ntx * nty * nzc, ncomp), dim3(CXP_NT), 0,
-                     s, t, c.lx, ntx, nty, zchunk);
+  const dim3 grid(ntx * nty * nzc, ncomp);
+  // uniform stencils (VVV / CCC operators): the kernel compiled for them
+  if (t.sid[0] == t.sid[1] && t.sid[1] == t.sid[2]) {
+    switch (t.sid[0]) {
+#define CXP_SB(S)                                                              \
+  case S:                                                                      \
+    hipLaunchKernelGGL(k_sep_box<S>, grid, dim3(CXP_NT), 0, s, t, c.lx, ntx,  \
+                       nty, zchunk);                                           \
+    return hip_status(hipGetLastError());
+      CXP_LINEAR_SIDS(CXP_SB)
+#undef CXP_SB
+    }
+  }
+  hipLaunchKernelGGL(k_sep_box<-1>, grid, dim3(CXP_NT), 0, s, t, c.lx, ntx,
+                     nty, zchunk);
   return hip_status(hipGetLastError());
 }
 
@@ -419,6 +432,7 @@ struct cxp_worklist_s {
   bool uploaded = false;
   int grid = 0;
   int lds = 0;  // bytes of dynamic LDS of k_fill_batched
+  int uniform_sid = -1;  // the one stencil of every separable task, or -1
   int64_t fine_points = 0, algo_bytes = 0;
 };
 
@@ -437,7 +451,7 @@ int64_t patch_coarse_points(const cxp_worklist_s::Patch &p) {
          (chi[2] - clo[2] + 1);
 }
 
-int persistent_grid(int ntiles, int lds);
+int persistent_grid(int ntiles, int lds, int su);
 
 // Orders the tile list into one contiguous range per workgroup of the
 // persistent grid (see k_fill_batched).  Cost estimates, in units of one
@@ -604,7 +618,19 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
     }
     return n;
   };
-  const long long slots = 4LL * persistent_grid(1 << 30, wl->lds);
+  // one stencil in every direction of every task: the specialised kernel
+  wl->uniform_sid = -1;
+  for (size_t i = 0; i < wl->tasks.size(); ++i) {
+    const SepTask &t = wl->tasks[i];
+    const bool uni = t.sid[0] == t.sid[1] && t.sid[1] == t.sid[2];
+    if (!uni || (i > 0 && t.sid[0] != wl->uniform_sid)) {
+      wl->uniform_sid = -1;
+      break;
+    }
+    wl->uniform_sid = t.sid[0];
+  }
+  const long long slots =
+      4LL * persistent_grid(1 << 30, wl->lds, wl->uniform_sid);
   int zdef = 64;
   while (zdef > 8 && count(zdef) < slots)
     zdef /= 2;
@@ -628,7 +654,7 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
             wl->tiles.push_back(tl);
           }
   }
-  wl->grid = persistent_grid((int)wl->tiles.size(), wl->lds);
+  wl->grid = persistent_grid((int)wl->tiles.size(), wl->lds, wl->uniform_sid);
   schedule_tiles(wl, wl->grid);
 }
 
@@ -728,7 +754,7 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
   return CXP_OK;
 }
 
-int persistent_grid(int ntiles, int lds) {
+int persistent_grid(int ntiles, int lds, int su) {
   static int cus = 0;
   if (!cus) {
     int dev = 0;
@@ -740,9 +766,11 @@ int persistent_grid(int ntiles, int lds) {
       cus = 256;
   }
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fill_batched,
-                                                   CXP_NT, lds) != hipSuccess ||
-      per_cu <= 0)
+  const hipError_t e = with_fill_kernel(su, [&](auto kern) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, CXP_NT,
+                                                        lds);
+  });
+  if (e != hipSuccess || per_cu <= 0)
     per_cu = 2;
   int g = cus * per_cu;
   g -= g % 8;
@@ -753,9 +781,11 @@ int persistent_grid(int ntiles, int lds) {
 int launch_sep_list(cxp_worklist_s *wl, hipStream_t s) {
   if (wl->tiles.empty())
     return CXP_OK;
-  hipLaunchKernelGGL(k_fill_batched, dim3(wl->grid), dim3(CXP_NT),
-                     (size_t)wl->lds, s, wl->d_tasks, wl->d_tiles,
-                     wl->d_wg_start);
+  with_fill_kernel(wl->uniform_sid, [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(wl->grid), dim3(CXP_NT), (size_t)wl->lds, s,
+                       wl->d_tasks, wl->d_tiles, wl->d_wg_start);
+    return 0;
+  });
   return hip_status(hipGetLastError());
 }
 

@@ -184,6 +184,9 @@ __device__ __forceinline__ double sep_zval(const double (&w)[CXP_ZW],
 
 // Process one tile [x0, x0+TX) x [y0, y0+TY) x [z0, z1] of a task, clipped
 // to the task's fine region (x0, y0 even).  All threads of the block call it.
+// SU >= 0: every direction uses stencil SU, compiled without dispatch;
+// SU < 0: per-pass runtime dispatch on the task's stencils.
+template <int SU = -1>
 __device__ inline void sep_tile(const Geom &g, const SepTask &t,
                                 long long comp, int x0, int y0, int z0, int z1,
                                 double *smem) {
@@ -273,13 +276,17 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     for (int r = 0; r < CXP_PRE; ++r)
       pre[r] = bld(rc, (more && soff[r] >= 0) ? pbase + loff[r] : CXP_OOB);
     __syncthreads();
-    switch (sx) {
+    if constexpr (SU >= 0) {
+      sep_xpass<SU>(g, Cs, Xs, CY, npx, x0, ci0, tid);
+    } else {
+      switch (sx) {
 #define CXP_X(S)                                                               \
   case S:                                                                      \
     sep_xpass<S>(g, Cs, Xs, CY, npx, x0, ci0, tid);                            \
     break;
-      CXP_LINEAR_SIDS(CXP_X)
+        CXP_LINEAR_SIDS(CXP_X)
 #undef CXP_X
+      }
     }
     __syncthreads();
     // y-pass into the register window (rotated by one coarse plane)
@@ -290,13 +297,17 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     }
     double ya = 0.0, yb = 0.0;
     if (own) {
-      switch (sy) {
+      if constexpr (SU >= 0) {
+        sep_ypass<SU>(g, Xs, row0, fi, ya, yb);
+      } else {
+        switch (sy) {
 #define CXP_Y(S)                                                               \
   case S:                                                                      \
     sep_ypass<S>(g, Xs, row0, fi, ya, yb);                                     \
     break;
-        CXP_LINEAR_SIDS(CXP_Y)
+          CXP_LINEAR_SIDS(CXP_Y)
 #undef CXP_Y
+        }
       }
     }
     w0[CXP_ZW - 1] = ya;
@@ -307,7 +318,13 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     const bool zact = kc >= kc0 && kc <= kc1;
     double v00 = 0.0, v01 = 0.0, v10 = 0.0, v11 = 0.0;
     if (zact) {
-      switch (sz) {
+      if constexpr (SU >= 0) {
+        v00 = sep_zval<SU>(w0, 0);
+        v01 = sep_zval<SU>(w0, 1);
+        v10 = sep_zval<SU>(w1, 0);
+        v11 = sep_zval<SU>(w1, 1);
+      } else {
+        switch (sz) {
 #define CXP_Z(S)                                                               \
   case S:                                                                      \
     v00 = sep_zval<S>(w0, 0);                                                  \
@@ -315,8 +332,9 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     v10 = sep_zval<S>(w1, 0);                                                  \
     v11 = sep_zval<S>(w1, 1);                                                  \
     break;
-        CXP_LINEAR_SIDS(CXP_Z)
+          CXP_LINEAR_SIDS(CXP_Z)
 #undef CXP_Z
+        }
       }
     }
     const int fz0 = 2 * kc, fz1 = 2 * kc + 1;
@@ -333,6 +351,7 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
 
 // Per-box launch: grid.x = tiles (x fastest, then y, then z-chunks),
 // grid.y = component.  The tile shape is uniform over the launch.
+template <int SU>
 __global__ void __launch_bounds__(CXP_NT)
     k_sep_box(SepTask t, int lx, int ntx, int nty, int zchunk) {
   __shared__ double smem[kLdsDoubles];
@@ -345,7 +364,7 @@ __global__ void __launch_bounds__(CXP_NT)
   const int y0 = (t.rlo[1] & ~1) + ty * g.ty;
   const int z0 = t.rlo[2] + tz * zchunk;
   const int z1 = min(t.rhi[2], z0 + zchunk - 1);
-  sep_tile(g, t, blockIdx.y, x0, y0, z0, z1, smem);
+  sep_tile<SU>(g, t, blockIdx.y, x0, y0, z0, z1, smem);
 }
 
 // ---------------------------------------------------------------------------
