@@ -325,13 +325,6 @@ __global__ void __launch_bounds__(256)
 // --------------------------------------------------------------------------
 // ENO / linear-fallback adjoint
 
-// Branch code of a fine point: (sx+1) + 3 (sy+1) + 9 (sz+1) + 27 need.
-__device__ __forceinline__ unsigned char adj_code(const int shift[3],
-                                                  bool need) {
-  return (unsigned char)((shift[0] + 1) + 3 * (shift[1] + 1) +
-                         9 * (shift[2] + 1) + (need ? 27 : 0));
-}
-
 // Records the branch of every fine point of the region (per component in
 // per-var mode, shared in per-group mode).
 __global__ void __launch_bounds__(256)

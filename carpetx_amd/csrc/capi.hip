@@ -388,9 +388,26 @@ int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
   const size_t nbytes = (size_t)npts * (per_group ? 1 : ncomp);
   if (hipMallocAsync((void **)&codes, nbytes, s) != hipSuccess)
     return CXP_ERR_HIP;
-  hipLaunchKernelGGL(k_adj_decide,
-                     dim3((unsigned)((npts + 255) / 256), per_group ? 1 : ncomp),
-                     dim3(256), 0, s, t.pt, codes, t.per_group);
+  // the branch record: LDS-tiled like the forward when a tile's coarse
+  // block fits, else one thread per point
+  const size_t lds =
+      8 * (size_t)pt_tile_block(t.pt.sid) * (size_t)((per_group ? ncomp : 1) + 3);
+  if (lds <= 48 * 1024) {
+    int nt[3];
+    for (int d = 0; d < 3; ++d)
+      nt[d] = (rhi[d] - (rlo[d] & ~1) + CXP_PT_TILE) / CXP_PT_TILE;
+    const dim3 grid((unsigned)(nt[0] * nt[1] * nt[2]), per_group ? 1 : ncomp);
+    if (per_group)
+      hipLaunchKernelGGL((k_point_tiled<true, true>), grid, dim3(256), lds, s,
+                         t.pt, nt[0], nt[1], codes);
+    else
+      hipLaunchKernelGGL((k_point_tiled<false, true>), grid, dim3(256), lds, s,
+                         t.pt, nt[0], nt[1], codes);
+  } else {
+    hipLaunchKernelGGL(k_adj_decide,
+                       dim3((unsigned)((npts + 255) / 256), per_group ? 1 : ncomp),
+                       dim3(256), 0, s, t.pt, codes, t.per_group);
+  }
   int st = hip_status(hipGetLastError());
   if (st == CXP_OK) {
     hipLaunchKernelGGL(k_adj_gather,

@@ -742,9 +742,20 @@ __host__ __device__ inline int pt_tile_block(const int sid[3]) {
   return v;
 }
 
-template <bool PG>
+// Branch code of a fine point (for the adjoint): (sx+1) + 3 (sy+1) +
+// 9 (sz+1) + 27 need.
+__device__ __forceinline__ unsigned char adj_code(const int shift[3],
+                                                  bool need) {
+  return (unsigned char)((shift[0] + 1) + 3 * (shift[1] + 1) +
+                         9 * (shift[2] + 1) + (need ? 27 : 0));
+}
+
+// CODES: record each fine point's branch in codes (per component in per-var
+// mode, region-linear, x fastest) instead of writing its value.
+template <bool PG, bool CODES = false>
 __global__ void __launch_bounds__(256)
-    k_point_tiled(PointTask t, int ntx, int nty) {
+    k_point_tiled(PointTask t, int ntx, int nty,
+                  unsigned char *__restrict__ codes = nullptr) {
   extern __shared__ double ptile[];
   const int b = blockIdx.x;
   const int tv[3] = {b % ntx, (b / ntx) % nty, b / (ntx * nty)};
@@ -806,7 +817,21 @@ __global__ void __launch_bounds__(256)
       off[d] = f[d] & 1;
     }
     bool need;
-    if (!PG) {
+    if constexpr (CODES) {
+      const long long rnx = t.rhi[0] - t.rlo[0] + 1;
+      const long long rny = t.rhi[1] - t.rlo[1] + 1;
+      const long long gid = (f[0] - t.rlo[0]) + (f[1] - t.rlo[1]) * rnx +
+                            (f[2] - t.rlo[2]) * rnx * rny;
+      if (PG) {
+        double vals[16];
+        pt_decide_group(tl, off, ic, shift, vals, need);
+        codes[gid] = adj_code(shift, need);
+      } else {
+        double val;
+        pt_decide_var(tl, 0, off, ic, shift, val, need);
+        codes[(long long)blockIdx.y * pt_npts(t) + gid] = adj_code(shift, need);
+      }
+    } else if (!PG) {
       double val;
       pt_decide_var(tl, 0, off, ic, shift, val, need);
       double res = val;

@@ -128,26 +128,28 @@ def test_nonlinear_adjoint_against_dense_jacobian(fam, order, it, pg):
     assert np.max(np.abs(got - expect)) <= 1e-6 * max(1.0, np.max(np.abs(expect)))
 
 
-@pytest.mark.parametrize("fam,order,it,pg", NONLINEAR[:4])
-def test_nonlinear_adjoint_dot_product(fam, order, it, pg):
+@pytest.mark.parametrize("fam,order,it,pg,nc", [x + (2,) for x in NONLINEAR[:4]] +
+                         [("poly-cons3lfb", 5, (1, 0, 1), True, 12)])
+def test_nonlinear_adjoint_dot_product(fam, order, it, pg, nc):
     """<J du, v> == <du, J^T v> on a larger region, J du from the GPU
-    forward by central differences."""
+    forward by central differences (12 per-group components: the branch
+    record no longer fits in LDS and runs one thread per point)."""
     rng = np.random.default_rng(21)
     rlo, rhi = (-4, 3, 0), (37, 22, 13)
-    P, cb, fbox, u, v = _fabs((fam, order, it), rlo, rhi, 2, rng)
+    P, cb, fbox, u, v = _fabs((fam, order, it), rlo, rhi, nc, rng)
     du = rng.uniform(-1, 1, size=u.shape)
     eps = 1e-7
 
     def fwd(x):
-        crse = FArrayBox(cb, 2, tensor=torch.from_numpy(x).cuda())
-        fine = FArrayBox(fbox, 2)
-        P.interp(crse, 0, fine, 0, 2, fbox, per_group=pg)
+        crse = FArrayBox(cb, nc, tensor=torch.from_numpy(x).cuda())
+        fine = FArrayBox(fbox, nc)
+        P.interp(crse, 0, fine, 0, nc, fbox, per_group=pg)
         return fine.t
     jdu = (fwd(u + eps * du) - fwd(u - eps * du)) / (2 * eps)
-    vb = FArrayBox(fbox, 2, tensor=torch.from_numpy(v).cuda())
-    cbar = FArrayBox(cb, 2)
-    crse = FArrayBox(cb, 2, tensor=torch.from_numpy(u).cuda())
-    P.interp_adjoint(cbar, 0, vb, 0, 2, fbox, crse_in=crse, per_group=pg)
+    vb = FArrayBox(fbox, nc, tensor=torch.from_numpy(v).cuda())
+    cbar = FArrayBox(cb, nc)
+    crse = FArrayBox(cb, nc, tensor=torch.from_numpy(u).cuda())
+    P.interp_adjoint(cbar, 0, vb, 0, nc, fbox, crse_in=crse, per_group=pg)
     torch.cuda.synchronize()
     lhs = float((jdu * vb.t).sum())
     rhs = float((torch.from_numpy(du).cuda() * cbar.t).sum())
