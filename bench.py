@@ -189,7 +189,7 @@ def bench_adjoint(stream, log):
             rec.update(adj_ms=round(ms_a, 4), adj_over_fwd=round(ms_a / ms_f, 3),
                        adj_hbm_frac=round(algo / (ms_a * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
         except Exception as e:  # operators without a device adjoint
-            rec.update(adj_ms=None, adj_note=str(e).split(":")[0][:80])
+            rec.update(adj_ms=None, adj_note=str(e)[:120])
         out[name] = rec
         log("adjoint %s: %s" % (name, rec))
     return out

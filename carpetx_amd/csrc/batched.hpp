@@ -23,22 +23,71 @@ namespace cxp {
 // SU >= 0: every separable task of the list uses stencil SU in all three
 // directions (a list of one VVV or CCC operator), and the marching routine
 // is compiled for it without dispatch.
+//
+// Dynamic tail: the static ranges hold most of the work, but the SIMDs
+// issue the oldest resident wave first, so workgroups dispatched later on a
+// compute unit progress more slowly than any fixed estimate says.  The
+// smallest tiles of each XCD class x therefore sit in a queue, tiles
+// [q.start[x], q.start[x + 1]), which the class's workgroups claim q.chunk
+// at a time through counter q.ctr[32 x] once their own range is done.  The
+// last workgroup to finish (counter q.ctr[32 q.nq]) zeroes the counters for
+// the next launch, so launches of one list must not overlap.
+struct FillQueue {
+  int *ctr;  // null: no queue
+  int start[9];
+  int nq;
+  int chunk;
+};
+
+template <int SU>
+__device__ __forceinline__ void fill_one(const SepTask *__restrict__ tasks,
+                                         const SepTile *__restrict__ tiles,
+                                         int n, double *smem) {
+  const SepTile tl = tiles[n];
+  const SepTask t = tasks[tl.task];
+  if (tl.cfg < 0)
+    thin_tile(t, tl.comp, tl.x0, tl.y0, smem);
+  else
+    sep_tile<SU>(make_geom(tl.cfg), t, tl.comp, tl.x0, tl.y0, tl.z0, tl.z1,
+                 smem);
+  __syncthreads();  // the next tile rewrites the LDS buffers
+}
+
 template <int SU>
 __global__ void __launch_bounds__(CXP_NT, 4)
     k_fill_batched(const SepTask *__restrict__ tasks,
                    const SepTile *__restrict__ tiles,
-                   const int *__restrict__ wg_start) {
+                   const int *__restrict__ wg_start, FillQueue q) {
   extern __shared__ double fill_smem[];
+  __shared__ int claim;
   const int n0 = wg_start[blockIdx.x], n1 = wg_start[blockIdx.x + 1];
-  for (int n = n0; n < n1; ++n) {
-    const SepTile tl = tiles[n];
-    const SepTask t = tasks[tl.task];
-    if (tl.cfg < 0)
-      thin_tile(t, tl.comp, tl.x0, tl.y0, fill_smem);
-    else
-      sep_tile<SU>(make_geom(tl.cfg), t, tl.comp, tl.x0, tl.y0, tl.z0, tl.z1,
-                   fill_smem);
-    __syncthreads();  // the next tile rewrites the LDS buffers
+  for (int n = n0; n < n1; ++n)
+    fill_one<SU>(tasks, tiles, n, fill_smem);
+  if (q.ctr == nullptr)
+    return;
+  const int x = blockIdx.x % q.nq;
+  const int p0 = q.start[x], p1 = q.start[x + 1];
+  for (;;) {
+    if (threadIdx.x == 0)
+      claim = p0 + q.chunk * __hip_atomic_fetch_add(q.ctr + 32 * x, 1,
+                                                    __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    // every later write of claim follows at least one tile's barrier
+    const int c0 = claim;
+    if (c0 >= p1)
+      break;
+    const int c1 = min(c0 + q.chunk, p1);
+    for (int n = c0; n < c1; ++n)
+      fill_one<SU>(tasks, tiles, n, fill_smem);
+  }
+  if (threadIdx.x == 0 &&
+      __hip_atomic_fetch_add(q.ctr + 32 * q.nq, 1, __ATOMIC_ACQ_REL,
+                             __HIP_MEMORY_SCOPE_AGENT) ==
+          (int)gridDim.x - 1) {
+    for (int i = 0; i <= q.nq; ++i)
+      __hip_atomic_store(q.ctr + 32 * i, 0, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

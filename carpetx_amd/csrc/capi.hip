@@ -440,6 +440,8 @@ struct cxp_worklist_s {
   std::vector<int> wg_start;  // per-workgroup tile ranges of k_fill_batched
   int *d_wg_start = nullptr;
   size_t cap_wg = 0;
+  std::vector<int> qstart;  // per-XCD-class queues after the ranges, or empty
+  int *d_ctr = nullptr;     // their claim counters and the finish counter
   size_t cap_tasks = 0, cap_tiles = 0;
   std::vector<PoisonRun> poison_runs;  // one per (patch, component)
   std::vector<PoisonChunk> poison_chunks;
@@ -470,11 +472,51 @@ int64_t patch_coarse_points(const cxp_worklist_s::Patch &p) {
 
 int persistent_grid(int ntiles, int lds, int su);
 
+// Compute units of the current device.
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cus = prop.multiProcessorCount;
+    if (cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
+// Percent of each XCD class's estimated cost that k_fill_batched hands out
+// dynamically (its smallest tiles).  On the uneven 512-patch level, 20 to
+// 45 percent all give the same time and 10 percent is slower.
+constexpr int kQueuePct = 25;
+
+// Relative time a workgroup of the persistent grid needs for one unit of
+// work, by its dispatch slot on its compute unit.  Workgroups are dispatched
+// round-robin over the compute units, so block b is the (b / cus)-th
+// resident workgroup of its unit; the SIMDs issue the oldest ready wave
+// first, and the later slots run slower.  Measured on MI355X with 8 slots
+// per unit (the uneven 512-patch level, equal work per workgroup: mean
+// busy time by slot relative to slot 0); other slot counts use a linear
+// ramp of the same height.
+double slot_weight(int b, int cus, int grid) {
+  static const double w8[8] = {1.000, 1.082, 1.085, 1.100,
+                               1.204, 1.265, 1.251, 1.280};
+  const int nslots = std::max(1, grid / cus);
+  const int slot = std::min(b / cus, nslots - 1);
+  if (nslots == 8)
+    return w8[slot];
+  return 1.0 + 0.3 * slot / nslots;
+}
+
 // Orders the tile list into one contiguous range per workgroup of the
-// persistent grid (see k_fill_batched).  Cost estimates, in units of one
-// plane iteration of the z-marching routine: the coarse planes a separable
-// tile marches through plus a fixed per-tile overhead; a thin tile costs
-// about two plane iterations (measured on the uneven level).
+// persistent grid, followed, for lists of at least four tiles per
+// workgroup, by one queue per XCD class (see k_fill_batched).  Cost
+// estimates, in units of one plane iteration of the z-marching routine: the
+// coarse planes a separable tile marches through plus a fixed per-tile
+// overhead; a thin tile costs about two plane iterations (measured on the
+// uneven level).
 void schedule_tiles(cxp_worklist_s *wl, int grid) {
   struct Group {
     size_t first, count;
@@ -496,6 +538,7 @@ void schedule_tiles(cxp_worklist_s *wl, int grid) {
     groups.back().cost += cost[i];
   }
   const int nx = grid >= 8 ? 8 : 1;  // XCD classes of workgroups
+  const int cus = device_cus();
   // groups -> XCD, the most expensive first to the least loaded
   std::vector<size_t> order(groups.size());
   for (size_t i = 0; i < order.size(); ++i)
@@ -514,21 +557,38 @@ void schedule_tiles(cxp_worklist_s *wl, int grid) {
       xcd_tiles[best].push_back(groups[gi].first + k);
     xload[best] += groups[gi].cost;
   }
-  // within an XCD: longest tile first to the least loaded of its workgroups
-  std::vector<std::vector<size_t>> wg_tiles(grid);
+  // within an XCD: the smallest tiles, kQueuePct percent of the XCD's cost,
+  // form its queue; the others go longest first to the least loaded of its
+  // workgroups
+  std::vector<std::vector<size_t>> wg_tiles(grid), queue(nx);
+  const bool use_queue = nt >= 4 * (size_t)grid;
   for (int x = 0; x < nx; ++x) {
     std::vector<size_t> &ts = xcd_tiles[x];
     std::stable_sort(ts.begin(), ts.end(),
                      [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-    using Slot = std::pair<long long, int>;  // (load, workgroup)
+    if (use_queue) {
+      long long total = 0, tail = 0;
+      for (size_t i : ts)
+        total += cost[i];
+      size_t cut = ts.size();
+      while (cut > 0 && (tail + cost[ts[cut - 1]]) * 100 <= total * kQueuePct) {
+        --cut;
+        tail += cost[ts[cut]];
+      }
+      queue[x].assign(ts.begin() + cut, ts.end());
+      ts.resize(cut);
+    }
+    // (weighted load, workgroup); the weight is the workgroup's time per
+    // unit of work
+    using Slot = std::pair<double, int>;
     std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
     for (int b = x; b < grid; b += nx)
-      heap.push({0, b});
+      heap.push({0.0, b});
     for (size_t i : ts) {
       Slot sl = heap.top();
       heap.pop();
       wg_tiles[sl.second].push_back(i);
-      sl.first += cost[i];
+      sl.first += cost[i] * slot_weight(sl.second, cus, grid);
       heap.push(sl);
     }
   }
@@ -562,6 +622,16 @@ void schedule_tiles(cxp_worklist_s *wl, int grid) {
     }
   }
   wl->wg_start[grid] = (int)out.size();
+  wl->qstart.clear();
+  if (use_queue) {
+    // each queue longest first; equal costs keep their patch order
+    for (int x = 0; x < nx; ++x) {
+      wl->qstart.push_back((int)out.size());
+      for (size_t i : queue[x])
+        out.push_back(wl->tiles[i]);
+    }
+    wl->qstart.push_back((int)out.size());
+  }
   wl->tiles.swap(out);
 }
 
@@ -750,6 +820,12 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
     if (hipMalloc(&wl->d_wg_start, wl->cap_wg * sizeof(int)) != hipSuccess)
       return CXP_ERR_HIP;
   }
+  if (!wl->d_ctr) {
+    if (hipMalloc(&wl->d_ctr, 9 * 32 * sizeof(int)) != hipSuccess)
+      return CXP_ERR_HIP;
+    if (hipMemsetAsync(wl->d_ctr, 0, 9 * 32 * sizeof(int), s) != hipSuccess)
+      return CXP_ERR_HIP;
+  }
   if (!wl->wg_start.empty() &&
       hipMemcpyAsync(wl->d_wg_start, wl->wg_start.data(),
                      wl->wg_start.size() * sizeof(int), hipMemcpyHostToDevice,
@@ -772,16 +848,7 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
 }
 
 int persistent_grid(int ntiles, int lds, int su) {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess)
-      cus = prop.multiProcessorCount;
-    if (cus <= 0)
-      cus = 256;
-  }
+  const int cus = device_cus();
   int per_cu = 0;
   const hipError_t e = with_fill_kernel(su, [&](auto kern) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, CXP_NT,
@@ -798,9 +865,18 @@ int persistent_grid(int ntiles, int lds, int su) {
 int launch_sep_list(cxp_worklist_s *wl, hipStream_t s) {
   if (wl->tiles.empty())
     return CXP_OK;
+  FillQueue q{};
+  const int nq = (int)wl->qstart.size() - 1;
+  if (nq >= 1 && nq <= 8 && wl->d_ctr) {
+    q.ctr = wl->d_ctr;
+    q.nq = nq;
+    q.chunk = 1;
+    for (int i = 0; i <= nq; ++i)
+      q.start[i] = wl->qstart[i];
+  }
   with_fill_kernel(wl->uniform_sid, [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(wl->grid), dim3(CXP_NT), (size_t)wl->lds, s,
-                       wl->d_tasks, wl->d_tiles, wl->d_wg_start);
+                       wl->d_tasks, wl->d_tiles, wl->d_wg_start, q);
     return 0;
   });
   return hip_status(hipGetLastError());
@@ -1021,6 +1097,8 @@ int cxp_worklist_destroy(cxp_worklist wl) {
     (void)hipFree(wl->d_tiles);
   if (wl->d_wg_start)
     (void)hipFree(wl->d_wg_start);
+  if (wl->d_ctr)
+    (void)hipFree(wl->d_ctr);
   if (wl->d_poison_runs)
     (void)hipFree(wl->d_poison_runs);
   if (wl->d_poison_chunks)

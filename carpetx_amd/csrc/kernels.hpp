@@ -64,6 +64,14 @@ __device__ __forceinline__ double bld(__amdgpu_buffer_rsrc_t r, int boff) {
   return __builtin_bit_cast(double,
                             __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, 0));
 }
+// Load at voff + soff with a uniform soff.  Callers keep voff = CXP_OOB for
+// lanes that must read 0 and voff + soff inside the buffer for the others,
+// so the result does not depend on whether the range check includes soff.
+__device__ __forceinline__ double bld_s(__amdgpu_buffer_rsrc_t r, int voff,
+                                        int soff) {
+  return __builtin_bit_cast(
+      double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
 __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, int boff,
                                     double v) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(cxp_u32x2, v), r,

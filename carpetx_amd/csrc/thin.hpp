@@ -25,7 +25,6 @@ namespace cxp {
 
 #define CXP_THIN_NZ 4   // largest fine z extent of a thin patch
 #define CXP_THIN_P 18   // LDS pitch of a block row (doubles; even for 16-B stores)
-#define CXP_THIN_R 22   // coarse rows loaded per thread: 11 planes x 16 rows / 8
 #define CXP_THIN_KMAX 11  // coarse planes of the block: span_z + 3 <= 11
 
 // Static extents of the coarse block for stencil windows of span s = whi-wlo:
@@ -131,28 +130,37 @@ __device__ __forceinline__ void thin_tile(const SepTask &t, long long comp,
   const __amdgpu_buffer_rsrc_t rf = make_rsrc(fine.p, t.fbytes);
   const int nrows = CZ * CYn;
 
-  // coarse block -> LDS rows (k, cy): lane lx = coarse x, the rows spread
-  // over the 8 row groups; absent columns and rows read as 0
+  // coarse block -> LDS rows k CYn + cy: lane lx = coarse x, and thread row
+  // group ly takes rows cy = ly and ly + 8 of every plane k < CZ (CYn <= 16),
+  // so a lane's two row offsets and masks hold for all planes and the plane
+  // step k sz is a uniform offset.  Absent columns and rows read as 0.
   {
     const int lx = tid & 15, ly = tid >> 4;
-    const unsigned mg = (65536u + CYn - 1) / CYn;  // rho / CYn for rho < 256
-    const int xb = 8 * (ci0 + lx - crse.lo[0]);
-    double v[CXP_THIN_R];
+    const bool xok = lx < CX;
+    const bool hok1 = ly + 8 < CYn;
+    const unsigned sy8 = 8u * (unsigned)crse.sy, sz8 = 8u * (unsigned)crse.sz;
+    const unsigned base = 8u * (unsigned)(ci0 + lx - crse.lo[0]) +
+                          (unsigned)(cj0 + ly - crse.lo[1]) * sy8 +
+                          (unsigned)(kf - crse.lo[2]) * sz8;
+    const int vo0 = (xok && ly < CY) ? (int)base : CXP_OOB;
+    const int vo1 = (xok && ly + 8 < CY) ? (int)(base + 8u * sy8) : CXP_OOB;
+    double v[CXP_THIN_KMAX][2];
 #pragma unroll
-    for (int r = 0; r < CXP_THIN_R; ++r) {
-      const int rho = ly + 8 * r;
-      const int k = (int)(((unsigned)rho * mg) >> 16);
-      const int cy = rho - k * CYn;
-      const bool in = rho < nrows && lx < CX && cy < CY;
-      const int off = xb + 8 * ((cj0 + cy - crse.lo[1]) * crse.sy) +
-                      8 * (int)((long long)(kf + k - crse.lo[2]) * crse.sz);
-      v[r] = bld(rc, in ? off : CXP_OOB);
+    for (int k = 0; k < CXP_THIN_KMAX; ++k) {
+      if (k < CZ) {
+        v[k][0] = bld_s(rc, vo0, (int)(k * sz8));
+        if (CYn > 8)
+          v[k][1] = bld_s(rc, vo1, (int)(k * sz8));
+      }
     }
+    double *row0 = smem + ly * CXP_THIN_P + lx;
 #pragma unroll
-    for (int r = 0; r < CXP_THIN_R; ++r) {
-      const int rho = ly + 8 * r;
-      if (rho < nrows)
-        smem[rho * CXP_THIN_P + lx] = v[r];
+    for (int k = 0; k < CXP_THIN_KMAX; ++k) {
+      if (k < CZ) {
+        row0[k * CYn * CXP_THIN_P] = v[k][0];
+        if (hok1)
+          row0[(k * CYn + 8) * CXP_THIN_P] = v[k][1];
+      }
     }
   }
   __syncthreads();
@@ -183,6 +191,10 @@ __device__ __forceinline__ void thin_tile(const SepTask &t, long long comp,
   const int gx = x0 + fx;
   const bool xin = gx >= t.rlo[0] && gx <= t.rhi[0];
   const int par = z0 & 1;
+  const unsigned fsz8 = 8u * (unsigned)fine.sz;
+  const unsigned fz0 = (unsigned)(z0 - fine.lo[2]) * fsz8;
+  unsigned fo = 8u * (unsigned)(gx - fine.lo[0]) +
+                (unsigned)(y0 + 2 * fyp - fine.lo[1]) * (8u * (unsigned)fine.sy);
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int gy = y0 + 2 * fyp + h;
@@ -199,12 +211,12 @@ __device__ __forceinline__ void thin_tile(const SepTask &t, long long comp,
       CXP_LINEAR_SIDS(CXP_TZ)
 #undef CXP_TZ
     }
-    const int fo = 8 * ((gx - fine.lo[0]) + (gy - fine.lo[1]) * fine.sy);
 #pragma unroll
     for (int dz = 0; dz < 4; ++dz) {
-      const int zo = 8 * (int)((long long)(z0 + dz - fine.lo[2]) * fine.sz);
-      bst(rf, (own && dz < nzt) ? fo + zo : CXP_OOB, out[dz]);
+      if (dz < nzt)
+        bst(rf, own ? (int)(fo + fz0 + dz * fsz8) : CXP_OOB, out[dz]);
     }
+    fo += 8u * (unsigned)fine.sy;
   }
 }
 

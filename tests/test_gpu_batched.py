@@ -115,6 +115,44 @@ def test_graph_replays_match_eager():
     g.close()
 
 
+def test_dynamic_tail_covers_every_tile_on_every_launch():
+    """A list with at least four tiles per workgroup of the persistent grid,
+    so the fill kernel hands out its smallest tiles through per-XCD claim
+    counters.  The shell is poisoned before every launch, eager and
+    replayed: each launch must reproduce the per-patch fill bit for bit, so
+    no queued tile is skipped and the counters are back at zero for the
+    next launch."""
+    w = workloads.uneven_512(seed=11, nboxes=160, ncomp=10)
+    wl = WorkList()
+    for b in w["boxes"]:
+        wl.append_shell(b["op"], b["crse"], 0, b["fine"], 0, w["ncomp"], b["interior"])
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        wl.upload(s)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    assert wl.stats()["tiles"] >= 4 * 8 * cus
+    for b in w["boxes"]:
+        b["fine"].t.fill_(7.0)
+    with torch.cuda.stream(s):
+        wl.run_per_patch(s)
+    torch.cuda.synchronize()
+    ref = [b["fine"].t.clone() for b in w["boxes"]]
+    with torch.cuda.stream(s):
+        g = Graph(lambda: wl.run(s), s)
+    for launch in range(5):
+        with torch.cuda.stream(s):
+            wl.poison(stream=s)
+            if launch % 2:
+                g.launch(s)
+            else:
+                wl.run(s)
+        torch.cuda.synchronize()
+        for b, e in zip(w["boxes"], ref):
+            assert torch.equal(b["fine"].t, e), launch
+    g.close()
+    wl.close()
+
+
 def test_autotune_keeps_results_identical():
     w = small_uneven(nboxes=16, seed=7)
     wl = WorkList()
