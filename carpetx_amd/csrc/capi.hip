@@ -498,16 +498,15 @@ constexpr int kQueuePct = 25;
 // resident workgroup of its unit; the SIMDs issue the oldest ready wave
 // first, and the later slots run slower.  Measured on MI355X with 8 slots
 // per unit (the uneven 512-patch level, equal work per workgroup: mean
-// busy time by slot relative to slot 0); other slot counts use a linear
-// ramp of the same height.
+// busy time by slot relative to slot 0).  Other slot counts keep equal
+// weights: unmeasured, and a guessed ramp slowed the 10-slot lists of the
+// 64-box level.
 double slot_weight(int b, int cus, int grid) {
   static const double w8[8] = {1.000, 1.082, 1.085, 1.100,
                                1.204, 1.265, 1.251, 1.280};
-  const int nslots = std::max(1, grid / cus);
-  const int slot = std::min(b / cus, nslots - 1);
-  if (nslots == 8)
-    return w8[slot];
-  return 1.0 + 0.3 * slot / nslots;
+  if (grid != 8 * cus)
+    return 1.0;
+  return w8[std::min(b / cus, 7)];
 }
 
 // Orders the tile list into one contiguous range per workgroup of the
