@@ -152,6 +152,42 @@ def bench_level64(args, stream, log):
     return out
 
 
+def bench_level_fill(stream, log):
+    """Ghost fill of a whole fine level (64 boxes of 32^3, 3 ghosts) from a
+    coarse level of 2^3 FABs: the fine ghost points outside every fine valid
+    box, coarse patches gathered where a stencil spans coarse FABs."""
+    import torch
+    from carpetx_amd import workloads
+    from carpetx_amd.batched import Graph, WorkList
+    lv = workloads.level_fill_64()
+    out = {}
+    for name, d in lv.items():
+        wl = WorkList()
+        wl.append_level(d["op"], d["crse"], 0, d["fine"], 0, d["ncomp"])
+        with torch.cuda.stream(stream):
+            wl.upload(stream)
+            g = Graph(lambda: wl.run(stream), stream)
+            for _ in range(3):
+                g.launch(stream)
+            mb = back_to_back(lambda: g.launch(stream), 50, stream)
+            wl.run_per_patch(stream)
+            mp = back_to_back(lambda: wl.run_per_patch(stream), 10, stream)
+            wl.poison(stream=stream)
+            mf = back_to_back(lambda: wl.poison(stream=stream), 20, stream)
+        st = wl.stats()
+        out[name] = dict(
+            patches=st["patches"], fine_pts=st["fine_points"],
+            algo_MB=round(st["algo_bytes"] / 1e6, 2),
+            batched_ms=round(mb, 4), per_patch_ms=round(mp, 4),
+            store_floor_ms=round(mf, 4),
+            batched_pts_per_s=st["fine_points"] / (mb * 1e-3),
+            batched_hbm_frac=round(st["algo_bytes"] / (mb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        log("level fill %s: %s" % (name, out[name]))
+        g.close()
+        wl.close()
+    return out
+
+
 def bench_adjoint(stream, log):
     """96^3 -> 192^3 VC 5th-order prolongation: forward vs adjoint (per-box
     launches on one stream), plus the CCC minmod forward (its adjoint is not
@@ -339,6 +375,7 @@ def main():
             batched_over_per_box=round(res["per_box_ms"] / ms, 2))
     if not args.no_extra:
         line["extra"]["level_64"] = bench_level64(args, stream, log)
+        line["extra"]["level_fill_64"] = bench_level_fill(stream, log)
         line["extra"]["adjoint_96_192"] = bench_adjoint(stream, log)
         if not args.no_wave:
             line["extra"]["wave_two_level"] = bench_wave(log)

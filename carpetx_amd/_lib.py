@@ -32,6 +32,11 @@ class CxpFab(ctypes.Structure):
                 ("comp_stride", c_int64), ("ncomp", c_int)]
 
 
+class CxpLevel(ctypes.Structure):
+    _fields_ = [("nfabs", c_int), ("fabs", ctypes.POINTER(CxpFab)),
+                ("valid", P_INT), ("domain_lo", c_int * 3), ("domain_hi", c_int * 3)]
+
+
 class CxpError(RuntimeError):
     def __init__(self, func, status, msg):
         super().__init__("%s failed: %s (%d) %s" % (func, STATUS.get(status, "?"),
@@ -70,6 +75,9 @@ _SIGS = {
                                           ctypes.POINTER(CxpFab), c_int,
                                           ctypes.POINTER(CxpFab), c_int, c_int,
                                           P_INT, P_INT]),
+    "cxp_worklist_append_level": (c_int, [c_void_p, c_void_p,
+                                          ctypes.POINTER(CxpLevel), c_int,
+                                          ctypes.POINTER(CxpLevel), c_int, c_int]),
     "cxp_worklist_upload": (c_int, [c_void_p, c_void_p]),
     "cxp_worklist_stats": (c_int, [c_void_p, P_INT64, P_INT64, P_INT64, P_INT64]),
     "cxp_prolongate_batched": (c_int, [c_void_p, c_void_p]),

@@ -19,6 +19,30 @@ POISON_BITS = 0xfff80000deadbeef
 POISON = struct.unpack("<d", POISON_BITS.to_bytes(8, "little"))[0]
 
 
+class Level:
+    """A refinement level for ``WorkList.append_level``: FArrayBoxes (valid
+    region plus ghosts), the valid Box of each, and the level's domain (the
+    index range of its valid points, in the FABs' index type)."""
+
+    def __init__(self, fabs, valid, domain):
+        if len(fabs) != len(valid):
+            raise ValueError("one valid box per FAB")
+        self.fabs, self.valid, self.domain = list(fabs), list(valid), domain
+
+    def c(self):
+        n = len(self.fabs)
+        arr = (_lib.CxpFab * max(n, 1))(*[f.c() for f in self.fabs])
+        val = (ctypes.c_int * max(6 * n, 1))(
+            *[x for v in self.valid for x in tuple(v.lo) + tuple(v.hi)])
+        lv = _lib.CxpLevel()
+        lv.nfabs = n
+        lv.fabs = ctypes.cast(arr, ctypes.POINTER(_lib.CxpFab))
+        lv.valid = ctypes.cast(val, ctypes.POINTER(ctypes.c_int))
+        lv.domain_lo = i3(self.domain.lo)
+        lv.domain_hi = i3(self.domain.hi)
+        return lv, (arr, val)
+
+
 class WorkList:
     def __init__(self):
         self.h = ctypes.c_void_p()
@@ -50,6 +74,18 @@ class WorkList:
             self.h, op.handle, ctypes.byref(c), crse_comp, ctypes.byref(f),
             fine_comp, ncomp, i3(interior.lo), i3(interior.hi)))
         self._keep += [crse, fine]
+
+    def append_level(self, op, crse, crse_comp, fine, fine_comp, ncomp):
+        """Ghost fill of the fine Level from the coarse Level: every fine FAB
+        point in no fine valid box (FillPatch_ProlongateGhosts,
+        CarpetX/src/fillpatch.cxx:42-138); coarse patches that no single
+        coarse FAB covers are gathered on the device at each run."""
+        c, keep_c = crse.c()
+        f, keep_f = fine.c()
+        check("cxp_worklist_append_level", _lib.lib().cxp_worklist_append_level(
+            self.h, op.handle, ctypes.byref(c), crse_comp, ctypes.byref(f),
+            fine_comp, ncomp))
+        self._keep += list(crse.fabs) + list(fine.fabs)
 
     def upload(self, stream=None):
         check("cxp_worklist_upload", _lib.lib().cxp_worklist_upload(self.h, _stream(stream)))

@@ -430,6 +430,9 @@ struct cxp_worklist_s {
     cxp_fab crse, fine;
     int crse_comp, fine_comp, ncomp;
     int rlo[3], rhi[3];
+    // >= 0: crse is a gathered coarse patch at this offset (doubles) of the
+    // patch buffer, set at upload
+    int64_t gather_off = -1;
   };
   std::vector<Patch> patches;
   std::vector<SepTask> tasks;
@@ -443,6 +446,15 @@ struct cxp_worklist_s {
   std::vector<int> qstart;  // per-XCD-class queues after the ranges, or empty
   int *d_ctr = nullptr;     // their claim counters and the finish counter
   size_t cap_tasks = 0, cap_tiles = 0;
+  // coarse patches of level fills, gathered at the start of every run
+  std::vector<GatherRun> gather_runs;  // dst: offsets into d_gather
+  std::vector<GatherChunk> gather_chunks;
+  int64_t gather_doubles = 0;
+  double *d_gather = nullptr;
+  size_t cap_gather = 0;
+  GatherRun *d_gather_runs = nullptr;
+  GatherChunk *d_gather_chunks = nullptr;
+  size_t cap_gather_runs = 0, cap_gather_chunks = 0;
   std::vector<PoisonRun> poison_runs;  // one per (patch, component)
   std::vector<PoisonChunk> poison_chunks;
   PoisonRun *d_poison_runs = nullptr;
@@ -787,8 +799,45 @@ void build_poison(cxp_worklist_s *wl) {
   }
 }
 
+// Allocates the coarse-patch buffer of the list's level fills and points
+// their patches at it (before the tiles are built from the patches).
+int place_gathered(cxp_worklist_s *wl) {
+  if (wl->gather_doubles == 0)
+    return CXP_OK;
+  if ((size_t)wl->gather_doubles > wl->cap_gather) {
+    if (wl->d_gather)
+      (void)hipFree(wl->d_gather);
+    wl->d_gather = nullptr;
+    wl->cap_gather = (size_t)wl->gather_doubles;
+    if (hipMalloc(&wl->d_gather, wl->cap_gather * sizeof(double)) !=
+        hipSuccess) {
+      wl->cap_gather = 0;
+      return CXP_ERR_HIP;
+    }
+  }
+  for (auto &p : wl->patches)
+    if (p.gather_off >= 0)
+      p.crse.data = wl->d_gather + p.gather_off;
+  return CXP_OK;
+}
+
+int launch_gather(cxp_worklist_s *wl, hipStream_t s) {
+  if (wl->gather_chunks.empty())
+    return CXP_OK;
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)wl->gather_chunks.size()),
+                     dim3(256), 0, s, wl->d_gather_runs, wl->d_gather_chunks,
+                     wl->d_gather);
+  return hip_status(hipGetLastError());
+}
+
 int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
   build_poison(wl);
+  if (int e = upload_vec(wl->gather_runs, wl->d_gather_runs,
+                         wl->cap_gather_runs, s))
+    return e;
+  if (int e = upload_vec(wl->gather_chunks, wl->d_gather_chunks,
+                         wl->cap_gather_chunks, s))
+    return e;
   if (int e = upload_vec(wl->poison_runs, wl->d_poison_runs,
                          wl->cap_poison_runs, s))
     return e;
@@ -1098,6 +1147,12 @@ int cxp_worklist_destroy(cxp_worklist wl) {
     (void)hipFree(wl->d_wg_start);
   if (wl->d_ctr)
     (void)hipFree(wl->d_ctr);
+  if (wl->d_gather)
+    (void)hipFree(wl->d_gather);
+  if (wl->d_gather_runs)
+    (void)hipFree(wl->d_gather_runs);
+  if (wl->d_gather_chunks)
+    (void)hipFree(wl->d_gather_chunks);
   if (wl->d_poison_runs)
     (void)hipFree(wl->d_poison_runs);
   if (wl->d_poison_chunks)
@@ -1110,6 +1165,9 @@ int cxp_worklist_clear(cxp_worklist wl) {
   if (!wl)
     return CXP_ERR_ARG;
   wl->patches.clear();
+  wl->gather_runs.clear();
+  wl->gather_chunks.clear();
+  wl->gather_doubles = 0;
   wl->uploaded = false;
   wl->fine_points = wl->algo_bytes = 0;
   return CXP_OK;
@@ -1215,9 +1273,265 @@ int cxp_worklist_append_shell(cxp_worklist wl, cxp_op op, const cxp_fab *crse,
   return CXP_OK;
 }
 
+namespace {
+
+// Inclusive index box for the level fill's box algebra.
+struct IBox {
+  int lo[3], hi[3];
+  bool empty() const {
+    return hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2];
+  }
+  int64_t volume() const {
+    return empty() ? 0
+                   : int64_t(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) *
+                         (hi[2] - lo[2] + 1);
+  }
+};
+
+IBox ibox(const int lo[3], const int hi[3]) {
+  IBox b;
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = lo[d];
+    b.hi[d] = hi[d];
+  }
+  return b;
+}
+
+IBox intersect(const IBox &a, const IBox &b) {
+  IBox r;
+  for (int d = 0; d < 3; ++d) {
+    r.lo[d] = std::max(a.lo[d], b.lo[d]);
+    r.hi[d] = std::min(a.hi[d], b.hi[d]);
+  }
+  return r;
+}
+
+bool inside(const IBox &a, const IBox &b) {
+  return box_in(a.lo, a.hi, b.lo, b.hi);
+}
+
+// a minus b as disjoint boxes: z-low and z-high slabs over a's full x-y
+// extent, then y-low and y-high over the overlap's z range, then x-low and
+// x-high (the ghost-shell slab order of cxp_worklist_append_shell).
+void subtract(const IBox &a, const IBox &b, std::vector<IBox> &out) {
+  const IBox c = intersect(a, b);
+  if (c.empty()) {
+    if (!a.empty())
+      out.push_back(a);
+    return;
+  }
+  auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
+    IBox r{{x0, y0, z0}, {x1, y1, z1}};
+    if (!r.empty())
+      out.push_back(r);
+  };
+  add(a.lo[0], a.hi[0], a.lo[1], a.hi[1], a.lo[2], c.lo[2] - 1);
+  add(a.lo[0], a.hi[0], a.lo[1], a.hi[1], c.hi[2] + 1, a.hi[2]);
+  add(a.lo[0], a.hi[0], a.lo[1], c.lo[1] - 1, c.lo[2], c.hi[2]);
+  add(a.lo[0], a.hi[0], c.hi[1] + 1, a.hi[1], c.lo[2], c.hi[2]);
+  add(a.lo[0], c.lo[0] - 1, c.lo[1], c.hi[1], c.lo[2], c.hi[2]);
+  add(c.hi[0] + 1, a.hi[0], c.lo[1], c.hi[1], c.lo[2], c.hi[2]);
+}
+
+void subtract_all(std::vector<IBox> &pieces, const IBox &b) {
+  std::vector<IBox> next;
+  for (const IBox &p : pieces)
+    subtract(p, b, next);
+  pieces.swap(next);
+}
+
+bool level_ok(const cxp_level *L) {
+  if (!L || L->nfabs < 0 || (L->nfabs > 0 && (!L->fabs || !L->valid)))
+    return false;
+  for (int d = 0; d < 3; ++d)
+    if (L->domain_hi[d] < L->domain_lo[d])
+      return false;
+  for (int i = 0; i < L->nfabs; ++i) {
+    const int *v = L->valid + 6 * i;
+    if (!fab_ok(&L->fabs[i]) || !box_in(v, v + 3, L->fabs[i].lo, L->fabs[i].hi))
+      return false;
+    for (int d = 0; d < 3; ++d)
+      if (v[3 + d] < v[d])
+        return false;
+  }
+  return true;
+}
+
+// Distinct points of a union of boxes.
+int64_t union_points(const std::vector<IBox> &boxes) {
+  std::vector<IBox> acc;
+  int64_t n = 0;
+  for (const IBox &b : boxes) {
+    std::vector<IBox> fresh{b};
+    for (const IBox &a : acc)
+      subtract_all(fresh, a);
+    for (const IBox &f : fresh)
+      n += f.volume();
+    acc.push_back(b);
+  }
+  return n;
+}
+
+} // namespace
+
+int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
+                              const cxp_level *crse, int crse_comp,
+                              const cxp_level *fine, int fine_comp,
+                              int ncomp) {
+  if (!wl || !op || !level_ok(crse) || !level_ok(fine) || ncomp < 1)
+    return CXP_ERR_ARG;
+  for (const cxp_level *L : {crse, fine})
+    for (int i = 0; i < L->nfabs; ++i) {
+      const int c0 = L == crse ? crse_comp : fine_comp;
+      if (c0 < 0 || c0 + ncomp > L->fabs[i].ncomp)
+        return CXP_ERR_COMP;
+      if (fab_component_bytes(&L->fabs[i]) > kMaxComponentBytes)
+        return CXP_ERR_ARG;
+    }
+  // ghost width of the fine FABs and the grown fine domain (dest_domain,
+  // fillpatch.cxx:116-119)
+  int g[3] = {0, 0, 0};
+  for (int i = 0; i < fine->nfabs; ++i)
+    for (int d = 0; d < 3; ++d)
+      g[d] = std::max(g[d], std::max(fine->valid[6 * i + d] - fine->fabs[i].lo[d],
+                                     fine->fabs[i].hi[d] - fine->valid[6 * i + 3 + d]));
+  IBox fdom = ibox(fine->domain_lo, fine->domain_hi);
+  for (int d = 0; d < 3; ++d) {
+    fdom.lo[d] -= g[d];
+    fdom.hi[d] += g[d];
+  }
+  const IBox cdom = ibox(crse->domain_lo, crse->domain_hi);
+  auto cvalid = [&](int j) { return ibox(crse->valid + 6 * j, crse->valid + 6 * j + 3); };
+  auto cfabbox = [&](int j) { return ibox(crse->fabs[j].lo, crse->fabs[j].hi); };
+
+  std::vector<cxp_worklist_s::Patch> patches;
+  std::vector<GatherRun> runs;
+  std::vector<GatherChunk> chunks;
+  int64_t gather = wl->gather_doubles;
+  int64_t fine_pts = 0, coarse_pts = 0;
+  const int run0 = (int)wl->gather_runs.size();
+
+  // one copy of region X of coarse FAB j into the coarse patch C at offset
+  // off, every component
+  auto add_copy = [&](int j, const IBox &X, const IBox &C, int64_t off) {
+    const int64_t cvol = C.volume();
+    const int nx = X.hi[0] - X.lo[0] + 1, ny = X.hi[1] - X.lo[1] + 1;
+    const int nrows = ny * (X.hi[2] - X.lo[2] + 1);
+    const int per = std::max(1, 2048 / nx);
+    const int cx = C.hi[0] - C.lo[0] + 1, cy = C.hi[1] - C.lo[1] + 1;
+    for (int c = 0; c < ncomp; ++c) {
+      const DFab f = dfab(&crse->fabs[j], crse_comp + c);
+      GatherRun r;
+      r.src = f.p + didx(f, X.lo[0], X.lo[1], X.lo[2]);
+      r.src_sy = f.sy;
+      r.src_sz = f.sz;
+      r.dst = off + c * cvol + (X.lo[0] - C.lo[0]) +
+              (long long)(X.lo[1] - C.lo[1]) * cx +
+              (long long)(X.lo[2] - C.lo[2]) * cx * cy;
+      r.dst_sy = cx;
+      r.dst_sz = (long long)cx * cy;
+      r.nx = nx;
+      r.ny = ny;
+      const int run = run0 + (int)runs.size();
+      runs.push_back(r);
+      for (int row = 0; row < nrows; row += per)
+        chunks.push_back({run, row, std::min(per, nrows - row)});
+    }
+  };
+
+  for (int i = 0; i < fine->nfabs; ++i) {
+    const cxp_fab &ff = fine->fabs[i];
+    const IBox fv = ibox(fine->valid + 6 * i, fine->valid + 6 * i + 3);
+    std::vector<IBox> pieces;
+    subtract(intersect(ibox(ff.lo, ff.hi), fdom), fv, pieces);
+    for (int j = 0; j < fine->nfabs; ++j)
+      if (j != i)
+        subtract_all(pieces, ibox(fine->valid + 6 * j, fine->valid + 6 * j + 3));
+    std::vector<IBox> cboxes;
+    for (const IBox &P : pieces) {
+      IBox C;
+      op_coarse_box(op->d, P.lo, P.hi, C.lo, C.hi);
+      cboxes.push_back(C);
+      fine_pts += P.volume();
+      cxp_worklist_s::Patch p;
+      p.op = op;
+      p.fine = ff;
+      p.fine_comp = fine_comp;
+      p.ncomp = ncomp;
+      for (int d = 0; d < 3; ++d) {
+        p.rlo[d] = P.lo[d];
+        p.rhi[d] = P.hi[d];
+      }
+      // in place: one coarse FAB holds C, inside the coarse domain within
+      // its valid region
+      const IBox Cin = intersect(C, cdom);
+      int src = -1;
+      for (int j = 0; j < crse->nfabs && src < 0; ++j)
+        if (inside(C, cfabbox(j)) && (Cin.empty() || inside(Cin, cvalid(j))))
+          src = j;
+      if (src >= 0) {
+        p.crse = crse->fabs[src];
+        p.crse_comp = crse_comp;
+      } else {
+        // gathered: inside the domain from the valid regions, outside it
+        // from the first FAB holding each point
+        const int64_t off = gather;
+        std::vector<IBox> left;
+        if (!Cin.empty()) {
+          left.push_back(Cin);
+          for (int j = 0; j < crse->nfabs; ++j) {
+            const IBox X = intersect(Cin, cvalid(j));
+            if (!X.empty()) {
+              add_copy(j, X, C, off);
+              subtract_all(left, cvalid(j));
+            }
+          }
+          if (!left.empty())
+            return CXP_ERR_COARSE;
+        }
+        subtract(C, cdom, left);
+        for (int j = 0; j < crse->nfabs && !left.empty(); ++j) {
+          std::vector<IBox> rest;
+          for (const IBox &r : left) {
+            const IBox X = intersect(r, cfabbox(j));
+            if (!X.empty())
+              add_copy(j, X, C, off);
+            subtract(r, cfabbox(j), rest);
+          }
+          left.swap(rest);
+        }
+        if (!left.empty())
+          return CXP_ERR_COARSE;
+        p.crse.data = nullptr;  // placed at upload
+        for (int d = 0; d < 3; ++d) {
+          p.crse.lo[d] = C.lo[d];
+          p.crse.hi[d] = C.hi[d];
+        }
+        p.crse.comp_stride = C.volume();
+        p.crse.ncomp = ncomp;
+        p.crse_comp = 0;
+        p.gather_off = off;
+        gather += C.volume() * ncomp;
+      }
+      patches.push_back(p);
+    }
+    coarse_pts += union_points(cboxes);
+  }
+  wl->patches.insert(wl->patches.end(), patches.begin(), patches.end());
+  wl->gather_runs.insert(wl->gather_runs.end(), runs.begin(), runs.end());
+  wl->gather_chunks.insert(wl->gather_chunks.end(), chunks.begin(), chunks.end());
+  wl->gather_doubles = gather;
+  wl->uploaded = false;
+  wl->fine_points += fine_pts * ncomp;
+  wl->algo_bytes += 8 * (int64_t)ncomp * (fine_pts + coarse_pts);
+  return CXP_OK;
+}
+
 int cxp_worklist_upload(cxp_worklist wl, void *stream) {
   if (!wl)
     return CXP_ERR_ARG;
+  if (const int e = place_gathered(wl))
+    return e;
   build_tiles(wl);
   const int e = upload_lists(wl, (hipStream_t)stream);
   if (e)
@@ -1259,12 +1573,21 @@ int cxp_prolongate_batched(cxp_worklist wl, void *stream) {
     return CXP_ERR_ARG;
   if (!wl->uploaded)
     return CXP_ERR_STATE;
+  if (const int e = launch_gather(wl, (hipStream_t)stream))
+    return e;
   return launch_batched(wl, (hipStream_t)stream);
 }
 
 int cxp_prolongate_per_patch(cxp_worklist wl, void *stream) {
   if (!wl)
     return CXP_ERR_ARG;
+  if (wl->gather_doubles > 0) {
+    // gathered coarse patches live on the device only after an upload
+    if (!wl->uploaded)
+      return CXP_ERR_STATE;
+    if (const int e = launch_gather(wl, (hipStream_t)stream))
+      return e;
+  }
   for (const auto &p : wl->patches) {
     const int e = prolongate_impl(p.op, &p.crse, p.crse_comp, &p.fine,
                                   p.fine_comp, p.ncomp, p.rlo, p.rhi, 0,
@@ -1334,6 +1657,16 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
   if (!wl || reps < 1)
     return CXP_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
+  // gathered coarse patches must be in place before any fill runs
+  if (wl->gather_doubles > 0) {
+    if (int e = place_gathered(wl))
+      return e;
+    build_tiles(wl);
+    if (int e = upload_lists(wl, s))
+      return e;
+    if (int e = launch_gather(wl, s))
+      return e;
+  }
   // distinct keys in the list
   std::vector<TuneKey> keys;
   for (const auto &p : wl->patches) {

@@ -411,6 +411,45 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
+// Coarse-patch gather: copies of coarse-level regions into the coarse
+// patches a level fill reads when no single coarse FAB covers a patch's
+// CoarseBox (the reference's ParallelCopy into its coarse patch,
+// CarpetX/src/fillpatch.cxx:86-93).  A run is one (region, component): rows
+// of nx points from a source FAB to an offset in the list's patch buffer; a
+// chunk is about 2048 points of one run.
+
+struct GatherRun {
+  const double *src;  // the region's first point in the source component
+  long long dst;      // its offset (doubles) in the patch buffer
+  int src_sy;
+  long long src_sz;
+  int dst_sy;
+  long long dst_sz;
+  int nx, ny;
+};
+struct GatherChunk {
+  int run;
+  int row0, nrows;
+};
+
+__global__ void __launch_bounds__(256)
+    k_gather(const GatherRun *__restrict__ runs,
+             const GatherChunk *__restrict__ chunks, double *__restrict__ buf) {
+  const GatherChunk ch = chunks[blockIdx.x];
+  const GatherRun r = runs[ch.run];
+  const int npts = ch.nrows * r.nx;
+  for (int i = threadIdx.x; i < npts; i += 256) {
+    const int rr = i / r.nx;
+    const int x = i - rr * r.nx;
+    const int row = ch.row0 + rr;
+    const int z = row / r.ny;
+    const int y = row - z * r.ny;
+    const double v = gld(r.src, x + (long long)y * r.src_sy + z * r.src_sz);
+    gst(buf, r.dst + x + (long long)y * r.dst_sy + z * r.dst_sz, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Point kernels for ENO / fallback operators
 
 struct PointTask {

@@ -9,6 +9,11 @@
                    CCC conservative 2nd order and VVV 5th-order polynomial
                    prolongation of whole boxes; Gaussian pulse (amplitude 1,
                    width 0.1 L) plus 1e-3 uniform noise.
+* ``level_fill_64`` the same 128^3 level and 64 fine boxes as level_64, as
+                   two levels of FABs: the coarse level split into 2^3 FABs
+                   of 64^3 cells plus 6 ghost points (which hold the pulse,
+                   standing in for boundary values), the fine boxes with 3
+                   ghost points; the ghost fill of the whole fine level.
 * ``uneven_512``   512 fine boxes with per-dimension extents drawn from
                    {8, 16, ..., 64}, per-direction VC/CC centring and order
                    1/3/5 of the ``natural`` family; 10 grid functions of smooth
@@ -91,6 +96,46 @@ def level_64(seed=3, ncomp=4, nboxes=64, box_pts=32):
             fines.append(FArrayBox(region, ncomp))
             regions.append(region)
         out[name] = dict(op=op, crse=crse, fines=fines, regions=regions, ncomp=ncomp)
+    return out
+
+
+def level_fill_64(seed=3, ncomp=4, nboxes=64, box_pts=32, ghosts=3, cghosts=6):
+    """Returns {name: dict(op, crse, fine, ncomp)}, crse and fine Levels for
+    WorkList.append_level, for the two operators of level_64."""
+    from .batched import Level
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    nc = 128
+    slot = box_pts // 2
+    nslot = nc // slot
+    slots = rng.choice(nslot ** 3, size=nboxes, replace=False)
+    out = {}
+    for name, typ, order, it in [("cons2_ccc", "conservative", 1, (1, 1, 1)),
+                                 ("poly5_vvv", "interpolate", 5, (0, 0, 0))]:
+        op = get_interpolator(typ, order, it)
+        cdom = Box([0] * 3, [nc - it[d] for d in range(3)])
+        whole = _gaussian_level(cdom.grow(cghosts), it, ncomp, gen, nc)
+        cfabs, cvalid = [], []
+        for c in range(8):
+            lo = [0 if not (c >> d) & 1 else nc // 2 for d in range(3)]
+            hi = [nc // 2 - 1 if not (c >> d) & 1 else cdom.hi[d] for d in range(3)]
+            v = Box(lo, hi)
+            fab = FArrayBox(v.grow(cghosts), ncomp)
+            fab.t.copy_(whole.view(fab.box))
+            cfabs.append(fab)
+            cvalid.append(v)
+        del whole
+        ffabs, fvalid = [], []
+        for s in slots:
+            sk, sj, si = s // (nslot * nslot), (s // nslot) % nslot, s % nslot
+            lo = [2 * slot * si, 2 * slot * sj, 2 * slot * sk]
+            v = Box(lo, [lo[d] + box_pts - 1 for d in range(3)])
+            ffabs.append(FArrayBox(v.grow(ghosts), ncomp))
+            fvalid.append(v)
+        fdom = Box([0] * 3, [2 * nc - it[d] for d in range(3)])
+        out[name] = dict(op=op, crse=Level(cfabs, cvalid, cdom),
+                         fine=Level(ffabs, fvalid, fdom), ncomp=ncomp)
     return out
 
 

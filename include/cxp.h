@@ -132,6 +132,37 @@ int cxp_worklist_append_shell(cxp_worklist wl, cxp_op op, const cxp_fab *crse,
                               int crse_comp, const cxp_fab *fine,
                               int fine_comp, int ncomp, const int int_lo[3],
                               const int int_hi[3]);
+/* A refinement level: nfabs FABs, each covering its valid region plus
+ * ghost points, valid[6 i .. 6 i + 5] = valid lo[3], hi[3] of FAB i (valid
+ * regions do not overlap), and the level's domain, the index range of its
+ * valid points, in the FABs' index type.  Points outside the domain are
+ * physical-boundary ghost points. */
+typedef struct cxp_level {
+  int nfabs;
+  const cxp_fab *fabs;
+  const int *valid;
+  int domain_lo[3];
+  int domain_hi[3];
+} cxp_level;
+/* Adds the ghost fill of a whole fine level from the next coarser level:
+ * FillPatch_ProlongateGhosts (CarpetX/src/fillpatch.cxx:42-138) without its
+ * MPI copies.  Every point of every fine FAB that lies in no fine FAB's
+ * valid region (and within the fine domain grown by the FABs' ghost width)
+ * is prolongated; points in another FAB's valid region are left alone, as
+ * the reference leaves them to FillBoundary.  Coarse values come from the
+ * coarse valid regions only, and, for coarse points outside the coarse
+ * domain, from the coarse FABs' ghost points, which must hold the boundary
+ * values already (the reference applies the boundary conditions to its
+ * gathered coarse patch, fillpatch.cxx:107).  The fine patches are the
+ * box differences of each FAB's box and the valid regions, z-slabs first
+ * as in cxp_worklist_append_shell.  A patch whose CoarseBox lies in one
+ * coarse FAB's available points reads that FAB in place; the others read a
+ * coarse patch that the list gathers on the device at the start of every
+ * run (the reference's ParallelCopy into its coarse patch, :86-93).
+ * CXP_ERR_COARSE if a needed coarse point is in no FAB's available points. */
+int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
+                              const cxp_level *crse, int crse_comp,
+                              const cxp_level *fine, int fine_comp, int ncomp);
 /* Builds tiles (using the tuned tile shapes) and copies the list to the
  * device; must precede cxp_prolongate_batched. */
 int cxp_worklist_upload(cxp_worklist wl, void *stream);

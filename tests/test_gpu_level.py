@@ -1,0 +1,192 @@
+"""GPU level ghost fill (cxp_worklist_append_level) against the oracle.
+
+The fill follows FillPatch_ProlongateGhosts (CarpetX/src/fillpatch.cxx:
+42-138): every fine FAB point that lies in no fine valid region, inside the
+fine domain grown by the ghost width, is prolongated from the coarse level;
+coarse values inside the coarse domain come only from the coarse valid
+regions (the reference copies its coarse patch without coarse ghosts,
+:91-93), and outside it from the coarse FABs' ghost points, which hold the
+boundary values (the reference applies the boundary conditions to its
+coarse patch, :107).
+
+Expected values: the oracle prolongates each whole fine FAB box from one
+global coarse array (valid data inside the domain, boundary data outside),
+and the test keeps those values exactly where a fill must write and the
+original values elsewhere.  Coarse FAB ghost points inside the domain hold
+1e300, so reading them instead of a neighbour's valid data would show.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from carpetx_amd import _lib
+from carpetx_amd.batched import Graph, Level, WorkList
+from carpetx_amd.interp import Box, FArrayBox, get_interpolator
+
+pytestmark = pytest.mark.gpu
+
+N = 24        # coarse cells per direction
+CG = 6        # coarse FAB ghosts
+FG = 3        # fine FAB ghosts
+NC_CRSE = 3   # coarse components; the fill reads components 1..2
+NCOMP = 2
+FINE_VALID = [((0, 0, 0), (9, 9, 9)),          # reads one coarse FAB in place
+              ((10, 0, 0), (21, 9, 9)),        # beside the first: shared ghosts
+              ((16, 14, 12), (31, 29, 27)),    # straddles coarse FABs: gathered
+              ((36, 36, 36), (47, 47, 47))]    # upper corner: boundary ghosts
+
+
+def domains(it):
+    cdom = Box((0, 0, 0), tuple(N - it[d] for d in range(3)))
+    fdom = Box((0, 0, 0), tuple(2 * N - it[d] for d in range(3)))
+    return cdom, fdom
+
+
+def coarse_level(it, g, cg=CG):
+    """8 coarse FABs (valid split at 12) from the global array g, whose
+    box is the coarse domain grown by CG."""
+    cdom, _ = domains(it)
+    glo = tuple(cdom.lo[d] - CG for d in range(3))
+    fabs, valid = [], []
+    for c in range(8):
+        lo = tuple(0 if not (c >> d) & 1 else 12 for d in range(3))
+        hi = tuple(11 if not (c >> d) & 1 else cdom.hi[d] for d in range(3))
+        v = Box(lo, hi)
+        b = v.grow(cg)
+        sl = tuple(slice(b.lo[d] - glo[d], b.hi[d] - glo[d] + 1) for d in (2, 1, 0))
+        arr = g[(slice(None),) + sl].copy()
+        # ghost points inside the domain must not be read
+        idx = np.meshgrid(*[np.arange(b.lo[d], b.hi[d] + 1) for d in (2, 1, 0)], indexing="ij")
+        ind = np.ones(arr.shape[1:], dtype=bool)
+        inv = np.ones(arr.shape[1:], dtype=bool)
+        for a, d in zip(idx, (2, 1, 0)):
+            ind &= (a >= cdom.lo[d]) & (a <= cdom.hi[d])
+            inv &= (a >= v.lo[d]) & (a <= v.hi[d])
+        arr[:, ind & ~inv] = 1e300
+        fabs.append(FArrayBox(b, NC_CRSE, tensor=torch.from_numpy(arr).cuda()))
+        valid.append(v)
+    return Level(fabs, valid, cdom)
+
+
+def fine_level(it, rng):
+    _, fdom = domains(it)
+    fabs, valid = [], []
+    for lo, hi in FINE_VALID:
+        hi = tuple(min(hi[d], fdom.hi[d]) for d in range(3))
+        v = Box(lo, hi)
+        b = v.grow(FG)
+        t = torch.full((NCOMP,) + tuple(reversed(b.size)), 7.0, dtype=torch.float64)
+        inner = tuple(slice(FG, FG + v.size[d]) for d in (2, 1, 0))
+        t[(slice(None),) + inner] = torch.from_numpy(
+            rng.uniform(-1, 1, (NCOMP,) + tuple(reversed(v.size))))
+        fabs.append(FArrayBox(b, NCOMP, tensor=t.cuda()))
+        valid.append(v)
+    return Level(fabs, valid, fdom)
+
+
+def expected(op_args, it, g, fine):
+    cdom, fdom = domains(it)
+    oop = oracle.lookup(*op_args)
+    glo = tuple(cdom.lo[d] - CG for d in range(3))
+    ghi = tuple(cdom.hi[d] + CG for d in range(3))
+    ch = oracle.Fab(glo, ghi, NCOMP, np.ascontiguousarray(g[1:1 + NCOMP]))
+    out = []
+    for fab in fine.fabs:
+        b = fab.box
+        orig = fab.t.cpu().numpy()
+        fh = oracle.Fab(b.lo, b.hi, NCOMP, np.zeros_like(orig))
+        oracle.prolongate(oop, ch, fh, b.lo, b.hi)
+        idx = np.meshgrid(*[np.arange(b.lo[d], b.hi[d] + 1) for d in (2, 1, 0)], indexing="ij")
+        fill = np.ones(orig.shape[1:], dtype=bool)
+        for a, d in zip(idx, (2, 1, 0)):
+            fill &= (a >= fdom.lo[d] - FG) & (a <= fdom.hi[d] + FG)
+        for v in fine.valid:
+            inv = np.ones(orig.shape[1:], dtype=bool)
+            for a, d in zip(idx, (2, 1, 0)):
+                inv &= (a >= v.lo[d]) & (a <= v.hi[d])
+            fill &= ~inv
+        out.append(np.where(fill[None], fh.arr, orig))
+    return out
+
+
+OPS = [("natural", 5, (0, 0, 0)), ("natural", 3, (1, 1, 1)), ("ddf", 3, (0, 1, 0)),
+       ("minmod", 3, (1, 0, 1)), ("eno", 3, (1, 1, 1))]
+
+
+@pytest.mark.parametrize("op_args", OPS)
+def test_level_fill_matches_oracle(op_args):
+    """Batched run, per-patch run and two graph replays each equal the
+    oracle bit for bit; after the coarse data change, the next run reads the
+    new values (the gather runs at every launch)."""
+    it = op_args[2]
+    rng = np.random.default_rng(sum(it) + op_args[1])
+    cdom, _ = domains(it)
+    gshape = (NC_CRSE,) + tuple(cdom.size[d] + 2 * CG for d in (2, 1, 0))
+    g = rng.uniform(-1, 1, gshape)
+    crse = coarse_level(it, g)
+    fine = fine_level(it, rng)
+    originals = [f.t.clone() for f in fine.fabs]
+    want = expected(op_args, it, g, fine)
+    op = get_interpolator(*op_args)
+    wl = WorkList()
+    wl.append_level(op, crse, 1, fine, 0, NCOMP)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        wl.upload(s)
+        wl.run(s)
+    torch.cuda.synchronize()
+    for f, w in zip(fine.fabs, want):
+        assert np.array_equal(f.t.cpu().numpy(), w), op_args
+
+    def reset():
+        for f, o in zip(fine.fabs, originals):
+            f.t.copy_(o)
+        torch.cuda.synchronize()
+
+    reset()
+    with torch.cuda.stream(s):
+        wl.run_per_patch(s)
+    torch.cuda.synchronize()
+    for f, w in zip(fine.fabs, want):
+        assert np.array_equal(f.t.cpu().numpy(), w), ("per patch", op_args)
+    with torch.cuda.stream(s):
+        graph = Graph(lambda: wl.run(s), s)
+    for rep in range(2):
+        reset()
+        with torch.cuda.stream(s):
+            graph.launch(s)
+        torch.cuda.synchronize()
+        for f, w in zip(fine.fabs, want):
+            assert np.array_equal(f.t.cpu().numpy(), w), ("graph", rep, op_args)
+    # new coarse data: the next replay gathers it
+    g2 = rng.uniform(-1, 1, gshape)
+    crse2 = coarse_level(it, g2)
+    for a, b in zip(crse.fabs, crse2.fabs):
+        a.t.copy_(b.t)
+    reset()
+    want2 = expected(op_args, it, g2, fine)
+    with torch.cuda.stream(s):
+        graph.launch(s)
+    torch.cuda.synchronize()
+    for f, w in zip(fine.fabs, want2):
+        assert np.array_equal(f.t.cpu().numpy(), w), ("new coarse data", op_args)
+    graph.close()
+    wl.close()
+
+
+def test_level_fill_needs_coarse_boundary_points():
+    """Coarse FABs with one ghost point do not hold the boundary values that
+    the corner fine box needs: CXP_ERR_COARSE, and nothing is appended."""
+    it = (1, 1, 1)
+    rng = np.random.default_rng(3)
+    cdom, _ = domains(it)
+    g = rng.uniform(-1, 1, (NC_CRSE,) + tuple(cdom.size[d] + 2 * CG for d in (2, 1, 0)))
+    crse = coarse_level(it, g, cg=1)
+    fine = fine_level(it, rng)
+    wl = WorkList()
+    with pytest.raises(_lib.CxpError) as e:
+        wl.append_level(get_interpolator("natural", 3, it), crse, 1, fine, 0, NCOMP)
+    assert e.value.status == -5
+    assert wl.stats()["patches"] == 0
+    wl.close()

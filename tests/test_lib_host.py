@@ -145,6 +145,92 @@ def test_adjoint_argument_checks_before_device_work():
         assert st == want, (fam, st)
 
 
+def _level(boxes, domain, ncomp, ghosts):
+    """A cxp_level of FABs (valid box grown by ``ghosts``) with a placeholder
+    data pointer: appending to a work list reads only the boxes."""
+    n = len(boxes)
+    fabs = (_lib.CxpFab * n)()
+    for f, (lo, hi) in zip(fabs, boxes):
+        blo = tuple(lo[d] - ghosts for d in range(3))
+        bhi = tuple(hi[d] + ghosts for d in range(3))
+        f.data = ctypes.c_void_p(0x1000)
+        f.lo, f.hi = i3(blo), i3(bhi)
+        f.comp_stride = (bhi[0] - blo[0] + 1) * (bhi[1] - blo[1] + 1) * (bhi[2] - blo[2] + 1)
+        f.ncomp = ncomp
+    valid = (ctypes.c_int * (6 * n))(*[x for lo, hi in boxes for x in tuple(lo) + tuple(hi)])
+    lv = _lib.CxpLevel()
+    lv.nfabs = n
+    lv.fabs = ctypes.cast(fabs, ctypes.POINTER(_lib.CxpFab))
+    lv.valid = ctypes.cast(valid, ctypes.POINTER(ctypes.c_int))
+    lv.domain_lo, lv.domain_hi = i3(domain[0]), i3(domain[1])
+    return lv, (fabs, valid)
+
+
+def _fill_points(boxes, domain, ghosts):
+    """Points a level fill writes, counted point by point: each FAB's box
+    within the grown domain, minus every valid box."""
+    import numpy as np
+    total = 0
+    for lo, hi in boxes:
+        ax = [np.arange(max(lo[d] - ghosts, domain[0][d] - ghosts),
+                        min(hi[d] + ghosts, domain[1][d] + ghosts) + 1) for d in range(3)]
+        z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        keep = np.ones(x.shape, dtype=bool)
+        for vlo, vhi in boxes:
+            keep &= ~((x >= vlo[0]) & (x <= vhi[0]) & (y >= vlo[1]) & (y <= vhi[1]) &
+                      (z >= vlo[2]) & (z <= vhi[2]))
+        total += int(keep.sum())
+    return total
+
+
+def test_level_append_partitions_the_ghost_points():
+    """cxp_worklist_append_level on a coarse level of 8 FABs and fine boxes
+    that touch, straddle coarse FABs and reach the domain corner: the
+    patches cover exactly the points a point-by-point count says must be
+    filled (no overlap, none missing), for every centring; a coarse level
+    whose FABs lack the boundary points fails with CXP_ERR_COARSE and adds
+    nothing; bad component ranges fail with CXP_ERR_COMP."""
+    L = _lib.lib()
+    n = 24
+    fine_valid = [((0, 0, 0), (9, 9, 9)), ((10, 0, 0), (21, 9, 9)),
+                  ((16, 14, 12), (31, 29, 27)), ((36, 36, 36), (47, 47, 47)),
+                  ((22, 0, 0), (35, 13, 9))]
+    for it in [(0, 0, 0), (1, 1, 1), (0, 1, 0), (1, 0, 1)]:
+        cdom = ((0, 0, 0), tuple(n - it[d] for d in range(3)))
+        fdom = ((0, 0, 0), tuple(2 * n - it[d] for d in range(3)))
+        cboxes = []
+        for c in range(8):
+            cboxes.append((tuple(0 if not (c >> d) & 1 else 12 for d in range(3)),
+                           tuple(11 if not (c >> d) & 1 else cdom[1][d] for d in range(3))))
+        for fam, order in [("natural", 5), ("natural", 1), ("eno", 3)]:
+            h = _lookup(fam, order, it)[1]
+            crse, keep_c = _level(cboxes, cdom, 3, 6)
+            fine, keep_f = _level(fine_valid, fdom, 2, 3)
+            wl = ctypes.c_void_p()
+            assert L.cxp_worklist_create(ctypes.byref(wl)) == 0
+            assert L.cxp_worklist_append_level(wl, h, ctypes.byref(crse), 1,
+                                               ctypes.byref(fine), 0, 2) == 0
+            v = [ctypes.c_int64() for _ in range(4)]
+            assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+            assert v[2].value == 2 * _fill_points(fine_valid, fdom, 3), (it, fam)
+            assert v[0].value > 0
+            # bad component ranges
+            assert L.cxp_worklist_append_level(wl, h, ctypes.byref(crse), 2,
+                                               ctypes.byref(fine), 0, 2) == -9
+            assert L.cxp_worklist_append_level(wl, h, ctypes.byref(crse), 0,
+                                               ctypes.byref(fine), 1, 2) == -9
+            # coarse FABs with one ghost point: the corner box's coarse
+            # boundary points are missing
+            thin, keep_t = _level(cboxes, cdom, 3, 1)
+            npatch = v[0].value
+            assert L.cxp_worklist_append_level(wl, h, ctypes.byref(thin), 0,
+                                               ctypes.byref(fine), 0, 2) == -5
+            assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+            assert v[0].value == npatch
+            assert L.cxp_worklist_append_level(wl, h, None, 0, ctypes.byref(fine), 0, 2) == -3
+            assert L.cxp_worklist_destroy(wl) == 0
+
+
 def test_cpp_host_header_compiles():
     """The C++ host API (carpetx_amd/host/cxp.hpp) is self-contained C++17
     over the C header."""
