@@ -239,6 +239,11 @@ def bench_wave(log, n=128, steps=16, ck=4):
     from carpetx_amd.wave import TwoLevelWave, loss
     w = TwoLevelWave(n=n)
     sc0, sf0 = w.gaussian(width=0.05)
+    # warm-up: one step forward and backward (library and allocator set-up)
+    yc = sc0.clone().requires_grad_(True)
+    yf = sf0.clone().requires_grad_(True)
+    loss(*w.evolve(yc, yf, 1, checkpoint_every=1)).backward()
+    del yc, yf
     xc = sc0.clone().requires_grad_(True)
     xf = sf0.clone().requires_grad_(True)
     torch.cuda.synchronize()
@@ -250,10 +255,31 @@ def bench_wave(log, n=128, steps=16, ck=4):
     L.backward()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    # forward only, one CUDA graph per coarse step (every RK4 stage's ghost
+    # fills and the restriction replayed together)
+    advance, _, _ = w.graphed_step(sc0, sf0)
+    advance()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    for _ in range(steps):
+        advance()
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    with torch.no_grad():
+        c, f = sc0.clone(), sf0.clone()
+        w.step(c, f)
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        for _ in range(steps):
+            c, f = w.step(c, f)
+        torch.cuda.synchronize()
+        t6 = time.perf_counter()
     rec = dict(n=n, steps=steps, checkpoint_every=ck,
                fwd_ms_per_step=round(1e3 * (t1 - t0) / steps, 3),
                bwd_ms_per_step=round(1e3 * (t2 - t1) / steps, 3),
-               grad_norm=float(xc.grad.norm()), loss=float(L))
+               fwd_nograd_ms_per_step=round(1e3 * (t6 - t5) / steps, 3),
+               fwd_graph_ms_per_step=round(1e3 * (t4 - t3) / steps, 3),
+               grad_norm=float(xc.grad.norm()), loss=float(L.detach()))
     log("wave: %s" % rec)
     return rec
 

@@ -19,7 +19,9 @@ uses -1/2 for the +2 neighbour; this module does not reproduce that.)
 
 Orchestration mirrors an RK4 ODESolvers step with a SYNC after every
 substep (ODESolvers/src/solve.cxx:888-916); subcycling itself is our own
-(the reference's is disabled, CarpetX/param.ccl:460-462).
+(the reference's is disabled, CarpetX/param.ccl:460-462).  Without autograd,
+``graphed_step`` captures a whole coarse step, with every substep's ghost
+fills, as one CUDA graph and replays it.
 
 Every prolongation and restriction runs in libcxp.so and is differentiable
 through the hand-written adjoint kernels, so the gradient of a loss with
@@ -147,6 +149,31 @@ class TwoLevelWave:
         c1 = c1.clone()
         c1[:, cu.lo[2]:cu.hi[2] + 1, cu.lo[1]:cu.hi[1] + 1, cu.lo[0]:cu.hi[0] + 1] = inj
         return c1, sf
+
+    def graphed_step(self, sc, sf):
+        """One coarse step (coarse RK4, two fine RK4 steps with a ghost fill
+        at every stage, restriction) captured as one CUDA graph, forward
+        only.  Returns (advance, c, f): each advance() replays the graph and
+        moves the state in c and f on by one step.  The capture goes on
+        torch's capturing stream, which the transfer calls launch on."""
+        c = sc.detach().clone()
+        f = sf.detach().clone()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.no_grad(), torch.cuda.stream(side):
+            for _ in range(2):  # allocator and library set-up outside capture
+                self.step(c.clone(), f.clone())
+        torch.cuda.current_stream().wait_stream(side)
+        graph = torch.cuda.CUDAGraph()
+        with torch.no_grad(), torch.cuda.graph(graph):
+            out_c, out_f = self.step(c, f)
+
+        def advance():
+            graph.replay()
+            c.copy_(out_c)
+            f.copy_(out_f)
+        advance.graph = graph
+        return advance, c, f
 
     def evolve(self, sc, sf, nsteps, checkpoint_every=0):
         if checkpoint_every <= 0:

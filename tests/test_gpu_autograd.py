@@ -99,3 +99,23 @@ def test_wave_energy_bounded():
         sc, sf = w.evolve(sc, sf, 4)
     assert torch.isfinite(sc).all() and torch.isfinite(sf).all()
     assert float((sc[0] ** 2).sum()) < 2 * e0
+
+
+def test_wave_graphed_step_replays_match_eager():
+    """A whole coarse step captured as one CUDA graph (every RK4 stage's ghost
+    fill and the restriction inside it), replayed step after step with eager
+    work and synchronisations in between, reproduces the eager evolution bit
+    for bit at every step."""
+    from carpetx_amd.wave import TwoLevelWave
+    w = TwoLevelWave(n=32)
+    sc, sf = w.gaussian(width=0.06)
+    advance, c, f = w.graphed_step(sc, sf)
+    ec, ef = sc.clone(), sf.clone()
+    for step in range(4):
+        with torch.no_grad():
+            ec, ef = w.step(ec, ef)
+        torch.cuda.synchronize()
+        advance()
+        torch.cuda.synchronize()
+        assert torch.equal(c, ec), step
+        assert torch.equal(f, ef), step
