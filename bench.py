@@ -33,7 +33,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--tune", action="store_true", help="run the autotuner first")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tile shapes (skip the in-context autotuner)")
     ap.add_argument("--cpu-boxes", type=int, default=512)
     ap.add_argument("--no-wave", action="store_true")
     ap.add_argument("--no-per-box", action="store_true", help="skip timing the per-box path")
@@ -80,8 +81,16 @@ def bench_uneven(args, stream, log):
         wl.append_shell(b["op"], b["crse"], 0, b["fine"], 0, wlz["ncomp"], b["interior"])
     with torch.cuda.stream(stream):
         wl.upload(stream)
-        if args.tune:
+    tune_s = None
+    if not args.no_tune:
+        # tile shapes and z-chunks from the autotuner, timed in context
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        with torch.cuda.stream(stream):
             wl.autotune(reps=3, stream=stream)
+        torch.cuda.synchronize()
+        tune_s = time.perf_counter() - t1
+        log("autotuned in %.1f s" % tune_s)
     st = wl.stats()
     log("uneven_512: %d patches, %d tiles, %.3e fine pts, %.3f GB algorithmic (setup %.1fs)"
         % (st["patches"], st["tiles"], st["fine_points"], st["algo_bytes"] / 1e9,
@@ -97,7 +106,7 @@ def bench_uneven(args, stream, log):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t_start
     ms = sum(times) / len(times)
-    res = dict(ms=ms, wall_ms=1e3 * wall / args.steps, stats=st)
+    res = dict(ms=ms, wall_ms=1e3 * wall / args.steps, stats=st, tune_s=tune_s)
 
     # the list's write pattern alone: CarpetX's poisoning of the same points
     # (one store per point, no reads)
@@ -380,6 +389,8 @@ def main():
         "config": {
             "workload": "uneven_512: ghost-shell fill (3 pts) of 512 boxes, extents "
                         "{8..64}^3, mixed VC/CC, natural o1/3/5, 10 GF, one hipGraph",
+            "tile_shapes": ("autotuned in context (%.1f s)" % res["tune_s"]
+                            if res["tune_s"] is not None else "defaults"),
             "patches": st["patches"], "tiles": st["tiles"],
             "fine_points": st["fine_points"], "algo_bytes": st["algo_bytes"],
         },

@@ -1653,75 +1653,114 @@ int cxp_tile_config(int cfg, int *tx, int *ty, int *zwindow) {
   return CXP_OK;
 }
 
+// In-context tuning.  A tile shape and z-chunk that are fastest for one
+// class run alone need not be fastest inside the whole list, where the
+// persistent kernel overlaps every class's tiles; so every candidate is
+// timed on the whole list.  Size classes are tuned one after another
+// (coordinate descent): each class takes, for all its stencil triples, the
+// (shape, z-chunk) that gives the shortest whole-list time, if that beats
+// the best so far by more than kTuneGain; otherwise it keeps its setting.
+// Passes over the classes repeat until one changes nothing.
+constexpr double kTuneGain = 0.005;
+constexpr int kTunePasses = 3;
+
 int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
   if (!wl || reps < 1)
     return CXP_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  // gathered coarse patches must be in place before any fill runs
-  if (wl->gather_doubles > 0) {
-    if (int e = place_gathered(wl))
-      return e;
-    build_tiles(wl);
-    if (int e = upload_lists(wl, s))
-      return e;
-    if (int e = launch_gather(wl, s))
-      return e;
-  }
-  // distinct keys in the list
-  std::vector<TuneKey> keys;
+  if (int e = place_gathered(wl))
+    return e;
+  // separable, non-thin keys of the list, grouped by size class
+  std::map<int, std::vector<TuneKey>> classes;
   for (const auto &p : wl->patches) {
     if (!op_is_separable(p.op->d, false))
       continue;
     const SepTask t = make_sep_task(p.op->d, &p.crse, p.crse_comp, &p.fine,
                                     p.fine_comp, p.rlo, p.rhi);
+    if (thin_eligible(t))
+      continue;
     const TuneKey k = tune_key(t);
-    if (std::find(keys.begin(), keys.end(), k) == keys.end())
-      keys.push_back(k);
+    auto &ks = classes[std::get<3>(k)];
+    if (std::find(ks.begin(), ks.end(), k) == ks.end())
+      ks.push_back(k);
   }
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return CXP_ERR_HIP;
-  const int zchunks[4] = {8, 16, 32, 64};
   int status = CXP_OK;
-  for (const TuneKey &k : keys) {
-    float best = 1e30f;
-    int best_cfg = -1, best_zc = 16;
-    for (int cfg = 0; cfg < kNumTileConfigs && !status; ++cfg)
-      for (int zi = 0; zi < 4 && !status; ++zi) {
-        const int zc = zchunks[zi];
-        build_tiles(wl, (cfg << 8) | zc, &k);
-        bool fits = !wl->tasks.empty();
-        for (const SepTask &t : wl->tasks)
-          fits = fits && cfg_fits(cfg, t);
-        if (!fits)
-          continue;
-        if ((status = upload_lists(wl, s)))
-          break;
-        float tot = 0;
-        for (int r = 0; r <= reps; ++r) {
-          (void)hipEventRecord(e0, s);
-          if ((status = launch_sep_list(wl, s)))
-            break;
-          (void)hipEventRecord(e1, s);
-          if (hipEventSynchronize(e1) != hipSuccess) {
-            status = CXP_ERR_HIP;
-            break;
-          }
-          float ms = 0;
-          (void)hipEventElapsedTime(&ms, e0, e1);
-          if (r > 0)
-            tot += ms;  // first launch is a warm-up
-        }
-        if (!status && tot < best) {
-          best = tot;
-          best_cfg = cfg;
-          best_zc = zc;
-        }
-      }
-    if (best_cfg >= 0) {
-      std::lock_guard<std::mutex> lk(g_tune_mu);
-      g_tune[k] = {best_cfg, best_zc};
+  // median of reps whole-list launches (after one warm-up) with the table
+  // as it stands
+  auto time_list = [&](float &ms) {
+    build_tiles(wl);
+    if ((status = upload_lists(wl, s)))
+      return false;
+    if (wl->tiles.empty()) {
+      ms = 0;
+      return true;
     }
+    std::vector<float> t;
+    for (int r = 0; r <= reps; ++r) {
+      (void)hipEventRecord(e0, s);
+      if ((status = launch_sep_list(wl, s)))
+        return false;
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        status = CXP_ERR_HIP;
+        return false;
+      }
+      float x = 0;
+      (void)hipEventElapsedTime(&x, e0, e1);
+      if (r > 0)
+        t.push_back(x);
+    }
+    std::sort(t.begin(), t.end());
+    ms = t[t.size() / 2];
+    return true;
+  };
+  auto set_class = [&](const std::vector<TuneKey> &ks, int cfg, int zc) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const TuneKey &k : ks) {
+      if (cfg < 0)
+        g_tune.erase(k);
+      else
+        g_tune[k] = {cfg, zc};
+    }
+  };
+  // start from the defaults for the list's classes
+  for (const auto &cl : classes)
+    set_class(cl.second, -1, 0);
+  float best = 0;
+  if (!time_list(best))
+    classes.clear();
+  const int zchunks[4] = {0, 16, 32, 64};  // 0: the list-wide default
+  std::map<int, std::pair<int, int>> choice;  // class -> (cfg or -1, zchunk)
+  for (const auto &cl : classes)
+    choice[cl.first] = {-1, 0};
+  // passes until one changes nothing (at most kTunePasses)
+  for (int pass = 0; pass < kTunePasses && !status; ++pass) {
+    bool changed = false;
+    for (const auto &cl : classes) {
+      std::pair<int, int> &ch = choice[cl.first];
+      for (int cfg = 0; cfg < kNumTileConfigs && !status; ++cfg)
+        for (int zi = 0; zi < 4 && !status; ++zi) {
+          if (cfg == ch.first && zchunks[zi] == ch.second)
+            continue;
+          set_class(cl.second, cfg, zchunks[zi]);
+          float ms = 0;
+          if (!time_list(ms))
+            break;
+          if (ms < best * (1.0 - kTuneGain)) {
+            best = ms;
+            ch = {cfg, zchunks[zi]};
+            changed = true;
+          }
+        }
+      set_class(cl.second, ch.first, ch.second);
+      if (status)
+        break;
+    }
+    if (!changed)
+      break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

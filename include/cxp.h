@@ -188,9 +188,13 @@ int cxp_graph_capture_end(void *stream, cxp_graph *out);
 int cxp_graph_launch(cxp_graph g, void *stream);
 int cxp_graph_destroy(cxp_graph g);
 
-/* ---- Tile-shape autotuner.  Times every tile shape on the patches of each
- * (operator, size class) of the list and keeps the fastest; the table is
- * process-wide and can be saved/loaded as text. */
+/* ---- Tile-shape autotuner.  Tunes the list's separable size classes one
+ * after another: every tile shape x z-chunk of a class is timed on the whole
+ * list (the persistent kernel overlaps all classes), and a class keeps the
+ * fastest only if it beats the best so far by more than 0.5%; passes repeat
+ * (at most 3) until one changes nothing.  The table is
+ * process-wide, keyed by (stencil triple, size class), and can be
+ * saved/loaded as text.  Uploads the list again when done. */
 int cxp_autotune(cxp_worklist wl, int reps, void *stream);
 int cxp_tune_save(const char *path);
 int cxp_tune_load(const char *path);
