@@ -169,7 +169,9 @@ int cxp_worklist_upload(cxp_worklist wl, void *stream);
 int cxp_worklist_stats(cxp_worklist wl, int64_t *npatches, int64_t *ntiles,
                        int64_t *fine_points, int64_t *algo_bytes);
 /* One launch of the persistent kernel over the whole list (plus one
- * point-kernel launch per ENO/fallback patch). */
+ * point-kernel launch per ENO/fallback patch, and first the gather of a
+ * level fill's coarse patches).  Launches of one list must not overlap: the
+ * list's dynamic-tail claim counters and gathered patches are its own. */
 int cxp_prolongate_batched(cxp_worklist wl, void *stream);
 /* The plain path over the same list: one cxp_prolongate launch per patch. */
 int cxp_prolongate_per_patch(cxp_worklist wl, void *stream);
