@@ -201,7 +201,8 @@ def bench_adjoint(stream, log):
     """96^3 -> 192^3 VC 5th-order prolongation: forward vs adjoint (per-box
     launches on one stream), plus the CCC minmod forward (its adjoint is not
     available) and the CCC ENO forward and adjoint (linearised at the
-    forward input)."""
+    forward input: recomputing the branches, and from the forward's branch
+    record)."""
     import torch
     from carpetx_amd.interp import Box, FArrayBox, get_interpolator
     out = {}
@@ -235,6 +236,20 @@ def bench_adjoint(stream, log):
                        adj_hbm_frac=round(algo / (ms_a * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
         except Exception as e:  # operators without a device adjoint
             rec.update(adj_ms=None, adj_note=str(e)[:120])
+        nb = P.branch_bytes(region, 1)
+        if nb > 0 and rec.get("adj_ms") is not None:
+            # the forward records each fine point's branch, the adjoint is
+            # the gather alone (what the autograd wrapper runs)
+            br = torch.empty(nb, dtype=torch.uint8, device="cuda")
+            with torch.cuda.stream(stream):
+                P.interp_record(crse, 0, fine, 0, 1, region, br, stream=stream)
+                ms_fr = back_to_back(lambda: P.interp_record(
+                    crse, 0, fine, 0, 1, region, br, stream=stream), 20, stream)
+                P.interp_adjoint_recorded(br, cbar, 0, fine, 0, 1, region, stream=stream)
+                ms_ar = back_to_back(lambda: P.interp_adjoint_recorded(
+                    br, cbar, 0, fine, 0, 1, region, stream=stream), 20, stream)
+            rec.update(fwd_record_ms=round(ms_fr, 4), adj_recorded_ms=round(ms_ar, 4),
+                       adj_recorded_over_fwd_record=round(ms_ar / ms_fr, 3))
         out[name] = rec
         log("adjoint %s: %s" % (name, rec))
     return out

@@ -62,6 +62,14 @@ _SIGS = {
                                        ctypes.POINTER(CxpFab), c_int,
                                        ctypes.POINTER(CxpFab), c_int, c_int,
                                        P_INT, P_INT, c_int, c_void_p]),
+    "cxp_branch_bytes": (c_int, [c_void_p, P_INT, P_INT, c_int, c_int, P_INT64]),
+    "cxp_prolongate_record": (c_int, [c_void_p, ctypes.POINTER(CxpFab), c_int,
+                                      ctypes.POINTER(CxpFab), c_int, c_int, P_INT,
+                                      P_INT, c_int, c_void_p, c_void_p]),
+    "cxp_prolongate_adjoint_recorded": (c_int, [c_void_p, c_void_p,
+                                                ctypes.POINTER(CxpFab), c_int,
+                                                ctypes.POINTER(CxpFab), c_int, c_int,
+                                                P_INT, P_INT, c_int, c_void_p]),
     "cxp_restrict_adjoint": (c_int, [P_INT, ctypes.POINTER(CxpFab), c_int,
                                      ctypes.POINTER(CxpFab), c_int, c_int, P_INT,
                                      P_INT, c_void_p]),

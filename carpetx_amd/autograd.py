@@ -19,26 +19,36 @@ def _fab(t, box):
 
 
 class _Prolongate(torch.autograd.Function):
+    """Operators with branches (ENO shifts, linear fallback) record them in
+    the forward, so the backward is the adjoint's gather alone."""
+
     @staticmethod
     def forward(ctx, crse, P, crse_box, fine_region):
         crse = crse.contiguous()
         ncomp = crse.shape[0]
         fine = torch.empty((ncomp,) + tuple(reversed(fine_region.size)),
                            dtype=torch.float64, device=crse.device)
-        P.interp(_fab(crse, crse_box), 0, _fab(fine, fine_region), 0, ncomp,
-                 fine_region)
+        nb = P.branch_bytes(fine_region, ncomp)
+        branches = None
+        if nb > 0:
+            branches = torch.empty(nb, dtype=torch.uint8, device=crse.device)
+            P.interp_record(_fab(crse, crse_box), 0, _fab(fine, fine_region), 0,
+                            ncomp, fine_region, branches)
+        else:
+            P.interp(_fab(crse, crse_box), 0, _fab(fine, fine_region), 0, ncomp,
+                     fine_region)
         ctx.P, ctx.crse_box, ctx.fine_region = P, crse_box, fine_region
-        ctx.save_for_backward(crse)
+        ctx.shape = crse.shape
+        ctx.branches = branches
         return fine
 
     @staticmethod
     def backward(ctx, gfine):
-        (crse,) = ctx.saved_tensors
         gfine = gfine.contiguous()
-        gcrse = torch.zeros_like(crse)
-        ctx.P.interp_adjoint(_fab(gcrse, ctx.crse_box), 0,
-                             _fab(gfine, ctx.fine_region), 0, crse.shape[0],
-                             ctx.fine_region, crse_in=_fab(crse, ctx.crse_box))
+        gcrse = torch.zeros(ctx.shape, dtype=torch.float64, device=gfine.device)
+        ctx.P.interp_adjoint_recorded(ctx.branches, _fab(gcrse, ctx.crse_box), 0,
+                                      _fab(gfine, ctx.fine_region), 0, ctx.shape[0],
+                                      ctx.fine_region)
         return gcrse, None, None, None
 
 

@@ -431,6 +431,111 @@ __global__ void __launch_bounds__(256)
   gst(cb.p, ci, gld(cb.p, ci) + sum);
 }
 
+// The same sums, a coarse tile per workgroup.  The branch codes of the fine
+// points any of the tile's outputs can read are staged in LDS, and the
+// weights come from a table indexed by branch code: T[code][d][off][o + 4]
+// is the 1-D weight in direction d of a fine point of that code and fine
+// offset off for the coarse point o away from its parent (code
+// CXP_ADJ_NOCODE, outside the region, weighs 0).  Candidates are visited in
+// the same order and weights multiplied in the same order as in
+// k_adj_gather, so the sums are the same to the bit.
+#define CXP_ADJ_TX 8
+#define CXP_ADJ_TY 8
+#define CXP_ADJ_TZ 4
+#define CXP_ADJ_NOCODE 54
+#define CXP_ADJ_TROW 54  // 3 directions x 2 offsets x 9 offsets o
+
+// Fine window of a coarse tile [c0, c1] in direction d.
+__host__ __device__ inline int adj_window(int c0, int c1, int rlo_reach,
+                                          int rhi_reach, int flo, int fhi,
+                                          int &w0, int &w1) {
+  w0 = max(flo, 2 * (c0 - rhi_reach));
+  w1 = min(fhi, 2 * (c1 - rlo_reach) + 1);
+  return w1 - w0 + 1;
+}
+
+__global__ void __launch_bounds__(256)
+    k_adj_gather_tiled(AdjPointTask t, const unsigned char *__restrict__ codes,
+                       int ntx, int nty) {
+  extern __shared__ double gsm[];
+  double *T = gsm;
+  unsigned char *cs = (unsigned char *)(gsm + (CXP_ADJ_NOCODE + 1) * CXP_ADJ_TROW);
+  for (int e = threadIdx.x; e < (CXP_ADJ_NOCODE + 1) * CXP_ADJ_TROW;
+       e += blockDim.x) {
+    const int code = e / CXP_ADJ_TROW, r = e % CXP_ADJ_TROW;
+    const int d = r / 18, off = (r / 9) % 2, o = r % 9 - 4;
+    double v = 0.0;
+    if (code < CXP_ADJ_NOCODE) {
+      const int br = code >= 27 ? 1 : 0;
+      const int pw = d == 0 ? 1 : d == 1 ? 3 : 9;
+      const int sh = ((code % 27) / pw) % 3 - 1;
+      const int sid = br ? t.sl[d] : t.pt.sid[d];
+      const int shift = (sid == SID_ENO2 && !br) ? sh : 0;
+      int lo, hi;
+      stencil_radius(sid, shift, off, lo, hi);
+      if (o >= lo && o <= hi)
+        v = apply1d(sid, DeltaAcc{o}, shift, off);
+    }
+    T[e] = v;
+  }
+  const int b = blockIdx.x;
+  const int tv[3] = {b % ntx, (b / ntx) % nty, b / (ntx * nty)};
+  const int tsz[3] = {CXP_ADJ_TX, CXP_ADJ_TY, CXP_ADJ_TZ};
+  int c0[3], c1[3], w0[3], w1[3], wn[3];
+  for (int d = 0; d < 3; ++d) {
+    c0[d] = t.olo[d] + tsz[d] * tv[d];
+    c1[d] = min(c0[d] + tsz[d] - 1, t.ohi[d]);
+    wn[d] = adj_window(c0[d], c1[d], t.rlo_reach[d], t.rhi_reach[d],
+                       t.pt.rlo[d], t.pt.rhi[d], w0[d], w1[d]);
+  }
+  const long long comp = blockIdx.y;
+  const long long npts = pt_npts(t.pt);
+  const int rnx = t.pt.rhi[0] - t.pt.rlo[0] + 1;
+  const int rny = t.pt.rhi[1] - t.pt.rlo[1] + 1;
+  const unsigned char *cd = codes + (t.per_group ? 0 : comp * npts);
+  const bool any = wn[0] > 0 && wn[1] > 0 && wn[2] > 0;
+  const int wvol = any ? wn[0] * wn[1] * wn[2] : 0;
+  for (int e = threadIdx.x; e < wvol; e += blockDim.x) {
+    const int x = e % wn[0], y = (e / wn[0]) % wn[1], z = e / (wn[0] * wn[1]);
+    cs[e] = cd[(w0[0] + x - t.pt.rlo[0]) +
+               (long long)(w0[1] + y - t.pt.rlo[1]) * rnx +
+               (long long)(w0[2] + z - t.pt.rlo[2]) * rnx * rny];
+  }
+  __syncthreads();
+  const int c[3] = {c0[0] + (int)threadIdx.x % CXP_ADJ_TX,
+                    c0[1] + ((int)threadIdx.x / CXP_ADJ_TX) % CXP_ADJ_TY,
+                    c0[2] + (int)threadIdx.x / (CXP_ADJ_TX * CXP_ADJ_TY)};
+  if (c[0] > c1[0] || c[1] > c1[1] || c[2] > c1[2])
+    return;
+  int flo[3], fhi[3];
+  for (int d = 0; d < 3; ++d) {
+    flo[d] = max(w0[d], 2 * (c[d] - t.rhi_reach[d]));
+    fhi[d] = min(w1[d], 2 * (c[d] - t.rlo_reach[d]) + 1);
+  }
+  DFab fb = t.fbar;
+  fb.p += comp * t.fcs;
+  double sum = 0.0;
+  for (int fz = flo[2]; fz <= fhi[2]; ++fz) {
+    const int rz = 36 + (fz & 1) * 9 + c[2] - (fz >> 1) + 4;
+    for (int fy = flo[1]; fy <= fhi[1]; ++fy) {
+      const int ry = 18 + (fy & 1) * 9 + c[1] - (fy >> 1) + 4;
+      const unsigned char *crow =
+          cs + ((fz - w0[2]) * wn[1] + (fy - w0[1])) * wn[0];
+      for (int fx = flo[0]; fx <= fhi[0]; ++fx) {
+        const double *tc = T + crow[fx - w0[0]] * CXP_ADJ_TROW;
+        const double wt =
+            tc[(fx & 1) * 9 + c[0] - (fx >> 1) + 4] * tc[ry] * tc[rz];
+        if (wt != 0.0)
+          sum += wt * gld(fb.p, didx(fb, fx, fy, fz));
+      }
+    }
+  }
+  DFab cb = t.cbar;
+  cb.p += comp * t.ccs;
+  const long long ci = didx(cb, c[0], c[1], c[2]);
+  gst(cb.p, ci, gld(cb.p, ci) + sum);
+}
+
 // --------------------------------------------------------------------------
 
 inline DFab adj_dfab(const cxp_fab *f, int comp) {

@@ -243,8 +243,8 @@ PointTask make_point_task(const OpDesc &op, const cxp_fab *crse, int crse_comp,
   return t;
 }
 
-int launch_point(const PointTask &t, int ncomp, int per_group,
-                 hipStream_t s) {
+int launch_point(const PointTask &t, int ncomp, int per_group, hipStream_t s,
+                 unsigned char *codes = nullptr) {
   const long long n = (long long)(t.rhi[0] - t.rlo[0] + 1) *
                       (t.rhi[1] - t.rlo[1] + 1) * (t.rhi[2] - t.rlo[2] + 1);
   // LDS-tiled kernels when a tile's coarse block fits in 48 KB (always in
@@ -259,17 +259,18 @@ int launch_point(const PointTask &t, int ncomp, int per_group,
     const dim3 grid((unsigned)(nt[0] * nt[1] * nt[2]), per_group ? 1 : ncomp);
     if (per_group)
       hipLaunchKernelGGL(k_point_tiled<true>, grid, dim3(256), lds, s, t,
-                         nt[0], nt[1]);
+                         nt[0], nt[1], codes);
     else
       hipLaunchKernelGGL(k_point_tiled<false>, grid, dim3(256), lds, s, t,
-                         nt[0], nt[1]);
+                         nt[0], nt[1], codes);
     return hip_status(hipGetLastError());
   }
   const unsigned nb = (unsigned)((n + 255) / 256);
   if (per_group)
-    hipLaunchKernelGGL(k_point_per_group, dim3(nb), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(k_point_per_group, dim3(nb), dim3(256), 0, s, t, codes);
   else
-    hipLaunchKernelGGL(k_point_per_var, dim3(nb, ncomp), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(k_point_per_var, dim3(nb, ncomp), dim3(256), 0, s, t,
+                       codes);
   return hip_status(hipGetLastError());
 }
 
@@ -334,18 +335,14 @@ int prolongate_impl(const cxp_op_s *op, const cxp_fab *crse, int crse_comp,
   return launch_point(t, ncomp, per_group, s);
 }
 
-// Adjoint of an ENO or linear-fallback operator, linearised at crse_in:
-// record every fine point's branch, then gather per coarse point.
-int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
-                         const cxp_fab *crse_bar, int crse_comp,
-                         const cxp_fab *fine_bar, int fine_comp, int ncomp,
-                         const int rlo[3], const int rhi[3], int per_group,
-                         hipStream_t s) {
-  if (!crse_in)
-    return CXP_ERR_ARG;
-  if (const int e = check_prolong(op, crse_in, crse_comp, fine_bar, fine_comp,
-                                  ncomp, rlo, rhi, per_group))
-    return e;
+// The gather of an ENO or linear-fallback adjoint: regions, branch
+// stencils and the coarse points any branch reaches.  crse_in only sets the
+// point task's coarse geometry here.
+AdjPointTask make_adj_point_task(const cxp_op_s *op, const cxp_fab *crse_in,
+                                 const cxp_fab *crse_bar, int crse_comp,
+                                 const cxp_fab *fine_bar, int fine_comp,
+                                 int ncomp, const int rlo[3], const int rhi[3],
+                                 int per_group) {
   const OpDesc &od = op->d;
   AdjPointTask t;
   t.pt = make_point_task(od, crse_in, crse_comp, fine_bar, fine_comp, ncomp,
@@ -379,11 +376,55 @@ int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
     t.olo[d] = std::max(clo[d], (rlo[d] >> 1) + lo);
     t.ohi[d] = std::min(chi[d], (rhi[d] >> 1) + hi);
   }
-  long long npts = 1, nout = 1;
+  return t;
+}
+
+// crse_bar += P^T fine_bar over the recorded branches: one thread per coarse
+// output point and component, coarse tiles with their codes in LDS when the
+// window fits.
+int launch_adj_gather(const AdjPointTask &t, const unsigned char *codes,
+                      int ncomp, hipStream_t s) {
+  long long nout = 1;
+  int ntl[3], wmax = 1;
+  const int tsz[3] = {CXP_ADJ_TX, CXP_ADJ_TY, CXP_ADJ_TZ};
   for (int d = 0; d < 3; ++d) {
-    npts *= rhi[d] - rlo[d] + 1;
     nout *= t.ohi[d] - t.olo[d] + 1;
+    ntl[d] = (t.ohi[d] - t.olo[d] + tsz[d]) / tsz[d];
+    wmax *= 2 * (tsz[d] + t.rhi_reach[d] - t.rlo_reach[d]);
   }
+  if (nout <= 0)
+    return CXP_OK;
+  const size_t glds =
+      8 * (size_t)(CXP_ADJ_NOCODE + 1) * CXP_ADJ_TROW + (size_t)wmax;
+  if (glds <= 64 * 1024)
+    hipLaunchKernelGGL(k_adj_gather_tiled,
+                       dim3((unsigned)(ntl[0] * ntl[1] * ntl[2]), ncomp),
+                       dim3(256), glds, s, t, codes, ntl[0], ntl[1]);
+  else
+    hipLaunchKernelGGL(k_adj_gather,
+                       dim3((unsigned)((nout + 255) / 256), ncomp), dim3(256),
+                       0, s, t, codes);
+  return hip_status(hipGetLastError());
+}
+
+// Adjoint of an ENO or linear-fallback operator, linearised at crse_in:
+// record every fine point's branch, then gather per coarse point.
+int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
+                         const cxp_fab *crse_bar, int crse_comp,
+                         const cxp_fab *fine_bar, int fine_comp, int ncomp,
+                         const int rlo[3], const int rhi[3], int per_group,
+                         hipStream_t s) {
+  if (!crse_in)
+    return CXP_ERR_ARG;
+  if (const int e = check_prolong(op, crse_in, crse_comp, fine_bar, fine_comp,
+                                  ncomp, rlo, rhi, per_group))
+    return e;
+  const AdjPointTask t =
+      make_adj_point_task(op, crse_in, crse_bar, crse_comp, fine_bar,
+                          fine_comp, ncomp, rlo, rhi, per_group);
+  long long npts = 1;
+  for (int d = 0; d < 3; ++d)
+    npts *= rhi[d] - rlo[d] + 1;
   unsigned char *codes = nullptr;
   const size_t nbytes = (size_t)npts * (per_group ? 1 : ncomp);
   if (hipMallocAsync((void **)&codes, nbytes, s) != hipSuccess)
@@ -409,12 +450,8 @@ int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
                        dim3(256), 0, s, t.pt, codes, t.per_group);
   }
   int st = hip_status(hipGetLastError());
-  if (st == CXP_OK) {
-    hipLaunchKernelGGL(k_adj_gather,
-                       dim3((unsigned)((nout + 255) / 256), ncomp), dim3(256),
-                       0, s, t, (const unsigned char *)codes);
-    st = hip_status(hipGetLastError());
-  }
+  if (st == CXP_OK)
+    st = launch_adj_gather(t, codes, ncomp, s);
   if (hipFreeAsync(codes, s) != hipSuccess && st == CXP_OK)
     st = CXP_ERR_HIP;
   return st;
@@ -1125,6 +1162,74 @@ int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
   return adjoint_launch(od, crse_in, crse_bar, crse_comp, fine_bar,
                         fine_comp, ncomp, rlo, rhi, per_group,
                         (hipStream_t)stream);
+}
+
+// Operators whose forward takes a branch per fine point (ENO shift, linear
+// fallback) and has a device adjoint; MINMOD is excluded (no adjoint).
+bool op_records_branches(const OpDesc &od, int per_group) {
+  for (int d = 0; d < 3; ++d)
+    if (od.sid[d] == SID_MINMOD)
+      return false;
+  return !op_is_separable(od, per_group != 0);
+}
+
+int cxp_branch_bytes(cxp_op op, const int rlo[3], const int rhi[3], int ncomp,
+                     int per_group, int64_t *bytes) {
+  if (!op || !rlo || !rhi || !bytes || ncomp < 1)
+    return CXP_ERR_ARG;
+  int64_t n = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (rhi[d] < rlo[d])
+      return CXP_ERR_ARG;
+    n *= rhi[d] - rlo[d] + 1;
+  }
+  *bytes = op_records_branches(op->d, per_group) ? n * (per_group ? 1 : ncomp)
+                                                 : 0;
+  return CXP_OK;
+}
+
+int cxp_prolongate_record(cxp_op op, const cxp_fab *crse, int crse_comp,
+                          const cxp_fab *fine, int fine_comp, int ncomp,
+                          const int rlo[3], const int rhi[3], int per_group,
+                          unsigned char *branches, void *stream) {
+  const int e = check_prolong(op, crse, crse_comp, fine, fine_comp, ncomp, rlo,
+                              rhi, per_group);
+  if (e)
+    return e;
+  if (!op_records_branches(op->d, per_group))
+    return prolongate_impl(op, crse, crse_comp, fine, fine_comp, ncomp, rlo,
+                           rhi, per_group, (hipStream_t)stream);
+  if (!branches)
+    return CXP_ERR_ARG;
+  const PointTask t = make_point_task(op->d, crse, crse_comp, fine, fine_comp,
+                                      ncomp, rlo, rhi);
+  return launch_point(t, ncomp, per_group, (hipStream_t)stream, branches);
+}
+
+int cxp_prolongate_adjoint_recorded(cxp_op op, const unsigned char *branches,
+                                    const cxp_fab *crse_bar, int crse_comp,
+                                    const cxp_fab *fine_bar, int fine_comp,
+                                    int ncomp, const int rlo[3],
+                                    const int rhi[3], int per_group,
+                                    void *stream) {
+  const int e = check_prolong(op, crse_bar, crse_comp, fine_bar, fine_comp,
+                              ncomp, rlo, rhi, per_group);
+  if (e)
+    return e;
+  const OpDesc &od = op->d;
+  for (int d = 0; d < 3; ++d)
+    if (od.sid[d] == SID_MINMOD)
+      return CXP_ERR_UNSUPPORTED;
+  if (!op_records_branches(od, per_group))
+    return adjoint_launch(od, nullptr, crse_bar, crse_comp, fine_bar,
+                          fine_comp, ncomp, rlo, rhi, per_group,
+                          (hipStream_t)stream);
+  if (!branches)
+    return CXP_ERR_ARG;
+  const AdjPointTask t =
+      make_adj_point_task(op, crse_bar, crse_bar, crse_comp, fine_bar,
+                          fine_comp, ncomp, rlo, rhi, per_group);
+  return launch_adj_gather(t, branches, ncomp, (hipStream_t)stream);
 }
 
 // ---- worklists

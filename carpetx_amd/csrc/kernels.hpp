@@ -499,80 +499,92 @@ __device__ __forceinline__ double undiv2(double m, double c, double p) {
 }
 
 // ENO shift search (:1417-1517 per-var; per-group takes the max over comps).
+// Score of one shift candidate (si, sj, sk): the penalised largest
+// undivided difference over the transverse stencil lines of every ENO
+// direction (prolongate_3d_rf2_impl.hxx:1424-1503).
+__device__ inline double pt_eno_dd(const PointTask &t, long long comp0,
+                                   int ncomp, const int off[3],
+                                   const int ic[3], int si, int sj, int sk) {
+  int use[3];
+  for (int d = 0; d < 3; ++d)
+    use[d] = t.sid[d] == SID_ENO2;
+  const int sv[3] = {si, sj, sk};
+  int r[3][2];
+  for (int d = 0; d < 3; ++d) {
+    stencil_radius(t.sid[d], sv[d], off[d], r[d][0], r[d][1]);
+    if (use[d]) {
+      r[d][0] -= sv[d];
+      r[d][1] -= sv[d];
+    }
+  }
+  double ddx = 0, ddy = 0, ddz = 0;
+  if (t.d2) {
+    const DFab &c = t.crse;
+    const double *dx2 = t.d2, *dy2 = t.d2 + t.ccs, *dz2 = t.d2 + 2 * t.ccs;
+    if (use[0])
+      for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
+        for (int dj = r[1][0]; dj <= r[1][1]; ++dj)
+          ddx = fmax(ddx, dx2[didx(c, ic[0] + si, ic[1] + sj + dj,
+                                   ic[2] + sk + dk)]);
+    if (use[1])
+      for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
+        for (int di = r[0][0]; di <= r[0][1]; ++di)
+          ddy = fmax(ddy, dy2[didx(c, ic[0] + si + di, ic[1] + sj,
+                                   ic[2] + sk + dk)]);
+    if (use[2])
+      for (int dj = r[1][0]; dj <= r[1][1]; ++dj)
+        for (int di = r[0][0]; di <= r[0][1]; ++di)
+          ddz = fmax(ddz, dz2[didx(c, ic[0] + si + di, ic[1] + sj + dj,
+                                   ic[2] + sk)]);
+  }
+  for (int cc = 0; cc < (t.d2 ? 0 : ncomp); ++cc) {
+    DFab c = t.crse;
+    c.p += (comp0 + cc) * t.ccs;
+    if (use[0])
+      for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
+        for (int dj = r[1][0]; dj <= r[1][1]; ++dj) {
+          const int i = ic[0] + si, j = ic[1] + sj + dj, k = ic[2] + sk + dk;
+          ddx = fmax(ddx, fabs(undiv2(crse_at(c, i - 1, j, k),
+                                      crse_at(c, i, j, k),
+                                      crse_at(c, i + 1, j, k))));
+        }
+    if (use[1])
+      for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
+        for (int di = r[0][0]; di <= r[0][1]; ++di) {
+          const int i = ic[0] + si + di, j = ic[1] + sj, k = ic[2] + sk + dk;
+          ddy = fmax(ddy, fabs(undiv2(crse_at(c, i, j - 1, k),
+                                      crse_at(c, i, j, k),
+                                      crse_at(c, i, j + 1, k))));
+        }
+    if (use[2])
+      for (int dj = r[1][0]; dj <= r[1][1]; ++dj)
+        for (int di = r[0][0]; di <= r[0][1]; ++di) {
+          const int i = ic[0] + si + di, j = ic[1] + sj + dj, k = ic[2] + sk;
+          ddz = fmax(ddz, fabs(undiv2(crse_at(c, i, j, k - 1),
+                                      crse_at(c, i, j, k),
+                                      crse_at(c, i, j, k + 1))));
+        }
+  }
+  // 1 + sqrt(DBL_EPSILON) * (|si| + |sj| + |sk|); sqrt(eps) = 2^-26
+  const double penalty = 1.0 + 0x1p-26 * (double)(abs(si) + abs(sj) + abs(sk));
+  return penalty * fmax(fmax(ddx, ddy), ddz);
+}
+
+// ENO shift search: candidates in the reference's lexicographic order (sk
+// outermost, si innermost), the first strictly smallest score wins
+// (:1418-1517).
 __device__ inline void pt_eno_search(const PointTask &t, long long comp0,
                                      int ncomp, const int off[3],
                                      const int ic[3], int shift[3]) {
-  int use[3], ms[3];
-  for (int d = 0; d < 3; ++d) {
-    use[d] = t.sid[d] == SID_ENO2;
-    ms[d] = use[d] ? 1 : 0;
-  }
+  int ms[3];
+  for (int d = 0; d < 3; ++d)
+    ms[d] = t.sid[d] == SID_ENO2 ? 1 : 0;
   shift[0] = shift[1] = shift[2] = 0;
   double min_dd = __builtin_inf();
   for (int sk = -ms[2]; sk <= ms[2]; ++sk)
     for (int sj = -ms[1]; sj <= ms[1]; ++sj)
       for (int si = -ms[0]; si <= ms[0]; ++si) {
-        const int sv[3] = {si, sj, sk};
-        int r[3][2];
-        for (int d = 0; d < 3; ++d) {
-          stencil_radius(t.sid[d], sv[d], off[d], r[d][0], r[d][1]);
-          if (use[d]) {
-            r[d][0] -= sv[d];
-            r[d][1] -= sv[d];
-          }
-        }
-        double ddx = 0, ddy = 0, ddz = 0;
-        if (t.d2) {
-          const DFab &c = t.crse;
-          const double *dx2 = t.d2, *dy2 = t.d2 + t.ccs, *dz2 = t.d2 + 2 * t.ccs;
-          if (use[0])
-            for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
-              for (int dj = r[1][0]; dj <= r[1][1]; ++dj)
-                ddx = fmax(ddx, dx2[didx(c, ic[0] + si, ic[1] + sj + dj,
-                                         ic[2] + sk + dk)]);
-          if (use[1])
-            for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
-              for (int di = r[0][0]; di <= r[0][1]; ++di)
-                ddy = fmax(ddy, dy2[didx(c, ic[0] + si + di, ic[1] + sj,
-                                         ic[2] + sk + dk)]);
-          if (use[2])
-            for (int dj = r[1][0]; dj <= r[1][1]; ++dj)
-              for (int di = r[0][0]; di <= r[0][1]; ++di)
-                ddz = fmax(ddz, dz2[didx(c, ic[0] + si + di, ic[1] + sj + dj,
-                                         ic[2] + sk)]);
-        }
-        for (int cc = 0; cc < (t.d2 ? 0 : ncomp); ++cc) {
-          DFab c = t.crse;
-          c.p += (comp0 + cc) * t.ccs;
-          if (use[0])
-            for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
-              for (int dj = r[1][0]; dj <= r[1][1]; ++dj) {
-                const int i = ic[0] + si, j = ic[1] + sj + dj, k = ic[2] + sk + dk;
-                ddx = fmax(ddx, fabs(undiv2(crse_at(c, i - 1, j, k),
-                                            crse_at(c, i, j, k),
-                                            crse_at(c, i + 1, j, k))));
-              }
-          if (use[1])
-            for (int dk = r[2][0]; dk <= r[2][1]; ++dk)
-              for (int di = r[0][0]; di <= r[0][1]; ++di) {
-                const int i = ic[0] + si + di, j = ic[1] + sj, k = ic[2] + sk + dk;
-                ddy = fmax(ddy, fabs(undiv2(crse_at(c, i, j - 1, k),
-                                            crse_at(c, i, j, k),
-                                            crse_at(c, i, j + 1, k))));
-              }
-          if (use[2])
-            for (int dj = r[1][0]; dj <= r[1][1]; ++dj)
-              for (int di = r[0][0]; di <= r[0][1]; ++di) {
-                const int i = ic[0] + si + di, j = ic[1] + sj + dj, k = ic[2] + sk;
-                ddz = fmax(ddz, fabs(undiv2(crse_at(c, i, j, k - 1),
-                                            crse_at(c, i, j, k),
-                                            crse_at(c, i, j, k + 1))));
-              }
-        }
-        // 1 + sqrt(DBL_EPSILON) * (|si| + |sj| + |sk|); sqrt(eps) = 2^-26
-        const double penalty =
-            1.0 + 0x1p-26 * (double)(abs(si) + abs(sj) + abs(sk));
-        const double dd = penalty * fmax(fmax(ddx, ddy), ddz);
+        const double dd = pt_eno_dd(t, comp0, ncomp, off, ic, si, sj, sk);
         if (dd < min_dd) {
           min_dd = dd;
           shift[0] = si;
@@ -625,15 +637,20 @@ __device__ inline bool pt_slope_change(const DFab &c, int d, const int sr[3][2],
 // shift and, for fallback operators, whether the linear value replaces the
 // high-order one (prolongate_3d_rf2_impl.hxx:1417-1667).  val is the
 // high-order value.
+// given: the ENO shift, already searched (or null: search here).
 __device__ inline void pt_decide_var(const PointTask &t, long long comp,
                                      const int off[3], const int ic[3],
-                                     int shift[3], double &val, bool &need) {
+                                     int shift[3], double &val, bool &need,
+                                     const int *given = nullptr) {
   DFab c = t.crse;
   c.p += comp * t.ccs;
   shift[0] = shift[1] = shift[2] = 0;
   const bool any_shift =
       t.sid[0] == SID_ENO2 || t.sid[1] == SID_ENO2 || t.sid[2] == SID_ENO2;
-  if (any_shift)
+  if (given)
+    for (int d = 0; d < 3; ++d)
+      shift[d] = given[d];
+  else if (any_shift)
     pt_eno_search(t, comp, 1, off, ic, shift);
   val = pt_tensor(c, t.sid, shift, off, ic);
   need = false;
@@ -657,12 +674,16 @@ __device__ inline void pt_decide_var(const PointTask &t, long long comp,
 // high-order values.
 __device__ inline void pt_decide_group(const PointTask &t, const int off[3],
                                        const int ic[3], int shift[3],
-                                       double (&vals)[16], bool &need) {
+                                       double (&vals)[16], bool &need,
+                                       const int *given = nullptr) {
   const int nc = t.ncomp;
   shift[0] = shift[1] = shift[2] = 0;
   const bool any_shift =
       t.sid[0] == SID_ENO2 || t.sid[1] == SID_ENO2 || t.sid[2] == SID_ENO2;
-  if (any_shift)
+  if (given)
+    for (int d = 0; d < 3; ++d)
+      shift[d] = given[d];
+  else if (any_shift)
     pt_eno_search(t, 0, nc, off, ic, shift);
   for (int cc = 0; cc < nc; ++cc) {
     DFab c = t.crse;
@@ -705,12 +726,23 @@ __device__ __forceinline__ void pt_point(const PointTask &t, long long gid,
   }
 }
 
+// Branch code of a fine point (for the adjoint): (sx+1) + 3 (sy+1) +
+// 9 (sz+1) + 27 need.
+__device__ __forceinline__ unsigned char adj_code(const int shift[3],
+                                                  bool need) {
+  return (unsigned char)((shift[0] + 1) + 3 * (shift[1] + 1) +
+                         9 * (shift[2] + 1) + (need ? 27 : 0));
+}
+
 __device__ __forceinline__ long long pt_npts(const PointTask &t) {
   return (long long)(t.rhi[0] - t.rlo[0] + 1) * (t.rhi[1] - t.rlo[1] + 1) *
          (t.rhi[2] - t.rlo[2] + 1);
 }
 
-__global__ void __launch_bounds__(256) k_point_per_var(PointTask t) {
+// codes (or null): also record each fine point's branch (adj_code), per
+// component, region-linear.
+__global__ void __launch_bounds__(256)
+    k_point_per_var(PointTask t, unsigned char *__restrict__ codes = nullptr) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= pt_npts(t))
     return;
@@ -720,6 +752,8 @@ __global__ void __launch_bounds__(256) k_point_per_var(PointTask t) {
   double val;
   bool need;
   pt_decide_var(t, comp, off, ic, shift, val, need);
+  if (codes)
+    codes[comp * pt_npts(t) + gid] = adj_code(shift, need);
   double res = val;
   if (need) {
     DFab c = t.crse;
@@ -735,7 +769,8 @@ __global__ void __launch_bounds__(256) k_point_per_var(PointTask t) {
   fo.p[didx(fo, f[0], f[1], f[2])] = res;
 }
 
-__global__ void __launch_bounds__(256) k_point_per_group(PointTask t) {
+__global__ void __launch_bounds__(256)
+    k_point_per_group(PointTask t, unsigned char *__restrict__ codes = nullptr) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= pt_npts(t))
     return;
@@ -744,6 +779,8 @@ __global__ void __launch_bounds__(256) k_point_per_group(PointTask t) {
   double vals[16];
   bool need;
   pt_decide_group(t, off, ic, shift, vals, need);
+  if (codes)
+    codes[gid] = adj_code(shift, need);
   int sl[3];
   for (int d = 0; d < 3; ++d)
     sl[d] = sid_linear_per_group(t.sid[d]);
@@ -789,16 +826,26 @@ __host__ __device__ inline int pt_tile_block(const int sid[3]) {
   return v;
 }
 
-// Branch code of a fine point (for the adjoint): (sx+1) + 3 (sy+1) +
-// 9 (sz+1) + 27 need.
-__device__ __forceinline__ unsigned char adj_code(const int shift[3],
-                                                  bool need) {
-  return (unsigned char)((shift[0] + 1) + 3 * (shift[1] + 1) +
-                         9 * (shift[2] + 1) + (need ? 27 : 0));
+// The ENO shift search of a fine point reads the coarse parent, the shift
+// candidates and, per direction, the stencil radius at the point's fine
+// offset (prolongate_3d_rf2_impl.hxx:1418-1517).  In a direction whose
+// radius does not depend on the offset (every ENO direction: the support
+// moves with the shift only), the two fine children of a coarse point run
+// the same search.
+__host__ __device__ inline bool radius_depends_on_off(int sid) {
+  int a0, a1, b0, b1;
+  stencil_radius(sid, 0, 0, a0, a1);
+  stencil_radius(sid, 0, 1, b0, b1);
+  return a0 != b0 || a1 != b1;
 }
+// Search units of a tile: (coarse parent, fine offset in the directions
+// whose radius depends on it), at most (CXP_PT_TILE / 2 + 1)^3 * 8.
+#define CXP_PT_UNITS 1000
+
 
 // CODES: record each fine point's branch in codes (per component in per-var
-// mode, region-linear, x fastest) instead of writing its value.
+// mode, region-linear, x fastest) instead of writing its value; without
+// CODES, a non-null codes records the branches beside the values.
 template <bool PG, bool CODES = false>
 __global__ void __launch_bounds__(256)
     k_point_tiled(PointTask t, int ntx, int nty,
@@ -854,14 +901,84 @@ __global__ void __launch_bounds__(256)
   tl.crse.sz = (long long)n[0] * n[1];
   tl.ccs = vol;
   tl.d2 = d2;
+  // one search per unit (the same code, so the same shift as each of the
+  // unit's fine points would find), kept in LDS for the fine points
+  __shared__ unsigned char ushift[CXP_PT_UNITS];
+  const bool any_shift =
+      t.sid[0] == SID_ENO2 || t.sid[1] == SID_ENO2 || t.sid[2] == SID_ENO2;
+  int pc0[3], dep[3], nu[3];
+  for (int d = 0; d < 3; ++d) {
+    pc0[d] = f0[d] >> 1;
+    dep[d] = radius_depends_on_off(t.sid[d]);
+    nu[d] = ((f1[d] >> 1) - pc0[d] + 1) * (dep[d] ? 2 : 1);
+  }
+  if (any_shift) {
+    // a group of 32 lanes per unit, a lane per candidate: lane c scores
+    // candidate c (lexicographic, si fastest), and the group keeps the
+    // smallest valid score, the smallest index on ties -- the candidate the
+    // reference's scan (strict <, from an infinite minimum) ends with; no
+    // valid score leaves the zero shift
+    int ms[3];
+    for (int d = 0; d < 3; ++d)
+      ms[d] = t.sid[d] == SID_ENO2 ? 1 : 0;
+    const int nw[3] = {2 * ms[0] + 1, 2 * ms[1] + 1, 2 * ms[2] + 1};
+    const int ncand = nw[0] * nw[1] * nw[2];
+    const int nunits = nu[0] * nu[1] * nu[2];
+    for (int w = threadIdx.x; w < nunits * 32; w += blockDim.x) {
+      const int u = w >> 5, c = w & 31;
+      const int uv[3] = {u % nu[0], (u / nu[0]) % nu[1], u / (nu[0] * nu[1])};
+      int ic[3], off[3];
+      for (int d = 0; d < 3; ++d) {
+        ic[d] = pc0[d] + (dep[d] ? uv[d] >> 1 : uv[d]);
+        off[d] = dep[d] ? uv[d] & 1 : 0;
+      }
+      double dd = 0.0;
+      int valid = 0;
+      if (c < ncand) {
+        dd = pt_eno_dd(tl, 0, PG ? t.ncomp : 1, off, ic, c % nw[0] - ms[0],
+                       (c / nw[0]) % nw[1] - ms[1], c / (nw[0] * nw[1]) - ms[2]);
+        valid = dd < __builtin_inf();  // false for inf and NaN
+      }
+      int idx = c;
+      for (int m = 1; m < 32; m <<= 1) {
+        const double od = __shfl_xor(dd, m, 32);
+        const int oi = __shfl_xor(idx, m, 32);
+        const int ov = __shfl_xor(valid, m, 32);
+        if (ov && (!valid || od < dd || (od == dd && oi < idx))) {
+          dd = od;
+          idx = oi;
+          valid = ov;
+        }
+      }
+      if (c == 0) {
+        int sh[3] = {0, 0, 0};
+        if (valid) {
+          sh[0] = idx % nw[0] - ms[0];
+          sh[1] = (idx / nw[0]) % nw[1] - ms[1];
+          sh[2] = idx / (nw[0] * nw[1]) - ms[2];
+        }
+        ushift[u] = (unsigned char)((sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1));
+      }
+    }
+    __syncthreads();
+  }
   const int m[3] = {f1[0] - f0[0] + 1, f1[1] - f0[1] + 1, f1[2] - f0[2] + 1};
   for (int p = threadIdx.x; p < m[0] * m[1] * m[2]; p += blockDim.x) {
     const int f[3] = {f0[0] + p % m[0], f0[1] + (p / m[0]) % m[1],
                       f0[2] + p / (m[0] * m[1])};
-    int ic[3], off[3], shift[3];
+    int ic[3], off[3], shift[3], given[3] = {0, 0, 0};
     for (int d = 0; d < 3; ++d) {
       ic[d] = f[d] >> 1;
       off[d] = f[d] & 1;
+    }
+    if (any_shift) {
+      int uv[3];
+      for (int d = 0; d < 3; ++d)
+        uv[d] = dep[d] ? 2 * (ic[d] - pc0[d]) + off[d] : ic[d] - pc0[d];
+      const int code = ushift[uv[0] + nu[0] * (uv[1] + nu[1] * uv[2])];
+      given[0] = code % 3 - 1;
+      given[1] = (code / 3) % 3 - 1;
+      given[2] = code / 9 - 1;
     }
     bool need;
     if constexpr (CODES) {
@@ -871,16 +988,23 @@ __global__ void __launch_bounds__(256)
                             (f[2] - t.rlo[2]) * rnx * rny;
       if (PG) {
         double vals[16];
-        pt_decide_group(tl, off, ic, shift, vals, need);
+        pt_decide_group(tl, off, ic, shift, vals, need, given);
         codes[gid] = adj_code(shift, need);
       } else {
         double val;
-        pt_decide_var(tl, 0, off, ic, shift, val, need);
+        pt_decide_var(tl, 0, off, ic, shift, val, need, given);
         codes[(long long)blockIdx.y * pt_npts(t) + gid] = adj_code(shift, need);
       }
     } else if (!PG) {
       double val;
-      pt_decide_var(tl, 0, off, ic, shift, val, need);
+      pt_decide_var(tl, 0, off, ic, shift, val, need, given);
+      if (codes) {
+        const long long rnx = t.rhi[0] - t.rlo[0] + 1;
+        const long long rny = t.rhi[1] - t.rlo[1] + 1;
+        codes[(long long)blockIdx.y * pt_npts(t) + (f[0] - t.rlo[0]) +
+              (f[1] - t.rlo[1]) * rnx + (f[2] - t.rlo[2]) * rnx * rny] =
+            adj_code(shift, need);
+      }
       double res = val;
       if (need) {
         int sl[3];
@@ -894,7 +1018,13 @@ __global__ void __launch_bounds__(256)
       fo.p[didx(fo, f[0], f[1], f[2])] = res;
     } else {
       double vals[16];
-      pt_decide_group(tl, off, ic, shift, vals, need);
+      pt_decide_group(tl, off, ic, shift, vals, need, given);
+      if (codes) {
+        const long long rnx = t.rhi[0] - t.rlo[0] + 1;
+        const long long rny = t.rhi[1] - t.rlo[1] + 1;
+        codes[(f[0] - t.rlo[0]) + (f[1] - t.rlo[1]) * rnx +
+              (f[2] - t.rlo[2]) * rnx * rny] = adj_code(shift, need);
+      }
       int sl[3];
       for (int d = 0; d < 3; ++d)
         sl[d] = sid_linear_per_group(t.sid[d]);

@@ -137,6 +137,44 @@ public:
           "cxp_prolongate_adjoint");
   }
 
+  // Bytes of the branch record of a forward over fine_region (0 for
+  // operators without ENO shifts or fallback).
+  int64_t branch_bytes(const Box &fine_region, int ncomp,
+                       bool per_group = false) const {
+    int64_t n = 0;
+    check(cxp_branch_bytes(op_, fine_region.lo.data(), fine_region.hi.data(),
+                           ncomp, per_group ? 1 : 0, &n),
+          "cxp_branch_bytes");
+    return n;
+  }
+
+  // interp that also records every fine point's branch (device memory of
+  // branch_bytes bytes; may be null when that is 0).
+  void interp_record(const Fab &crse, int crse_comp, const Fab &fine,
+                     int fine_comp, int ncomp, const Box &fine_region,
+                     unsigned char *branches, bool per_group = false,
+                     void *stream = nullptr) const {
+    const cxp_fab c = crse.c(), f = fine.c();
+    check(cxp_prolongate_record(op_, &c, crse_comp, &f, fine_comp, ncomp,
+                                fine_region.lo.data(), fine_region.hi.data(),
+                                per_group ? 1 : 0, branches, stream),
+          "cxp_prolongate_record");
+  }
+
+  // crse_bar += P^T fine_bar from the branches of interp_record.
+  void interp_adjoint_recorded(const unsigned char *branches,
+                               const Fab &crse_bar, int crse_comp,
+                               const Fab &fine_bar, int fine_comp, int ncomp,
+                               const Box &fine_region, bool per_group = false,
+                               void *stream = nullptr) const {
+    const cxp_fab cb = crse_bar.c(), fb = fine_bar.c();
+    check(cxp_prolongate_adjoint_recorded(
+              op_, branches, &cb, crse_comp, &fb, fine_comp, ncomp,
+              fine_region.lo.data(), fine_region.hi.data(), per_group ? 1 : 0,
+              stream),
+          "cxp_prolongate_adjoint_recorded");
+  }
+
   // Per-direction (centring, interpolation, order, ghosts) and fallback.
   void describe(IntVect &cent, IntVect &intp, IntVect &order, IntVect &ghosts,
                 int &fb) const {

@@ -162,6 +162,38 @@ class Prolongate3DRF2:
             fine_comp, ncomp, i3(fine_region.lo), i3(fine_region.hi),
             int(bool(per_group)), _stream(stream)))
 
+    def branch_bytes(self, fine_region, ncomp, per_group=False):
+        """Size of the branch record of a forward over ``fine_region`` (0 for
+        operators without ENO shifts or fallback)."""
+        n = ctypes.c_int64()
+        check("cxp_branch_bytes", _lib.lib().cxp_branch_bytes(
+            self.handle, i3(fine_region.lo), i3(fine_region.hi), int(ncomp),
+            int(bool(per_group)), ctypes.byref(n)))
+        return n.value
+
+    def interp_record(self, crse, crse_comp, fine, fine_comp, ncomp, fine_region,
+                      branches, per_group=False, stream=None):
+        """interp that also records every fine point's branch in ``branches``
+        (a uint8 device tensor of branch_bytes bytes, or None when 0)."""
+        c, f = crse.c(), fine.c()
+        ptr = ctypes.c_void_p(branches.data_ptr()) if branches is not None else None
+        check("cxp_prolongate_record", _lib.lib().cxp_prolongate_record(
+            self.handle, ctypes.byref(c), crse_comp, ctypes.byref(f), fine_comp,
+            ncomp, i3(fine_region.lo), i3(fine_region.hi), int(bool(per_group)),
+            ptr, _stream(stream)))
+
+    def interp_adjoint_recorded(self, branches, crse_bar, crse_comp, fine_bar,
+                                fine_comp, ncomp, fine_region, per_group=False,
+                                stream=None):
+        """crse_bar += P^T fine_bar with the branches of interp_record."""
+        cb, fb = crse_bar.c(), fine_bar.c()
+        ptr = ctypes.c_void_p(branches.data_ptr()) if branches is not None else None
+        check("cxp_prolongate_adjoint_recorded",
+              _lib.lib().cxp_prolongate_adjoint_recorded(
+                  self.handle, ptr, ctypes.byref(cb), crse_comp, ctypes.byref(fb),
+                  fine_comp, ncomp, i3(fine_region.lo), i3(fine_region.hi),
+                  int(bool(per_group)), _stream(stream)))
+
     def __repr__(self):
         return "Prolongate3DRF2(%s, %d, %s)" % (
             self.prolongation_type, self.prolongation_order, self.indextype)

@@ -107,6 +107,32 @@ int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
                            const int region_lo[3], const int region_hi[3],
                            int per_group, void *stream);
 
+/* Bytes of the branch record of a forward over [region_lo, region_hi]
+ * (no reference counterpart): one byte per fine point (per component in
+ * per-var mode) for ENO and linear-fallback operators, 0 for the others. */
+int cxp_branch_bytes(cxp_op op, const int region_lo[3],
+                     const int region_hi[3], int ncomp, int per_group,
+                     int64_t *bytes);
+/* cxp_prolongate that also writes each fine point's branch (ENO shift,
+ * fallback decision) into branches, device memory of cxp_branch_bytes bytes
+ * (no reference counterpart).  Operators without branches ignore branches
+ * (may be null); for the others it is required (CXP_ERR_ARG). */
+int cxp_prolongate_record(cxp_op op, const cxp_fab *crse, int crse_comp,
+                          const cxp_fab *fine, int fine_comp, int ncomp,
+                          const int region_lo[3], const int region_hi[3],
+                          int per_group, unsigned char *branches,
+                          void *stream);
+/* cxp_prolongate_adjoint from a branch record of cxp_prolongate_record over
+ * the same region, components and mode (no reference counterpart): the
+ * gather alone, without recomputing the branches from crse_in.  Linear
+ * operators ignore branches; MINMOD returns CXP_ERR_UNSUPPORTED. */
+int cxp_prolongate_adjoint_recorded(cxp_op op, const unsigned char *branches,
+                                    const cxp_fab *crse_bar, int crse_comp,
+                                    const cxp_fab *fine_bar, int fine_comp,
+                                    int ncomp, const int region_lo[3],
+                                    const int region_hi[3], int per_group,
+                                    void *stream);
+
 /* Adjoint of cxp_restrict: fine_bar += R^T crse_bar. */
 int cxp_restrict_adjoint(const int indextype[3], const cxp_fab *crse_bar,
                          int crse_comp, const cxp_fab *fine_bar, int fine_comp,

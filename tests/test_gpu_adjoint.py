@@ -157,6 +157,37 @@ def test_nonlinear_adjoint_dot_product(fam, order, it, pg, nc):
     assert abs(lhs - rhs) <= 1e-6 * scale, (lhs, rhs)
 
 
+@pytest.mark.parametrize("fam,order,it,pg,nc", [x + (2,) for x in NONLINEAR] +
+                         [("eno", 3, (1, 1, 1), True, 12), ("natural", 5, (0, 0, 0), False, 2)])
+def test_branch_record_matches_recomputed_branches(fam, order, it, pg, nc):
+    """cxp_prolongate_record writes the same fine values as cxp_prolongate,
+    and the adjoint from its branch record equals the adjoint that recomputes
+    the branches from crse_in, bit for bit (per-var and per-group; 12
+    per-group components run the one-thread-per-point kernels).  A linear
+    operator (poly-cons3lfb of order 1 included: its fallback is inactive)
+    records nothing and its recorded adjoint is its adjoint."""
+    rng = np.random.default_rng(31)
+    rlo, rhi = (-4, 3, 0), (29, 22, 13)
+    P, cb, fbox, u, v = _fabs((fam, order, it), rlo, rhi, nc, rng)
+    crse = FArrayBox(cb, nc, tensor=torch.from_numpy(u).cuda())
+    f1, f2 = FArrayBox(fbox, nc), FArrayBox(fbox, nc)
+    P.interp(crse, 0, f1, 0, nc, fbox, per_group=pg)
+    nb = P.branch_bytes(fbox, nc, per_group=pg)
+    # ENO shifts, or a fallback that is active (a CONS direction of order > 1)
+    branches = fam in ("eno", "poly-eno3lfb") or (fam == "poly-cons3lfb" and order > 1)
+    assert nb == (fbox.numPts * (1 if pg else nc) if branches else 0)
+    br = torch.empty(nb, dtype=torch.uint8, device="cuda") if nb else None
+    P.interp_record(crse, 0, f2, 0, nc, fbox, br, per_group=pg)
+    torch.cuda.synchronize()
+    assert torch.equal(f1.t, f2.t)
+    vb = FArrayBox(fbox, nc, tensor=torch.from_numpy(v).cuda())
+    a1, a2 = FArrayBox(cb, nc), FArrayBox(cb, nc)
+    P.interp_adjoint(a1, 0, vb, 0, nc, fbox, crse_in=crse, per_group=pg)
+    P.interp_adjoint_recorded(br, a2, 0, vb, 0, nc, fbox, per_group=pg)
+    torch.cuda.synchronize()
+    assert torch.equal(a1.t, a2.t)
+
+
 MINMOD_OFF = ("the MINMOD adjoint is switched off: its first implementation hung the "
               "GPU twice in testing and was removed; the entry point returns "
               "CXP_ERR_UNSUPPORTED until it is rewritten")

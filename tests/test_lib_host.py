@@ -145,6 +145,45 @@ def test_adjoint_argument_checks_before_device_work():
         assert st == want, (fam, st)
 
 
+def test_branch_record_sizes_and_checks_without_device_work():
+    """cxp_branch_bytes: one byte per fine point (per component in per-var
+    mode) for ENO and fallback operators, 0 for linear ones and MINMOD; the
+    record and the recorded adjoint reject a missing record, and MINMOD has
+    no recorded adjoint, all before any device work."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+
+    def fab(lo, hi, nc):
+        f = _lib.CxpFab()
+        f.data = fake
+        f.lo, f.hi = i3(lo), i3(hi)
+        f.comp_stride = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+        f.ncomp = nc
+        return f
+    rlo, rhi = (0, 0, 0), (15, 9, 7)
+    npts = 16 * 10 * 8
+    crse, fine = fab((-4, -4, -4), (20, 20, 20), 3), fab((0, 0, 0), (15, 9, 7), 3)
+    n = ctypes.c_int64()
+    for fam, order, it, pg, want in [("eno", 3, (1, 1, 1), 0, 3 * npts),
+                                     ("eno", 3, (1, 1, 1), 1, npts),
+                                     ("poly-cons3lfb", 3, (1, 0, 1), 0, 3 * npts),
+                                     ("natural", 5, (0, 0, 0), 0, 0),
+                                     ("minmod", 3, (1, 1, 1), 0, 0)]:
+        h = _lookup(fam, order, it)[1]
+        assert L.cxp_branch_bytes(h, i3(rlo), i3(rhi), 3, pg, ctypes.byref(n)) == 0
+        assert n.value == want, (fam, pg, n.value)
+    h = _lookup("eno", 3, (1, 1, 1))[1]
+    assert L.cxp_prolongate_record(h, ctypes.byref(crse), 0, ctypes.byref(fine), 0, 3,
+                                   i3(rlo), i3(rhi), 0, None, None) == -3
+    assert L.cxp_prolongate_adjoint_recorded(h, None, ctypes.byref(crse), 0,
+                                             ctypes.byref(fine), 0, 3, i3(rlo),
+                                             i3(rhi), 0, None) == -3
+    hm = _lookup("minmod", 3, (1, 1, 1))[1]
+    assert L.cxp_prolongate_adjoint_recorded(hm, None, ctypes.byref(crse), 0,
+                                             ctypes.byref(fine), 0, 3, i3(rlo),
+                                             i3(rhi), 0, None) == -11
+
+
 def _level(boxes, domain, ncomp, ghosts):
     """A cxp_level of FABs (valid box grown by ``ghosts``) with a placeholder
     data pointer: appending to a work list reads only the boxes."""
