@@ -142,6 +142,8 @@ def bench_level64(args, stream, log):
             wl.append(d["op"], d["crse"], 0, fab, 0, d["ncomp"], reg)
         with torch.cuda.stream(stream):
             wl.upload(stream)
+            if not args.no_tune:
+                wl.autotune(reps=5, stream=stream)
             g = Graph(lambda: wl.run(stream), stream)
             for _ in range(3):
                 g.launch(stream)
@@ -161,7 +163,7 @@ def bench_level64(args, stream, log):
     return out
 
 
-def bench_level_fill(stream, log):
+def bench_level_fill(args, stream, log):
     """Ghost fill of a whole fine level (64 boxes of 32^3, 3 ghosts) from a
     coarse level of 2^3 FABs: the fine ghost points outside every fine valid
     box, coarse patches gathered where a stencil spans coarse FABs."""
@@ -175,6 +177,8 @@ def bench_level_fill(stream, log):
         wl.append_level(d["op"], d["crse"], 0, d["fine"], 0, d["ncomp"])
         with torch.cuda.stream(stream):
             wl.upload(stream)
+            if not args.no_tune:
+                wl.autotune(reps=5, stream=stream)
             g = Graph(lambda: wl.run(stream), stream)
             for _ in range(3):
                 g.launch(stream)
@@ -427,7 +431,7 @@ def main():
             batched_over_per_box=round(res["per_box_ms"] / ms, 2))
     if not args.no_extra:
         line["extra"]["level_64"] = bench_level64(args, stream, log)
-        line["extra"]["level_fill_64"] = bench_level_fill(stream, log)
+        line["extra"]["level_fill_64"] = bench_level_fill(args, stream, log)
         line["extra"]["adjoint_96_192"] = bench_adjoint(stream, log)
         if not args.no_wave:
             line["extra"]["wave_two_level"] = bench_wave(log)
