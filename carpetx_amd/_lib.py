@@ -17,6 +17,7 @@ c_void_p = ctypes.c_void_p
 c_char_p = ctypes.c_char_p
 P_INT = ctypes.POINTER(ctypes.c_int)
 P_INT64 = ctypes.POINTER(ctypes.c_int64)
+P_DOUBLE = ctypes.POINTER(ctypes.c_double)
 
 CXP_OK = 0
 STATUS = {
@@ -35,6 +36,12 @@ class CxpFab(ctypes.Structure):
 class CxpLevel(ctypes.Structure):
     _fields_ = [("nfabs", c_int), ("fabs", ctypes.POINTER(CxpFab)),
                 ("valid", P_INT), ("domain_lo", c_int * 3), ("domain_hi", c_int * 3)]
+
+
+class CxpBc(ctypes.Structure):
+    _fields_ = [("type", (c_int * 3) * 2), ("dirichlet_values", P_DOUBLE),
+                ("robin_values", P_DOUBLE), ("parities", P_INT),
+                ("xmin", ctypes.c_double * 3), ("dx", ctypes.c_double * 3)]
 
 
 class CxpError(RuntimeError):
@@ -86,6 +93,10 @@ _SIGS = {
     "cxp_worklist_append_level": (c_int, [c_void_p, c_void_p,
                                           ctypes.POINTER(CxpLevel), c_int,
                                           ctypes.POINTER(CxpLevel), c_int, c_int]),
+    "cxp_worklist_append_level_bc": (c_int, [c_void_p, c_void_p,
+                                             ctypes.POINTER(CxpLevel), c_int,
+                                             ctypes.POINTER(CxpLevel), c_int, c_int,
+                                             ctypes.POINTER(CxpBc)]),
     "cxp_worklist_upload": (c_int, [c_void_p, c_void_p]),
     "cxp_worklist_stats": (c_int, [c_void_p, P_INT64, P_INT64, P_INT64, P_INT64]),
     "cxp_prolongate_batched": (c_int, [c_void_p, c_void_p]),

@@ -43,6 +43,55 @@ class Level:
         return lv, (arr, val)
 
 
+# Symmetry and boundary condition types (CXP_BC_* of the C interface):
+# CarpetX's boundary_t values and its reflection symmetry
+# (CarpetX/src/boundaries.hxx, boundaries_impl.hxx:150-188).
+BC_TYPES = {"none": 0, "dirichlet": 1, "linear_extrapolation": 2,
+            "neumann": 3, "robin": 4, "reflection": 5}
+
+
+class BoundaryConditions:
+    """The coarse level's conditions for ``WorkList.append_level``.
+
+    ``faces[f][d]`` names the condition of face f (0 lower, 1 upper) in
+    direction d, a key of BC_TYPES.  ``dirichlet_values`` and
+    ``robin_values`` hold one value per appended component and
+    ``parities`` three (+1 or -1, one per direction) per component, as the
+    reference's GroupData does; the Robin condition's coordinates are
+    x = xmin + i dx (BoundaryCondition's xmin and dx, boundaries.cxx:34-56).
+    """
+
+    def __init__(self, faces, dirichlet_values=None, robin_values=None,
+                 parities=None, xmin=(0.0, 0.0, 0.0), dx=(1.0, 1.0, 1.0)):
+        self.faces = [[BC_TYPES[faces[f][d]] for d in range(3)] for f in range(2)]
+        self.dirichlet_values = dirichlet_values
+        self.robin_values = robin_values
+        self.parities = parities
+        self.xmin, self.dx = tuple(xmin), tuple(dx)
+
+    def c(self):
+        b = _lib.CxpBc()
+        for f in range(2):
+            for d in range(3):
+                b.type[f][d] = self.faces[f][d]
+        keep = []
+
+        def arr(ct, vals):
+            if vals is None:
+                return None
+            a = (ct * len(vals))(*vals)
+            keep.append(a)
+            return ctypes.cast(a, ctypes.POINTER(ct))
+
+        b.dirichlet_values = arr(ctypes.c_double, self.dirichlet_values)
+        b.robin_values = arr(ctypes.c_double, self.robin_values)
+        b.parities = arr(ctypes.c_int, None if self.parities is None else
+                         [int(x) for p in self.parities for x in p])
+        b.xmin = (ctypes.c_double * 3)(*self.xmin)
+        b.dx = (ctypes.c_double * 3)(*self.dx)
+        return b, keep
+
+
 class WorkList:
     def __init__(self):
         self.h = ctypes.c_void_p()
@@ -75,16 +124,27 @@ class WorkList:
             fine_comp, ncomp, i3(interior.lo), i3(interior.hi)))
         self._keep += [crse, fine]
 
-    def append_level(self, op, crse, crse_comp, fine, fine_comp, ncomp):
+    def append_level(self, op, crse, crse_comp, fine, fine_comp, ncomp, bc=None):
         """Ghost fill of the fine Level from the coarse Level: every fine FAB
         point in no fine valid box (FillPatch_ProlongateGhosts,
         CarpetX/src/fillpatch.cxx:42-138); coarse patches that no single
-        coarse FAB covers are gathered on the device at each run."""
+        coarse FAB covers are gathered on the device at each run.  With
+        ``bc`` (BoundaryConditions), coarse points outside the coarse domain
+        take the boundary values computed on the gathered patch, as the
+        reference applies them to its coarse patch (fillpatch.cxx:107);
+        without it they come from the coarse FABs' ghost points."""
         c, keep_c = crse.c()
         f, keep_f = fine.c()
-        check("cxp_worklist_append_level", _lib.lib().cxp_worklist_append_level(
-            self.h, op.handle, ctypes.byref(c), crse_comp, ctypes.byref(f),
-            fine_comp, ncomp))
+        if bc is None:
+            check("cxp_worklist_append_level", _lib.lib().cxp_worklist_append_level(
+                self.h, op.handle, ctypes.byref(c), crse_comp, ctypes.byref(f),
+                fine_comp, ncomp))
+        else:
+            b, keep_b = bc.c()
+            check("cxp_worklist_append_level_bc",
+                  _lib.lib().cxp_worklist_append_level_bc(
+                      self.h, op.handle, ctypes.byref(c), crse_comp,
+                      ctypes.byref(f), fine_comp, ncomp, ctypes.byref(b)))
         self._keep += list(crse.fabs) + list(fine.fabs)
 
     def upload(self, stream=None):

@@ -492,6 +492,12 @@ struct cxp_worklist_s {
   GatherRun *d_gather_runs = nullptr;
   GatherChunk *d_gather_chunks = nullptr;
   size_t cap_gather_runs = 0, cap_gather_chunks = 0;
+  // boundary values of gathered coarse patches, after the gather
+  std::vector<BcRun> bc_runs;  // off: offsets into d_gather
+  std::vector<BcChunk> bc_chunks;
+  BcRun *d_bc_runs = nullptr;
+  BcChunk *d_bc_chunks = nullptr;
+  size_t cap_bc_runs = 0, cap_bc_chunks = 0;
   std::vector<PoisonRun> poison_runs;  // one per (patch, component)
   std::vector<PoisonChunk> poison_chunks;
   PoisonRun *d_poison_runs = nullptr;
@@ -858,12 +864,20 @@ int place_gathered(cxp_worklist_s *wl) {
   return CXP_OK;
 }
 
+// Fills the gathered coarse patches: copies from the coarse FABs, then the
+// boundary values that are computed from those copies.
 int launch_gather(cxp_worklist_s *wl, hipStream_t s) {
   if (wl->gather_chunks.empty())
     return CXP_OK;
   hipLaunchKernelGGL(k_gather, dim3((unsigned)wl->gather_chunks.size()),
                      dim3(256), 0, s, wl->d_gather_runs, wl->d_gather_chunks,
                      wl->d_gather);
+  if (const int e = hip_status(hipGetLastError()))
+    return e;
+  if (wl->bc_chunks.empty())
+    return CXP_OK;
+  hipLaunchKernelGGL(k_bc, dim3((unsigned)wl->bc_chunks.size()), dim3(256), 0,
+                     s, wl->d_bc_runs, wl->d_bc_chunks, wl->d_gather);
   return hip_status(hipGetLastError());
 }
 
@@ -874,6 +888,10 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
     return e;
   if (int e = upload_vec(wl->gather_chunks, wl->d_gather_chunks,
                          wl->cap_gather_chunks, s))
+    return e;
+  if (int e = upload_vec(wl->bc_runs, wl->d_bc_runs, wl->cap_bc_runs, s))
+    return e;
+  if (int e = upload_vec(wl->bc_chunks, wl->d_bc_chunks, wl->cap_bc_chunks, s))
     return e;
   if (int e = upload_vec(wl->poison_runs, wl->d_poison_runs,
                          wl->cap_poison_runs, s))
@@ -1258,6 +1276,10 @@ int cxp_worklist_destroy(cxp_worklist wl) {
     (void)hipFree(wl->d_gather_runs);
   if (wl->d_gather_chunks)
     (void)hipFree(wl->d_gather_chunks);
+  if (wl->d_bc_runs)
+    (void)hipFree(wl->d_bc_runs);
+  if (wl->d_bc_chunks)
+    (void)hipFree(wl->d_bc_chunks);
   if (wl->d_poison_runs)
     (void)hipFree(wl->d_poison_runs);
   if (wl->d_poison_chunks)
@@ -1272,6 +1294,8 @@ int cxp_worklist_clear(cxp_worklist wl) {
   wl->patches.clear();
   wl->gather_runs.clear();
   wl->gather_chunks.clear();
+  wl->bc_runs.clear();
+  wl->bc_chunks.clear();
   wl->gather_doubles = 0;
   wl->uploaded = false;
   wl->fine_points = wl->algo_bytes = 0;
@@ -1477,13 +1501,89 @@ int64_t union_points(const std::vector<IBox> &boxes) {
   return n;
 }
 
-} // namespace
+// Reflection offsets of a domain: src = offset - dst mirrors about the
+// first and last points for vertex centring and half a cell outside them
+// for cell centring (boundaries_impl.hxx:286-298).
+int refl_offset(const IBox &dom, int face, int d, int cell) {
+  return face ? 2 * dom.hi[d] + cell : 2 * dom.lo[d] - cell;
+}
 
-int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
-                              const cxp_level *crse, int crse_comp,
-                              const cxp_level *fine, int fine_comp,
-                              int ncomp) {
+// The box a coarse patch with boundary points must hold: C and every point
+// BoundaryCondition::apply reads for C's points outside the domain
+// (boundaries_impl.hxx:249-298 and :322-360); those all lie inside the
+// domain.  CXP_ERR_ARG if C reaches a face without a condition or the
+// domain is too narrow for the sources.
+int bc_closure(const cxp_bc &bc, const OpDesc &od, const IBox &dom,
+               const IBox &C, IBox &E) {
+  E = C;
+  for (int d = 0; d < 3; ++d)
+    for (int face = 0; face < 2; ++face) {
+      if (face ? C.hi[d] <= dom.hi[d] : C.lo[d] >= dom.lo[d])
+        continue;
+      const int edge = face ? dom.hi[d] : dom.lo[d];
+      auto need = [&](int x) {
+        if (x < dom.lo[d] || x > dom.hi[d])
+          return false;
+        E.lo[d] = std::min(E.lo[d], x);
+        E.hi[d] = std::max(E.hi[d], x);
+        return true;
+      };
+      switch (bc.type[face][d]) {
+      case CXP_BC_DIRICHLET:
+        break;
+      case CXP_BC_NEUMANN:
+      case CXP_BC_ROBIN:
+        need(edge);
+        break;
+      case CXP_BC_LINEAR_EXTRAPOLATION:
+        if (!need(edge) || !need(face ? edge - 1 : edge + 1))
+          return CXP_ERR_ARG;
+        break;
+      case CXP_BC_REFLECTION: {
+        // mirrors of C's outside range [a, b] in direction d
+        const int a = face ? std::max(C.lo[d], dom.hi[d] + 1) : C.lo[d];
+        const int b = face ? C.hi[d] : std::min(C.hi[d], dom.lo[d] - 1);
+        const int off = refl_offset(dom, face, d, od.indextype[d]);
+        if (!need(off - a) || !need(off - b))
+          return CXP_ERR_ARG;
+        break;
+      }
+      default:
+        return CXP_ERR_ARG;
+      }
+    }
+  return CXP_OK;
+}
+
+bool bc_ok(const cxp_bc &bc, int ncomp) {
+  if (ncomp > 16)
+    return false;
+  bool dir = false, rob = false, ref = false;
+  for (int f = 0; f < 2; ++f)
+    for (int d = 0; d < 3; ++d) {
+      const int t = bc.type[f][d];
+      if (t < CXP_BC_NONE || t > CXP_BC_REFLECTION)
+        return false;
+      dir |= t == CXP_BC_DIRICHLET;
+      rob |= t == CXP_BC_ROBIN;
+      ref |= t == CXP_BC_REFLECTION;
+    }
+  if ((dir && !bc.dirichlet_values) || (rob && !bc.robin_values) ||
+      (ref && !bc.parities))
+    return false;
+  if (bc.parities)
+    for (int i = 0; i < 3 * ncomp; ++i)
+      if (bc.parities[i] != 1 && bc.parities[i] != -1)
+        return false;
+  return true;
+}
+
+int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
+                      int crse_comp, const cxp_level *fine, int fine_comp,
+                      int ncomp, const cxp_bc *bc) {
   if (!wl || !op || !level_ok(crse) || !level_ok(fine) || ncomp < 1)
+    return CXP_ERR_ARG;
+  if (bc && !bc_ok(*bc, ncomp))
     return CXP_ERR_ARG;
   for (const cxp_level *L : {crse, fine})
     for (int i = 0; i < L->nfabs; ++i) {
@@ -1512,9 +1612,12 @@ int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
   std::vector<cxp_worklist_s::Patch> patches;
   std::vector<GatherRun> runs;
   std::vector<GatherChunk> chunks;
+  std::vector<BcRun> bruns;
+  std::vector<BcChunk> bchunks;
   int64_t gather = wl->gather_doubles;
   int64_t fine_pts = 0, coarse_pts = 0;
   const int run0 = (int)wl->gather_runs.size();
+  const int brun0 = (int)wl->bc_runs.size();
 
   // one copy of region X of coarse FAB j into the coarse patch C at offset
   // off, every component
@@ -1543,6 +1646,48 @@ int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
         chunks.push_back({run, row, std::min(per, nrows - row)});
     }
   };
+  // boundary values of the coarse patch E at offset off, every component:
+  // one run each, chunks over the parts of E outside the domain
+  auto add_bc = [&](const IBox &E, int64_t off) {
+    std::vector<IBox> outs;
+    subtract(E, cdom, outs);
+    for (int c = 0; c < ncomp; ++c) {
+      BcRun r;
+      r.off = off + c * E.volume();
+      for (int d = 0; d < 3; ++d) {
+        r.lo[d] = E.lo[d];
+        r.hi[d] = E.hi[d];
+        r.imin[d] = cdom.lo[d];
+        r.imax[d] = cdom.hi[d] + 1;
+        for (int f = 0; f < 2; ++f) {
+          r.type[f][d] = bc->type[f][d];
+          r.refl[f][d] = refl_offset(cdom, f, d, op->d.indextype[d]);
+        }
+        r.parity[d] = bc->parities ? bc->parities[3 * c + d] : 1;
+        r.xmin[d] = bc->xmin[d];
+        r.dx[d] = bc->dx[d];
+      }
+      r.dirichlet = bc->dirichlet_values ? bc->dirichlet_values[c] : 0.0;
+      r.robin = bc->robin_values ? bc->robin_values[c] : 0.0;
+      const int run = brun0 + (int)bruns.size();
+      bruns.push_back(r);
+      for (const IBox &P : outs) {
+        const int nx = P.hi[0] - P.lo[0] + 1, ny = P.hi[1] - P.lo[1] + 1;
+        const int vol = (int)P.volume();
+        for (int q = 0; q < vol; q += 1024) {
+          BcChunk ch;
+          ch.run = run;
+          for (int d = 0; d < 3; ++d)
+            ch.blo[d] = P.lo[d];
+          ch.bnx = nx;
+          ch.bny = ny;
+          ch.first = q;
+          ch.count = std::min(1024, vol - q);
+          bchunks.push_back(ch);
+        }
+      }
+    }
+  };
 
   for (int i = 0; i < fine->nfabs; ++i) {
     const cxp_fab &ff = fine->fabs[i];
@@ -1568,55 +1713,70 @@ int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
         p.rhi[d] = P.hi[d];
       }
       // in place: one coarse FAB holds C, inside the coarse domain within
-      // its valid region
-      const IBox Cin = intersect(C, cdom);
+      // its valid region (and, with boundary conditions to apply, C lies
+      // inside the domain)
+      const bool bcp = bc && !inside(C, cdom);
       int src = -1;
-      for (int j = 0; j < crse->nfabs && src < 0; ++j)
-        if (inside(C, cfabbox(j)) && (Cin.empty() || inside(Cin, cvalid(j))))
+      for (int j = 0; j < crse->nfabs && src < 0 && !bcp; ++j)
+        if (inside(C, cfabbox(j)) &&
+            (intersect(C, cdom).empty() || inside(intersect(C, cdom), cvalid(j))))
           src = j;
       if (src >= 0) {
         p.crse = crse->fabs[src];
         p.crse_comp = crse_comp;
       } else {
-        // gathered: inside the domain from the valid regions, outside it
-        // from the first FAB holding each point
+        // gathered into box E: inside the domain from the valid regions;
+        // outside it the boundary values (E is C and their sources), or
+        // without boundary conditions (E = C) the first FAB holding each
+        // point
+        IBox E = C;
+        if (bcp)
+          if (const int e = bc_closure(*bc, op->d, cdom, C, E))
+            return e;
+        if (E.volume() > (int64_t)INT32_MAX)
+          return CXP_ERR_ARG;
+        const IBox Ein = intersect(E, cdom);
         const int64_t off = gather;
         std::vector<IBox> left;
-        if (!Cin.empty()) {
-          left.push_back(Cin);
+        if (!Ein.empty()) {
+          left.push_back(Ein);
           for (int j = 0; j < crse->nfabs; ++j) {
-            const IBox X = intersect(Cin, cvalid(j));
+            const IBox X = intersect(Ein, cvalid(j));
             if (!X.empty()) {
-              add_copy(j, X, C, off);
+              add_copy(j, X, E, off);
               subtract_all(left, cvalid(j));
             }
           }
           if (!left.empty())
             return CXP_ERR_COARSE;
         }
-        subtract(C, cdom, left);
-        for (int j = 0; j < crse->nfabs && !left.empty(); ++j) {
-          std::vector<IBox> rest;
-          for (const IBox &r : left) {
-            const IBox X = intersect(r, cfabbox(j));
-            if (!X.empty())
-              add_copy(j, X, C, off);
-            subtract(r, cfabbox(j), rest);
+        if (bcp) {
+          add_bc(E, off);
+        } else {
+          subtract(E, cdom, left);
+          for (int j = 0; j < crse->nfabs && !left.empty(); ++j) {
+            std::vector<IBox> rest;
+            for (const IBox &r : left) {
+              const IBox X = intersect(r, cfabbox(j));
+              if (!X.empty())
+                add_copy(j, X, E, off);
+              subtract(r, cfabbox(j), rest);
+            }
+            left.swap(rest);
           }
-          left.swap(rest);
+          if (!left.empty())
+            return CXP_ERR_COARSE;
         }
-        if (!left.empty())
-          return CXP_ERR_COARSE;
         p.crse.data = nullptr;  // placed at upload
         for (int d = 0; d < 3; ++d) {
-          p.crse.lo[d] = C.lo[d];
-          p.crse.hi[d] = C.hi[d];
+          p.crse.lo[d] = E.lo[d];
+          p.crse.hi[d] = E.hi[d];
         }
-        p.crse.comp_stride = C.volume();
+        p.crse.comp_stride = E.volume();
         p.crse.ncomp = ncomp;
         p.crse_comp = 0;
         p.gather_off = off;
-        gather += C.volume() * ncomp;
+        gather += E.volume() * ncomp;
       }
       patches.push_back(p);
     }
@@ -1625,11 +1785,31 @@ int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
   wl->patches.insert(wl->patches.end(), patches.begin(), patches.end());
   wl->gather_runs.insert(wl->gather_runs.end(), runs.begin(), runs.end());
   wl->gather_chunks.insert(wl->gather_chunks.end(), chunks.begin(), chunks.end());
+  wl->bc_runs.insert(wl->bc_runs.end(), bruns.begin(), bruns.end());
+  wl->bc_chunks.insert(wl->bc_chunks.end(), bchunks.begin(), bchunks.end());
   wl->gather_doubles = gather;
   wl->uploaded = false;
   wl->fine_points += fine_pts * ncomp;
   wl->algo_bytes += 8 * (int64_t)ncomp * (fine_pts + coarse_pts);
   return CXP_OK;
+}
+
+} // namespace
+
+int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
+                              const cxp_level *crse, int crse_comp,
+                              const cxp_level *fine, int fine_comp,
+                              int ncomp) {
+  return append_level_impl(wl, op, crse, crse_comp, fine, fine_comp, ncomp,
+                           nullptr);
+}
+
+int cxp_worklist_append_level_bc(cxp_worklist wl, cxp_op op,
+                                 const cxp_level *crse, int crse_comp,
+                                 const cxp_level *fine, int fine_comp,
+                                 int ncomp, const cxp_bc *bc) {
+  return append_level_impl(wl, op, crse, crse_comp, fine, fine_comp, ncomp,
+                           bc);
 }
 
 int cxp_worklist_upload(cxp_worklist wl, void *stream) {

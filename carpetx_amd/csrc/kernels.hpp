@@ -23,6 +23,7 @@
 #include <cstdint>
 
 #include "stencil1d.hpp"
+#include "../../include/cxp.h"
 
 namespace cxp {
 
@@ -446,6 +447,128 @@ __global__ void __launch_bounds__(256)
     const int y = row - z * r.ny;
     const double v = gld(r.src, x + (long long)y * r.src_sy + z * r.src_sz);
     gst(buf, r.dst + x + (long long)y * r.dst_sy + z * r.dst_sz, v);
+  }
+}
+
+// Boundary values of a gathered coarse patch: every point of the patch box
+// outside the domain [imin, imax) takes the value CarpetX's
+// BoundaryCondition::apply gives it (CarpetX/src/boundaries_impl.hxx:
+// 322-437), from the patch's own points inside the domain.  Sources always
+// lie inside the domain, so no point both reads and is written.  A run is
+// one (patch, component); a chunk is a range of the points (x fastest) of
+// one box of the patch that lies outside the domain.
+struct BcRun {
+  long long off;        // the component's first point in the patch buffer
+  int lo[3], hi[3];     // patch box
+  int imin[3], imax[3]; // domain, imax exclusive
+  int type[2][3];       // CXP_BC_* per (face, direction)
+  int refl[2][3];       // reflection offsets: src = refl - dst
+  int parity[3];
+  double dirichlet, robin;
+  double xmin[3], dx[3];
+};
+struct BcChunk {
+  int run;
+  int blo[3];    // the box's first point
+  int bnx, bny;  // its x and y extents
+  int first, count;
+};
+
+__global__ void __launch_bounds__(256)
+    k_bc(const BcRun *__restrict__ runs, const BcChunk *__restrict__ chunks,
+         double *__restrict__ buf) {
+  const BcChunk ch = chunks[blockIdx.x];
+  const BcRun r = runs[ch.run];
+  const int nx = r.hi[0] - r.lo[0] + 1, ny = r.hi[1] - r.lo[1] + 1;
+  double *v = buf + r.off;
+  auto at = [&](const int p[3]) {
+    return (long long)(p[0] - r.lo[0]) + (long long)(p[1] - r.lo[1]) * nx +
+           (long long)(p[2] - r.lo[2]) * nx * ny;
+  };
+  // the host sizes the patch to hold every source; a source outside it
+  // would be a host error, and gives NaN instead of a stray read
+  auto in_box = [&](const int p[3]) {
+    return p[0] >= r.lo[0] && p[0] <= r.hi[0] && p[1] >= r.lo[1] &&
+           p[1] <= r.hi[1] && p[2] >= r.lo[2] && p[2] <= r.hi[2];
+  };
+  for (int q = ch.first + (int)threadIdx.x; q < ch.first + ch.count;
+       q += 256) {
+    const int dst[3] = {ch.blo[0] + q % ch.bnx, ch.blo[1] + (q / ch.bnx) % ch.bny,
+                        ch.blo[2] + q / (ch.bnx * ch.bny)};
+    int src[3] = {dst[0], dst[1], dst[2]}, delta[3] = {0, 0, 0};
+    bool outside = false, dirichlet = false, lin = false, refl = false;
+    int robin = 0;  // bit d: Robin in direction d
+    double parity = 1.0;
+    for (int d = 0; d < 3; ++d) {
+      if (dst[d] >= r.imin[d] && dst[d] < r.imax[d])
+        continue;  // not a boundary direction of this point
+      const int face = dst[d] >= r.imax[d] ? 1 : 0;
+      if (r.type[face][d] == CXP_BC_NONE)
+        continue;  // nor is a face without a condition
+      outside = true;
+      const int edge = face ? r.imax[d] - 1 : r.imin[d];
+      switch (r.type[face][d]) {
+      case CXP_BC_DIRICHLET:
+        dirichlet = true;
+        break;
+      case CXP_BC_LINEAR_EXTRAPOLATION:
+        src[d] = edge;
+        delta[d] = face ? -1 : 1;  // -inormal
+        lin = true;
+        break;
+      case CXP_BC_NEUMANN:
+        src[d] = edge;
+        break;
+      case CXP_BC_ROBIN:
+        src[d] = edge;
+        robin |= 1 << d;
+        break;
+      case CXP_BC_REFLECTION:
+        src[d] = r.refl[face][d] - dst[d];
+        parity *= (double)r.parity[d];
+        refl = true;
+        break;
+      default:
+        break;
+      }
+    }
+    if (!outside)
+      continue;
+    const int sp[3] = {src[0] + delta[0], src[1] + delta[1],
+                       src[2] + delta[2]};
+    double val;
+    if (dirichlet) {
+      val = r.dirichlet;
+    } else if (!in_box(src) || !in_box(sp)) {
+      val = __builtin_nan("");
+    } else {
+      val = v[at(src)];
+      for (int d = 0; d < 3; ++d)
+        if (robin & (1 << d)) {
+          double xb[3], xi[3];
+          for (int e = 0; e < 3; ++e) {
+            xb[e] = r.xmin[e] + (double)dst[e] * r.dx[e];
+            xi[e] = r.xmin[e] + (double)src[e] * r.dx[e];
+          }
+          const double rb = sqrt(xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2]);
+          const double ri = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+          const double qr = ri / rb;
+          val = r.robin + qr * (val - r.robin);
+        }
+      if (lin) {
+        const double grad = val - v[at(sp)];
+        int num = 0, den = 0;
+        for (int e = 0; e < 3; ++e) {
+          num += (dst[e] - src[e]) * (dst[e] - src[e]);
+          den += delta[e] * delta[e];
+        }
+        // integer quotient, as the reference divides two int sums
+        val += sqrt((double)(num / den)) * grad;
+      }
+      if (refl)
+        val *= parity;
+    }
+    v[at(dst)] = val;
   }
 }
 

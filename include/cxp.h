@@ -189,6 +189,40 @@ typedef struct cxp_level {
 int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
                               const cxp_level *crse, int crse_comp,
                               const cxp_level *fine, int fine_comp, int ncomp);
+/* Symmetry and boundary conditions of a level
+ * (CarpetX/src/boundaries.hxx, boundaries_impl.hxx:150-476), one per face:
+ * type[face][d], face 0 the lower and 1 the upper face in direction d.
+ * The reference's reflection symmetry is CXP_BC_REFLECTION; otherwise the
+ * group's boundary condition applies. */
+#define CXP_BC_NONE 0
+#define CXP_BC_DIRICHLET 1
+#define CXP_BC_LINEAR_EXTRAPOLATION 2
+#define CXP_BC_NEUMANN 3
+#define CXP_BC_ROBIN 4
+#define CXP_BC_REFLECTION 5
+typedef struct cxp_bc {
+  int type[2][3];
+  const double *dirichlet_values; /* host, one per component, or null */
+  const double *robin_values;     /* host, one per component, or null */
+  const int *parities;            /* host, 3 per component (+1 / -1), or null */
+  double xmin[3], dx[3];          /* coordinate of index 0 and spacing, as the
+                                     reference's BoundaryCondition (robin) */
+} cxp_bc;
+/* cxp_worklist_append_level with the coarse level's boundary conditions
+ * applied to the coarse patches, as FillPatch_ProlongateGhosts applies
+ * them to its coarse patch (fillpatch.cxx:107) before interpolating.  Every
+ * patch whose CoarseBox reaches outside the coarse domain reads a gathered
+ * coarse patch: valid coarse data inside the domain, and outside it the
+ * boundary values BoundaryCondition::apply computes from the patch's own
+ * interior points (all directions at once, as on faces, edges and corners
+ * there).  A face that such a patch reaches must have a type other than
+ * CXP_BC_NONE, and the per-component arrays its type uses must be given
+ * (CXP_ERR_ARG); at most 16 components.  bc = null is
+ * cxp_worklist_append_level. */
+int cxp_worklist_append_level_bc(cxp_worklist wl, cxp_op op,
+                                 const cxp_level *crse, int crse_comp,
+                                 const cxp_level *fine, int fine_comp,
+                                 int ncomp, const cxp_bc *bc);
 /* Builds tiles (using the tuned tile shapes) and copies the list to the
  * device; must precede cxp_prolongate_batched. */
 int cxp_worklist_upload(cxp_worklist wl, void *stream);

@@ -21,7 +21,8 @@ import torch
 
 import oracle
 from carpetx_amd import _lib
-from carpetx_amd.batched import Graph, Level, WorkList
+from carpetx_amd.batched import BoundaryConditions, Graph, Level, WorkList
+from oracle.boundary import apply_bc
 from carpetx_amd.interp import Box, FArrayBox, get_interpolator
 
 pytestmark = pytest.mark.gpu
@@ -43,9 +44,10 @@ def domains(it):
     return cdom, fdom
 
 
-def coarse_level(it, g, cg=CG):
+def coarse_level(it, g, cg=CG, poison_outside=False):
     """8 coarse FABs (valid split at 12) from the global array g, whose
-    box is the coarse domain grown by CG."""
+    box is the coarse domain grown by CG; with poison_outside, the points
+    outside the domain hold 1e300 as well."""
     cdom, _ = domains(it)
     glo = tuple(cdom.lo[d] - CG for d in range(3))
     fabs, valid = [], []
@@ -64,6 +66,8 @@ def coarse_level(it, g, cg=CG):
             ind &= (a >= cdom.lo[d]) & (a <= cdom.hi[d])
             inv &= (a >= v.lo[d]) & (a <= v.hi[d])
         arr[:, ind & ~inv] = 1e300
+        if poison_outside:
+            arr[:, ~ind] = 1e300
         fabs.append(FArrayBox(b, NC_CRSE, tensor=torch.from_numpy(arr).cuda()))
         valid.append(v)
     return Level(fabs, valid, cdom)
@@ -189,4 +193,106 @@ def test_level_fill_needs_coarse_boundary_points():
         wl.append_level(get_interpolator("natural", 3, it), crse, 1, fine, 0, NCOMP)
     assert e.value.status == -5
     assert wl.stats()["patches"] == 0
+    wl.close()
+
+
+BC_KINDS = {"none": 0, "dirichlet": 1, "linear_extrapolation": 2,
+            "neumann": 3, "robin": 4, "reflection": 5}
+BC_CASES = [
+    (("natural", 5, (0, 0, 0)),
+     [["reflection", "linear_extrapolation", "robin"],
+      ["dirichlet", "neumann", "reflection"]]),
+    (("natural", 3, (1, 1, 1)),
+     [["neumann", "reflection", "dirichlet"],
+      ["linear_extrapolation", "robin", "reflection"]]),
+    (("interpolate", 7, (1, 0, 1)),
+     [["reflection"] * 3, ["linear_extrapolation"] * 3]),
+    (("eno", 3, (1, 1, 1)),
+     [["reflection", "robin", "linear_extrapolation"], ["neumann"] * 3]),
+    (("minmod", 3, (0, 1, 0)),
+     [["dirichlet", "reflection", "neumann"], ["reflection"] * 3]),
+]
+
+
+@pytest.mark.parametrize("op_args,faces", BC_CASES)
+def test_level_fill_with_boundary_conditions_matches_oracle(op_args, faces):
+    """cxp_worklist_append_level_bc: coarse points outside the coarse domain
+    are computed from the gathered coarse patch, never read from the coarse
+    FABs (which hold 1e300 there).  Expected: the oracle's boundary
+    conditions applied to the global coarse array, then the oracle's
+    prolongation of every fine FAB box, kept where the fill writes.  The
+    sources of every boundary point lie inside the domain, so a patch's
+    boundary values equal the global array's.  Tolerance 1e-13 (the Robin
+    and linear-extrapolation values take a square root); batched, per-patch
+    and graph runs, then new coarse data through the same graph."""
+    it = op_args[2]
+    rng = np.random.default_rng(7 + sum(it) + op_args[1])
+    cdom, _ = domains(it)
+    gshape = (NC_CRSE,) + tuple(cdom.size[d] + 2 * CG for d in (2, 1, 0))
+    glo = tuple(cdom.lo[d] - CG for d in range(3))
+    kw = dict(dirichlet=[0.5, -1.5], robin=[0.25, 2.0],
+              parities=[(-1, 1, -1), (1, -1, 1)],
+              xmin=(0.37, -1.21, 0.53), dx=(0.05, 0.07, 0.06))
+    bc = BoundaryConditions(faces, dirichlet_values=kw["dirichlet"],
+                            robin_values=kw["robin"], parities=kw["parities"],
+                            xmin=kw["xmin"], dx=kw["dx"])
+    codes = [[BC_KINDS[faces[f][d]] for d in range(3)] for f in range(2)]
+
+    def with_bc(g):
+        gb = g.copy()
+        sub = np.ascontiguousarray(gb[1:1 + NCOMP])
+        apply_bc(sub, glo, cdom.lo, cdom.hi, codes, it, **kw)
+        gb[1:1 + NCOMP] = sub
+        return gb
+
+    g = rng.uniform(-1, 1, gshape)
+    crse = coarse_level(it, g, poison_outside=True)
+    fine = fine_level(it, rng)
+    originals = [f.t.clone() for f in fine.fabs]
+    want = expected(op_args, it, with_bc(g), fine)
+    op = get_interpolator(*op_args)
+    wl = WorkList()
+    wl.append_level(op, crse, 1, fine, 0, NCOMP, bc=bc)
+    s = torch.cuda.Stream()
+
+    def check(tag, want):
+        for f, w in zip(fine.fabs, want):
+            got = f.t.cpu().numpy()
+            np.testing.assert_allclose(got, w, rtol=1e-13, atol=1e-13,
+                                       err_msg=str((tag, op_args, faces)))
+
+    def reset():
+        for f, o in zip(fine.fabs, originals):
+            f.t.copy_(o)
+        torch.cuda.synchronize()
+
+    with torch.cuda.stream(s):
+        wl.upload(s)
+        wl.run(s)
+    torch.cuda.synchronize()
+    check("batched", want)
+    reset()
+    with torch.cuda.stream(s):
+        wl.run_per_patch(s)
+    torch.cuda.synchronize()
+    check("per patch", want)
+    with torch.cuda.stream(s):
+        graph = Graph(lambda: wl.run(s), s)
+    for rep in range(2):
+        reset()
+        with torch.cuda.stream(s):
+            graph.launch(s)
+        torch.cuda.synchronize()
+        check(("graph", rep), want)
+    g2 = rng.uniform(-1, 1, gshape)
+    crse2 = coarse_level(it, g2, poison_outside=True)
+    for a, b in zip(crse.fabs, crse2.fabs):
+        a.t.copy_(b.t)
+    reset()
+    want2 = expected(op_args, it, with_bc(g2), fine)
+    with torch.cuda.stream(s):
+        graph.launch(s)
+    torch.cuda.synchronize()
+    check("new coarse data", want2)
+    graph.close()
     wl.close()

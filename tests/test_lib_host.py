@@ -270,6 +270,89 @@ def test_level_append_partitions_the_ghost_points():
             assert L.cxp_worklist_destroy(wl) == 0
 
 
+def _bc(faces, ncomp, dirichlet=True, robin=True, parities=True):
+    b = _lib.CxpBc()
+    for f in range(2):
+        for d in range(3):
+            b.type[f][d] = faces[f][d]
+    keep = [(ctypes.c_double * ncomp)(*[1.0] * ncomp),
+            (ctypes.c_double * ncomp)(*[0.5] * ncomp),
+            (ctypes.c_int * (3 * ncomp))(*[1, -1, 1] * ncomp)]
+    if dirichlet:
+        b.dirichlet_values = ctypes.cast(keep[0], _lib.P_DOUBLE)
+    if robin:
+        b.robin_values = ctypes.cast(keep[1], _lib.P_DOUBLE)
+    if parities:
+        b.parities = ctypes.cast(keep[2], _lib.P_INT)
+    b.dx = (ctypes.c_double * 3)(1.0, 1.0, 1.0)
+    return b, keep
+
+
+def test_level_append_with_boundary_conditions_checks_its_arguments():
+    """cxp_worklist_append_level_bc computes the coarse boundary points, so
+    coarse FABs with one ghost point suffice (cxp_worklist_append_level
+    needs them in the FABs); it fills the same fine points; a face that a
+    coarse patch reaches without a condition, a condition without its
+    values, a parity other than +-1 or more than 16 components fail with
+    CXP_ERR_ARG and add nothing."""
+    L = _lib.lib()
+    n = 24
+    fine_valid = [((0, 0, 0), (9, 9, 9)), ((16, 14, 12), (31, 29, 27)),
+                  ((36, 36, 36), (47, 47, 47))]
+    D, LE, NM, RB, RF = 1, 2, 3, 4, 5
+    all_faces = [[RF, LE, RB], [D, NM, RF]]
+    for it in [(0, 0, 0), (1, 1, 1), (1, 0, 1)]:
+        cdom = ((0, 0, 0), tuple(n - it[d] for d in range(3)))
+        fdom = ((0, 0, 0), tuple(2 * n - it[d] for d in range(3)))
+        cboxes = []
+        for c in range(8):
+            cboxes.append((tuple(0 if not (c >> d) & 1 else 12 for d in range(3)),
+                           tuple(11 if not (c >> d) & 1 else cdom[1][d] for d in range(3))))
+        h = _lookup("natural", 5, it)[1]
+        thin, keep_t = _level(cboxes, cdom, 3, 1)
+        fine, keep_f = _level(fine_valid, fdom, 2, 3)
+        wl = ctypes.c_void_p()
+        assert L.cxp_worklist_create(ctypes.byref(wl)) == 0
+        assert L.cxp_worklist_append_level(wl, h, ctypes.byref(thin), 0,
+                                           ctypes.byref(fine), 0, 2) == -5
+        b, keep_b = _bc(all_faces, 2)
+        assert L.cxp_worklist_append_level_bc(wl, h, ctypes.byref(thin), 0,
+                                              ctypes.byref(fine), 0, 2, ctypes.byref(b)) == 0
+        v = [ctypes.c_int64() for _ in range(4)]
+        assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+        assert v[2].value == 2 * _fill_points(fine_valid, fdom, 3), it
+        npatch = v[0].value
+        bad = []
+        for f, d in [(0, 0), (1, 2), (0, 1)]:
+            faces = [list(r) for r in all_faces]
+            faces[f][d] = 0
+            bad.append(_bc(faces, 2))
+        bad.append(_bc(all_faces, 2, dirichlet=False))
+        bad.append(_bc(all_faces, 2, robin=False))
+        bad.append(_bc(all_faces, 2, parities=False))
+        p0 = _bc(all_faces, 2)
+        p0[1][2][4] = 0
+        bad.append(p0)
+        bad.append(_bc([[7, 1, 1], [1, 1, 1]], 2))
+        for bb, kb in bad:
+            assert L.cxp_worklist_append_level_bc(
+                wl, h, ctypes.byref(thin), 0, ctypes.byref(fine), 0, 2,
+                ctypes.byref(bb)) == -3
+        # more than 16 components
+        many_c, k1 = _level(cboxes, cdom, 17, 1)
+        many_f, k2 = _level(fine_valid, fdom, 17, 3)
+        b17, k3 = _bc(all_faces, 17)
+        assert L.cxp_worklist_append_level_bc(
+            wl, h, ctypes.byref(many_c), 0, ctypes.byref(many_f), 0, 17,
+            ctypes.byref(b17)) == -3
+        assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+        assert v[0].value == npatch
+        # a null condition set is cxp_worklist_append_level
+        assert L.cxp_worklist_append_level_bc(wl, h, ctypes.byref(thin), 0,
+                                              ctypes.byref(fine), 0, 2, None) == -5
+        assert L.cxp_worklist_destroy(wl) == 0
+
+
 def test_cpp_host_header_compiles():
     """The C++ host API (carpetx_amd/host/cxp.hpp) is self-contained C++17
     over the C header."""
