@@ -166,15 +166,24 @@ def bench_level64(args, stream, log):
 def bench_level_fill(args, stream, log):
     """Ghost fill of a whole fine level (64 boxes of 32^3, 3 ghosts) from a
     coarse level of 2^3 FABs: the fine ghost points outside every fine valid
-    box, coarse patches gathered where a stencil spans coarse FABs."""
+    box, coarse patches gathered where a stencil spans coarse FABs.  The
+    "_bc" rows fill the same level with the coarse boundary points computed
+    by the boundary conditions (reflection on the lower faces, linear
+    extrapolation on the upper ones) instead of read from the coarse FABs."""
     import torch
     from carpetx_amd import workloads
-    from carpetx_amd.batched import Graph, WorkList
+    from carpetx_amd.batched import BoundaryConditions, Graph, WorkList
     lv = workloads.level_fill_64()
-    out = {}
+    cases = []
     for name, d in lv.items():
+        cases.append((name, d, None))
+        cases.append((name + "_bc", d, BoundaryConditions(
+            [["reflection"] * 3, ["linear_extrapolation"] * 3],
+            parities=[(1, 1, 1)] * d["ncomp"])))
+    out = {}
+    for name, d, bc in cases:
         wl = WorkList()
-        wl.append_level(d["op"], d["crse"], 0, d["fine"], 0, d["ncomp"])
+        wl.append_level(d["op"], d["crse"], 0, d["fine"], 0, d["ncomp"], bc=bc)
         with torch.cuda.stream(stream):
             wl.upload(stream)
             if not args.no_tune:

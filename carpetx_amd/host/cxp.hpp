@@ -21,6 +21,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <vector>
 
 #include "../../include/cxp.h"
 
@@ -221,6 +222,26 @@ inline void restrict_fab(const IntVect &indextype, const Fab &fine,
 }
 
 // A device-resident list of interp() calls, run by one persistent kernel.
+// A refinement level: FABs, the valid box of each (valid boxes do not
+// overlap) and the domain, the index range of the valid points.
+struct Level {
+  std::vector<Fab> fabs;
+  std::vector<Box> valid;
+  Box domain;
+};
+
+// Symmetry and boundary conditions of a level, per face (0 lower, 1 upper)
+// and direction: CXP_BC_* (CarpetX/src/boundaries.hxx).  dirichlet and
+// robin hold one value per appended component, parities three per
+// component; the Robin condition's coordinates are x = xmin + i dx.
+struct BoundaryConditions {
+  int type[2][3] = {{CXP_BC_NONE, CXP_BC_NONE, CXP_BC_NONE},
+                    {CXP_BC_NONE, CXP_BC_NONE, CXP_BC_NONE}};
+  std::vector<double> dirichlet, robin;
+  std::vector<IntVect> parities;
+  std::array<double, 3> xmin{0.0, 0.0, 0.0}, dx{1.0, 1.0, 1.0};
+};
+
 class WorkList {
 public:
   WorkList() { check(cxp_worklist_create(&wl_), "cxp_worklist_create"); }
@@ -247,6 +268,59 @@ public:
                                     fine_comp, ncomp, interior.lo.data(),
                                     interior.hi.data()),
           "cxp_worklist_append_shell");
+  }
+  // The ghost fill of the whole fine level from the coarse level
+  // (FillPatch_ProlongateGhosts); with bc, the coarse points outside the
+  // coarse domain take the boundary values.
+  void append_level(const Prolongate3DRF2 &op, const Level &crse,
+                    int crse_comp, const Level &fine, int fine_comp, int ncomp,
+                    const BoundaryConditions *bc = nullptr) {
+    struct C {
+      std::vector<cxp_fab> fabs;
+      std::vector<int> valid;
+      cxp_level l;
+    };
+    auto conv = [](const Level &L, C &c) {
+      for (const Fab &f : L.fabs)
+        c.fabs.push_back(f.c());
+      for (const Box &b : L.valid)
+        for (const IntVect *v : {&b.lo, &b.hi})
+          c.valid.insert(c.valid.end(), v->begin(), v->end());
+      c.l.nfabs = (int)L.fabs.size();
+      c.l.fabs = c.fabs.data();
+      c.l.valid = c.valid.data();
+      for (int d = 0; d < 3; ++d) {
+        c.l.domain_lo[d] = L.domain.lo[d];
+        c.l.domain_hi[d] = L.domain.hi[d];
+      }
+      return L.fabs.size() == L.valid.size();
+    };
+    C c, f;
+    if (!conv(crse, c) || !conv(fine, f))
+      throw Error("append_level: one valid box per FAB", CXP_ERR_ARG);
+    if (!bc) {
+      check(cxp_worklist_append_level(wl_, op.handle(), &c.l, crse_comp, &f.l,
+                                      fine_comp, ncomp),
+            "cxp_worklist_append_level");
+      return;
+    }
+    cxp_bc b;
+    std::vector<int> par;
+    for (const IntVect &p : bc->parities)
+      par.insert(par.end(), p.begin(), p.end());
+    for (int side = 0; side < 2; ++side)
+      for (int d = 0; d < 3; ++d)
+        b.type[side][d] = bc->type[side][d];
+    b.dirichlet_values = bc->dirichlet.empty() ? nullptr : bc->dirichlet.data();
+    b.robin_values = bc->robin.empty() ? nullptr : bc->robin.data();
+    b.parities = par.empty() ? nullptr : par.data();
+    for (int d = 0; d < 3; ++d) {
+      b.xmin[d] = bc->xmin[d];
+      b.dx[d] = bc->dx[d];
+    }
+    check(cxp_worklist_append_level_bc(wl_, op.handle(), &c.l, crse_comp, &f.l,
+                                       fine_comp, ncomp, &b),
+          "cxp_worklist_append_level_bc");
   }
   void upload(void *stream = nullptr) {
     check(cxp_worklist_upload(wl_, stream), "cxp_worklist_upload");

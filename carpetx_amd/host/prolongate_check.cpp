@@ -15,6 +15,10 @@
 //      or fallback (interpolate, conservative, ddf, hermite, natural,
 //      minmod) in all 8 centrings: exact for polynomials of its
 //      per-direction order.
+//   5. A level ghost fill with reflection symmetry on the lower faces: the
+//      coarse FAB holds NaN outside the domain, the fill computes those
+//      points by reflection, and data even (component 0) or odd in x
+//      (component 1, parity -1) about the faces is reproduced exactly.
 //
 // Exit status 0 when every check passes.
 #include <hip/hip_runtime.h>
@@ -232,6 +236,65 @@ int main() {
               exact, checked);
   expect(checked > 100 && exact == checked,
          "every operator reproduces polynomials of its order");
+
+  // 5. level fill with reflection on the lower faces
+  {
+    const cxp::Box cdom({0, 0, 0}, {n, n, n});
+    const cxp::Box cfab = cdom.grow(3);
+    const cxp::Box fvalid({0, 0, 0}, {15, 15, 15});
+    const cxp::Box ffab = fvalid.grow(3);
+    // component 0: (1 + x^2/2)(2 - y^2)(1 + z^2); component 1: (x - x^3) y^2 (3 + z^2)
+    auto f = [](int c, double x, double y, double z) {
+      return c == 0 ? (1 + 0.5 * x * x) * (2 - y * y) * (1 + z * z)
+                    : (x - x * x * x) * (y * y) * (3 + z * z);
+    };
+    auto fill = [&](const cxp::Box &b, double h, bool nan_outside) {
+      std::vector<double> v;
+      for (int c = 0; c < 2; ++c)
+        for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+          for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+            for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
+              const bool in = i >= 0 && j >= 0 && k >= 0 && i <= n && j <= n && k <= n;
+              v.push_back(nan_outside && !in ? std::nan("") : f(c, i * h, j * h, k * h));
+            }
+      return v;
+    };
+    DevFab cr(cfab, 2), fi(ffab, 2);
+    cr.put(fill(cfab, hc, true));
+    std::vector<double> init(ffab.numPts() * 2, 0.0);
+    fi.put(init);
+    cxp::Level cl{{cr.fab}, {cdom}, cdom};
+    cxp::Level fl{{fi.fab}, {fvalid}, cxp::Box({0, 0, 0}, {2 * n, 2 * n, 2 * n})};
+    cxp::BoundaryConditions bc;
+    for (int d = 0; d < 3; ++d)
+      bc.type[0][d] = CXP_BC_REFLECTION;
+    bc.parities = {{1, 1, 1}, {-1, 1, 1}};
+    const auto &P3 = cxp::get_interpolator("interpolate", 3, {0, 0, 0});
+    cxp::WorkList wl;
+    wl.append_level(P3, cl, 0, fl, 0, 2, &bc);
+    wl.upload(s);
+    wl.run(s);
+    (void)hipStreamSynchronize(s);
+    const auto got = fi.get();
+    const auto want = fill(ffab, hf, false);
+    double m = 0;
+    size_t q = 0, ghosts = 0;
+    for (int c = 0; c < 2; ++c)
+      for (int k = ffab.lo[2]; k <= ffab.hi[2]; ++k)
+        for (int j = ffab.lo[1]; j <= ffab.hi[1]; ++j)
+          for (int i = ffab.lo[0]; i <= ffab.hi[0]; ++i, ++q) {
+            const bool valid = i >= 0 && j >= 0 && k >= 0 && i <= 15 && j <= 15 && k <= 15;
+            if (valid) {
+              m = std::fmax(m, std::fabs(got[q]));  // left alone: 0
+            } else {
+              ++ghosts;
+              m = std::fmax(m, std::fabs(got[q] - want[q]));
+            }
+          }
+    std::printf("      level fill, %zu ghost points: max error %.3e\n", ghosts, m);
+    expect(ghosts > 0 && m < 1e-12,
+           "level fill with reflection reproduces even and odd data");
+  }
 
   (void)hipStreamDestroy(s);
   std::printf("%s\n", failures ? "FAILED" : "all checks passed");
