@@ -97,6 +97,17 @@ _SIGS = {
                                              ctypes.POINTER(CxpLevel), c_int,
                                              ctypes.POINTER(CxpLevel), c_int, c_int,
                                              ctypes.POINTER(CxpBc)]),
+    "cxp_worklist_append_new_level": (c_int, [c_void_p, c_void_p,
+                                              ctypes.POINTER(CxpLevel), c_int,
+                                              ctypes.POINTER(CxpLevel), c_int, c_int,
+                                              ctypes.POINTER(CxpBc)]),
+    "cxp_worklist_append_remake_level": (c_int, [c_void_p, c_void_p,
+                                                 ctypes.POINTER(CxpLevel), c_int,
+                                                 ctypes.POINTER(CxpLevel),
+                                                 ctypes.POINTER(CxpLevel), c_int, c_int,
+                                                 ctypes.POINTER(CxpBc)]),
+    "cxp_worklist_append_boundaries": (c_int, [c_void_p, ctypes.POINTER(CxpLevel), P_INT,
+                                               c_int, c_int, ctypes.POINTER(CxpBc)]),
     "cxp_worklist_upload": (c_int, [c_void_p, c_void_p]),
     "cxp_worklist_stats": (c_int, [c_void_p, P_INT64, P_INT64, P_INT64, P_INT64]),
     "cxp_prolongate_batched": (c_int, [c_void_p, c_void_p]),

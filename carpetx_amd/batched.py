@@ -147,6 +147,46 @@ class WorkList:
                       ctypes.byref(f), fine_comp, ncomp, ctypes.byref(b)))
         self._keep += list(crse.fabs) + list(fine.fabs)
 
+    def append_new_level(self, op, crse, crse_comp, fine, fine_comp, ncomp, bc=None):
+        """Fill of a new fine Level (FillPatch_NewLevel, CarpetX/src/
+        fillpatch.cxx:140-181): every fine FAB's whole box within the fine
+        domain, prolongated from the coarse Level; ``bc`` as for
+        append_level."""
+        c, keep_c = crse.c()
+        f, keep_f = fine.c()
+        b, keep_b = (None, None) if bc is None else bc.c()
+        check("cxp_worklist_append_new_level", _lib.lib().cxp_worklist_append_new_level(
+            self.h, op.handle, ctypes.byref(c), crse_comp, ctypes.byref(f),
+            fine_comp, ncomp, None if b is None else ctypes.byref(b)))
+        self._keep += list(crse.fabs) + list(fine.fabs)
+
+    def append_remake_level(self, op, crse, crse_comp, old_fine, fine, fine_comp,
+                            ncomp, bc=None):
+        """Fill of a regridded fine Level (FillPatch_RemakeLevel,
+        fillpatch.cxx:183-227): the old fine Level's valid data where it
+        exists, prolongation from the coarse Level elsewhere within the
+        fine domain grown by the ghost width."""
+        c, keep_c = crse.c()
+        o, keep_o = old_fine.c()
+        f, keep_f = fine.c()
+        b, keep_b = (None, None) if bc is None else bc.c()
+        check("cxp_worklist_append_remake_level",
+              _lib.lib().cxp_worklist_append_remake_level(
+                  self.h, op.handle, ctypes.byref(c), crse_comp, ctypes.byref(o),
+                  ctypes.byref(f), fine_comp, ncomp,
+                  None if b is None else ctypes.byref(b)))
+        self._keep += list(crse.fabs) + list(old_fine.fabs) + list(fine.fabs)
+
+    def append_boundaries(self, level, indextype, comp, ncomp, bc):
+        """The Level's own boundary conditions, in place, after the list's
+        fills (apply_boundary_conditions, CarpetX/src/boundaries_impl.hxx:
+        150-437)."""
+        lv, keep = level.c()
+        b, keep_b = bc.c()
+        check("cxp_worklist_append_boundaries", _lib.lib().cxp_worklist_append_boundaries(
+            self.h, ctypes.byref(lv), i3(indextype), comp, ncomp, ctypes.byref(b)))
+        self._keep += list(level.fabs)
+
     def upload(self, stream=None):
         check("cxp_worklist_upload", _lib.lib().cxp_worklist_upload(self.h, _stream(stream)))
 

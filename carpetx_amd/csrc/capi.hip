@@ -498,6 +498,12 @@ struct cxp_worklist_s {
   BcRun *d_bc_runs = nullptr;
   BcChunk *d_bc_chunks = nullptr;
   size_t cap_bc_runs = 0, cap_bc_chunks = 0;
+  // boundary conditions of levels, in place, after the fills
+  std::vector<BcRun> post_runs;
+  std::vector<BcChunk> post_chunks;
+  BcRun *d_post_runs = nullptr;
+  BcChunk *d_post_chunks = nullptr;
+  size_t cap_post_runs = 0, cap_post_chunks = 0;
   std::vector<PoisonRun> poison_runs;  // one per (patch, component)
   std::vector<PoisonChunk> poison_chunks;
   PoisonRun *d_poison_runs = nullptr;
@@ -881,6 +887,16 @@ int launch_gather(cxp_worklist_s *wl, hipStream_t s) {
   return hip_status(hipGetLastError());
 }
 
+// The levels' own boundary conditions, after the fills (their sources are
+// points the fills and copies wrote).
+int launch_post_bc(cxp_worklist_s *wl, hipStream_t s) {
+  if (wl->post_chunks.empty())
+    return CXP_OK;
+  hipLaunchKernelGGL(k_bc, dim3((unsigned)wl->post_chunks.size()), dim3(256),
+                     0, s, wl->d_post_runs, wl->d_post_chunks, nullptr);
+  return hip_status(hipGetLastError());
+}
+
 int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
   build_poison(wl);
   if (int e = upload_vec(wl->gather_runs, wl->d_gather_runs,
@@ -892,6 +908,11 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
   if (int e = upload_vec(wl->bc_runs, wl->d_bc_runs, wl->cap_bc_runs, s))
     return e;
   if (int e = upload_vec(wl->bc_chunks, wl->d_bc_chunks, wl->cap_bc_chunks, s))
+    return e;
+  if (int e = upload_vec(wl->post_runs, wl->d_post_runs, wl->cap_post_runs, s))
+    return e;
+  if (int e = upload_vec(wl->post_chunks, wl->d_post_chunks,
+                         wl->cap_post_chunks, s))
     return e;
   if (int e = upload_vec(wl->poison_runs, wl->d_poison_runs,
                          wl->cap_poison_runs, s))
@@ -1280,6 +1301,10 @@ int cxp_worklist_destroy(cxp_worklist wl) {
     (void)hipFree(wl->d_bc_runs);
   if (wl->d_bc_chunks)
     (void)hipFree(wl->d_bc_chunks);
+  if (wl->d_post_runs)
+    (void)hipFree(wl->d_post_runs);
+  if (wl->d_post_chunks)
+    (void)hipFree(wl->d_post_chunks);
   if (wl->d_poison_runs)
     (void)hipFree(wl->d_poison_runs);
   if (wl->d_poison_chunks)
@@ -1296,6 +1321,8 @@ int cxp_worklist_clear(cxp_worklist wl) {
   wl->gather_chunks.clear();
   wl->bc_runs.clear();
   wl->bc_chunks.clear();
+  wl->post_runs.clear();
+  wl->post_chunks.clear();
   wl->gather_doubles = 0;
   wl->uploaded = false;
   wl->fine_points = wl->algo_bytes = 0;
@@ -1578,15 +1605,108 @@ bool bc_ok(const cxp_bc &bc, int ncomp) {
   return true;
 }
 
+// Consistency of new chunks with the runs they index (runs[k] is run
+// run0 + k): every chunk names one of them, and a boundary chunk's points
+// lie in its run's box.  Checked on the host before anything reaches the
+// device.
+bool gather_chunks_ok(const std::vector<GatherChunk> &chunks, int run0,
+                      size_t nruns) {
+  for (const GatherChunk &c : chunks)
+    if (c.run < run0 || c.run >= run0 + (int)nruns || c.row0 < 0 ||
+        c.nrows < 1)
+      return false;
+  return true;
+}
+bool bc_chunks_ok(const std::vector<BcChunk> &chunks, int run0,
+                  const std::vector<BcRun> &runs) {
+  for (const BcChunk &c : chunks) {
+    if (c.run < run0 || c.run >= run0 + (int)runs.size() || c.bnx < 1 ||
+        c.bny < 1 || c.first < 0 || c.count < 1)
+      return false;
+    const BcRun &r = runs[c.run - run0];
+    const int last = c.first + c.count - 1;
+    const int ext[3] = {c.bnx - 1, c.bny - 1, last / (c.bnx * c.bny)};
+    const int ext0[3] = {0, 0, c.first / (c.bnx * c.bny)};
+    for (int d = 0; d < 3; ++d)
+      if (c.blo[d] + ext0[d] < r.lo[d] || c.blo[d] + ext[d] > r.hi[d])
+        return false;
+  }
+  return true;
+}
+
+// Boundary runs of box E (its ncomp components at p + off + c comp_stride;
+// p null: the patch buffer) over a domain: one run per component, chunks
+// of up to 1024 points over the parts of E outside the domain.  runs[k]
+// becomes run run0 + k.
+void add_bc_runs(const cxp_bc &bc, const int it[3], const IBox &E,
+                 const IBox &dom, double *p, int64_t off, int64_t comp_stride,
+                 int ncomp, int run0, std::vector<BcRun> &runs,
+                 std::vector<BcChunk> &chunks) {
+  std::vector<IBox> outs;
+  subtract(E, dom, outs);
+  for (int c = 0; c < ncomp; ++c) {
+    BcRun r;
+    r.p = p;
+    r.off = off + c * comp_stride;
+    for (int d = 0; d < 3; ++d) {
+      r.lo[d] = E.lo[d];
+      r.hi[d] = E.hi[d];
+      r.imin[d] = dom.lo[d];
+      r.imax[d] = dom.hi[d] + 1;
+      for (int f = 0; f < 2; ++f) {
+        r.type[f][d] = bc.type[f][d];
+        r.refl[f][d] = refl_offset(dom, f, d, it[d]);
+      }
+      r.parity[d] = bc.parities ? bc.parities[3 * c + d] : 1;
+      r.xmin[d] = bc.xmin[d];
+      r.dx[d] = bc.dx[d];
+    }
+    r.dirichlet = bc.dirichlet_values ? bc.dirichlet_values[c] : 0.0;
+    r.robin = bc.robin_values ? bc.robin_values[c] : 0.0;
+    const int run = run0 + (int)runs.size();
+    runs.push_back(r);
+    for (const IBox &P : outs) {
+      const int nx = P.hi[0] - P.lo[0] + 1, ny = P.hi[1] - P.lo[1] + 1;
+      const int vol = (int)P.volume();
+      for (int q = 0; q < vol; q += 1024) {
+        BcChunk ch;
+        ch.run = run;
+        for (int d = 0; d < 3; ++d)
+          ch.blo[d] = P.lo[d];
+        ch.bnx = nx;
+        ch.bny = ny;
+        ch.first = q;
+        ch.count = std::min(1024, vol - q);
+        chunks.push_back(ch);
+      }
+    }
+  }
+}
+
+// What a level fill prolongates in each fine FAB:
+//  * ghosts: its box within the fine domain grown by the ghost width,
+//    minus every fine valid box (FillPatch_ProlongateGhosts);
+//  * whole: its box within the fine domain (FillPatch_NewLevel,
+//    fillpatch.cxx:160-170: the domain is grown in periodic directions
+//    only, and none are modelled);
+//  * remake: its box within the grown fine domain minus every valid box of
+//    the old fine level, whose data is copied into the rest
+//    (FillPatch_RemakeLevel, fillpatch.cxx:185-227).
+enum class FillMode { ghosts, whole, remake };
+
 int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
                       int crse_comp, const cxp_level *fine, int fine_comp,
-                      int ncomp, const cxp_bc *bc) {
+                      int ncomp, const cxp_bc *bc,
+                      FillMode mode = FillMode::ghosts,
+                      const cxp_level *old = nullptr) {
   if (!wl || !op || !level_ok(crse) || !level_ok(fine) || ncomp < 1)
+    return CXP_ERR_ARG;
+  if (mode == FillMode::remake && !level_ok(old))
     return CXP_ERR_ARG;
   if (bc && !bc_ok(*bc, ncomp))
     return CXP_ERR_ARG;
-  for (const cxp_level *L : {crse, fine})
-    for (int i = 0; i < L->nfabs; ++i) {
+  for (const cxp_level *L : {crse, fine, old})
+    for (int i = 0; L && i < L->nfabs; ++i) {
       const int c0 = L == crse ? crse_comp : fine_comp;
       if (c0 < 0 || c0 + ncomp > L->fabs[i].ncomp)
         return CXP_ERR_COMP;
@@ -1601,7 +1721,7 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
       g[d] = std::max(g[d], std::max(fine->valid[6 * i + d] - fine->fabs[i].lo[d],
                                      fine->fabs[i].hi[d] - fine->valid[6 * i + 3 + d]));
   IBox fdom = ibox(fine->domain_lo, fine->domain_hi);
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < 3 && mode != FillMode::whole; ++d) {
     fdom.lo[d] -= g[d];
     fdom.hi[d] += g[d];
   }
@@ -1633,6 +1753,7 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
       r.src = f.p + didx(f, X.lo[0], X.lo[1], X.lo[2]);
       r.src_sy = f.sy;
       r.src_sz = f.sz;
+      r.dstp = nullptr;
       r.dst = off + c * cvol + (X.lo[0] - C.lo[0]) +
               (long long)(X.lo[1] - C.lo[1]) * cx +
               (long long)(X.lo[2] - C.lo[2]) * cx * cy;
@@ -1646,45 +1767,40 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
         chunks.push_back({run, row, std::min(per, nrows - row)});
     }
   };
-  // boundary values of the coarse patch E at offset off, every component:
-  // one run each, chunks over the parts of E outside the domain
+  // boundary values of the coarse patch E at offset off, every component
   auto add_bc = [&](const IBox &E, int64_t off) {
-    std::vector<IBox> outs;
-    subtract(E, cdom, outs);
-    for (int c = 0; c < ncomp; ++c) {
-      BcRun r;
-      r.off = off + c * E.volume();
-      for (int d = 0; d < 3; ++d) {
-        r.lo[d] = E.lo[d];
-        r.hi[d] = E.hi[d];
-        r.imin[d] = cdom.lo[d];
-        r.imax[d] = cdom.hi[d] + 1;
-        for (int f = 0; f < 2; ++f) {
-          r.type[f][d] = bc->type[f][d];
-          r.refl[f][d] = refl_offset(cdom, f, d, op->d.indextype[d]);
-        }
-        r.parity[d] = bc->parities ? bc->parities[3 * c + d] : 1;
-        r.xmin[d] = bc->xmin[d];
-        r.dx[d] = bc->dx[d];
-      }
-      r.dirichlet = bc->dirichlet_values ? bc->dirichlet_values[c] : 0.0;
-      r.robin = bc->robin_values ? bc->robin_values[c] : 0.0;
-      const int run = brun0 + (int)bruns.size();
-      bruns.push_back(r);
-      for (const IBox &P : outs) {
-        const int nx = P.hi[0] - P.lo[0] + 1, ny = P.hi[1] - P.lo[1] + 1;
-        const int vol = (int)P.volume();
-        for (int q = 0; q < vol; q += 1024) {
-          BcChunk ch;
-          ch.run = run;
-          for (int d = 0; d < 3; ++d)
-            ch.blo[d] = P.lo[d];
-          ch.bnx = nx;
-          ch.bny = ny;
-          ch.first = q;
-          ch.count = std::min(1024, vol - q);
-          bchunks.push_back(ch);
-        }
+    add_bc_runs(*bc, op->d.indextype, E, cdom, nullptr, off, E.volume(), ncomp,
+                brun0, bruns, bchunks);
+  };
+  // copies of the old fine level's valid data into fine FAB i (remake)
+  int64_t copied = 0;
+  auto add_fab_copies = [&](int i) {
+    const IBox B = ibox(fine->fabs[i].lo, fine->fabs[i].hi);
+    for (int j = 0; j < old->nfabs; ++j) {
+      const IBox X = intersect(B, ibox(old->valid + 6 * j, old->valid + 6 * j + 3));
+      if (X.empty())
+        continue;
+      copied += X.volume();
+      const int nx = X.hi[0] - X.lo[0] + 1, ny = X.hi[1] - X.lo[1] + 1;
+      const int nrows = ny * (X.hi[2] - X.lo[2] + 1);
+      const int per = std::max(1, 2048 / nx);
+      for (int c = 0; c < ncomp; ++c) {
+        const DFab f = dfab(&old->fabs[j], fine_comp + c);
+        const DFab t = dfab(&fine->fabs[i], fine_comp + c);
+        GatherRun r;
+        r.src = f.p + didx(f, X.lo[0], X.lo[1], X.lo[2]);
+        r.src_sy = f.sy;
+        r.src_sz = f.sz;
+        r.dstp = t.p;
+        r.dst = didx(t, X.lo[0], X.lo[1], X.lo[2]);
+        r.dst_sy = t.sy;
+        r.dst_sz = t.sz;
+        r.nx = nx;
+        r.ny = ny;
+        const int run = run0 + (int)runs.size();
+        runs.push_back(r);
+        for (int row = 0; row < nrows; row += per)
+          chunks.push_back({run, row, std::min(per, nrows - row)});
       }
     }
   };
@@ -1693,10 +1809,21 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
     const cxp_fab &ff = fine->fabs[i];
     const IBox fv = ibox(fine->valid + 6 * i, fine->valid + 6 * i + 3);
     std::vector<IBox> pieces;
-    subtract(intersect(ibox(ff.lo, ff.hi), fdom), fv, pieces);
-    for (int j = 0; j < fine->nfabs; ++j)
-      if (j != i)
-        subtract_all(pieces, ibox(fine->valid + 6 * j, fine->valid + 6 * j + 3));
+    if (mode == FillMode::ghosts) {
+      subtract(intersect(ibox(ff.lo, ff.hi), fdom), fv, pieces);
+      for (int j = 0; j < fine->nfabs; ++j)
+        if (j != i)
+          subtract_all(pieces, ibox(fine->valid + 6 * j, fine->valid + 6 * j + 3));
+    } else {
+      const IBox B = intersect(ibox(ff.lo, ff.hi), fdom);
+      if (!B.empty())
+        pieces.push_back(B);
+      if (mode == FillMode::remake) {
+        for (int j = 0; j < old->nfabs; ++j)
+          subtract_all(pieces, ibox(old->valid + 6 * j, old->valid + 6 * j + 3));
+        add_fab_copies(i);
+      }
+    }
     std::vector<IBox> cboxes;
     for (const IBox &P : pieces) {
       IBox C;
@@ -1782,6 +1909,9 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
     }
     coarse_pts += union_points(cboxes);
   }
+  if (!gather_chunks_ok(chunks, run0, runs.size()) ||
+      !bc_chunks_ok(bchunks, brun0, bruns))
+    return CXP_ERR_STATE;
   wl->patches.insert(wl->patches.end(), patches.begin(), patches.end());
   wl->gather_runs.insert(wl->gather_runs.end(), runs.begin(), runs.end());
   wl->gather_chunks.insert(wl->gather_chunks.end(), chunks.begin(), chunks.end());
@@ -1790,7 +1920,7 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
   wl->gather_doubles = gather;
   wl->uploaded = false;
   wl->fine_points += fine_pts * ncomp;
-  wl->algo_bytes += 8 * (int64_t)ncomp * (fine_pts + coarse_pts);
+  wl->algo_bytes += 8 * (int64_t)ncomp * (fine_pts + coarse_pts + 2 * copied);
   return CXP_OK;
 }
 
@@ -1810,6 +1940,74 @@ int cxp_worklist_append_level_bc(cxp_worklist wl, cxp_op op,
                                  int ncomp, const cxp_bc *bc) {
   return append_level_impl(wl, op, crse, crse_comp, fine, fine_comp, ncomp,
                            bc);
+}
+
+int cxp_worklist_append_new_level(cxp_worklist wl, cxp_op op,
+                                  const cxp_level *crse, int crse_comp,
+                                  const cxp_level *fine, int fine_comp,
+                                  int ncomp, const cxp_bc *bc) {
+  return append_level_impl(wl, op, crse, crse_comp, fine, fine_comp, ncomp,
+                           bc, FillMode::whole);
+}
+
+int cxp_worklist_append_remake_level(cxp_worklist wl, cxp_op op,
+                                     const cxp_level *crse, int crse_comp,
+                                     const cxp_level *old_fine,
+                                     const cxp_level *fine, int fine_comp,
+                                     int ncomp, const cxp_bc *bc) {
+  if (!old_fine)
+    return CXP_ERR_ARG;
+  return append_level_impl(wl, op, crse, crse_comp, fine, fine_comp, ncomp,
+                           bc, FillMode::remake, old_fine);
+}
+
+int cxp_worklist_append_boundaries(cxp_worklist wl, const cxp_level *level,
+                                   const int indextype[3], int comp,
+                                   int ncomp, const cxp_bc *bc) {
+  if (!wl || !level_ok(level) || !indextype || !bc || ncomp < 1 ||
+      !bc_ok(*bc, ncomp))
+    return CXP_ERR_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (indextype[d] != 0 && indextype[d] != 1)
+      return CXP_ERR_ARG;
+  for (int i = 0; i < level->nfabs; ++i) {
+    if (comp < 0 || comp + ncomp > level->fabs[i].ncomp)
+      return CXP_ERR_COMP;
+    if (fab_component_bytes(&level->fabs[i]) > kMaxComponentBytes)
+      return CXP_ERR_ARG;
+  }
+  const IBox dom = ibox(level->domain_lo, level->domain_hi);
+  std::vector<BcRun> runs;
+  std::vector<BcChunk> chunks;
+  for (int i = 0; i < level->nfabs; ++i) {
+    const cxp_fab &f = level->fabs[i];
+    const IBox B = ibox(f.lo, f.hi);
+    if (inside(B, dom))
+      continue;
+    // every source of the FAB's boundary points must lie in the FAB, as
+    // the reference asserts (boundaries_impl.hxx:249-298); faces without a
+    // condition are left alone
+    cxp_bc only = *bc;
+    for (int side = 0; side < 2; ++side)
+      for (int d = 0; d < 3; ++d)
+        if (only.type[side][d] == CXP_BC_NONE)
+          only.type[side][d] = CXP_BC_DIRICHLET;  // reads nothing
+    OpDesc od;
+    for (int d = 0; d < 3; ++d)
+      od.indextype[d] = indextype[d];
+    IBox E;
+    if (bc_closure(only, od, dom, B, E) || !inside(E, B))
+      return CXP_ERR_ARG;
+    const DFab d0 = dfab(&f, comp);
+    add_bc_runs(*bc, indextype, B, dom, d0.p, 0, (int64_t)f.comp_stride, ncomp,
+                (int)wl->post_runs.size(), runs, chunks);
+  }
+  if (!bc_chunks_ok(chunks, (int)wl->post_runs.size(), runs))
+    return CXP_ERR_STATE;
+  wl->post_runs.insert(wl->post_runs.end(), runs.begin(), runs.end());
+  wl->post_chunks.insert(wl->post_chunks.end(), chunks.begin(), chunks.end());
+  wl->uploaded = false;
+  return CXP_OK;
 }
 
 int cxp_worklist_upload(cxp_worklist wl, void *stream) {
@@ -1860,14 +2058,17 @@ int cxp_prolongate_batched(cxp_worklist wl, void *stream) {
     return CXP_ERR_STATE;
   if (const int e = launch_gather(wl, (hipStream_t)stream))
     return e;
-  return launch_batched(wl, (hipStream_t)stream);
+  if (const int e = launch_batched(wl, (hipStream_t)stream))
+    return e;
+  return launch_post_bc(wl, (hipStream_t)stream);
 }
 
 int cxp_prolongate_per_patch(cxp_worklist wl, void *stream) {
   if (!wl)
     return CXP_ERR_ARG;
-  if (wl->gather_doubles > 0) {
-    // gathered coarse patches live on the device only after an upload
+  if (!wl->gather_runs.empty() || !wl->post_runs.empty()) {
+    // gathers, copies and boundary runs live on the device only after an
+    // upload
     if (!wl->uploaded)
       return CXP_ERR_STATE;
     if (const int e = launch_gather(wl, (hipStream_t)stream))
@@ -1880,7 +2081,7 @@ int cxp_prolongate_per_patch(cxp_worklist wl, void *stream) {
     if (e)
       return e;
   }
-  return CXP_OK;
+  return launch_post_bc(wl, (hipStream_t)stream);
 }
 
 // ---- graphs

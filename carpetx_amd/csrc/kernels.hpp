@@ -421,7 +421,8 @@ __global__ void __launch_bounds__(256)
 
 struct GatherRun {
   const double *src;  // the region's first point in the source component
-  long long dst;      // its offset (doubles) in the patch buffer
+  double *dstp;       // destination component, or null: the patch buffer
+  long long dst;      // the region's first point (doubles) from dstp
   int src_sy;
   long long src_sz;
   int dst_sy;
@@ -438,6 +439,7 @@ __global__ void __launch_bounds__(256)
              const GatherChunk *__restrict__ chunks, double *__restrict__ buf) {
   const GatherChunk ch = chunks[blockIdx.x];
   const GatherRun r = runs[ch.run];
+  double *const d = r.dstp ? r.dstp : buf;
   const int npts = ch.nrows * r.nx;
   for (int i = threadIdx.x; i < npts; i += 256) {
     const int rr = i / r.nx;
@@ -446,19 +448,20 @@ __global__ void __launch_bounds__(256)
     const int z = row / r.ny;
     const int y = row - z * r.ny;
     const double v = gld(r.src, x + (long long)y * r.src_sy + z * r.src_sz);
-    gst(buf, r.dst + x + (long long)y * r.dst_sy + z * r.dst_sz, v);
+    gst(d, r.dst + x + (long long)y * r.dst_sy + z * r.dst_sz, v);
   }
 }
 
-// Boundary values of a gathered coarse patch: every point of the patch box
-// outside the domain [imin, imax) takes the value CarpetX's
-// BoundaryCondition::apply gives it (CarpetX/src/boundaries_impl.hxx:
-// 322-437), from the patch's own points inside the domain.  Sources always
-// lie inside the domain, so no point both reads and is written.  A run is
-// one (patch, component); a chunk is a range of the points (x fastest) of
-// one box of the patch that lies outside the domain.
+// Boundary values of a box (a gathered coarse patch, or a FAB in place):
+// every point of the box outside the domain [imin, imax) takes the value
+// CarpetX's BoundaryCondition::apply gives it (CarpetX/src/
+// boundaries_impl.hxx:322-437), from the box's own points: inside the
+// domain in every direction whose face has a condition.  So no point both
+// reads and is written.  A run is one (box, component); a chunk is a range
+// of the points (x fastest) of one part of the box outside the domain.
 struct BcRun {
-  long long off;        // the component's first point in the patch buffer
+  double *p;            // the component's box, or null: the patch buffer
+  long long off;        // the component's first point from p
   int lo[3], hi[3];     // patch box
   int imin[3], imax[3]; // domain, imax exclusive
   int type[2][3];       // CXP_BC_* per (face, direction)
@@ -480,7 +483,7 @@ __global__ void __launch_bounds__(256)
   const BcChunk ch = chunks[blockIdx.x];
   const BcRun r = runs[ch.run];
   const int nx = r.hi[0] - r.lo[0] + 1, ny = r.hi[1] - r.lo[1] + 1;
-  double *v = buf + r.off;
+  double *v = (r.p ? r.p : buf) + r.off;
   auto at = [&](const int p[3]) {
     return (long long)(p[0] - r.lo[0]) + (long long)(p[1] - r.lo[1]) * nx +
            (long long)(p[2] - r.lo[2]) * nx * ny;

@@ -223,15 +223,52 @@ int cxp_worklist_append_level_bc(cxp_worklist wl, cxp_op op,
                                  const cxp_level *crse, int crse_comp,
                                  const cxp_level *fine, int fine_comp,
                                  int ncomp, const cxp_bc *bc);
+/* A new fine level's fill, FillPatch_NewLevel (CarpetX/src/fillpatch.cxx:
+ * 140-181) without its MPI copies: each fine FAB's whole box within the
+ * fine domain (valid and ghost points; the domain is not grown, as the
+ * reference grows it in periodic directions only) is prolongated from the
+ * coarse level, coarse patches as in cxp_worklist_append_level_bc (bc may
+ * be null: the coarse FABs' boundary ghosts are read).  The fine level's
+ * own boundary conditions are cxp_worklist_append_boundaries. */
+int cxp_worklist_append_new_level(cxp_worklist wl, cxp_op op,
+                                  const cxp_level *crse, int crse_comp,
+                                  const cxp_level *fine, int fine_comp,
+                                  int ncomp, const cxp_bc *bc);
+/* A regridded fine level's fill, FillPatch_RemakeLevel (fillpatch.cxx:
+ * 183-227): in each new fine FAB, the points within the fine domain grown
+ * by the ghost width that lie in no valid region of the old fine level are
+ * prolongated (coarse patches as in cxp_worklist_append_level_bc), and the
+ * old level's valid data is copied into the rest of the FAB's box, on the
+ * device at every run.  Both levels hold components fine_comp ..
+ * fine_comp + ncomp - 1; the old and new FABs must not overlap in memory. */
+int cxp_worklist_append_remake_level(cxp_worklist wl, cxp_op op,
+                                     const cxp_level *crse, int crse_comp,
+                                     const cxp_level *old_fine,
+                                     const cxp_level *fine, int fine_comp,
+                                     int ncomp, const cxp_bc *bc);
+/* A level's own boundary conditions, in place, after the list's fills
+ * (groupdata.apply_boundary_conditions(mfab), fillpatch.cxx:133,180,225;
+ * BoundaryCondition::apply, boundaries_impl.hxx:150-437): in every FAB,
+ * each point outside the domain in a direction whose face has a condition
+ * takes its boundary value, computed from the FAB's own points.  Faces
+ * with CXP_BC_NONE are left alone.  indextype: 1 for cell-centred
+ * directions.  CXP_ERR_ARG if a FAB does not hold the sources of its
+ * boundary points (the reference asserts this), or as for
+ * cxp_worklist_append_level_bc. */
+int cxp_worklist_append_boundaries(cxp_worklist wl, const cxp_level *level,
+                                   const int indextype[3], int comp,
+                                   int ncomp, const cxp_bc *bc);
 /* Builds tiles (using the tuned tile shapes) and copies the list to the
  * device; must precede cxp_prolongate_batched. */
 int cxp_worklist_upload(cxp_worklist wl, void *stream);
 int cxp_worklist_stats(cxp_worklist wl, int64_t *npatches, int64_t *ntiles,
                        int64_t *fine_points, int64_t *algo_bytes);
 /* One launch of the persistent kernel over the whole list (plus one
- * point-kernel launch per ENO/fallback patch, and first the gather of a
- * level fill's coarse patches).  Launches of one list must not overlap: the
- * list's dynamic-tail claim counters and gathered patches are its own. */
+ * point-kernel launch per ENO/fallback patch; first the gather of the level
+ * fills' coarse patches and old-level copies, then the coarse patches'
+ * boundary values; last the levels' own boundary conditions).  Launches of
+ * one list must not overlap: the list's dynamic-tail claim counters and
+ * gathered patches are its own. */
 int cxp_prolongate_batched(cxp_worklist wl, void *stream);
 /* The plain path over the same list: one cxp_prolongate launch per patch. */
 int cxp_prolongate_per_patch(cxp_worklist wl, void *stream);

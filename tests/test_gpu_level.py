@@ -73,17 +73,20 @@ def coarse_level(it, g, cg=CG, poison_outside=False):
     return Level(fabs, valid, cdom)
 
 
-def fine_level(it, rng):
+def fine_level(it, rng, boxes=FINE_VALID, fill_valid=True):
+    """Fine FABs of the given valid boxes grown by FG: uniform[-1, 1] in the
+    valid region (7.0 with fill_valid False), 7.0 in the ghosts."""
     _, fdom = domains(it)
     fabs, valid = [], []
-    for lo, hi in FINE_VALID:
+    for lo, hi in boxes:
         hi = tuple(min(hi[d], fdom.hi[d]) for d in range(3))
         v = Box(lo, hi)
         b = v.grow(FG)
         t = torch.full((NCOMP,) + tuple(reversed(b.size)), 7.0, dtype=torch.float64)
         inner = tuple(slice(FG, FG + v.size[d]) for d in (2, 1, 0))
-        t[(slice(None),) + inner] = torch.from_numpy(
-            rng.uniform(-1, 1, (NCOMP,) + tuple(reversed(v.size))))
+        if fill_valid:
+            t[(slice(None),) + inner] = torch.from_numpy(
+                rng.uniform(-1, 1, (NCOMP,) + tuple(reversed(v.size))))
         fabs.append(FArrayBox(b, NCOMP, tensor=t.cuda()))
         valid.append(v)
     return Level(fabs, valid, fdom)
@@ -295,4 +298,185 @@ def test_level_fill_with_boundary_conditions_matches_oracle(op_args, faces):
     torch.cuda.synchronize()
     check("new coarse data", want2)
     graph.close()
+    wl.close()
+
+
+NEW_VALID = [((0, 0, 0), (15, 11, 9)), ((16, 0, 0), (31, 13, 9)),
+             ((30, 30, 30), (47, 47, 47))]
+REGRID_CASES = [
+    (("natural", 3, (1, 1, 1)), None, None),
+    (("natural", 5, (0, 0, 0)),
+     [["reflection", "linear_extrapolation", "robin"],
+      ["dirichlet", "neumann", "reflection"]],
+     [["neumann", "reflection", "linear_extrapolation"],
+      ["reflection", "dirichlet", "neumann"]]),
+    (("eno", 3, (1, 0, 1)),
+     [["reflection"] * 3, ["linear_extrapolation"] * 3],
+     [["linear_extrapolation"] * 3, ["reflection"] * 3]),
+]
+BC_KW = dict(dirichlet=[0.5, -1.5], robin=[0.25, 2.0],
+             parities=[(-1, 1, -1), (1, -1, 1)],
+             xmin=(0.37, -1.21, 0.53), dx=(0.05, 0.07, 0.06))
+
+
+def _bc(faces):
+    return None if faces is None else BoundaryConditions(
+        faces, dirichlet_values=BC_KW["dirichlet"], robin_values=BC_KW["robin"],
+        parities=BC_KW["parities"], xmin=BC_KW["xmin"], dx=BC_KW["dx"])
+
+
+def _codes(faces):
+    return [[BC_KINDS[faces[f][d]] for d in range(3)] for f in range(2)]
+
+
+def _coarse_global(it, g, faces):
+    """The global coarse array as the fill must see it: boundary values from
+    the conditions, or the array itself without them."""
+    if faces is None:
+        return g
+    cdom, _ = domains(it)
+    glo = tuple(cdom.lo[d] - CG for d in range(3))
+    gb = g.copy()
+    sub = np.ascontiguousarray(gb[1:1 + NCOMP])
+    apply_bc(sub, glo, cdom.lo, cdom.hi, _codes(faces), it, **BC_KW)
+    gb[1:1 + NCOMP] = sub
+    return gb
+
+
+def _prolongated(op_args, it, gc, fab):
+    """The oracle's prolongation of the whole fine FAB box from the global
+    coarse array gc."""
+    cdom, _ = domains(it)
+    glo = tuple(cdom.lo[d] - CG for d in range(3))
+    ghi = tuple(cdom.hi[d] + CG for d in range(3))
+    ch = oracle.Fab(glo, ghi, NCOMP, np.ascontiguousarray(gc[1:1 + NCOMP]))
+    b = fab.box
+    fh = oracle.Fab(b.lo, b.hi, NCOMP, np.zeros((NCOMP,) + tuple(reversed(b.size))))
+    oracle.prolongate(oracle.lookup(*op_args), ch, fh, b.lo, b.hi)
+    return fh.arr
+
+
+def _mask(b, lo, hi):
+    idx = np.meshgrid(*[np.arange(b.lo[d], b.hi[d] + 1) for d in (2, 1, 0)], indexing="ij")
+    m = np.ones(idx[0].shape, dtype=bool)
+    for a, d in zip(idx, (2, 1, 0)):
+        m &= (a >= lo[d]) & (a <= hi[d])
+    return m
+
+
+def _run_all_ways(wl, fabs, originals, want, tag):
+    """Batched, per-patch and two graph replays, each from the original
+    fine data, compared with want within 1e-13."""
+    s = torch.cuda.Stream()
+
+    def reset():
+        for f, o in zip(fabs, originals):
+            f.t.copy_(o)
+        torch.cuda.synchronize()
+
+    def check(how):
+        for f, w in zip(fabs, want):
+            np.testing.assert_allclose(f.t.cpu().numpy(), w, rtol=1e-13, atol=1e-13,
+                                       err_msg=str((how, tag)))
+
+    with torch.cuda.stream(s):
+        wl.upload(s)
+        wl.run(s)
+    torch.cuda.synchronize()
+    check("batched")
+    reset()
+    with torch.cuda.stream(s):
+        wl.run_per_patch(s)
+    torch.cuda.synchronize()
+    check("per patch")
+    with torch.cuda.stream(s):
+        graph = Graph(lambda: wl.run(s), s)
+    for rep in range(2):
+        reset()
+        with torch.cuda.stream(s):
+            graph.launch(s)
+        torch.cuda.synchronize()
+        check(("graph", rep))
+    graph.close()
+
+
+@pytest.mark.parametrize("op_args,cfaces,ffaces", REGRID_CASES)
+def test_new_level_matches_oracle(op_args, cfaces, ffaces):
+    """cxp_worklist_append_new_level (FillPatch_NewLevel): every fine FAB's
+    box within the fine domain is prolongated, valid points included;
+    points outside the fine domain are left alone or, with the fine level's
+    conditions appended (cxp_worklist_append_boundaries), take their
+    boundary values after the fill, as the oracle's conditions give them on
+    each prolongated FAB."""
+    it = op_args[2]
+    rng = np.random.default_rng(11 + sum(it) + op_args[1])
+    cdom, fdom = domains(it)
+    gshape = (NC_CRSE,) + tuple(cdom.size[d] + 2 * CG for d in (2, 1, 0))
+    g = rng.uniform(-1, 1, gshape)
+    crse = coarse_level(it, g, poison_outside=cfaces is not None)
+    fine = fine_level(it, rng, boxes=NEW_VALID, fill_valid=False)
+    originals = [f.t.clone() for f in fine.fabs]
+    gc = _coarse_global(it, g, cfaces)
+    want = []
+    for fab, orig in zip(fine.fabs, originals):
+        p = _prolongated(op_args, it, gc, fab)
+        w = np.where(_mask(fab.box, fdom.lo, fdom.hi)[None], p, orig.cpu().numpy())
+        if ffaces is not None:
+            w = np.ascontiguousarray(w)
+            apply_bc(w, fab.box.lo, fdom.lo, fdom.hi, _codes(ffaces), it, **BC_KW)
+        want.append(w)
+    wl = WorkList()
+    wl.append_new_level(get_interpolator(*op_args), crse, 1, fine, 0, NCOMP, bc=_bc(cfaces))
+    if ffaces is not None:
+        wl.append_boundaries(fine, it, 0, NCOMP, _bc(ffaces))
+    _run_all_ways(wl, fine.fabs, originals, want, ("new level", op_args))
+    wl.close()
+
+
+@pytest.mark.parametrize("op_args,cfaces,ffaces", REGRID_CASES)
+def test_remake_level_matches_oracle(op_args, cfaces, ffaces):
+    """cxp_worklist_append_remake_level (FillPatch_RemakeLevel): in each new
+    fine FAB the old fine level's valid data where an old valid box covers
+    the point, the oracle's prolongation elsewhere within the fine domain
+    grown by the ghost width, the original value outside it; then the fine
+    level's conditions when appended."""
+    it = op_args[2]
+    rng = np.random.default_rng(23 + sum(it) + op_args[1])
+    cdom, fdom = domains(it)
+    gshape = (NC_CRSE,) + tuple(cdom.size[d] + 2 * CG for d in (2, 1, 0))
+    g = rng.uniform(-1, 1, gshape)
+    crse = coarse_level(it, g, poison_outside=cfaces is not None)
+    old = fine_level(it, rng)
+    fine = fine_level(it, rng, boxes=NEW_VALID, fill_valid=False)
+    originals = [f.t.clone() for f in fine.fabs]
+    gc = _coarse_global(it, g, cfaces)
+    want = []
+    for fab, orig in zip(fine.fabs, originals):
+        b = fab.box
+        w = orig.cpu().numpy().copy()
+        grown = (tuple(fdom.lo[d] - FG for d in range(3)), tuple(fdom.hi[d] + FG for d in range(3)))
+        p = _prolongated(op_args, it, gc, fab)
+        w = np.where(_mask(b, *grown)[None], p, w)
+        for ofab, ov in zip(old.fabs, old.valid):
+            m = _mask(b, ov.lo, ov.hi)
+            if not m.any():
+                continue
+            # the old FAB's values at the same global points
+            x = ofab.t.cpu().numpy()
+            lo = [max(b.lo[d], ov.lo[d]) for d in range(3)]
+            hi = [min(b.hi[d], ov.hi[d]) for d in range(3)]
+            src = x[(slice(None),) + tuple(slice(lo[d] - ofab.box.lo[d], hi[d] - ofab.box.lo[d] + 1)
+                                           for d in (2, 1, 0))]
+            w[(slice(None),) + tuple(slice(lo[d] - b.lo[d], hi[d] - b.lo[d] + 1)
+                                     for d in (2, 1, 0))] = src
+        if ffaces is not None:
+            w = np.ascontiguousarray(w)
+            apply_bc(w, b.lo, fdom.lo, fdom.hi, _codes(ffaces), it, **BC_KW)
+        want.append(w)
+    wl = WorkList()
+    wl.append_remake_level(get_interpolator(*op_args), crse, 1, old, fine, 0, NCOMP,
+                           bc=_bc(cfaces))
+    if ffaces is not None:
+        wl.append_boundaries(fine, it, 0, NCOMP, _bc(ffaces))
+    _run_all_ways(wl, fine.fabs, originals, want, ("remake", op_args))
     wl.close()

@@ -353,6 +353,121 @@ def test_level_append_with_boundary_conditions_checks_its_arguments():
         assert L.cxp_worklist_destroy(wl) == 0
 
 
+def _count(boxes_fn, shape_lo, shape_hi):
+    """Points p of the box [shape_lo, shape_hi] with boxes_fn(x, y, z) true."""
+    import numpy as np
+    ax = [np.arange(shape_lo[d], shape_hi[d] + 1) for d in range(3)]
+    z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    return int(boxes_fn(x, y, z).sum())
+
+
+def _in(x, y, z, lo, hi):
+    return ((x >= lo[0]) & (x <= hi[0]) & (y >= lo[1]) & (y <= hi[1]) &
+            (z >= lo[2]) & (z <= hi[2]))
+
+
+def test_new_and_remake_level_partition_the_points():
+    """cxp_worklist_append_new_level prolongates each fine FAB's box within
+    the fine domain; cxp_worklist_append_remake_level the points of each new
+    FAB within the grown fine domain that no old valid box covers, counted
+    point by point; the copied points add 16 bytes each.  A missing old
+    level or too few old components fail and add nothing."""
+    L = _lib.lib()
+    n = 24
+    new_valid = [((0, 0, 0), (15, 11, 9)), ((16, 0, 0), (31, 13, 9)),
+                 ((30, 30, 30), (47, 47, 47))]
+    old_valid = [((0, 0, 0), (9, 9, 9)), ((10, 0, 0), (21, 9, 9)),
+                 ((16, 14, 12), (31, 29, 27)), ((36, 36, 36), (47, 47, 47))]
+    for it in [(0, 0, 0), (1, 1, 1), (1, 0, 1)]:
+        cdom = ((0, 0, 0), tuple(n - it[d] for d in range(3)))
+        fdom = ((0, 0, 0), tuple(2 * n - it[d] for d in range(3)))
+        cboxes = []
+        for c in range(8):
+            cboxes.append((tuple(0 if not (c >> d) & 1 else 12 for d in range(3)),
+                           tuple(11 if not (c >> d) & 1 else cdom[1][d] for d in range(3))))
+        h = _lookup("natural", 3, it)[1]
+        crse, kc = _level(cboxes, cdom, 3, 6)
+        fine, kf = _level(new_valid, fdom, 2, 3)
+        old, ko = _level(old_valid, fdom, 2, 3)
+        v = [ctypes.c_int64() for _ in range(4)]
+        wl = ctypes.c_void_p()
+        assert L.cxp_worklist_create(ctypes.byref(wl)) == 0
+        assert L.cxp_worklist_append_new_level(wl, h, ctypes.byref(crse), 1,
+                                               ctypes.byref(fine), 0, 2, None) == 0
+        assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+        want = 0
+        for lo, hi in new_valid:
+            blo = tuple(max(lo[d] - 3, fdom[0][d]) for d in range(3))
+            bhi = tuple(min(hi[d] + 3, fdom[1][d]) for d in range(3))
+            want += _count(lambda x, y, z: x == x, blo, bhi)
+        assert v[2].value == 2 * want and v[0].value == len(new_valid), it
+        assert L.cxp_worklist_destroy(wl) == 0
+
+        assert L.cxp_worklist_create(ctypes.byref(wl)) == 0
+        assert L.cxp_worklist_append_remake_level(
+            wl, h, ctypes.byref(crse), 1, ctypes.byref(old), ctypes.byref(fine), 0, 2, None) == 0
+        assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+        prol = copied = 0
+        for lo, hi in new_valid:
+            blo = tuple(lo[d] - 3 for d in range(3))
+            bhi = tuple(hi[d] + 3 for d in range(3))
+
+            def in_old(x, y, z):
+                m = _in(x, y, z, *old_valid[0])
+                for b in old_valid[1:]:
+                    m = m | _in(x, y, z, *b)
+                return m
+            gdom = (tuple(-3 for d in range(3)), tuple(fdom[1][d] + 3 for d in range(3)))
+            prol += _count(lambda x, y, z: _in(x, y, z, *gdom) & ~in_old(x, y, z), blo, bhi)
+            copied += _count(in_old, blo, bhi)
+        assert v[2].value == 2 * prol, it
+        # prolongation bytes (fine + coarse) plus 16 B per copied point
+        assert v[3].value > 2 * 16 * copied
+        npatch = v[0].value
+        assert L.cxp_worklist_append_remake_level(
+            wl, h, ctypes.byref(crse), 1, None, ctypes.byref(fine), 0, 2, None) == -3
+        few, kk = _level(old_valid, fdom, 1, 3)
+        assert L.cxp_worklist_append_remake_level(
+            wl, h, ctypes.byref(crse), 1, ctypes.byref(few), ctypes.byref(fine), 0, 2, None) == -9
+        assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+        assert v[0].value == npatch
+        assert L.cxp_worklist_destroy(wl) == 0
+
+
+def test_level_boundaries_check_their_sources():
+    """cxp_worklist_append_boundaries: FABs whose boxes hold the sources of
+    their boundary points are accepted, faces without a condition are left
+    alone; a FAB too thin for a reflection's mirror points, a bad index
+    type or no conditions fail with CXP_ERR_ARG."""
+    L = _lib.lib()
+    fdom = ((0, 0, 0), (47, 47, 47))
+    it = (1, 1, 1)
+    ok_level, k1 = _level([((0, 0, 0), (15, 15, 15)), ((32, 32, 32), (47, 47, 47))], fdom, 2, 3)
+    D, LE, NM, RF = 1, 2, 3, 5
+    b, kb = _bc([[RF, LE, NM], [D, 0, RF]], 2)
+    wl = ctypes.c_void_p()
+    assert L.cxp_worklist_create(ctypes.byref(wl)) == 0
+    assert L.cxp_worklist_append_boundaries(wl, ctypes.byref(ok_level), i3(it), 0, 2,
+                                            ctypes.byref(b)) == 0
+    # a FAB whose box ends at x = 1 cannot hold the mirror of x = -3 (x = 2)
+    thin, k2 = _level([((0, 0, 0), (1, 15, 15))], fdom, 2, 3)
+    f = k2[0][0]
+    f.hi = i3((1, 18, 18))
+    f.comp_stride = 5 * 22 * 22
+    assert L.cxp_worklist_append_boundaries(wl, ctypes.byref(thin), i3(it), 0, 2,
+                                            ctypes.byref(b)) == -3
+    nb, k3 = _bc([[NM, NM, NM], [NM, NM, NM]], 2)
+    assert L.cxp_worklist_append_boundaries(wl, ctypes.byref(thin), i3(it), 0, 2,
+                                            ctypes.byref(nb)) == 0
+    assert L.cxp_worklist_append_boundaries(wl, ctypes.byref(ok_level), i3((2, 0, 0)), 0, 2,
+                                            ctypes.byref(b)) == -3
+    assert L.cxp_worklist_append_boundaries(wl, ctypes.byref(ok_level), i3(it), 0, 2,
+                                            None) == -3
+    assert L.cxp_worklist_append_boundaries(wl, ctypes.byref(ok_level), i3(it), 1, 2,
+                                            ctypes.byref(b)) == -9
+    assert L.cxp_worklist_destroy(wl) == 0
+
+
 def test_cpp_host_header_compiles():
     """The C++ host API (carpetx_amd/host/cxp.hpp) is self-contained C++17
     over the C header."""
