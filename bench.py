@@ -210,6 +210,53 @@ def bench_level_fill(args, stream, log):
     return out
 
 
+def bench_regrid(args, stream, log):
+    """Regrid-time fills of the 64-box level: a new level of 64 boxes (half
+    at old boxes' slots) filled whole from the coarse level (new_level), or
+    from the old fine level where it has valid data and the coarse level
+    elsewhere (remake_level); both with reflection on the lower faces and
+    linear extrapolation on the upper ones, on the coarse patches and then
+    on the new level's own boundary points."""
+    import torch
+    from carpetx_amd import workloads
+    from carpetx_amd.batched import BoundaryConditions, Graph, WorkList
+    lv = workloads.level_fill_64(regrid=True)
+    out = {}
+    for name, d in lv.items():
+        bc = BoundaryConditions([["reflection"] * 3, ["linear_extrapolation"] * 3],
+                                parities=[(1, 1, 1)] * d["ncomp"])
+        for kind in ("new_level", "remake_level"):
+            wl = WorkList()
+            if kind == "new_level":
+                wl.append_new_level(d["op"], d["crse"], 0, d["new"], 0, d["ncomp"], bc=bc)
+            else:
+                wl.append_remake_level(d["op"], d["crse"], 0, d["fine"], d["new"], 0,
+                                       d["ncomp"], bc=bc)
+            wl.append_boundaries(d["new"], d["indextype"], 0, d["ncomp"], bc)
+            with torch.cuda.stream(stream):
+                wl.upload(stream)
+                if not args.no_tune:
+                    wl.autotune(reps=5, stream=stream)
+                g = Graph(lambda: wl.run(stream), stream)
+                for _ in range(3):
+                    g.launch(stream)
+                mb = back_to_back(lambda: g.launch(stream), 30, stream)
+                wl.run_per_patch(stream)
+                mp = back_to_back(lambda: wl.run_per_patch(stream), 5, stream)
+            st = wl.stats()
+            key = "%s_%s" % (name, kind)
+            out[key] = dict(
+                patches=st["patches"], fine_pts=st["fine_points"],
+                algo_MB=round(st["algo_bytes"] / 1e6, 2),
+                batched_ms=round(mb, 4), per_patch_ms=round(mp, 4),
+                batched_pts_per_s=st["fine_points"] / (mb * 1e-3),
+                batched_hbm_frac=round(st["algo_bytes"] / (mb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+            log("regrid %s: %s" % (key, out[key]))
+            g.close()
+            wl.close()
+    return out
+
+
 def bench_adjoint(stream, log):
     """96^3 -> 192^3 VC 5th-order prolongation: forward vs adjoint (per-box
     launches on one stream), plus the CCC minmod forward (its adjoint is not
@@ -441,6 +488,7 @@ def main():
     if not args.no_extra:
         line["extra"]["level_64"] = bench_level64(args, stream, log)
         line["extra"]["level_fill_64"] = bench_level_fill(args, stream, log)
+        line["extra"]["regrid_64"] = bench_regrid(args, stream, log)
         line["extra"]["adjoint_96_192"] = bench_adjoint(stream, log)
         if not args.no_wave:
             line["extra"]["wave_two_level"] = bench_wave(log)

@@ -242,7 +242,58 @@ struct BoundaryConditions {
   std::array<double, 3> xmin{0.0, 0.0, 0.0}, dx{1.0, 1.0, 1.0};
 };
 
+namespace detail_ {
+// C views of a Level and of BoundaryConditions, valid while they live.
+struct CLevel {
+  std::vector<cxp_fab> fabs;
+  std::vector<int> valid;
+  cxp_level l;
+  explicit CLevel(const Level &L) {
+    if (L.fabs.size() != L.valid.size())
+      throw Error("Level: one valid box per FAB", CXP_ERR_ARG);
+    for (const Fab &f : L.fabs)
+      fabs.push_back(f.c());
+    for (const Box &b : L.valid)
+      for (const IntVect *v : {&b.lo, &b.hi})
+        valid.insert(valid.end(), v->begin(), v->end());
+    l.nfabs = (int)L.fabs.size();
+    l.fabs = fabs.data();
+    l.valid = valid.data();
+    for (int d = 0; d < 3; ++d) {
+      l.domain_lo[d] = L.domain.lo[d];
+      l.domain_hi[d] = L.domain.hi[d];
+    }
+  }
+};
+struct CBc {
+  std::vector<int> par;
+  cxp_bc b{};
+  explicit CBc(const BoundaryConditions *bc) {
+    if (bc)
+      set(*bc);
+  }
+  explicit CBc(const BoundaryConditions &bc) { set(bc); }
+  void set(const BoundaryConditions &bc) {
+    for (const IntVect &p : bc.parities)
+      par.insert(par.end(), p.begin(), p.end());
+    for (int side = 0; side < 2; ++side)
+      for (int d = 0; d < 3; ++d)
+        b.type[side][d] = bc.type[side][d];
+    b.dirichlet_values = bc.dirichlet.empty() ? nullptr : bc.dirichlet.data();
+    b.robin_values = bc.robin.empty() ? nullptr : bc.robin.data();
+    b.parities = par.empty() ? nullptr : par.data();
+    for (int d = 0; d < 3; ++d) {
+      b.xmin[d] = bc.xmin[d];
+      b.dx[d] = bc.dx[d];
+    }
+  }
+};
+} // namespace detail_
+
 class WorkList {
+  using CLevel = detail_::CLevel;
+  using CBc = detail_::CBc;
+
 public:
   WorkList() { check(cxp_worklist_create(&wl_), "cxp_worklist_create"); }
   ~WorkList() {
@@ -275,52 +326,49 @@ public:
   void append_level(const Prolongate3DRF2 &op, const Level &crse,
                     int crse_comp, const Level &fine, int fine_comp, int ncomp,
                     const BoundaryConditions *bc = nullptr) {
-    struct C {
-      std::vector<cxp_fab> fabs;
-      std::vector<int> valid;
-      cxp_level l;
-    };
-    auto conv = [](const Level &L, C &c) {
-      for (const Fab &f : L.fabs)
-        c.fabs.push_back(f.c());
-      for (const Box &b : L.valid)
-        for (const IntVect *v : {&b.lo, &b.hi})
-          c.valid.insert(c.valid.end(), v->begin(), v->end());
-      c.l.nfabs = (int)L.fabs.size();
-      c.l.fabs = c.fabs.data();
-      c.l.valid = c.valid.data();
-      for (int d = 0; d < 3; ++d) {
-        c.l.domain_lo[d] = L.domain.lo[d];
-        c.l.domain_hi[d] = L.domain.hi[d];
-      }
-      return L.fabs.size() == L.valid.size();
-    };
-    C c, f;
-    if (!conv(crse, c) || !conv(fine, f))
-      throw Error("append_level: one valid box per FAB", CXP_ERR_ARG);
+    CLevel c(crse), f(fine);
     if (!bc) {
       check(cxp_worklist_append_level(wl_, op.handle(), &c.l, crse_comp, &f.l,
                                       fine_comp, ncomp),
             "cxp_worklist_append_level");
       return;
     }
-    cxp_bc b;
-    std::vector<int> par;
-    for (const IntVect &p : bc->parities)
-      par.insert(par.end(), p.begin(), p.end());
-    for (int side = 0; side < 2; ++side)
-      for (int d = 0; d < 3; ++d)
-        b.type[side][d] = bc->type[side][d];
-    b.dirichlet_values = bc->dirichlet.empty() ? nullptr : bc->dirichlet.data();
-    b.robin_values = bc->robin.empty() ? nullptr : bc->robin.data();
-    b.parities = par.empty() ? nullptr : par.data();
-    for (int d = 0; d < 3; ++d) {
-      b.xmin[d] = bc->xmin[d];
-      b.dx[d] = bc->dx[d];
-    }
+    CBc b(*bc);
     check(cxp_worklist_append_level_bc(wl_, op.handle(), &c.l, crse_comp, &f.l,
-                                       fine_comp, ncomp, &b),
+                                       fine_comp, ncomp, &b.b),
           "cxp_worklist_append_level_bc");
+  }
+  // A new level's fill (FillPatch_NewLevel): whole fine boxes.
+  void append_new_level(const Prolongate3DRF2 &op, const Level &crse,
+                        int crse_comp, const Level &fine, int fine_comp,
+                        int ncomp, const BoundaryConditions *bc = nullptr) {
+    CLevel c(crse), f(fine);
+    CBc b(bc);
+    check(cxp_worklist_append_new_level(wl_, op.handle(), &c.l, crse_comp, &f.l,
+                                        fine_comp, ncomp, bc ? &b.b : nullptr),
+          "cxp_worklist_append_new_level");
+  }
+  // A regridded level's fill (FillPatch_RemakeLevel): the old fine level's
+  // valid data where it exists, prolongation elsewhere.
+  void append_remake_level(const Prolongate3DRF2 &op, const Level &crse,
+                           int crse_comp, const Level &old_fine,
+                           const Level &fine, int fine_comp, int ncomp,
+                           const BoundaryConditions *bc = nullptr) {
+    CLevel c(crse), o(old_fine), f(fine);
+    CBc b(bc);
+    check(cxp_worklist_append_remake_level(wl_, op.handle(), &c.l, crse_comp,
+                                           &o.l, &f.l, fine_comp, ncomp,
+                                           bc ? &b.b : nullptr),
+          "cxp_worklist_append_remake_level");
+  }
+  // The level's own boundary conditions, in place, after the fills.
+  void append_boundaries(const Level &level, const IntVect &indextype,
+                         int comp, int ncomp, const BoundaryConditions &bc) {
+    CLevel l(level);
+    CBc b(bc);
+    check(cxp_worklist_append_boundaries(wl_, &l.l, indextype.data(), comp,
+                                         ncomp, &b.b),
+          "cxp_worklist_append_boundaries");
   }
   void upload(void *stream = nullptr) {
     check(cxp_worklist_upload(wl_, stream), "cxp_worklist_upload");

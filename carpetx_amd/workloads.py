@@ -99,9 +99,13 @@ def level_64(seed=3, ncomp=4, nboxes=64, box_pts=32):
     return out
 
 
-def level_fill_64(seed=3, ncomp=4, nboxes=64, box_pts=32, ghosts=3, cghosts=6):
+def level_fill_64(seed=3, ncomp=4, nboxes=64, box_pts=32, ghosts=3, cghosts=6,
+                  regrid=False):
     """Returns {name: dict(op, crse, fine, ncomp)}, crse and fine Levels for
-    WorkList.append_level, for the two operators of level_64."""
+    WorkList.append_level, for the two operators of level_64.  With regrid,
+    each entry also holds "new": a fine Level of nboxes boxes on the same
+    slot lattice, half of them at old boxes' slots and half at slots the
+    old level leaves free (no data yet), for the regrid-time fills."""
     from .batched import Level
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
@@ -110,6 +114,19 @@ def level_fill_64(seed=3, ncomp=4, nboxes=64, box_pts=32, ghosts=3, cghosts=6):
     slot = box_pts // 2
     nslot = nc // slot
     slots = rng.choice(nslot ** 3, size=nboxes, replace=False)
+    free = np.setdiff1d(np.arange(nslot ** 3), slots)
+    new_slots = np.concatenate([slots[:nboxes // 2],
+                                rng.choice(free, size=nboxes - nboxes // 2, replace=False)])
+
+    def fine_level(slot_ids, fdom):
+        ffabs, fvalid = [], []
+        for s in slot_ids:
+            sk, sj, si = s // (nslot * nslot), (s // nslot) % nslot, s % nslot
+            lo = [2 * slot * si, 2 * slot * sj, 2 * slot * sk]
+            v = Box(lo, [lo[d] + box_pts - 1 for d in range(3)])
+            ffabs.append(FArrayBox(v.grow(ghosts), ncomp))
+            fvalid.append(v)
+        return Level(ffabs, fvalid, fdom)
     out = {}
     for name, typ, order, it in [("cons2_ccc", "conservative", 1, (1, 1, 1)),
                                  ("poly5_vvv", "interpolate", 5, (0, 0, 0))]:
@@ -126,16 +143,11 @@ def level_fill_64(seed=3, ncomp=4, nboxes=64, box_pts=32, ghosts=3, cghosts=6):
             cfabs.append(fab)
             cvalid.append(v)
         del whole
-        ffabs, fvalid = [], []
-        for s in slots:
-            sk, sj, si = s // (nslot * nslot), (s // nslot) % nslot, s % nslot
-            lo = [2 * slot * si, 2 * slot * sj, 2 * slot * sk]
-            v = Box(lo, [lo[d] + box_pts - 1 for d in range(3)])
-            ffabs.append(FArrayBox(v.grow(ghosts), ncomp))
-            fvalid.append(v)
         fdom = Box([0] * 3, [2 * nc - it[d] for d in range(3)])
         out[name] = dict(op=op, crse=Level(cfabs, cvalid, cdom),
-                         fine=Level(ffabs, fvalid, fdom), ncomp=ncomp)
+                         fine=fine_level(slots, fdom), ncomp=ncomp, indextype=it)
+        if regrid:
+            out[name]["new"] = fine_level(new_slots, fdom)
     return out
 
 
