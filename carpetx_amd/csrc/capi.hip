@@ -1605,6 +1605,12 @@ bool bc_ok(const cxp_bc &bc, int ncomp) {
   return true;
 }
 
+// Points per chunk of the level fills' copy and boundary launches: one or
+// two points per thread keep more independent loads in flight than longer
+// per-thread loops (both kernels are latency-bound).
+constexpr int kGatherChunkPts = 512;
+constexpr int kBcChunkPts = 256;
+
 // Consistency of new chunks with the runs they index (runs[k] is run
 // run0 + k): every chunk names one of them, and a boundary chunk's points
 // lie in its run's box.  Checked on the host before anything reaches the
@@ -1668,7 +1674,7 @@ void add_bc_runs(const cxp_bc &bc, const int it[3], const IBox &E,
     for (const IBox &P : outs) {
       const int nx = P.hi[0] - P.lo[0] + 1, ny = P.hi[1] - P.lo[1] + 1;
       const int vol = (int)P.volume();
-      for (int q = 0; q < vol; q += 1024) {
+      for (int q = 0; q < vol; q += kBcChunkPts) {
         BcChunk ch;
         ch.run = run;
         for (int d = 0; d < 3; ++d)
@@ -1676,7 +1682,7 @@ void add_bc_runs(const cxp_bc &bc, const int it[3], const IBox &E,
         ch.bnx = nx;
         ch.bny = ny;
         ch.first = q;
-        ch.count = std::min(1024, vol - q);
+        ch.count = std::min(kBcChunkPts, vol - q);
         chunks.push_back(ch);
       }
     }
@@ -1745,7 +1751,7 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
     const int64_t cvol = C.volume();
     const int nx = X.hi[0] - X.lo[0] + 1, ny = X.hi[1] - X.lo[1] + 1;
     const int nrows = ny * (X.hi[2] - X.lo[2] + 1);
-    const int per = std::max(1, 2048 / nx);
+    const int per = std::max(1, kGatherChunkPts / nx);
     const int cx = C.hi[0] - C.lo[0] + 1, cy = C.hi[1] - C.lo[1] + 1;
     for (int c = 0; c < ncomp; ++c) {
       const DFab f = dfab(&crse->fabs[j], crse_comp + c);
@@ -1783,7 +1789,7 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
       copied += X.volume();
       const int nx = X.hi[0] - X.lo[0] + 1, ny = X.hi[1] - X.lo[1] + 1;
       const int nrows = ny * (X.hi[2] - X.lo[2] + 1);
-      const int per = std::max(1, 2048 / nx);
+      const int per = std::max(1, kGatherChunkPts / nx);
       for (int c = 0; c < ncomp; ++c) {
         const DFab f = dfab(&old->fabs[j], fine_comp + c);
         const DFab t = dfab(&fine->fabs[i], fine_comp + c);
