@@ -75,6 +75,19 @@ def lib(kind="strict"):
                                   ctypes.POINTER(OrFab), ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.or_num_threads.restype = ctypes.c_int
+        # the C interface's own signatures (cxp_ref.h); cxp_fab structs are
+        # laid out as in include/cxp.h
+        P = ctypes.POINTER
+        L.cxp_ref_op_lookup.argtypes = [ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int),
+                                        P(ctypes.c_void_p)]
+        L.cxp_ref_coarse_box.argtypes = [ctypes.c_void_p] + [P(ctypes.c_int)] * 4
+        L.cxp_ref_prolongate.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         P(ctypes.c_int), P(ctypes.c_int), ctypes.c_int,
+                                         ctypes.c_void_p]
+        L.cxp_ref_restrict.argtypes = [P(ctypes.c_int), ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                       P(ctypes.c_int), P(ctypes.c_int), ctypes.c_void_p]
         _libs[kind] = L
     return _libs[kind]
 

@@ -144,3 +144,63 @@ def test_thin_slabs_every_orientation():
                 got, ref = run_case("natural", 5, it, tuple(lo), tuple(hi), ncomp=1,
                                     seed=th + 10 * d)
                 assert_parity(got, ref, "slab it=%s d=%d th=%d" % (it, d, th))
+
+
+@pytest.mark.parametrize("fam,order,it,pg", [("natural", 5, (0, 0, 0), 0),
+                                              ("conservative", 1, (1, 1, 1), 0),
+                                              ("eno", 3, (1, 0, 1), 1),
+                                              ("poly-eno3lfb", 3, (1, 1, 1), 0),
+                                              ("minmod", 3, (1, 1, 0), 1)])
+def test_same_call_through_the_device_and_oracle_libraries(fam, order, it, pg):
+    """One argument list, two libraries: cxp_prolongate on device FABs and
+    cxp_ref_prolongate (the oracle behind the same signature) on host
+    copies give identical fine values and status; then cxp_restrict and
+    cxp_ref_restrict on the result."""
+    import ctypes
+    from carpetx_amd import _lib as L_
+    from carpetx_amd._lib import i3
+    D, R = L_.lib(), oracle.lib()
+
+    def fab(lo, hi, ncomp, ptr):
+        f = L_.CxpFab()
+        f.data = ctypes.c_void_p(ptr)
+        f.lo, f.hi = i3(lo), i3(hi)
+        f.comp_stride = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+        f.ncomp = ncomp
+        return f
+    hd, hr = ctypes.c_void_p(), ctypes.c_void_p()
+    assert D.cxp_op_lookup(fam.encode(), order, i3(it), ctypes.byref(hd)) == 0
+    assert R.cxp_ref_op_lookup(fam.encode(), order, i3(it), ctypes.byref(hr)) == 0
+    rlo, rhi = (-2, 0, 3), (29, 21, 16)
+    clo, chi = i3((0, 0, 0)), i3((0, 0, 0))
+    assert D.cxp_coarse_box(hd, i3(rlo), i3(rhi), clo, chi) == 0
+    clo, chi = tuple(clo), tuple(chi)
+    rng = np.random.default_rng(order + 7 * pg)
+    cshape = (3,) + tuple(chi[d] - clo[d] + 1 for d in (2, 1, 0))
+    fshape = (3,) + tuple(rhi[d] - rlo[d] + 1 for d in (2, 1, 0))
+    c_h = rng.uniform(-1, 1, cshape)
+    f_h = np.full(fshape, 5.0)
+    c_d = torch.from_numpy(c_h).cuda()
+    f_d = torch.from_numpy(f_h).cuda()
+    args = (1, None, 1, 2, i3(rlo), i3(rhi), pg, None)
+    cd, fd = fab(clo, chi, 3, c_d.data_ptr()), fab(rlo, rhi, 3, f_d.data_ptr())
+    ch, fh = fab(clo, chi, 3, c_h.ctypes.data), fab(rlo, rhi, 3, f_h.ctypes.data)
+    sd = D.cxp_prolongate(hd, ctypes.byref(cd), 1, ctypes.byref(fd), 1, 2, i3(rlo), i3(rhi),
+                          pg, None)
+    sr = R.cxp_ref_prolongate(hr, ctypes.byref(ch), 1, ctypes.byref(fh), 1, 2, i3(rlo),
+                              i3(rhi), pg, None)
+    torch.cuda.synchronize()
+    assert sd == sr == 0
+    assert np.array_equal(f_d.cpu().numpy(), f_h), (fam, order, it, pg)
+    # restriction of the prolongated values back onto a coarse region
+    qlo = tuple((rlo[d] + 1) // 2 for d in range(3))
+    qhi = tuple((rhi[d] - it[d]) // 2 for d in range(3))
+    q_h = np.zeros(cshape)
+    q_d = torch.zeros(cshape, dtype=torch.float64, device="cuda")
+    sd = D.cxp_restrict(i3(it), ctypes.byref(fd), 1, ctypes.byref(fab(clo, chi, 3, q_d.data_ptr())),
+                        0, 2, i3(qlo), i3(qhi), None)
+    sr = R.cxp_ref_restrict(i3(it), ctypes.byref(fh), 1, ctypes.byref(fab(clo, chi, 3, q_h.ctypes.data)),
+                            0, 2, i3(qlo), i3(qhi), None)
+    torch.cuda.synchronize()
+    assert sd == sr == 0
+    np.testing.assert_allclose(q_d.cpu().numpy(), q_h, rtol=1e-13, atol=1e-15)
