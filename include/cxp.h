@@ -293,7 +293,10 @@ int cxp_graph_destroy(cxp_graph g);
  * fastest only if it beats the best so far by more than 0.5%; passes repeat
  * (at most 3) until one changes nothing.  The table is
  * process-wide, keyed by (stencil triple, size class), and can be
- * saved/loaded as text.  Uploads the list again when done. */
+ * saved/loaded as text.  Uploads the list again when done.  The timed
+ * launches are the fill kernel alone (no gathers, copies or boundary
+ * values) and write the list's fine points with values that mean nothing:
+ * run the list after tuning. */
 int cxp_autotune(cxp_worklist wl, int reps, void *stream);
 int cxp_tune_save(const char *path);
 int cxp_tune_load(const char *path);
