@@ -44,10 +44,10 @@ class Level:
 
 
 # Symmetry and boundary condition types (CXP_BC_* of the C interface):
-# CarpetX's boundary_t values and its reflection symmetry
+# CarpetX's boundary_t values, its reflection symmetry and periodicity
 # (CarpetX/src/boundaries.hxx, boundaries_impl.hxx:150-188).
 BC_TYPES = {"none": 0, "dirichlet": 1, "linear_extrapolation": 2,
-            "neumann": 3, "robin": 4, "reflection": 5}
+            "neumann": 3, "robin": 4, "reflection": 5, "periodic": 6}
 
 
 class BoundaryConditions:

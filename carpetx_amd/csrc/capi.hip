@@ -1557,6 +1557,7 @@ int bc_closure(const cxp_bc &bc, const OpDesc &od, const IBox &dom,
       };
       switch (bc.type[face][d]) {
       case CXP_BC_DIRICHLET:
+      case CXP_BC_PERIODIC:  // images, not sources in the box
         break;
       case CXP_BC_NEUMANN:
       case CXP_BC_ROBIN:
@@ -1589,7 +1590,10 @@ bool bc_ok(const cxp_bc &bc, int ncomp) {
   for (int f = 0; f < 2; ++f)
     for (int d = 0; d < 3; ++d) {
       const int t = bc.type[f][d];
-      if (t < CXP_BC_NONE || t > CXP_BC_REFLECTION)
+      if (t < CXP_BC_NONE || t > CXP_BC_PERIODIC)
+        return false;
+      // periodic directions are periodic on both faces
+      if ((t == CXP_BC_PERIODIC) != (bc.type[1 - f][d] == CXP_BC_PERIODIC))
         return false;
       dir |= t == CXP_BC_DIRICHLET;
       rob |= t == CXP_BC_ROBIN;
@@ -1726,11 +1730,19 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
     for (int d = 0; d < 3; ++d)
       g[d] = std::max(g[d], std::max(fine->valid[6 * i + d] - fine->fabs[i].lo[d],
                                      fine->fabs[i].hi[d] - fine->valid[6 * i + 3 + d]));
-  IBox fdom = ibox(fine->domain_lo, fine->domain_hi);
-  for (int d = 0; d < 3 && mode != FillMode::whole; ++d) {
-    fdom.lo[d] -= g[d];
-    fdom.hi[d] += g[d];
+  // periodic directions of the coarse conditions, and their periods
+  bool periodic[3] = {false, false, false};
+  int period[3] = {0, 0, 0};
+  for (int d = 0; d < 3 && bc; ++d) {
+    periodic[d] = bc->type[0][d] == CXP_BC_PERIODIC;
+    period[d] = crse->domain_hi[d] - crse->domain_lo[d] + op->d.indextype[d];
   }
+  IBox fdom = ibox(fine->domain_lo, fine->domain_hi);
+  for (int d = 0; d < 3; ++d)
+    if (mode != FillMode::whole || periodic[d]) {
+      fdom.lo[d] -= g[d];
+      fdom.hi[d] += g[d];
+    }
   const IBox cdom = ibox(crse->domain_lo, crse->domain_hi);
   auto cvalid = [&](int j) { return ibox(crse->valid + 6 * j, crse->valid + 6 * j + 3); };
   auto cfabbox = [&](int j) { return ibox(crse->fabs[j].lo, crse->fabs[j].hi); };
@@ -1747,8 +1759,11 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
 
   // one copy of region X of coarse FAB j into the coarse patch C at offset
   // off, every component
-  auto add_copy = [&](int j, const IBox &X, const IBox &C, int64_t off) {
+  // (with shift sh, into X + sh: a periodic image)
+  auto add_copy = [&](int j, const IBox &X, const IBox &C, int64_t off,
+                      const int *sh = nullptr) {
     const int64_t cvol = C.volume();
+    const int d0 = sh ? sh[0] : 0, d1 = sh ? sh[1] : 0, d2 = sh ? sh[2] : 0;
     const int nx = X.hi[0] - X.lo[0] + 1, ny = X.hi[1] - X.lo[1] + 1;
     const int nrows = ny * (X.hi[2] - X.lo[2] + 1);
     const int per = std::max(1, kGatherChunkPts / nx);
@@ -1760,9 +1775,9 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
       r.src_sy = f.sy;
       r.src_sz = f.sz;
       r.dstp = nullptr;
-      r.dst = off + c * cvol + (X.lo[0] - C.lo[0]) +
-              (long long)(X.lo[1] - C.lo[1]) * cx +
-              (long long)(X.lo[2] - C.lo[2]) * cx * cy;
+      r.dst = off + c * cvol + (X.lo[0] + d0 - C.lo[0]) +
+              (long long)(X.lo[1] + d1 - C.lo[1]) * cx +
+              (long long)(X.lo[2] + d2 - C.lo[2]) * cx * cy;
       r.dst_sy = cx;
       r.dst_sz = (long long)cx * cy;
       r.nx = nx;
@@ -1868,21 +1883,52 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
             return e;
         if (E.volume() > (int64_t)INT32_MAX)
           return CXP_ERR_ARG;
-        const IBox Ein = intersect(E, cdom);
         const int64_t off = gather;
         std::vector<IBox> left;
-        if (!Ein.empty()) {
-          left.push_back(Ein);
-          for (int j = 0; j < crse->nfabs; ++j) {
-            const IBox X = intersect(Ein, cvalid(j));
-            if (!X.empty()) {
-              add_copy(j, X, E, off);
-              subtract_all(left, cvalid(j));
-            }
+        // valid data: the part of E inside the domain, and in periodic
+        // directions the parts one period away, read from their images.
+        // The unshifted part goes first and each image takes only points no
+        // earlier one covered (with vertex centring the domain's first and
+        // last points are images of each other; the point inside is kept).
+        std::vector<IBox> covered;
+        for (int s0 = 0; s0 < 27; ++s0) {
+          const int s = (s0 + 13) % 27;  // (0, 0, 0) first
+          const int sh[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+          if ((sh[0] && !periodic[0]) || (sh[1] && !periodic[1]) ||
+              (sh[2] && !periodic[2]))
+            continue;
+          IBox img = cdom;  // the domain moved by the shift
+          int dsh[3];
+          for (int d = 0; d < 3; ++d) {
+            dsh[d] = sh[d] * period[d];
+            img.lo[d] += dsh[d];
+            img.hi[d] += dsh[d];
           }
-          if (!left.empty())
-            return CXP_ERR_COARSE;
+          const IBox part = intersect(E, img);
+          if (part.empty())
+            continue;
+          std::vector<IBox> pieces{part};
+          for (const IBox &cv : covered)
+            subtract_all(pieces, cv);
+          covered.push_back(part);
+          for (IBox piece : pieces) {
+            for (int d = 0; d < 3; ++d) {  // its source, inside the domain
+              piece.lo[d] -= dsh[d];
+              piece.hi[d] -= dsh[d];
+            }
+            left.assign(1, piece);
+            for (int j = 0; j < crse->nfabs; ++j) {
+              const IBox X = intersect(piece, cvalid(j));
+              if (!X.empty()) {
+                add_copy(j, X, E, off, dsh);
+                subtract_all(left, cvalid(j));
+              }
+            }
+            if (!left.empty())
+              return CXP_ERR_COARSE;
+          }
         }
+        left.clear();
         if (bcp) {
           add_bc(E, off);
         } else {
@@ -1996,7 +2042,8 @@ int cxp_worklist_append_boundaries(cxp_worklist wl, const cxp_level *level,
     cxp_bc only = *bc;
     for (int side = 0; side < 2; ++side)
       for (int d = 0; d < 3; ++d)
-        if (only.type[side][d] == CXP_BC_NONE)
+        if (only.type[side][d] == CXP_BC_NONE ||
+            only.type[side][d] == CXP_BC_PERIODIC)
           only.type[side][d] = CXP_BC_DIRICHLET;  // reads nothing
     OpDesc od;
     for (int d = 0; d < 3; ++d)

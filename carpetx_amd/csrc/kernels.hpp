@@ -506,8 +506,8 @@ __global__ void __launch_bounds__(256)
       if (dst[d] >= r.imin[d] && dst[d] < r.imax[d])
         continue;  // not a boundary direction of this point
       const int face = dst[d] >= r.imax[d] ? 1 : 0;
-      if (r.type[face][d] == CXP_BC_NONE)
-        continue;  // nor is a face without a condition
+      if (r.type[face][d] == CXP_BC_NONE || r.type[face][d] == CXP_BC_PERIODIC)
+        continue;  // nor is a face without a condition, or a periodic one
       outside = true;
       const int edge = face ? r.imax[d] - 1 : r.imin[d];
       switch (r.type[face][d]) {

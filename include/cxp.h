@@ -200,6 +200,14 @@ int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
 #define CXP_BC_NEUMANN 3
 #define CXP_BC_ROBIN 4
 #define CXP_BC_REFLECTION 5
+/* Periodic direction (both faces): coarse points outside the domain are the
+ * periodic images of valid points, as the reference's coarse-patch copy
+ * uses the coarse geometry's periodicity (fillpatch.cxx:91-93,175), with
+ * period hi - lo + 1 (cell-centred) or hi - lo (vertex-centred) in the
+ * domain's index range; a new level's fine domain is grown by the ghost
+ * width in such directions (:160-170).  A level's own periodic ghosts are
+ * left alone (the reference fills them by FillBoundary). */
+#define CXP_BC_PERIODIC 6
 typedef struct cxp_bc {
   int type[2][3];
   const double *dirichlet_values; /* host, one per component, or null */

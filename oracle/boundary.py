@@ -14,7 +14,8 @@ direction d, with the face's condition (:322-371):
   * neumann, robin:        src[d] = edge;
   * reflection:            src[d] = offset - dst[d], offset = 2 imin - cell
                            (lower) or 2 (imax - 1) + cell (upper), :286-298;
-  * none:                  the direction is not a boundary (src[d] = dst[d]).
+  * none or periodic:      the direction is not a boundary (src[d] = dst[d]);
+                           periodic images are the coarse-patch copy's job.
 If every such direction is "none" the point is left alone.  The value
 (:380-417): any dirichlet direction -> the component's Dirichlet value;
 otherwise var(src), then for each Robin direction
@@ -31,7 +32,39 @@ import math
 
 import numpy as np
 
-NONE, DIRICHLET, LINEAR_EXTRAPOLATION, NEUMANN, ROBIN, REFLECTION = range(6)
+NONE, DIRICHLET, LINEAR_EXTRAPOLATION, NEUMANN, ROBIN, REFLECTION, PERIODIC = range(7)
+
+
+def periodic_fill(arr, lo, dom_lo, dom_hi, dirs, cell):
+    """The coarse-patch copy's periodic images (the reference copies with
+    the geometry's periodicity, CarpetX/src/fillpatch.cxx:91-93): every
+    point of arr outside the domain only in periodic directions ``dirs``
+    takes the value of its image inside the domain, period dom_hi - dom_lo
+    + cell[d].  Points inside the domain keep their values."""
+    hi = [lo[d] + arr.shape[3 - d] - 1 for d in range(3)]
+    src = arr.copy()
+    for k in range(lo[2], hi[2] + 1):
+        for j in range(lo[1], hi[1] + 1):
+            for i in range(lo[0], hi[0] + 1):
+                p = [i, j, k]
+                q = list(p)
+                ok, moved = True, False
+                for d in range(3):
+                    if dom_lo[d] <= p[d] <= dom_hi[d]:
+                        continue
+                    if d not in dirs:
+                        ok = False
+                        break
+                    per = dom_hi[d] - dom_lo[d] + cell[d]
+                    while q[d] < dom_lo[d]:
+                        q[d] += per
+                    while q[d] > dom_hi[d]:
+                        q[d] -= per
+                    moved = True
+                if ok and moved:
+                    arr[:, k - lo[2], j - lo[1], i - lo[0]] = \
+                        src[:, q[2] - lo[2], q[1] - lo[1], q[0] - lo[0]]
+    return arr
 
 
 def apply_bc(arr, lo, dom_lo, dom_hi, faces, cell, dirichlet=None, robin=None,
@@ -62,8 +95,8 @@ def apply_bc(arr, lo, dom_lo, dom_hi, faces, cell, dirichlet=None, robin=None,
                     face = 1 if dst[d] >= imax[d] else 0
                     edge = imax[d] - 1 if face else imin[d]
                     t = faces[face][d]
-                    if t == NONE:
-                        continue
+                    if t in (NONE, PERIODIC):
+                        continue  # periodic faces are translated to none (:160-161)
                     active = True
                     if t == DIRICHLET:
                         dirich = True

@@ -22,7 +22,7 @@ import torch
 import oracle
 from carpetx_amd import _lib
 from carpetx_amd.batched import BoundaryConditions, Graph, Level, WorkList
-from oracle.boundary import apply_bc
+from oracle.boundary import apply_bc, periodic_fill
 from carpetx_amd.interp import Box, FArrayBox, get_interpolator
 
 pytestmark = pytest.mark.gpu
@@ -200,7 +200,11 @@ def test_level_fill_needs_coarse_boundary_points():
 
 
 BC_KINDS = {"none": 0, "dirichlet": 1, "linear_extrapolation": 2,
-            "neumann": 3, "robin": 4, "reflection": 5}
+            "neumann": 3, "robin": 4, "reflection": 5, "periodic": 6}
+
+
+def _periodic_dirs(faces):
+    return tuple(d for d in range(3) if faces[0][d] == "periodic")
 BC_CASES = [
     (("natural", 5, (0, 0, 0)),
      [["reflection", "linear_extrapolation", "robin"],
@@ -214,6 +218,10 @@ BC_CASES = [
      [["reflection", "robin", "linear_extrapolation"], ["neumann"] * 3]),
     (("minmod", 3, (0, 1, 0)),
      [["dirichlet", "reflection", "neumann"], ["reflection"] * 3]),
+    (("natural", 5, (0, 0, 0)),
+     [["periodic", "reflection", "neumann"], ["periodic", "linear_extrapolation", "dirichlet"]]),
+    (("natural", 3, (1, 1, 1)),
+     [["reflection", "periodic", "periodic"], ["neumann", "periodic", "periodic"]]),
 ]
 
 
@@ -244,6 +252,7 @@ def test_level_fill_with_boundary_conditions_matches_oracle(op_args, faces):
     def with_bc(g):
         gb = g.copy()
         sub = np.ascontiguousarray(gb[1:1 + NCOMP])
+        periodic_fill(sub, glo, cdom.lo, cdom.hi, _periodic_dirs(faces), it)
         apply_bc(sub, glo, cdom.lo, cdom.hi, codes, it, **kw)
         gb[1:1 + NCOMP] = sub
         return gb
@@ -306,6 +315,9 @@ NEW_VALID = [((0, 0, 0), (15, 11, 9)), ((16, 0, 0), (31, 13, 9)),
 REGRID_CASES = [
     (("natural", 3, (1, 1, 1)), None, None),
     (("natural", 5, (0, 0, 0)),
+     [["periodic", "reflection", "neumann"], ["periodic", "linear_extrapolation", "dirichlet"]],
+     [["periodic", "neumann", "reflection"], ["periodic", "dirichlet", "neumann"]]),
+    (("natural", 5, (0, 0, 0)),
      [["reflection", "linear_extrapolation", "robin"],
       ["dirichlet", "neumann", "reflection"]],
      [["neumann", "reflection", "linear_extrapolation"],
@@ -338,6 +350,7 @@ def _coarse_global(it, g, faces):
     glo = tuple(cdom.lo[d] - CG for d in range(3))
     gb = g.copy()
     sub = np.ascontiguousarray(gb[1:1 + NCOMP])
+    periodic_fill(sub, glo, cdom.lo, cdom.hi, _periodic_dirs(faces), it)
     apply_bc(sub, glo, cdom.lo, cdom.hi, _codes(faces), it, **BC_KW)
     gb[1:1 + NCOMP] = sub
     return gb
@@ -417,10 +430,14 @@ def test_new_level_matches_oracle(op_args, cfaces, ffaces):
     fine = fine_level(it, rng, boxes=NEW_VALID, fill_valid=False)
     originals = [f.t.clone() for f in fine.fabs]
     gc = _coarse_global(it, g, cfaces)
+    # the new level's domain, grown by the ghost width in periodic directions
+    per = () if cfaces is None else _periodic_dirs(cfaces)
+    nlo = tuple(fdom.lo[d] - (FG if d in per else 0) for d in range(3))
+    nhi = tuple(fdom.hi[d] + (FG if d in per else 0) for d in range(3))
     want = []
     for fab, orig in zip(fine.fabs, originals):
         p = _prolongated(op_args, it, gc, fab)
-        w = np.where(_mask(fab.box, fdom.lo, fdom.hi)[None], p, orig.cpu().numpy())
+        w = np.where(_mask(fab.box, nlo, nhi)[None], p, orig.cpu().numpy())
         if ffaces is not None:
             w = np.ascontiguousarray(w)
             apply_bc(w, fab.box.lo, fdom.lo, fdom.hi, _codes(ffaces), it, **BC_KW)

@@ -291,9 +291,10 @@ def _bc(faces, ncomp, dirichlet=True, robin=True, parities=True):
 def test_level_append_with_boundary_conditions_checks_its_arguments():
     """cxp_worklist_append_level_bc computes the coarse boundary points, so
     coarse FABs with one ghost point suffice (cxp_worklist_append_level
-    needs them in the FABs); it fills the same fine points; a face that a
-    coarse patch reaches without a condition, a condition without its
-    values, a parity other than +-1 or more than 16 components fail with
+    needs them in the FABs); it fills the same fine points, also with
+    periodic directions; a face that a coarse patch reaches without a
+    condition, a condition without its values, a parity other than +-1, an
+    unpaired periodic face or more than 16 components fail with
     CXP_ERR_ARG and add nothing."""
     L = _lib.lib()
     n = 24
@@ -321,6 +322,11 @@ def test_level_append_with_boundary_conditions_checks_its_arguments():
         v = [ctypes.c_int64() for _ in range(4)]
         assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
         assert v[2].value == 2 * _fill_points(fine_valid, fdom, 3), it
+        pb, kp = _bc([[6, LE, 6], [6, NM, 6]], 2)
+        assert L.cxp_worklist_append_level_bc(wl, h, ctypes.byref(thin), 0,
+                                              ctypes.byref(fine), 0, 2, ctypes.byref(pb)) == 0
+        assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+        assert v[2].value == 4 * _fill_points(fine_valid, fdom, 3), it
         npatch = v[0].value
         bad = []
         for f, d in [(0, 0), (1, 2), (0, 1)]:
@@ -333,7 +339,9 @@ def test_level_append_with_boundary_conditions_checks_its_arguments():
         p0 = _bc(all_faces, 2)
         p0[1][2][4] = 0
         bad.append(p0)
-        bad.append(_bc([[7, 1, 1], [1, 1, 1]], 2))
+        bad.append(_bc([[8, 1, 1], [1, 1, 1]], 2))
+        # a periodic direction is periodic on both faces
+        bad.append(_bc([[6, 1, 1], [1, 1, 1]], 2))
         for bb, kb in bad:
             assert L.cxp_worklist_append_level_bc(
                 wl, h, ctypes.byref(thin), 0, ctypes.byref(fine), 0, 2,

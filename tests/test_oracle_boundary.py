@@ -161,3 +161,28 @@ def test_faces_without_condition_are_left_alone():
     assert np.all(arr[0][m] == 9.0)
     m = (y < 0) & ~o[0] & ~o[2]
     assert np.array_equal(arr[0][m], f[(z - LO)[m], 3, (x - LO)[m]])
+
+
+def test_periodic_images_are_the_points_one_period_away():
+    """periodic_fill: data (i mod p) + 100 (k mod p) with p the period (the
+    domain's extent, one less for vertex centring, whose first and last
+    points coincide) is reproduced at every point outside the domain only
+    in the periodic directions; points also outside in another direction,
+    and the domain itself, are left alone; periodic faces are 'none' for
+    the boundary conditions."""
+    from oracle.boundary import PERIODIC, periodic_fill
+    x, y, z = grid()
+    for cell in (1, 0):
+        p = 6 - (1 - cell)  # domain [0, 5]: 6 cells, or 5 vertex intervals
+        f = (x % p) + 100.0 * (z % p)
+        arr, out = fab([f], fill=np.nan)
+        periodic_fill(arr, (LO, LO, LO), DOM_LO, DOM_HI, (0, 2), (cell, 0, cell))
+        o = outside_dirs()
+        m = (o[0] | o[2]) & ~o[1]
+        assert np.array_equal(arr[0][m], f[m]), cell
+        assert np.all(np.isnan(arr[0][o[1]]))
+        assert np.array_equal(arr[0][~out], f[~out])
+        # the y faces' condition reads the images (a periodic x face is none)
+        run(arr, [[PERIODIC, NEUMANN, PERIODIC], [PERIODIC, NEUMANN, PERIODIC]], cell=(cell, 0, cell))
+        yc = np.clip(y, 0, 5)
+        assert np.array_equal(arr[0], np.take_along_axis(arr[0], (yc - LO), axis=1))
