@@ -202,3 +202,97 @@ def test_worklist_rejects_bad_patches():
     assert e.value.status == -5
     with pytest.raises(CxpError):
         wl.run()   # not uploaded
+
+
+def test_full_uneven_level_properties():
+    """The whole 512-box, 10-field uneven level (the headline workload), by
+    size-independent properties: after poisoning the shells, one batched
+    fill leaves no poison in any shell and every interior bit as it was;
+    the per-patch path and a graph replay give the same bits."""
+    w = workloads.uneven_512()
+    assert len(w["boxes"]) == 512 and w["ncomp"] == 10
+    wl = WorkList()
+    for b in w["boxes"]:
+        wl.append_shell(b["op"], b["crse"], 0, b["fine"], 0, w["ncomp"], b["interior"])
+    wl.upload()
+    before = [b["fine"].t.clone() for b in w["boxes"]]
+    wl.poison()
+    wl.run()
+    torch.cuda.synchronize()
+    bits = torch.tensor(POISON_BITS - (1 << 64), dtype=torch.int64)
+    filled = []
+    for b, e in zip(w["boxes"], before):
+        t = b["fine"].t
+        m = _shell_mask(b)
+        assert not torch.any(t.view(torch.int64)[m] == bits), "poison left in a shell"
+        assert torch.equal(t.view(torch.int64)[~m], e.view(torch.int64)[~m])
+        filled.append(t.clone())
+    for b in w["boxes"]:
+        b["fine"].t.fill_(7.0)
+    wl.run_per_patch()
+    torch.cuda.synchronize()
+    for b, f in zip(w["boxes"], filled):
+        m = _shell_mask(b)
+        assert torch.equal(b["fine"].t.view(torch.int64)[m], f.view(torch.int64)[m])
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        g = Graph(lambda: wl.run(s), s)
+        wl.poison(stream=s)
+        g.launch(s)
+    torch.cuda.synchronize()
+    for b, f in zip(w["boxes"], filled):
+        m = _shell_mask(b)
+        assert torch.equal(b["fine"].t.view(torch.int64)[m], f.view(torch.int64)[m])
+    g.close()
+    wl.close()
+
+
+def test_full_level_fill_with_boundary_conditions_batched_equals_per_patch():
+    """The whole 64-box level ghost fill with boundary conditions on the
+    coarse patches (reflection below, linear extrapolation above) and on
+    the fine level.  Ghosts start as NaN except where another FAB's valid
+    region covers them, which hold that FAB's values (the same-level sync
+    the reference runs before its boundary conditions).  Then every point
+    is written (no NaN left), valid and synced points keep their bits, and
+    batched and per-patch runs give the same bits."""
+    from carpetx_amd.batched import BoundaryConditions
+    lv = workloads.level_fill_64()
+    for name, d in lv.items():
+        bc = BoundaryConditions([["reflection"] * 3, ["linear_extrapolation"] * 3],
+                                parities=[(1, 1, 1)] * d["ncomp"])
+        fine = d["fine"]
+        valid_data = [torch.rand_like(f.t) for f in fine.fabs]
+
+        def sl(f, lo, hi):
+            return (slice(None),) + tuple(slice(lo[k] - f.box.lo[k], hi[k] - f.box.lo[k] + 1)
+                                          for k in (2, 1, 0))
+
+        def reset():
+            for f in fine.fabs:
+                f.t.fill_(float("nan"))
+            for j, (fj, vj) in enumerate(zip(fine.fabs, fine.valid)):
+                for f in fine.fabs:
+                    lo = [max(vj.lo[k], f.box.lo[k]) for k in range(3)]
+                    hi = [min(vj.hi[k], f.box.hi[k]) for k in range(3)]
+                    if all(lo[k] <= hi[k] for k in range(3)):
+                        f.t[sl(f, lo, hi)] = valid_data[j][sl(fj, lo, hi)]
+            torch.cuda.synchronize()
+        wl = WorkList()
+        wl.append_level(d["op"], d["crse"], 0, fine, 0, d["ncomp"], bc=bc)
+        wl.append_boundaries(fine, d["indextype"], 0, d["ncomp"], bc)
+        wl.upload()
+        reset()
+        start = [f.t.clone() for f in fine.fabs]
+        wl.run()
+        torch.cuda.synchronize()
+        got = [f.t.clone() for f in fine.fabs]
+        for g_, s0 in zip(got, start):
+            assert not torch.isnan(g_).any(), name
+            kept = ~torch.isnan(s0)
+            assert torch.equal(g_[kept], s0[kept]), name
+        reset()
+        wl.run_per_patch()
+        torch.cuda.synchronize()
+        for f, g_ in zip(fine.fabs, got):
+            assert torch.equal(f.t.view(torch.int64), g_.view(torch.int64)), name
+        wl.close()
