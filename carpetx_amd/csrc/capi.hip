@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -1793,37 +1794,70 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
     add_bc_runs(*bc, op->d.indextype, E, cdom, nullptr, off, E.volume(), ncomp,
                 brun0, bruns, bchunks);
   };
-  // copies of the old fine level's valid data into fine FAB i (remake)
+  // the fine level's periodic shifts (fine index units), the unshifted one
+  // first: a valid box also covers its periodic images (the reference's
+  // copies and syncs use the geometry's periodicity)
+  std::vector<std::array<int, 3>> fshifts;
+  for (int s0 = 0; s0 < 27; ++s0) {
+    const int s = (s0 + 13) % 27;
+    const int sh[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+    if ((sh[0] && !periodic[0]) || (sh[1] && !periodic[1]) ||
+        (sh[2] && !periodic[2]))
+      continue;
+    std::array<int, 3> v;
+    for (int d = 0; d < 3; ++d)
+      v[d] = sh[d] * (fine->domain_hi[d] - fine->domain_lo[d] +
+                      op->d.indextype[d]);
+    fshifts.push_back(v);
+  }
+  auto shifted = [](IBox b, const std::array<int, 3> &v) {
+    for (int d = 0; d < 3; ++d) {
+      b.lo[d] += v[d];
+      b.hi[d] += v[d];
+    }
+    return b;
+  };
+  // copies of the old fine level's valid data (and its periodic images)
+  // into fine FAB i (remake); each point from the first image covering it
   int64_t copied = 0;
   auto add_fab_copies = [&](int i) {
     const IBox B = ibox(fine->fabs[i].lo, fine->fabs[i].hi);
-    for (int j = 0; j < old->nfabs; ++j) {
-      const IBox X = intersect(B, ibox(old->valid + 6 * j, old->valid + 6 * j + 3));
-      if (X.empty())
-        continue;
-      copied += X.volume();
-      const int nx = X.hi[0] - X.lo[0] + 1, ny = X.hi[1] - X.lo[1] + 1;
-      const int nrows = ny * (X.hi[2] - X.lo[2] + 1);
-      const int per = std::max(1, kGatherChunkPts / nx);
-      for (int c = 0; c < ncomp; ++c) {
-        const DFab f = dfab(&old->fabs[j], fine_comp + c);
-        const DFab t = dfab(&fine->fabs[i], fine_comp + c);
-        GatherRun r;
-        r.src = f.p + didx(f, X.lo[0], X.lo[1], X.lo[2]);
-        r.src_sy = f.sy;
-        r.src_sz = f.sz;
-        r.dstp = t.p;
-        r.dst = didx(t, X.lo[0], X.lo[1], X.lo[2]);
-        r.dst_sy = t.sy;
-        r.dst_sz = t.sz;
-        r.nx = nx;
-        r.ny = ny;
-        const int run = run0 + (int)runs.size();
-        runs.push_back(r);
-        for (int row = 0; row < nrows; row += per)
-          chunks.push_back({run, row, std::min(per, nrows - row)});
+    std::vector<IBox> covered;
+    for (const auto &v : fshifts)
+      for (int j = 0; j < old->nfabs; ++j) {
+        const IBox X = intersect(
+            B, shifted(ibox(old->valid + 6 * j, old->valid + 6 * j + 3), v));
+        if (X.empty())
+          continue;
+        std::vector<IBox> parts{X};
+        for (const IBox &cv : covered)
+          subtract_all(parts, cv);
+        covered.push_back(X);
+        for (const IBox &P : parts) {
+          copied += P.volume();
+          const int nx = P.hi[0] - P.lo[0] + 1, ny = P.hi[1] - P.lo[1] + 1;
+          const int nrows = ny * (P.hi[2] - P.lo[2] + 1);
+          const int per = std::max(1, kGatherChunkPts / nx);
+          for (int c = 0; c < ncomp; ++c) {
+            const DFab f = dfab(&old->fabs[j], fine_comp + c);
+            const DFab t = dfab(&fine->fabs[i], fine_comp + c);
+            GatherRun r;
+            r.src = f.p + didx(f, P.lo[0] - v[0], P.lo[1] - v[1], P.lo[2] - v[2]);
+            r.src_sy = f.sy;
+            r.src_sz = f.sz;
+            r.dstp = t.p;
+            r.dst = didx(t, P.lo[0], P.lo[1], P.lo[2]);
+            r.dst_sy = t.sy;
+            r.dst_sz = t.sz;
+            r.nx = nx;
+            r.ny = ny;
+            const int run = run0 + (int)runs.size();
+            runs.push_back(r);
+            for (int row = 0; row < nrows; row += per)
+              chunks.push_back({run, row, std::min(per, nrows - row)});
+          }
+        }
       }
-    }
   };
 
   for (int i = 0; i < fine->nfabs; ++i) {
@@ -1832,16 +1866,20 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
     std::vector<IBox> pieces;
     if (mode == FillMode::ghosts) {
       subtract(intersect(ibox(ff.lo, ff.hi), fdom), fv, pieces);
-      for (int j = 0; j < fine->nfabs; ++j)
-        if (j != i)
-          subtract_all(pieces, ibox(fine->valid + 6 * j, fine->valid + 6 * j + 3));
+      for (const auto &v : fshifts)
+        for (int j = 0; j < fine->nfabs; ++j)
+          if (j != i || v[0] || v[1] || v[2])
+            subtract_all(pieces, shifted(ibox(fine->valid + 6 * j,
+                                              fine->valid + 6 * j + 3), v));
     } else {
       const IBox B = intersect(ibox(ff.lo, ff.hi), fdom);
       if (!B.empty())
         pieces.push_back(B);
       if (mode == FillMode::remake) {
-        for (int j = 0; j < old->nfabs; ++j)
-          subtract_all(pieces, ibox(old->valid + 6 * j, old->valid + 6 * j + 3));
+        for (const auto &v : fshifts)
+          for (int j = 0; j < old->nfabs; ++j)
+            subtract_all(pieces, shifted(ibox(old->valid + 6 * j,
+                                              old->valid + 6 * j + 3), v));
         add_fab_copies(i);
       }
     }

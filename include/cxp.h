@@ -205,8 +205,11 @@ int cxp_worklist_append_level(cxp_worklist wl, cxp_op op,
  * uses the coarse geometry's periodicity (fillpatch.cxx:91-93,175), with
  * period hi - lo + 1 (cell-centred) or hi - lo (vertex-centred) in the
  * domain's index range; a new level's fine domain is grown by the ghost
- * width in such directions (:160-170).  A level's own periodic ghosts are
- * left alone (the reference fills them by FillBoundary). */
+ * width in such directions (:160-170).  Fine valid boxes cover their
+ * periodic images too: a ghost fill leaves points under another box's
+ * image alone, and a remake copies old data from images (:222-223).  A
+ * level's own periodic ghosts are left alone (the reference fills them by
+ * FillBoundary). */
 #define CXP_BC_PERIODIC 6
 typedef struct cxp_bc {
   int type[2][3];

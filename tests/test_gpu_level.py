@@ -497,3 +497,88 @@ def test_remake_level_matches_oracle(op_args, cfaces, ffaces):
         wl.append_boundaries(fine, it, 0, NCOMP, _bc(ffaces))
     _run_all_ways(wl, fine.fabs, originals, want, ("remake", op_args))
     wl.close()
+
+
+PER_FACES = [["periodic", "reflection", "neumann"], ["periodic", "linear_extrapolation", "dirichlet"]]
+
+
+def _covering(b, boxes, shifts):
+    """For each point of box b: the index into (shift, box) pairs of the
+    first image covering it (shifts in the given order, boxes within a
+    shift), or -1."""
+    idx = np.meshgrid(*[np.arange(b.lo[d], b.hi[d] + 1) for d in (2, 1, 0)], indexing="ij")
+    out = np.full(idx[0].shape, -1)
+    k = 0
+    pairs = []
+    for sh in shifts:
+        for bi, v in enumerate(boxes):
+            m = np.ones(idx[0].shape, dtype=bool)
+            for a, d in zip(idx, (2, 1, 0)):
+                m &= (a >= v.lo[d] + sh[d]) & (a <= v.hi[d] + sh[d])
+            out[(out < 0) & m] = k
+            pairs.append((sh, bi))
+            k += 1
+    return out, pairs
+
+
+@pytest.mark.parametrize("op_args", [("natural", 3, (1, 1, 1)), ("natural", 5, (0, 0, 0))])
+def test_periodic_fine_images(op_args):
+    """x periodic on both levels, fine boxes at both x faces, so that each
+    box's x ghosts lie in the other's periodic image.  Ghost fill: those
+    ghosts are left alone (the same-level sync fills them in the
+    reference).  Remake: new-level points covered by an old valid box or
+    by its periodic image take the old data (the unshifted box first),
+    the rest of the grown domain is prolongated."""
+    it = op_args[2]
+    rng = np.random.default_rng(31 + sum(it))
+    cdom, fdom = domains(it)
+    per = fdom.hi[0] - fdom.lo[0] + it[0]
+    shifts = [(0, 0, 0), (-per, 0, 0), (per, 0, 0)]
+    gshape = (NC_CRSE,) + tuple(cdom.size[d] + 2 * CG for d in (2, 1, 0))
+    g = rng.uniform(-1, 1, gshape)
+    crse = coarse_level(it, g, poison_outside=True)
+    gc = _coarse_global(it, g, PER_FACES)
+    grown = (tuple(fdom.lo[d] - FG for d in range(3)), tuple(fdom.hi[d] + FG for d in range(3)))
+    boxes = [((0, 0, 0), (9, 9, 9)), ((38, 0, 0), (fdom.hi[0], 9, 9))]
+    op = get_interpolator(*op_args)
+    # ghost fill
+    fine = fine_level(it, rng, boxes=boxes)
+    originals = [f.t.clone() for f in fine.fabs]
+    want = []
+    for f, o in zip(fine.fabs, originals):
+        cov, _ = _covering(f.box, fine.valid, shifts)
+        p = _prolongated(op_args, it, gc, f)
+        m = _mask(f.box, *grown) & (cov < 0)
+        want.append(np.where(m[None], p, o.cpu().numpy()))
+    assert any((_covering(f.box, fine.valid, shifts)[0] >= 3).any() for f in fine.fabs)
+    wl = WorkList()
+    wl.append_level(op, crse, 1, fine, 0, NCOMP, bc=_bc(PER_FACES))
+    _run_all_ways(wl, fine.fabs, originals, want, ("periodic ghosts", op_args))
+    wl.close()
+    # remake from that level onto shifted boxes
+    old = fine
+    new_boxes = [((0, 2, 0), (11, 11, 9)), ((36, 0, 0), (fdom.hi[0], 9, 11))]
+    new = fine_level(it, rng, boxes=new_boxes, fill_valid=False)
+    originals = [f.t.clone() for f in new.fabs]
+    old_np = [f.t.cpu().numpy() for f in old.fabs]
+    want = []
+    for f, o in zip(new.fabs, originals):
+        b = f.box
+        cov, pairs = _covering(b, old.valid, shifts)
+        p = _prolongated(op_args, it, gc, f)
+        w = np.where(_mask(b, *grown)[None], p, o.cpu().numpy())
+        idx = np.meshgrid(*[np.arange(b.lo[d], b.hi[d] + 1) for d in (2, 1, 0)], indexing="ij")
+        for k, (sh, bi) in enumerate(pairs):
+            m = cov == k
+            if not m.any():
+                continue
+            ob = old.fabs[bi].box
+            zz, yy, xx = (idx[0][m] - sh[2] - ob.lo[2], idx[1][m] - sh[1] - ob.lo[1],
+                          idx[2][m] - sh[0] - ob.lo[0])
+            w[:, m] = old_np[bi][:, zz, yy, xx]
+        want.append(w)
+    assert any((_covering(f.box, old.valid, shifts)[0] >= 2).any() for f in new.fabs)
+    wl = WorkList()
+    wl.append_remake_level(op, crse, 1, old, new, 0, NCOMP, bc=_bc(PER_FACES))
+    _run_all_ways(wl, new.fabs, originals, want, ("periodic remake", op_args))
+    wl.close()
