@@ -108,6 +108,7 @@ _SIGS = {
                                                  ctypes.POINTER(CxpBc)]),
     "cxp_worklist_append_boundaries": (c_int, [c_void_p, ctypes.POINTER(CxpLevel), P_INT,
                                                c_int, c_int, ctypes.POINTER(CxpBc)]),
+    "cxp_worklist_set_per_group": (c_int, [c_void_p, c_int]),
     "cxp_worklist_upload": (c_int, [c_void_p, c_void_p]),
     "cxp_worklist_stats": (c_int, [c_void_p, P_INT64, P_INT64, P_INT64, P_INT64]),
     "cxp_prolongate_batched": (c_int, [c_void_p, c_void_p]),

@@ -187,6 +187,12 @@ class WorkList:
             self.h, ctypes.byref(lv), i3(indextype), comp, ncomp, ctypes.byref(b)))
         self._keep += list(level.fabs)
 
+    def set_per_group(self, per_group):
+        """The reference's prolongate_per_group parameter for this list's
+        ENO and fallback patches (takes effect at the next upload)."""
+        check("cxp_worklist_set_per_group",
+              _lib.lib().cxp_worklist_set_per_group(self.h, 1 if per_group else 0))
+
     def upload(self, stream=None):
         check("cxp_worklist_upload", _lib.lib().cxp_worklist_upload(self.h, _stream(stream)))
 

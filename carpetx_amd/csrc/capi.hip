@@ -511,6 +511,9 @@ struct cxp_worklist_s {
   PoisonChunk *d_poison_chunks = nullptr;
   size_t cap_poison_runs = 0, cap_poison_chunks = 0;
   bool uploaded = false;
+  // the reference's prolongate_per_group parameter, for ENO and fallback
+  // patches
+  int per_group = 0;
   int grid = 0;
   int lds = 0;  // bytes of dynamic LDS of k_fill_batched
   int uniform_sid = -1;  // the one stencil of every separable task, or -1
@@ -711,7 +714,7 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
   std::vector<Plan> plans;
   for (size_t n = 0; n < wl->patches.size(); ++n) {
     const auto &p = wl->patches[n];
-    if (!op_is_separable(p.op->d, false)) {
+    if (!op_is_separable(p.op->d, wl->per_group != 0)) {
       wl->point_patches.push_back((int)n);
       continue;
     }
@@ -1016,7 +1019,7 @@ int launch_batched(cxp_worklist_s *wl, hipStream_t s) {
     const PointTask t = make_point_task(p.op->d, &p.crse, p.crse_comp,
                                         &p.fine, p.fine_comp, p.ncomp, p.rlo,
                                         p.rhi);
-    const int e = launch_point(t, p.ncomp, 0, s);
+    const int e = launch_point(t, p.ncomp, wl->per_group, s);
     if (e)
       return e;
   }
@@ -2101,9 +2104,22 @@ int cxp_worklist_append_boundaries(cxp_worklist wl, const cxp_level *level,
   return CXP_OK;
 }
 
+int cxp_worklist_set_per_group(cxp_worklist wl, int per_group) {
+  if (!wl || (per_group != 0 && per_group != 1))
+    return CXP_ERR_ARG;
+  wl->per_group = per_group;
+  wl->uploaded = false;
+  return CXP_OK;
+}
+
 int cxp_worklist_upload(cxp_worklist wl, void *stream) {
   if (!wl)
     return CXP_ERR_ARG;
+  // per-group decisions cover at most 16 components (impl.hxx:1707-1708)
+  if (wl->per_group)
+    for (const auto &p : wl->patches)
+      if (p.ncomp > 16 && !op_is_separable(p.op->d, true))
+        return CXP_ERR_NCOMP;
   if (const int e = place_gathered(wl))
     return e;
   build_tiles(wl);
@@ -2167,8 +2183,8 @@ int cxp_prolongate_per_patch(cxp_worklist wl, void *stream) {
   }
   for (const auto &p : wl->patches) {
     const int e = prolongate_impl(p.op, &p.crse, p.crse_comp, &p.fine,
-                                  p.fine_comp, p.ncomp, p.rlo, p.rhi, 0,
-                                  (hipStream_t)stream);
+                                  p.fine_comp, p.ncomp, p.rlo, p.rhi,
+                                  wl->per_group, (hipStream_t)stream);
     if (e)
       return e;
   }
@@ -2250,7 +2266,7 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
   // separable, non-thin keys of the list, grouped by size class
   std::map<int, std::vector<TuneKey>> classes;
   for (const auto &p : wl->patches) {
-    if (!op_is_separable(p.op->d, false))
+    if (!op_is_separable(p.op->d, wl->per_group != 0))
       continue;
     const SepTask t = make_sep_task(p.op->d, &p.crse, p.crse_comp, &p.fine,
                                     p.fine_comp, p.rlo, p.rhi);

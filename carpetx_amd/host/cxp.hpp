@@ -370,6 +370,11 @@ public:
                                          ncomp, &b.b),
           "cxp_worklist_append_boundaries");
   }
+  // the reference's prolongate_per_group parameter for this list
+  void set_per_group(bool per_group) {
+    check(cxp_worklist_set_per_group(wl_, per_group ? 1 : 0),
+          "cxp_worklist_set_per_group");
+  }
   void upload(void *stream = nullptr) {
     check(cxp_worklist_upload(wl_, stream), "cxp_worklist_upload");
   }

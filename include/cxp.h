@@ -269,6 +269,13 @@ int cxp_worklist_append_remake_level(cxp_worklist wl, cxp_op op,
 int cxp_worklist_append_boundaries(cxp_worklist wl, const cxp_level *level,
                                    const int indextype[3], int comp,
                                    int ncomp, const cxp_bc *bc);
+/* The reference's `prolongate_per_group` parameter for the list's calls
+ * (CarpetX/param.ccl:449-451; impl.hxx:2179-2187): 0 (the default) runs
+ * ENO and fallback patches per variable, 1 with per-group decisions, as
+ * cxp_prolongate's per_group.  Operators without branches give the same
+ * values either way.  Takes effect at the next upload, which returns
+ * CXP_ERR_NCOMP if a per-group patch has more than 16 components. */
+int cxp_worklist_set_per_group(cxp_worklist wl, int per_group);
 /* Builds tiles (using the tuned tile shapes) and copies the list to the
  * device; must precede cxp_prolongate_batched. */
 int cxp_worklist_upload(cxp_worklist wl, void *stream);

@@ -92,7 +92,7 @@ def fine_level(it, rng, boxes=FINE_VALID, fill_valid=True):
     return Level(fabs, valid, fdom)
 
 
-def expected(op_args, it, g, fine):
+def expected(op_args, it, g, fine, per_group=False):
     cdom, fdom = domains(it)
     oop = oracle.lookup(*op_args)
     glo = tuple(cdom.lo[d] - CG for d in range(3))
@@ -103,7 +103,7 @@ def expected(op_args, it, g, fine):
         b = fab.box
         orig = fab.t.cpu().numpy()
         fh = oracle.Fab(b.lo, b.hi, NCOMP, np.zeros_like(orig))
-        oracle.prolongate(oop, ch, fh, b.lo, b.hi)
+        oracle.prolongate(oop, ch, fh, b.lo, b.hi, per_group=per_group)
         idx = np.meshgrid(*[np.arange(b.lo[d], b.hi[d] + 1) for d in (2, 1, 0)], indexing="ij")
         fill = np.ones(orig.shape[1:], dtype=bool)
         for a, d in zip(idx, (2, 1, 0)):
@@ -581,4 +581,33 @@ def test_periodic_fine_images(op_args):
     wl = WorkList()
     wl.append_remake_level(op, crse, 1, old, new, 0, NCOMP, bc=_bc(PER_FACES))
     _run_all_ways(wl, new.fabs, originals, want, ("periodic remake", op_args))
+    wl.close()
+
+
+@pytest.mark.parametrize("op_args", [("eno", 3, (1, 1, 1)), ("poly-cons3lfb", 3, (1, 0, 1)),
+                                     ("poly-eno3lfb", 3, (1, 1, 1))])
+def test_level_fill_per_group_matches_oracle(op_args):
+    """cxp_worklist_set_per_group(1): the list's ENO and fallback patches
+    take the per-group decisions (one shift and one fallback choice for all
+    components), as the reference's prolongate_per_group parameter selects;
+    batched, per-patch and graph runs equal the oracle's per-group
+    prolongation bit for bit, and switching back to per-variable gives the
+    per-variable values."""
+    it = op_args[2]
+    rng = np.random.default_rng(41 + sum(it))
+    cdom, _ = domains(it)
+    gshape = (NC_CRSE,) + tuple(cdom.size[d] + 2 * CG for d in (2, 1, 0))
+    g = rng.uniform(-1, 1, gshape)
+    crse = coarse_level(it, g)
+    fine = fine_level(it, rng)
+    originals = [f.t.clone() for f in fine.fabs]
+    want_pg = expected(op_args, it, g, fine, per_group=True)
+    want_pv = expected(op_args, it, g, fine, per_group=False)
+    assert any(not np.array_equal(a, b) for a, b in zip(want_pg, want_pv))
+    wl = WorkList()
+    wl.append_level(get_interpolator(*op_args), crse, 1, fine, 0, NCOMP)
+    wl.set_per_group(True)
+    _run_all_ways(wl, fine.fabs, originals, want_pg, ("per group", op_args))
+    wl.set_per_group(False)
+    _run_all_ways(wl, fine.fabs, originals, want_pv, ("per var", op_args))
     wl.close()

@@ -476,6 +476,24 @@ def test_level_boundaries_check_their_sources():
     assert L.cxp_worklist_destroy(wl) == 0
 
 
+def test_worklist_per_group_setting_checks():
+    """cxp_worklist_set_per_group takes 0 or 1; an upload in per-group mode
+    with a branching patch of more than 16 components fails with
+    CXP_ERR_NCOMP before any device work."""
+    L = _lib.lib()
+    wl = ctypes.c_void_p()
+    assert L.cxp_worklist_create(ctypes.byref(wl)) == 0
+    assert L.cxp_worklist_set_per_group(None, 1) == -3
+    assert L.cxp_worklist_set_per_group(wl, 2) == -3
+    h = _lookup("eno", 3, (1, 1, 1))[1]
+    crse, k1 = _level([((0, 0, 0), (23, 23, 23))], ((0, 0, 0), (23, 23, 23)), 17, 6)
+    fine, k2 = _level([((8, 8, 8), (31, 31, 31))], ((0, 0, 0), (47, 47, 47)), 17, 3)
+    assert L.cxp_worklist_append_level(wl, h, ctypes.byref(crse), 0, ctypes.byref(fine), 0, 17) == 0
+    assert L.cxp_worklist_set_per_group(wl, 1) == 0
+    assert L.cxp_worklist_upload(wl, None) == -6
+    assert L.cxp_worklist_destroy(wl) == 0
+
+
 def test_cpp_host_header_compiles():
     """The C++ host API (carpetx_amd/host/cxp.hpp) is self-contained C++17
     over the C header."""
