@@ -1863,6 +1863,93 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
       }
   };
 
+  // Gathers coarse box C at offset `gather` of the patch buffer, into box
+  // E: inside the domain from the valid regions (and in periodic
+  // directions from the images one period away); outside it, with
+  // boundary conditions, the boundary values (E is then C and their
+  // sources), without, the first FAB holding each point (E = C).  On
+  // failure nothing is added.
+  auto gather_box = [&](const IBox &C, IBox &E) -> int {
+    const size_t n_runs = runs.size(), n_chunks = chunks.size();
+    const size_t n_bruns = bruns.size(), n_bchunks = bchunks.size();
+    auto fail = [&](int e) {
+      runs.resize(n_runs);
+      chunks.resize(n_chunks);
+      bruns.resize(n_bruns);
+      bchunks.resize(n_bchunks);
+      return e;
+    };
+    const bool use_bc = bc && !inside(C, cdom);
+    E = C;
+    if (use_bc)
+      if (const int e = bc_closure(*bc, op->d, cdom, C, E))
+        return fail(e);
+    if (E.volume() > (int64_t)INT32_MAX)
+      return fail(CXP_ERR_ARG);
+    const int64_t off = gather;
+    std::vector<IBox> left;
+    // The unshifted part goes first and each image takes only points no
+    // earlier one covered (with vertex centring the domain's first and last
+    // points are images of each other; the point inside is kept).
+    std::vector<IBox> covered;
+    for (int s0 = 0; s0 < 27; ++s0) {
+      const int s = (s0 + 13) % 27;  // (0, 0, 0) first
+      const int sh[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+      if ((sh[0] && !periodic[0]) || (sh[1] && !periodic[1]) ||
+          (sh[2] && !periodic[2]))
+        continue;
+      IBox img = cdom;  // the domain moved by the shift
+      int dsh[3];
+      for (int d = 0; d < 3; ++d) {
+        dsh[d] = sh[d] * period[d];
+        img.lo[d] += dsh[d];
+        img.hi[d] += dsh[d];
+      }
+      const IBox part = intersect(E, img);
+      if (part.empty())
+        continue;
+      std::vector<IBox> pieces{part};
+      for (const IBox &cv : covered)
+        subtract_all(pieces, cv);
+      covered.push_back(part);
+      for (IBox piece : pieces) {
+        for (int d = 0; d < 3; ++d) {  // its source, inside the domain
+          piece.lo[d] -= dsh[d];
+          piece.hi[d] -= dsh[d];
+        }
+        left.assign(1, piece);
+        for (int j = 0; j < crse->nfabs; ++j) {
+          const IBox X = intersect(piece, cvalid(j));
+          if (!X.empty()) {
+            add_copy(j, X, E, off, dsh);
+            subtract_all(left, cvalid(j));
+          }
+        }
+        if (!left.empty())
+          return fail(CXP_ERR_COARSE);
+      }
+    }
+    left.clear();
+    if (use_bc) {
+      add_bc(E, off);
+    } else {
+      subtract(E, cdom, left);
+      for (int j = 0; j < crse->nfabs && !left.empty(); ++j) {
+        std::vector<IBox> rest;
+        for (const IBox &r : left) {
+          const IBox X = intersect(r, cfabbox(j));
+          if (!X.empty())
+            add_copy(j, X, E, off);
+          subtract(r, cfabbox(j), rest);
+        }
+        left.swap(rest);
+      }
+      if (!left.empty())
+        return fail(CXP_ERR_COARSE);
+    }
+    return CXP_OK;
+  };
+
   for (int i = 0; i < fine->nfabs; ++i) {
     const cxp_fab &ff = fine->fabs[i];
     const IBox fv = ibox(fine->valid + 6 * i, fine->valid + 6 * i + 3);
@@ -1886,7 +1973,14 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
         add_fab_copies(i);
       }
     }
+    // patches of this FAB: in place where one coarse FAB holds the coarse
+    // box (inside the coarse domain within its valid region, and inside
+    // the domain when boundary conditions apply); the others share one
+    // gathered box, the hull of their coarse boxes (their boxes overlap at
+    // edges and corners), or, if that hull cannot be filled, one each
     std::vector<IBox> cboxes;
+    std::vector<cxp_worklist_s::Patch> fp;
+    std::vector<size_t> gathered;
     for (const IBox &P : pieces) {
       IBox C;
       op_coarse_box(op->d, P.lo, P.hi, C.lo, C.hi);
@@ -1901,9 +1995,6 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
         p.rlo[d] = P.lo[d];
         p.rhi[d] = P.hi[d];
       }
-      // in place: one coarse FAB holds C, inside the coarse domain within
-      // its valid region (and, with boundary conditions to apply, C lies
-      // inside the domain)
       const bool bcp = bc && !inside(C, cdom);
       int src = -1;
       for (int j = 0; j < crse->nfabs && src < 0 && !bcp; ++j)
@@ -1914,92 +2005,48 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
         p.crse = crse->fabs[src];
         p.crse_comp = crse_comp;
       } else {
-        // gathered into box E: inside the domain from the valid regions;
-        // outside it the boundary values (E is C and their sources), or
-        // without boundary conditions (E = C) the first FAB holding each
-        // point
-        IBox E = C;
-        if (bcp)
-          if (const int e = bc_closure(*bc, op->d, cdom, C, E))
-            return e;
-        if (E.volume() > (int64_t)INT32_MAX)
-          return CXP_ERR_ARG;
-        const int64_t off = gather;
-        std::vector<IBox> left;
-        // valid data: the part of E inside the domain, and in periodic
-        // directions the parts one period away, read from their images.
-        // The unshifted part goes first and each image takes only points no
-        // earlier one covered (with vertex centring the domain's first and
-        // last points are images of each other; the point inside is kept).
-        std::vector<IBox> covered;
-        for (int s0 = 0; s0 < 27; ++s0) {
-          const int s = (s0 + 13) % 27;  // (0, 0, 0) first
-          const int sh[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
-          if ((sh[0] && !periodic[0]) || (sh[1] && !periodic[1]) ||
-              (sh[2] && !periodic[2]))
-            continue;
-          IBox img = cdom;  // the domain moved by the shift
-          int dsh[3];
-          for (int d = 0; d < 3; ++d) {
-            dsh[d] = sh[d] * period[d];
-            img.lo[d] += dsh[d];
-            img.hi[d] += dsh[d];
-          }
-          const IBox part = intersect(E, img);
-          if (part.empty())
-            continue;
-          std::vector<IBox> pieces{part};
-          for (const IBox &cv : covered)
-            subtract_all(pieces, cv);
-          covered.push_back(part);
-          for (IBox piece : pieces) {
-            for (int d = 0; d < 3; ++d) {  // its source, inside the domain
-              piece.lo[d] -= dsh[d];
-              piece.hi[d] -= dsh[d];
-            }
-            left.assign(1, piece);
-            for (int j = 0; j < crse->nfabs; ++j) {
-              const IBox X = intersect(piece, cvalid(j));
-              if (!X.empty()) {
-                add_copy(j, X, E, off, dsh);
-                subtract_all(left, cvalid(j));
-              }
-            }
-            if (!left.empty())
-              return CXP_ERR_COARSE;
-          }
-        }
-        left.clear();
-        if (bcp) {
-          add_bc(E, off);
-        } else {
-          subtract(E, cdom, left);
-          for (int j = 0; j < crse->nfabs && !left.empty(); ++j) {
-            std::vector<IBox> rest;
-            for (const IBox &r : left) {
-              const IBox X = intersect(r, cfabbox(j));
-              if (!X.empty())
-                add_copy(j, X, E, off);
-              subtract(r, cfabbox(j), rest);
-            }
-            left.swap(rest);
-          }
-          if (!left.empty())
-            return CXP_ERR_COARSE;
-        }
-        p.crse.data = nullptr;  // placed at upload
-        for (int d = 0; d < 3; ++d) {
-          p.crse.lo[d] = E.lo[d];
-          p.crse.hi[d] = E.hi[d];
-        }
-        p.crse.comp_stride = E.volume();
-        p.crse.ncomp = ncomp;
-        p.crse_comp = 0;
-        p.gather_off = off;
-        gather += E.volume() * ncomp;
+        gathered.push_back(fp.size());
       }
-      patches.push_back(p);
+      fp.push_back(p);
     }
+    auto point_at = [&](cxp_worklist_s::Patch &p, const IBox &E, int64_t off) {
+      p.crse.data = nullptr;  // placed at upload
+      for (int d = 0; d < 3; ++d) {
+        p.crse.lo[d] = E.lo[d];
+        p.crse.hi[d] = E.hi[d];
+      }
+      p.crse.comp_stride = E.volume();
+      p.crse.ncomp = ncomp;
+      p.crse_comp = 0;
+      p.gather_off = off;
+    };
+    if (!gathered.empty()) {
+      IBox H = cboxes[gathered[0]];
+      int64_t separate = 0;
+      for (size_t k : gathered) {
+        separate += cboxes[k].volume();
+        for (int d = 0; d < 3; ++d) {
+          H.lo[d] = std::min(H.lo[d], cboxes[k].lo[d]);
+          H.hi[d] = std::max(H.hi[d], cboxes[k].hi[d]);
+        }
+      }
+      // shared only where it gathers no more than the separate boxes
+      IBox E;
+      if (gathered.size() > 1 && H.volume() <= separate &&
+          gather_box(H, E) == CXP_OK) {
+        for (size_t k : gathered)
+          point_at(fp[k], E, gather);
+        gather += E.volume() * ncomp;
+      } else {
+        for (size_t k : gathered) {
+          if (const int e = gather_box(cboxes[k], E))
+            return e;
+          point_at(fp[k], E, gather);
+          gather += E.volume() * ncomp;
+        }
+      }
+    }
+    patches.insert(patches.end(), fp.begin(), fp.end());
     coarse_pts += union_points(cboxes);
   }
   if (!gather_chunks_ok(chunks, run0, runs.size()) ||
