@@ -358,7 +358,37 @@ def bench_wave(log, n=128, steps=16, ck=4):
             c, f = w.step(c, f)
         torch.cuda.synchronize()
         t6 = time.perf_counter()
+    # forward only with the native RK4 stage (both levels) and batched
+    # fills, eagerly and as one CUDA graph per coarse step
+    from carpetx_amd.wave import NativeStep
+    with torch.no_grad():
+        ns = NativeStep(w, sc0, sf0)
+        ns.run()
+        torch.cuda.synchronize()
+        t7 = time.perf_counter()
+        for _ in range(steps):
+            ns.run()
+        torch.cuda.synchronize()
+        t8 = time.perf_counter()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            ns.run()
+        torch.cuda.current_stream().wait_stream(side)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            ns.run()
+        graph.replay()
+        torch.cuda.synchronize()
+        t9 = time.perf_counter()
+        for _ in range(steps):
+            graph.replay()
+        torch.cuda.synchronize()
+        t10 = time.perf_counter()
+        ns.close()
     rec = dict(n=n, steps=steps, checkpoint_every=ck,
+               fwd_native_ms_per_step=round(1e3 * (t8 - t7) / steps, 3),
+               fwd_native_graph_ms_per_step=round(1e3 * (t10 - t9) / steps, 3),
                fwd_ms_per_step=round(1e3 * (t1 - t0) / steps, 3),
                bwd_ms_per_step=round(1e3 * (t2 - t1) / steps, 3),
                fwd_nograd_ms_per_step=round(1e3 * (t6 - t5) / steps, 3),

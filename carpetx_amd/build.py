@@ -53,6 +53,17 @@ def build_lib():
         os.remove(o)
 
 
+WAVE_LIB = os.path.join(HERE, "libcxp_wave.so")
+
+
+def build_wave_lib():
+    """The wave harness's native RK4 stage (csrc/wave.hip), a library of its
+    own beside libcxp.so."""
+    tmp = WAVE_LIB + ".tmp"
+    run([HIPCC] + HIPFLAGS + ["-shared", "-o", tmp, os.path.join(CSRC, "wave.hip")])
+    os.replace(tmp, WAVE_LIB)
+
+
 HOST_CHECK = os.path.join(HERE, "host", "prolongate_check")
 
 
@@ -74,6 +85,7 @@ def build_oracle():
 def main():
     t0 = time.time()
     build_lib()
+    build_wave_lib()
     build_host_check()
     print("built %s in %.1fs" % (LIB, time.time() - t0))
 

@@ -29,13 +29,35 @@ respect to the initial data flows back through the whole evolution;
 ``checkpoint_every`` coarse steps are recomputed in the backward pass
 instead of being stored.
 """
+import ctypes
+import os
+
 import torch
 from torch.utils.checkpoint import checkpoint
 
 from .autograd import fill_ghosts, region_view, restrict
-from .interp import Box, get_interpolator
+from .interp import Box, FArrayBox, get_interpolator
 
 GHOSTS = 2
+
+_WAVE_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcxp_wave.so")
+_wave = None
+
+
+def wave_lib():
+    """libcxp_wave.so (the native RK4 stage); raises if it is not built."""
+    global _wave
+    if _wave is None:
+        if not os.path.exists(_WAVE_LIB):
+            raise RuntimeError("libcxp_wave.so is not built (%s); run "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`"
+                               % _WAVE_LIB)
+        L = ctypes.CDLL(_WAVE_LIB)
+        L.cxp_wave_rk4_stage.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 4 + \
+            [ctypes.c_int] * 5 + [ctypes.c_double] * 2 + [ctypes.c_void_p]
+        L.cxp_wave_rk4_stage.restype = ctypes.c_int
+        _wave = L
+    return _wave
 
 
 def laplacian4(u, h):
@@ -191,6 +213,90 @@ class TwoLevelWave:
             sc, sf = checkpoint(block, sc, sf, k, use_reentrant=False)
             done += k
         return sc, sf
+
+
+class NativeStep:
+    """TwoLevelWave.step without autograd, on fixed device buffers: each RK4
+    stage is one fused launch of the native stage (cxp_wave_rk4_stage, both
+    levels), each fine ghost fill one batched work-list launch from a
+    time-interpolated coarse sub-box; time interpolation and restriction as
+    in step().  Bit-identical to step() (same operations, same order).
+    ``run()`` advances ``c`` and ``f`` by one coarse step in place on the
+    current stream, so it can be captured in a CUDA graph."""
+
+    def __init__(self, wave, sc, sf):
+        from .batched import WorkList
+        self.w = wave
+        dev = sc.device
+        self.c = sc.detach().clone()
+        self.c1 = torch.empty_like(self.c)
+        self.ca = torch.empty_like(self.c)
+        self.cb = torch.empty_like(self.c)
+        self.cacc = torch.empty_like(self.c)
+        self.f = sf.detach().clone()
+        self.fa = self.f.clone()
+        self.fb = self.f.clone()
+        self.facc = torch.empty_like(self.f)
+        self.ci = FArrayBox(wave.crse_sub, 2, tensor=torch.zeros(
+            (2,) + tuple(reversed(wave.crse_sub.size)), dtype=torch.float64, device=dev))
+        self.lists = {}
+        for name in ("f", "fa", "fb"):
+            fab = FArrayBox(wave.fine_box, 2, tensor=getattr(self, name))
+            wl = WorkList()
+            wl.append_shell(wave.P, self.ci, 0, fab, 0, 2, wave.fine_int)
+            wl.upload()
+            self.lists[name] = wl
+
+    def close(self):
+        for wl in self.lists.values():
+            wl.close()
+
+    def _stage(self, stage, y, dst, base, acc, h, ghosts, periodic, a):
+        c = 1.0 / (12.0 * h * h)
+        nz, ny, nx = y.shape[1:]
+        st = wave_lib().cxp_wave_rk4_stage(
+            stage, y.data_ptr(), dst.data_ptr(), base.data_ptr(), acc.data_ptr(),
+            nx, ny, nz, ghosts, periodic, c, a,
+            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        if st != 0:
+            raise RuntimeError("cxp_wave_rk4_stage failed: %d" % st)
+
+    def _fill(self, name, th):
+        w = self.w
+        full = Box((0, 0, 0), (w.n - 1,) * 3)
+        c0 = region_view(self.c, full, w.crse_sub)
+        c1 = region_view(self.c1, full, w.crse_sub)
+        self.ci.t.copy_((1.0 - th) * c0 + th * c1)
+        self.lists[name].run()
+
+    def run(self):
+        w = self.w
+        dt = w.dt
+        h = w.h
+        # coarse RK4, periodic (rk4_coarse)
+        self._stage(1, self.c, self.ca, self.c, self.cacc, h, 0, 1, 0.5 * dt)
+        self._stage(2, self.ca, self.cb, self.c, self.cacc, h, 0, 1, 0.5 * dt)
+        self._stage(3, self.cb, self.ca, self.c, self.cacc, h, 0, 1, dt)
+        self._stage(4, self.ca, self.c1, self.c, self.cacc, h, 0, 1, dt / 6.0)
+        # two fine RK4 steps (rk4_fine), ghosts filled at every stage
+        dtf = 0.5 * dt
+        half = 0.5 * dtf / dt
+        hf = h / 2
+        for theta0 in (0.0, 0.5):
+            self._fill("f", theta0)
+            self._stage(1, self.f, self.fa, self.f, self.facc, hf, GHOSTS, 0, 0.5 * dtf)
+            self._fill("fa", theta0 + half)
+            self._stage(2, self.fa, self.fb, self.f, self.facc, hf, GHOSTS, 0, 0.5 * dtf)
+            self._fill("fb", theta0 + half)
+            self._stage(3, self.fb, self.fa, self.f, self.facc, hf, GHOSTS, 0, dtf)
+            self._fill("fa", theta0 + 2 * half)
+            self._stage(4, self.fa, self.f, self.f, self.facc, hf, GHOSTS, 0, dtf / 6.0)
+            self._fill("f", theta0 + 2 * half)
+        # restriction, then the new coarse state becomes the current one
+        inj = restrict(self.f, w.fine_box, w.crse_under, (0, 0, 0))
+        cu = w.crse_under
+        self.c1[:, cu.lo[2]:cu.hi[2] + 1, cu.lo[1]:cu.hi[1] + 1, cu.lo[0]:cu.hi[0] + 1] = inj
+        self.c.copy_(self.c1)
 
 
 def loss(sc, sf):

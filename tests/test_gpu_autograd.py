@@ -119,3 +119,45 @@ def test_wave_graphed_step_replays_match_eager():
         torch.cuda.synchronize()
         assert torch.equal(c, ec), step
         assert torch.equal(f, ef), step
+
+
+def test_wave_native_step_matches_step():
+    """NativeStep (the fused native RK4 stage on both levels, batched ghost
+    fills) advances the two-level wave exactly as TwoLevelWave.step does:
+    three steps eagerly, and three replays of its CUDA graph, bit for
+    bit."""
+    from carpetx_amd.wave import NativeStep, TwoLevelWave
+    w = TwoLevelWave(n=32)
+    sc, sf = w.gaussian(width=0.08)
+    with torch.no_grad():
+        c, f = sc.clone(), sf.clone()
+        ref = []
+        for _ in range(3):
+            c, f = w.step(c, f)
+            ref.append((c.clone(), f.clone()))
+        ns = NativeStep(w, sc, sf)
+        for k in range(3):
+            ns.run()
+            torch.cuda.synchronize()
+            assert torch.equal(ns.c, ref[k][0]), ("coarse", k)
+            assert torch.equal(ns.f, ref[k][1]), ("fine", k)
+        ns.close()
+        ng = NativeStep(w, sc, sf)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            ng.run()  # set-up outside capture
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        ng.c.copy_(sc)
+        ng.f.copy_(sf)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            ng.run()
+        torch.cuda.synchronize()
+        for k in range(3):
+            graph.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(ng.c, ref[k][0]), ("graph coarse", k)
+            assert torch.equal(ng.f, ref[k][1]), ("graph fine", k)
+        ng.close()
