@@ -504,3 +504,23 @@ def test_cpp_host_header_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra",
                         "-x", "c++", hdr], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_wave_library_exports_its_header():
+    """libcxp_wave.so exports every function include/cxp_wave.h declares,
+    and its argument checks run before any device work."""
+    import os
+    import re
+    from carpetx_amd.wave import wave_lib
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(root, "include", "cxp_wave.h")) as f:
+        names = sorted(set(re.findall(r"\b(cxp_[a-z0-9_]+)\s*\(", f.read())))
+    assert names == ["cxp_wave_rk4_stage"]
+    W = wave_lib()
+    for n in names:
+        assert hasattr(W, n)
+    fake = 0x1000
+    assert W.cxp_wave_rk4_stage(5, fake, fake + 8, fake, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
+    assert W.cxp_wave_rk4_stage(1, fake, fake, fake, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
+    assert W.cxp_wave_rk4_stage(1, fake, fake + 8, fake, fake, 8, 8, 8, 2, 1, 1.0, 1.0, None) == -3
+    assert W.cxp_wave_rk4_stage(1, fake, fake + 8, fake, fake, 4, 8, 8, 2, 0, 1.0, 1.0, None) == -3
