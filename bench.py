@@ -386,7 +386,29 @@ def bench_wave(log, n=128, steps=16, ck=4):
         torch.cuda.synchronize()
         t10 = time.perf_counter()
         ns.close()
+    # the same evolution with the native stage and its adjoint under
+    # autograd (TwoLevelWave(native=True))
+    wn = TwoLevelWave(n=n, native=True)
+    yc = sc0.clone().requires_grad_(True)
+    yf = sf0.clone().requires_grad_(True)
+    loss(*wn.evolve(yc, yf, 1, checkpoint_every=1)).backward()
+    del yc, yf
+    nc = sc0.clone().requires_grad_(True)
+    nf = sf0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t11 = time.perf_counter()
+    an, bn = wn.evolve(nc, nf, steps, checkpoint_every=ck)
+    Ln = loss(an, bn)
+    torch.cuda.synchronize()
+    t12 = time.perf_counter()
+    Ln.backward()
+    torch.cuda.synchronize()
+    t13 = time.perf_counter()
     rec = dict(n=n, steps=steps, checkpoint_every=ck,
+               fwd_native_autograd_ms_per_step=round(1e3 * (t12 - t11) / steps, 3),
+               bwd_native_ms_per_step=round(1e3 * (t13 - t12) / steps, 3),
+               native_grad_rel_diff=float((nc.grad - xc.grad).abs().max() /
+                                          xc.grad.abs().max()),
                fwd_native_ms_per_step=round(1e3 * (t8 - t7) / steps, 3),
                fwd_native_graph_ms_per_step=round(1e3 * (t10 - t9) / steps, 3),
                fwd_ms_per_step=round(1e3 * (t1 - t0) / steps, 3),

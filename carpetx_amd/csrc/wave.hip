@@ -57,7 +57,84 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Adjoint of one stage, in gather form (no atomics).  Given the gradients
+// dst_bar and, for stages 1-3, acc_out_bar, with k_bar = a dst_bar + w
+// acc_out_bar (w = 1 for stage 1, 2 for stages 2-3; stage 4: k_bar =
+// a dst_bar) at interior points:
+//   y_bar[rho](p) += k_bar[u](p)               (interior p)
+//   y_bar[u](q)   += c sum_p L(p, q) k_bar[rho](p)   (every q a stencil reaches)
+// L is symmetric in its offsets: -90 at q = p, 16 at distance 1 and -1 at
+// distance 2 along an axis.  One thread per point q of y (ghosts
+// included; with periodic, the wrapped neighbours).  The pointwise parts
+// (base_bar, acc_in_bar) are done by the caller.
+__global__ void __launch_bounds__(256)
+    k_wave_stage_adjoint(int stage, const double *__restrict__ dst_bar,
+                         const double *__restrict__ acc_bar,
+                         double *__restrict__ y_bar, int nx, int ny, int nz,
+                         int g, int periodic, double c, double a) {
+  const int qx = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int qy = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int qz = blockIdx.z;
+  if (qx >= nx || qy >= ny || qz >= nz)
+    return;
+  const long long sy = nx, sz = (long long)nx * ny, cs = sz * nz;
+  const double w = stage == 1 ? 1.0 : 2.0;
+  auto inside = [&](int x, int yy, int z) {
+    return x >= g && x < nx - g && yy >= g && yy < ny - g && z >= g &&
+           z < nz - g;
+  };
+  // k_bar of component m at interior point (x, yy, z)
+  auto kbar = [&](int m, int x, int yy, int z) {
+    const long long p = m * cs + x + yy * sy + z * sz;
+    return stage == 4 ? a * dst_bar[p] : a * dst_bar[p] + w * acc_bar[p];
+  };
+  const long long q = qx + qy * sy + qz * sz;
+  double su = 0.0;
+  // the point itself
+  if (inside(qx, qy, qz))
+    su += -90.0 * kbar(1, qx, qy, qz);
+  const int off[4] = {-2, -1, 1, 2};
+  const double wt[4] = {-1.0, 16.0, 16.0, -1.0};
+  for (int dir = 0; dir < 3; ++dir)
+    for (int m = 0; m < 4; ++m) {
+      int x = qx, yy = qy, z = qz;
+      if (dir == 0)
+        x += off[m];
+      else if (dir == 1)
+        yy += off[m];
+      else
+        z += off[m];
+      if (periodic) {
+        x = (x + nx) % nx;
+        yy = (yy + ny) % ny;
+        z = (z + nz) % nz;
+      }
+      if (inside(x, yy, z))
+        su += wt[m] * kbar(1, x, yy, z);
+    }
+  y_bar[q] += c * su;
+  if (inside(qx, qy, qz))
+    y_bar[cs + q] += kbar(0, qx, qy, qz);
+}
+
 } // namespace
+
+extern "C" int cxp_wave_rk4_stage_adjoint(int stage, const double *dst_bar,
+                                          const double *acc_bar, double *y_bar,
+                                          int nx, int ny, int nz, int ghosts,
+                                          int periodic, double c, double a,
+                                          void *stream) {
+  if (stage < 1 || stage > 4 || !dst_bar || !y_bar ||
+      (stage < 4 && !acc_bar) || ghosts < 0 || (periodic && ghosts != 0) ||
+      nx - 2 * ghosts < 1 || ny - 2 * ghosts < 1 || nz - 2 * ghosts < 1 ||
+      (periodic && (nx < 3 || ny < 3 || nz < 3)))
+    return CXP_ERR_ARG;
+  const dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
+  hipLaunchKernelGGL(k_wave_stage_adjoint, grid, dim3(256), 0,
+                     (hipStream_t)stream, stage, dst_bar, acc_bar, y_bar, nx,
+                     ny, nz, ghosts, periodic, c, a);
+  return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
+}
 
 extern "C" int cxp_wave_rk4_stage(int stage, const double *y, double *dst,
                                   const double *base, double *acc, int nx,

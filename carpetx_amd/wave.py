@@ -56,8 +56,67 @@ def wave_lib():
         L.cxp_wave_rk4_stage.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 4 + \
             [ctypes.c_int] * 5 + [ctypes.c_double] * 2 + [ctypes.c_void_p]
         L.cxp_wave_rk4_stage.restype = ctypes.c_int
+        L.cxp_wave_rk4_stage_adjoint.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3 + \
+            [ctypes.c_int] * 5 + [ctypes.c_double] * 2 + [ctypes.c_void_p]
+        L.cxp_wave_rk4_stage_adjoint.restype = ctypes.c_int
         _wave = L
     return _wave
+
+
+def _launch(name, *args):
+    st = getattr(wave_lib(), name)(*args, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    if st != 0:
+        raise RuntimeError("%s failed: %d" % (name, st))
+
+
+class _Stage(torch.autograd.Function):
+    """One native RK4 stage, out of place: (dst, acc_out) from (y, base,
+    acc_in); dst is base with its interior replaced (the ghosts are base's,
+    as in rk4_fine's pad_interior), acc_out is acc_in with its interior
+    updated (stage 4 leaves it as it is).  The backward runs the native
+    stage adjoint (the transposed Laplacian in gather form) for y and the
+    pointwise parts in torch."""
+
+    @staticmethod
+    def forward(ctx, y, base, acc_in, stage, h, ghosts, periodic, a):
+        y, base = y.contiguous(), base.contiguous()
+        dst = base.clone()
+        acc_out = torch.zeros_like(y) if stage == 1 else acc_in.contiguous().clone()
+        nz, ny, nx = y.shape[1:]
+        c = 1.0 / (12.0 * h * h)
+        _launch("cxp_wave_rk4_stage", stage, y.data_ptr(), dst.data_ptr(), base.data_ptr(),
+                acc_out.data_ptr(), nx, ny, nz, ghosts, periodic, c, a)
+        ctx.args = (stage, nx, ny, nz, ghosts, periodic, c, a)
+        return dst, acc_out
+
+    @staticmethod
+    def backward(ctx, dst_bar, acc_bar):
+        stage, nx, ny, nz, ghosts, periodic, c, a = ctx.args
+        shape = (2, nz, ny, nx)
+        dev = torch.device("cuda")
+        dst_bar = torch.zeros(shape, dtype=torch.float64, device=dev) if dst_bar is None \
+            else dst_bar.contiguous()
+        acc_bar = torch.zeros(shape, dtype=torch.float64, device=dev) if acc_bar is None \
+            else acc_bar.contiguous()
+        y_bar = torch.zeros(shape, dtype=torch.float64, device=dev)
+        _launch("cxp_wave_rk4_stage_adjoint", stage, dst_bar.data_ptr(), acc_bar.data_ptr(),
+                y_bar.data_ptr(), nx, ny, nz, ghosts, periodic, c, a)
+        base_bar = dst_bar
+        if stage == 1:
+            acc_in_bar = None
+        elif stage < 4:
+            acc_in_bar = acc_bar
+        else:
+            g = ghosts
+            inner = (slice(None),) + (slice(g, None if g == 0 else -g),) * 3
+            acc_in_bar = acc_bar.clone()
+            acc_in_bar[inner] += a * dst_bar[inner]
+        return y_bar, base_bar, acc_in_bar, None, None, None, None, None
+
+
+def stage(y, base, acc, k, h, ghosts, periodic, a):
+    """Differentiable native RK4 stage k (1..4); returns (dst, acc)."""
+    return _Stage.apply(y, base, acc, k, h, ghosts, periodic, a)
 
 
 def laplacian4(u, h):
@@ -81,8 +140,12 @@ def periodic_pad(u, g):
 
 
 class TwoLevelWave:
-    def __init__(self, n=128, cfl=0.25, prolongation_order=5, device="cuda"):
+    def __init__(self, n=128, cfl=0.25, prolongation_order=5, device="cuda",
+                 native=False):
+        """native: RK4 stages through the native stage kernel and its
+        adjoint (same values as the torch arithmetic, bit for bit)."""
         assert n % 4 == 0
+        self.native = native
         self.n = n
         self.h = 1.0 / n
         self.dt = cfl * self.h
@@ -131,6 +194,13 @@ class TwoLevelWave:
     # --- time stepping ------------------------------------------------------
     def rk4_coarse(self, s):
         dt = self.dt
+        if self.native:
+            h = self.h
+            y, acc = stage(s, s, None, 1, h, 0, 1, 0.5 * dt)
+            y, acc = stage(y, s, acc, 2, h, 0, 1, 0.5 * dt)
+            y, acc = stage(y, s, acc, 3, h, 0, 1, dt)
+            out, _ = stage(y, s, acc, 4, h, 0, 1, dt / 6.0)
+            return out
         k1 = self.rhs_coarse(s)
         k2 = self.rhs_coarse(s + 0.5 * dt * k1)
         k3 = self.rhs_coarse(s + 0.5 * dt * k2)
@@ -150,6 +220,14 @@ class TwoLevelWave:
             return out
         half = 0.5 * dtf / self.dt
         y1 = self.fill(sf, at(theta0))
+        if self.native:
+            hf = self.h / 2
+            y, acc = stage(y1, y1, None, 1, hf, GHOSTS, 0, 0.5 * dtf)
+            y, acc = stage(self.fill(y, at(theta0 + half)), y1, acc, 2, hf, GHOSTS, 0, 0.5 * dtf)
+            y, acc = stage(self.fill(y, at(theta0 + half)), y1, acc, 3, hf, GHOSTS, 0, dtf)
+            out, _ = stage(self.fill(y, at(theta0 + 2 * half)), y1, acc, 4, hf, GHOSTS, 0,
+                           dtf / 6.0)
+            return self.fill(out, at(theta0 + 2 * half))
         k1 = self.rhs_fine(y1)
         y2 = self.fill(pad_interior(y1[:, 2:-2, 2:-2, 2:-2] + 0.5 * dtf * k1, y1), at(theta0 + half))
         k2 = self.rhs_fine(y2)

@@ -33,6 +33,17 @@ int cxp_wave_rk4_stage(int stage, const double *y, double *dst,
                        int nz, int ghosts, int periodic, double c, double a,
                        void *stream);
 
+/* Adjoint of cxp_wave_rk4_stage with respect to y, in gather form:
+ * y_bar += (dk / dy)^T k_bar with k_bar = a dst_bar + w acc_bar at the
+ * interior points (w = 1 for stage 1, 2 for stages 2 and 3; stage 4: k_bar
+ * = a dst_bar, acc_bar unused).  y_bar covers the whole box (ghosts
+ * included); the gradients of base and of the incoming acc are pointwise
+ * and left to the caller. */
+int cxp_wave_rk4_stage_adjoint(int stage, const double *dst_bar,
+                               const double *acc_bar, double *y_bar, int nx,
+                               int ny, int nz, int ghosts, int periodic,
+                               double c, double a, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -515,7 +515,7 @@ def test_wave_library_exports_its_header():
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     with open(os.path.join(root, "include", "cxp_wave.h")) as f:
         names = sorted(set(re.findall(r"\b(cxp_[a-z0-9_]+)\s*\(", f.read())))
-    assert names == ["cxp_wave_rk4_stage"]
+    assert names == ["cxp_wave_rk4_stage", "cxp_wave_rk4_stage_adjoint"]
     W = wave_lib()
     for n in names:
         assert hasattr(W, n)
@@ -524,3 +524,9 @@ def test_wave_library_exports_its_header():
     assert W.cxp_wave_rk4_stage(1, fake, fake, fake, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
     assert W.cxp_wave_rk4_stage(1, fake, fake + 8, fake, fake, 8, 8, 8, 2, 1, 1.0, 1.0, None) == -3
     assert W.cxp_wave_rk4_stage(1, fake, fake + 8, fake, fake, 4, 8, 8, 2, 0, 1.0, 1.0, None) == -3
+    adj = W.cxp_wave_rk4_stage_adjoint
+    adj.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int] * 5 + \
+        [ctypes.c_double] * 2 + [ctypes.c_void_p]
+    assert adj(0, fake, fake, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
+    assert adj(2, fake, None, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
+    assert adj(4, fake, None, fake, 8, 8, 8, 1, 1, 1.0, 1.0, None) == -3
