@@ -482,9 +482,9 @@ def cpu_baseline(wlz, nboxes, log):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the transfer would shard); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the transfer path has no multi-GPU "
+              "launch (each GPU would fill its own share of the boxes); run with "
+              "--gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
