@@ -19,6 +19,10 @@
 //      coarse FAB holds NaN outside the domain, the fill computes those
 //      points by reflection, and data even (component 0) or odd in x
 //      (component 1, parity -1) about the faces is reproduced exactly.
+//   6. The regrid-time fills on the same data: a new level filled whole
+//      and then given its own reflection boundaries, and a remake of it
+//      onto a shifted box (old data where the old box covers, prolongation
+//      elsewhere), both reproduce the data at every point of the FABs.
 //
 // Exit status 0 when every check passes.
 #include <hip/hip_runtime.h>
@@ -294,6 +298,46 @@ int main() {
     std::printf("      level fill, %zu ghost points: max error %.3e\n", ghosts, m);
     expect(ghosts > 0 && m < 1e-12,
            "level fill with reflection reproduces even and odd data");
+
+    // 6. new level (whole box, then its own boundaries) and a remake
+    auto max_err = [&](const DevFab &d, const cxp::Box &b) {
+      const auto g = d.get();
+      const auto w = fill(b, hf, false);
+      double e = 0;
+      for (size_t i = 0; i < g.size(); ++i)
+        e = std::fmax(e, std::fabs(g[i] - w[i]));
+      return e;
+    };
+    const cxp::Box fdom({0, 0, 0}, {2 * n, 2 * n, 2 * n});
+    DevFab nf(ffab, 2);
+    nf.put(std::vector<double>(ffab.numPts() * 2, std::nan("")));
+    cxp::Level nl{{nf.fab}, {fvalid}, fdom};
+    cxp::WorkList wn;
+    wn.append_new_level(P3, cl, 0, nl, 0, 2, &bc);
+    wn.append_boundaries(nl, {0, 0, 0}, 0, 2, bc);
+    wn.upload(s);
+    wn.run(s);
+    (void)hipStreamSynchronize(s);
+    const double en = max_err(nf, ffab);
+    std::printf("      new level, %lld points: max error %.3e\n",
+                (long long)ffab.numPts() * 2, en);
+    expect(en < 1e-12, "new-level fill with boundaries reproduces the data everywhere");
+
+    const cxp::Box rvalid({4, 0, 2}, {19, 13, 15});
+    const cxp::Box rfab = rvalid.grow(3);
+    DevFab rf(rfab, 2);
+    rf.put(std::vector<double>(rfab.numPts() * 2, std::nan("")));
+    cxp::Level rl{{rf.fab}, {rvalid}, fdom};
+    cxp::WorkList wr;
+    wr.append_remake_level(P3, cl, 0, nl, rl, 0, 2, &bc);
+    wr.append_boundaries(rl, {0, 0, 0}, 0, 2, bc);
+    wr.upload(s);
+    wr.run(s);
+    (void)hipStreamSynchronize(s);
+    const double er = max_err(rf, rfab);
+    std::printf("      remake, %lld points: max error %.3e\n",
+                (long long)rfab.numPts() * 2, er);
+    expect(er < 1e-12, "remade-level fill with boundaries reproduces the data everywhere");
   }
 
   (void)hipStreamDestroy(s);
