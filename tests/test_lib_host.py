@@ -530,3 +530,52 @@ def test_wave_library_exports_its_header():
     assert adj(0, fake, fake, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
     assert adj(2, fake, None, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
     assert adj(4, fake, None, fake, 8, 8, 8, 1, 1, 1.0, 1.0, None) == -3
+
+
+def test_level_append_edge_cases():
+    """An empty fine level and fine FABs without ghost points append no
+    patches; anisotropic ghost widths fill exactly the points a
+    point-by-point count gives; a new level of an empty level is empty."""
+    import numpy as np
+    L = _lib.lib()
+    n = 24
+    cdom = ((0, 0, 0), (n, n, n))
+    fdom = ((0, 0, 0), (2 * n, 2 * n, 2 * n))
+    h = _lookup("natural", 3, (0, 0, 0))[1]
+    crse, kc = _level([(cdom[0], cdom[1])], cdom, 2, 6)
+    v = [ctypes.c_int64() for _ in range(4)]
+    wl = ctypes.c_void_p()
+    assert L.cxp_worklist_create(ctypes.byref(wl)) == 0
+    empty = _lib.CxpLevel()
+    empty.nfabs = 0
+    empty.domain_lo, empty.domain_hi = i3(fdom[0]), i3(fdom[1])
+    assert L.cxp_worklist_append_level(wl, h, ctypes.byref(crse), 0, ctypes.byref(empty), 0, 2) == 0
+    assert L.cxp_worklist_append_new_level(wl, h, ctypes.byref(crse), 0, ctypes.byref(empty), 0, 2,
+                                           None) == 0
+    nog, k0 = _level([((4, 4, 4), (19, 19, 19))], fdom, 2, 0)
+    assert L.cxp_worklist_append_level(wl, h, ctypes.byref(crse), 0, ctypes.byref(nog), 0, 2) == 0
+    assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+    assert v[0].value == 0 and v[2].value == 0
+    # ghost widths (1, 2, 3) in x, y, z around two boxes, one at the domain corner
+    boxes = [((0, 0, 0), (9, 11, 13)), ((20, 4, 6), (29, 15, 17))]
+    g = (1, 2, 3)
+    aniso, ka = _level(boxes, fdom, 2, 0)
+    for f, (lo, hi) in zip(ka[0], boxes):
+        f.lo = i3(tuple(lo[d] - g[d] for d in range(3)))
+        f.hi = i3(tuple(hi[d] + g[d] for d in range(3)))
+        f.comp_stride = int(np.prod([hi[d] - lo[d] + 1 + 2 * g[d] for d in range(3)]))
+    assert L.cxp_worklist_append_level(wl, h, ctypes.byref(crse), 0, ctypes.byref(aniso), 0, 2) == 0
+    assert L.cxp_worklist_stats(wl, *[ctypes.byref(x) for x in v]) == 0
+    want = 0
+    for lo, hi in boxes:
+        blo = [max(lo[d] - g[d], fdom[0][d] - g[d]) for d in range(3)]
+        bhi = [min(hi[d] + g[d], fdom[1][d] + g[d]) for d in range(3)]
+
+        def keep(x, y, z):
+            m = x == x
+            for vlo, vhi in boxes:
+                m = m & ~_in(x, y, z, vlo, vhi)
+            return m
+        want += _count(keep, blo, bhi)
+    assert v[2].value == 2 * want
+    assert L.cxp_worklist_destroy(wl) == 0
