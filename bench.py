@@ -163,6 +163,52 @@ def bench_level64(args, stream, log):
     return out
 
 
+def cpu_level_fill(d, log, whole=False):
+    """The oracle's OpenMP build on a level fill's work: every fine FAB's
+    ghost shell (its six slabs around its valid box, points in a
+    neighbour's valid box included), or with whole every new FAB's box
+    within the domain, prolongated from host copies of the same coarse
+    data; median of a few runs."""
+    import numpy as np
+    import oracle
+    from carpetx_amd.workloads import shell_slabs
+    crse, ncomp = d["crse"], d["ncomp"]
+    fine = d["new"] if whole else d["fine"]
+    glo = [min(f.box.lo[k] for f in crse.fabs) for k in range(3)]
+    ghi = [max(f.box.hi[k] for f in crse.fabs) for k in range(3)]
+    g = np.zeros((ncomp,) + tuple(ghi[k] - glo[k] + 1 for k in (2, 1, 0)))
+    for f in crse.fabs:
+        sl = tuple(slice(f.box.lo[k] - glo[k], f.box.hi[k] - glo[k] + 1) for k in (2, 1, 0))
+        g[(slice(None),) + sl] = f.t[:ncomp].cpu().numpy()
+    op_d = d["op"]
+    op = oracle.lookup(op_d.prolongation_type, op_d.prolongation_order, op_d.indextype,
+                       kind="fast")
+    ch = oracle.Fab(glo, ghi, ncomp, g)
+    items, npts = [], 0
+    dom = fine.domain
+    for f, v in zip(fine.fabs, fine.valid):
+        fh = oracle.Fab(f.box.lo, f.box.hi, ncomp)
+        if whole:
+            lo = [max(f.box.lo[k], dom.lo[k]) for k in range(3)]
+            hi = [min(f.box.hi[k], dom.hi[k]) for k in range(3)]
+            items.append((op, ch, fh, lo, hi, ncomp))
+            npts += int(np.prod([hi[k] - lo[k] + 1 for k in range(3)])) * ncomp
+            continue
+        for sb in shell_slabs(f.box, v):
+            items.append((op, ch, fh, sb.lo, sb.hi, ncomp))
+            npts += sb.numPts * ncomp
+    oracle.prolongate_many(items, kind="fast")  # warm-up
+    reps = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        oracle.prolongate_many(items, kind="fast")
+        reps.append(time.perf_counter() - t0)
+    med = sorted(reps)[len(reps) // 2]
+    threads = oracle.num_threads("fast")
+    return dict(cpu_ms=round(1e3 * med, 3), cpu_pts=npts, cpu_pts_per_s=npts / med,
+                cpu_threads=threads)
+
+
 def bench_level_fill(args, stream, log):
     """Ghost fill of a whole fine level (64 boxes of 32^3, 3 ghosts) from a
     coarse level of 2^3 FABs: the fine ghost points outside every fine valid
@@ -204,6 +250,8 @@ def bench_level_fill(args, stream, log):
             store_floor_ms=round(mf, 4),
             batched_pts_per_s=st["fine_points"] / (mb * 1e-3),
             batched_hbm_frac=round(st["algo_bytes"] / (mb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        if bc is None and not args.no_cpu:
+            out[name].update(cpu_level_fill(d, log))
         log("level fill %s: %s" % (name, out[name]))
         g.close()
         wl.close()
@@ -251,6 +299,8 @@ def bench_regrid(args, stream, log):
                 batched_ms=round(mb, 4), per_patch_ms=round(mp, 4),
                 batched_pts_per_s=st["fine_points"] / (mb * 1e-3),
                 batched_hbm_frac=round(st["algo_bytes"] / (mb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+            if kind == "new_level" and not args.no_cpu:
+                out[key].update(cpu_level_fill(d, log, whole=True))
             log("regrid %s: %s" % (key, out[key]))
             g.close()
             wl.close()
