@@ -495,10 +495,14 @@ struct cxp_worklist_s {
   size_t cap_gather_runs = 0, cap_gather_chunks = 0;
   // boundary values of gathered coarse patches, after the gather
   std::vector<BcRun> bc_runs;  // off: offsets into d_gather
-  std::vector<BcChunk> bc_chunks;
+  std::vector<BcChunk> bc_chunks;   // sources in the patch: after the gather
+  std::vector<BcChunk> bcd_chunks;  // sources in one coarse FAB: with it
+  std::vector<BcDirect> bc_direct;
   BcRun *d_bc_runs = nullptr;
-  BcChunk *d_bc_chunks = nullptr;
-  size_t cap_bc_runs = 0, cap_bc_chunks = 0;
+  BcChunk *d_bc_chunks = nullptr, *d_bcd_chunks = nullptr;
+  BcDirect *d_bc_direct = nullptr;
+  size_t cap_bc_runs = 0, cap_bc_chunks = 0, cap_bcd_chunks = 0,
+         cap_bc_direct = 0;
   // boundary conditions of levels, in place, after the fills
   std::vector<BcRun> post_runs;
   std::vector<BcChunk> post_chunks;
@@ -877,13 +881,15 @@ int place_gathered(cxp_worklist_s *wl) {
 // Fills the gathered coarse patches: copies from the coarse FABs, then the
 // boundary values that are computed from those copies.
 int launch_gather(cxp_worklist_s *wl, hipStream_t s) {
-  if (wl->gather_chunks.empty())
-    return CXP_OK;
-  hipLaunchKernelGGL(k_gather, dim3((unsigned)wl->gather_chunks.size()),
-                     dim3(256), 0, s, wl->d_gather_runs, wl->d_gather_chunks,
-                     wl->d_gather);
-  if (const int e = hip_status(hipGetLastError()))
-    return e;
+  const size_t nblocks = wl->gather_chunks.size() + wl->bcd_chunks.size();
+  if (nblocks > 0) {
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)nblocks), dim3(256), 0, s,
+                       wl->d_gather_runs, wl->d_gather_chunks,
+                       (int)wl->gather_chunks.size(), wl->d_bc_runs,
+                       wl->d_bcd_chunks, wl->d_bc_direct, wl->d_gather);
+    if (const int e = hip_status(hipGetLastError()))
+      return e;
+  }
   if (wl->bc_chunks.empty())
     return CXP_OK;
   hipLaunchKernelGGL(k_bc, dim3((unsigned)wl->bc_chunks.size()), dim3(256), 0,
@@ -912,6 +918,10 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
   if (int e = upload_vec(wl->bc_runs, wl->d_bc_runs, wl->cap_bc_runs, s))
     return e;
   if (int e = upload_vec(wl->bc_chunks, wl->d_bc_chunks, wl->cap_bc_chunks, s))
+    return e;
+  if (int e = upload_vec(wl->bcd_chunks, wl->d_bcd_chunks, wl->cap_bcd_chunks, s))
+    return e;
+  if (int e = upload_vec(wl->bc_direct, wl->d_bc_direct, wl->cap_bc_direct, s))
     return e;
   if (int e = upload_vec(wl->post_runs, wl->d_post_runs, wl->cap_post_runs, s))
     return e;
@@ -1305,6 +1315,10 @@ int cxp_worklist_destroy(cxp_worklist wl) {
     (void)hipFree(wl->d_bc_runs);
   if (wl->d_bc_chunks)
     (void)hipFree(wl->d_bc_chunks);
+  if (wl->d_bcd_chunks)
+    (void)hipFree(wl->d_bcd_chunks);
+  if (wl->d_bc_direct)
+    (void)hipFree(wl->d_bc_direct);
   if (wl->d_post_runs)
     (void)hipFree(wl->d_post_runs);
   if (wl->d_post_chunks)
@@ -1325,6 +1339,8 @@ int cxp_worklist_clear(cxp_worklist wl) {
   wl->gather_chunks.clear();
   wl->bc_runs.clear();
   wl->bc_chunks.clear();
+  wl->bcd_chunks.clear();
+  wl->bc_direct.clear();
   wl->post_runs.clear();
   wl->post_chunks.clear();
   wl->gather_doubles = 0;
@@ -1648,14 +1664,34 @@ bool bc_chunks_ok(const std::vector<BcChunk> &chunks, int run0,
   return true;
 }
 
+// For a gathered coarse patch: the coarse level, whose valid data holds
+// the sources of the patch's boundary points, the periodic directions and
+// periods, and where chunks that read it directly go.
+struct DirectSrc {
+  const cxp_level *crse;
+  int crse_comp;
+  const bool *periodic;
+  const int *period;
+  std::vector<BcChunk> *chunks;
+  std::vector<BcDirect> *dirs;
+  int dir0;
+};
+
 // Boundary runs of box E (its ncomp components at p + off + c comp_stride;
 // p null: the patch buffer) over a domain: one run per component, chunks
-// of up to 1024 points over the parts of E outside the domain.  runs[k]
-// becomes run run0 + k.
+// of kBcChunkPts points over the parts of E outside the domain.  runs[k]
+// becomes run run0 + k.  With direct, each part is split where each
+// direction is below, inside or above the domain; a part whose sources
+// (edge points, mirror images, periodic images, and for linear
+// extrapolation the points beside the edge) one coarse FAB's valid region
+// holds reads them there, in the gather launch; the others read the patch
+// after the gather.  Parts outside only in periodic directions are the
+// gather's images and get no chunks.
 void add_bc_runs(const cxp_bc &bc, const int it[3], const IBox &E,
                  const IBox &dom, double *p, int64_t off, int64_t comp_stride,
                  int ncomp, int run0, std::vector<BcRun> &runs,
-                 std::vector<BcChunk> &chunks) {
+                 std::vector<BcChunk> &chunks,
+                 const DirectSrc *direct = nullptr) {
   std::vector<IBox> outs;
   subtract(E, dom, outs);
   for (int c = 0; c < ncomp; ++c) {
@@ -1679,7 +1715,7 @@ void add_bc_runs(const cxp_bc &bc, const int it[3], const IBox &E,
     r.robin = bc.robin_values ? bc.robin_values[c] : 0.0;
     const int run = run0 + (int)runs.size();
     runs.push_back(r);
-    for (const IBox &P : outs) {
+    auto emit = [&](std::vector<BcChunk> &into, const IBox &P, int dir) {
       const int nx = P.hi[0] - P.lo[0] + 1, ny = P.hi[1] - P.lo[1] + 1;
       const int vol = (int)P.volume();
       for (int q = 0; q < vol; q += kBcChunkPts) {
@@ -1691,7 +1727,109 @@ void add_bc_runs(const cxp_bc &bc, const int it[3], const IBox &E,
         ch.bny = ny;
         ch.first = q;
         ch.count = std::min(kBcChunkPts, vol - q);
-        chunks.push_back(ch);
+        ch.direct = dir;
+        into.push_back(ch);
+      }
+    };
+    for (const IBox &P : outs) {
+      if (!direct) {
+        emit(chunks, P, -1);
+        continue;
+      }
+      for (int cl = 0; cl < 27; ++cl) {
+        const int cls[3] = {cl % 3, (cl / 3) % 3, cl / 9};  // below, in, above
+        IBox S;
+        for (int d = 0; d < 3; ++d) {
+          S.lo[d] = cls[d] == 0 ? P.lo[d]
+                  : cls[d] == 1 ? std::max(P.lo[d], dom.lo[d])
+                                : std::max(P.lo[d], dom.hi[d] + 1);
+          S.hi[d] = cls[d] == 0 ? std::min(P.hi[d], dom.lo[d] - 1)
+                  : cls[d] == 1 ? std::min(P.hi[d], dom.hi[d])
+                                : P.hi[d];
+        }
+        if (S.empty())
+          continue;
+        // how each direction maps S's points to their sources: identity
+        // (inside, or outside a periodic face: then shifted to the image),
+        // the edge (and for linear extrapolation the point beside it), or
+        // the mirror o - dst; a Dirichlet direction needs no source at all
+        enum { kIdentity, kEdge, kMirror };
+        int kind[3] = {kIdentity, kIdentity, kIdentity};
+        int shift[3] = {0, 0, 0}, e0[3] = {0, 0, 0}, e1[3] = {0, 0, 0},
+            mo[3] = {0, 0, 0};
+        bool cond = false, dirichlet = false;
+        for (int d = 0; d < 3; ++d) {
+          if (cls[d] == 1)
+            continue;
+          const int face = cls[d] == 2 ? 1 : 0;
+          const int t = bc.type[face][d];
+          const int edge = face ? dom.hi[d] : dom.lo[d];
+          if (t == CXP_BC_PERIODIC) {
+            shift[d] = face ? -direct->period[d] : direct->period[d];
+            continue;
+          }
+          if (t == CXP_BC_NONE)
+            continue;
+          cond = true;
+          if (t == CXP_BC_DIRICHLET) {
+            dirichlet = true;
+          } else if (t == CXP_BC_REFLECTION) {
+            kind[d] = kMirror;
+            mo[d] = refl_offset(dom, face, d, it[d]);
+          } else {
+            kind[d] = kEdge;
+            e0[d] = edge;
+            e1[d] = t == CXP_BC_LINEAR_EXTRAPOLATION
+                        ? (face ? edge - 1 : edge + 1)
+                        : edge;
+          }
+        }
+        if (!cond)
+          continue;  // outside only in periodic directions
+        // split S by the coarse FAB whose valid region holds the sources:
+        // the preimage of each valid box under the map
+        std::vector<IBox> left{S};
+        for (int k = 0; k < direct->crse->nfabs && !left.empty(); ++k) {
+          const IBox v = ibox(direct->crse->valid + 6 * k,
+                              direct->crse->valid + 6 * k + 3);
+          IBox Q = S;
+          for (int d = 0; d < 3 && !dirichlet; ++d) {
+            if (kind[d] == kIdentity) {
+              Q.lo[d] = std::max(S.lo[d], v.lo[d] - shift[d]);
+              Q.hi[d] = std::min(S.hi[d], v.hi[d] - shift[d]);
+            } else if (kind[d] == kEdge) {
+              if (e0[d] < v.lo[d] || e0[d] > v.hi[d] || e1[d] < v.lo[d] ||
+                  e1[d] > v.hi[d])
+                Q.hi[d] = Q.lo[d] - 1;
+            } else {
+              Q.lo[d] = std::max(S.lo[d], mo[d] - v.hi[d]);
+              Q.hi[d] = std::min(S.hi[d], mo[d] - v.lo[d]);
+            }
+          }
+          std::vector<IBox> rest;
+          for (const IBox &L : left) {
+            const IBox X = intersect(L, Q);
+            if (X.empty()) {
+              rest.push_back(L);
+              continue;
+            }
+            const DFab f = dfab(&direct->crse->fabs[k], direct->crse_comp + c);
+            BcDirect dd;
+            dd.src = f.p;
+            for (int d = 0; d < 3; ++d) {
+              dd.lo[d] = f.lo[d];
+              dd.shift[d] = shift[d];
+            }
+            dd.sy = f.sy;
+            dd.sz = f.sz;
+            direct->dirs->push_back(dd);
+            emit(*direct->chunks, X, direct->dir0 + (int)direct->dirs->size() - 1);
+            subtract(L, X, rest);
+          }
+          left.swap(rest);
+        }
+        for (const IBox &L : left)
+          emit(chunks, L, -1);
       }
     }
   }
@@ -1755,7 +1893,8 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
   std::vector<GatherRun> runs;
   std::vector<GatherChunk> chunks;
   std::vector<BcRun> bruns;
-  std::vector<BcChunk> bchunks;
+  std::vector<BcChunk> bchunks, bdchunks;
+  std::vector<BcDirect> bdirs;
   int64_t gather = wl->gather_doubles;
   int64_t fine_pts = 0, coarse_pts = 0;
   const int run0 = (int)wl->gather_runs.size();
@@ -1794,8 +1933,10 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
   };
   // boundary values of the coarse patch E at offset off, every component
   auto add_bc = [&](const IBox &E, int64_t off) {
+    const DirectSrc direct{crse, crse_comp, periodic, period, &bdchunks, &bdirs,
+                           (int)wl->bc_direct.size()};
     add_bc_runs(*bc, op->d.indextype, E, cdom, nullptr, off, E.volume(), ncomp,
-                brun0, bruns, bchunks);
+                brun0, bruns, bchunks, &direct);
   };
   // the fine level's periodic shifts (fine index units), the unshifted one
   // first: a valid box also covers its periodic images (the reference's
@@ -1872,11 +2013,14 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
   auto gather_box = [&](const IBox &C, IBox &E) -> int {
     const size_t n_runs = runs.size(), n_chunks = chunks.size();
     const size_t n_bruns = bruns.size(), n_bchunks = bchunks.size();
+    const size_t n_bdchunks = bdchunks.size(), n_bdirs = bdirs.size();
     auto fail = [&](int e) {
       runs.resize(n_runs);
       chunks.resize(n_chunks);
       bruns.resize(n_bruns);
       bchunks.resize(n_bchunks);
+      bdchunks.resize(n_bdchunks);
+      bdirs.resize(n_bdirs);
       return e;
     };
     const bool use_bc = bc && !inside(C, cdom);
@@ -2050,13 +2194,23 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
     coarse_pts += union_points(cboxes);
   }
   if (!gather_chunks_ok(chunks, run0, runs.size()) ||
-      !bc_chunks_ok(bchunks, brun0, bruns))
+      !bc_chunks_ok(bchunks, brun0, bruns) ||
+      !bc_chunks_ok(bdchunks, brun0, bruns))
     return CXP_ERR_STATE;
+  for (const BcChunk &c : bdchunks)
+    if (c.direct < (int)wl->bc_direct.size() ||
+        c.direct >= (int)(wl->bc_direct.size() + bdirs.size()))
+      return CXP_ERR_STATE;
+  for (const BcChunk &c : bchunks)
+    if (c.direct != -1)
+      return CXP_ERR_STATE;
   wl->patches.insert(wl->patches.end(), patches.begin(), patches.end());
   wl->gather_runs.insert(wl->gather_runs.end(), runs.begin(), runs.end());
   wl->gather_chunks.insert(wl->gather_chunks.end(), chunks.begin(), chunks.end());
   wl->bc_runs.insert(wl->bc_runs.end(), bruns.begin(), bruns.end());
   wl->bc_chunks.insert(wl->bc_chunks.end(), bchunks.begin(), bchunks.end());
+  wl->bcd_chunks.insert(wl->bcd_chunks.end(), bdchunks.begin(), bdchunks.end());
+  wl->bc_direct.insert(wl->bc_direct.end(), bdirs.begin(), bdirs.end());
   wl->gather_doubles = gather;
   wl->uploaded = false;
   wl->fine_points += fine_pts * ncomp;
@@ -2220,7 +2374,8 @@ int cxp_prolongate_batched(cxp_worklist wl, void *stream) {
 int cxp_prolongate_per_patch(cxp_worklist wl, void *stream) {
   if (!wl)
     return CXP_ERR_ARG;
-  if (!wl->gather_runs.empty() || !wl->post_runs.empty()) {
+  if (!wl->gather_runs.empty() || !wl->post_runs.empty() ||
+      !wl->bc_runs.empty()) {
     // gathers, copies and boundary runs live on the device only after an
     // upload
     if (!wl->uploaded)

@@ -434,10 +434,9 @@ struct GatherChunk {
   int row0, nrows;
 };
 
-__global__ void __launch_bounds__(256)
-    k_gather(const GatherRun *__restrict__ runs,
-             const GatherChunk *__restrict__ chunks, double *__restrict__ buf) {
-  const GatherChunk ch = chunks[blockIdx.x];
+__device__ __forceinline__ void gather_chunk(const GatherRun *__restrict__ runs,
+                                             const GatherChunk &ch,
+                                             double *__restrict__ buf) {
   const GatherRun r = runs[ch.run];
   double *const d = r.dstp ? r.dstp : buf;
   const int npts = ch.nrows * r.nx;
@@ -475,104 +474,160 @@ struct BcChunk {
   int blo[3];    // the box's first point
   int bnx, bny;  // its x and y extents
   int first, count;
+  int direct;    // >= 0: the sources' coarse FAB (a BcDirect); -1: the box
+};
+// The coarse FAB that holds every source of a chunk's points in its valid
+// region: the run's component there, and the periodic shift that takes a
+// source point (domain coordinates as the condition gives them) to the
+// point that holds its value.
+struct BcDirect {
+  const double *src;  // component at its FAB's lo
+  int lo[3];
+  int sy;
+  long long sz;
+  int shift[3];
 };
 
-__global__ void __launch_bounds__(256)
-    k_bc(const BcRun *__restrict__ runs, const BcChunk *__restrict__ chunks,
-         double *__restrict__ buf) {
-  const BcChunk ch = chunks[blockIdx.x];
-  const BcRun r = runs[ch.run];
+// The boundary value of point dst of run r, or false if dst is not a
+// boundary point of a face with a condition.  fetch(p) gives the value at
+// source point p.
+template <class Fetch>
+__device__ __forceinline__ bool bc_value(const BcRun &r, const int dst[3],
+                                         Fetch fetch, double &val) {
+  int src[3] = {dst[0], dst[1], dst[2]}, delta[3] = {0, 0, 0};
+  bool outside = false, dirichlet = false, lin = false, refl = false;
+  int robin = 0;  // bit d: Robin in direction d
+  double parity = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    if (dst[d] >= r.imin[d] && dst[d] < r.imax[d])
+      continue;  // not a boundary direction of this point
+    const int face = dst[d] >= r.imax[d] ? 1 : 0;
+    if (r.type[face][d] == CXP_BC_NONE || r.type[face][d] == CXP_BC_PERIODIC)
+      continue;  // nor is a face without a condition, or a periodic one
+    outside = true;
+    const int edge = face ? r.imax[d] - 1 : r.imin[d];
+    switch (r.type[face][d]) {
+    case CXP_BC_DIRICHLET:
+      dirichlet = true;
+      break;
+    case CXP_BC_LINEAR_EXTRAPOLATION:
+      src[d] = edge;
+      delta[d] = face ? -1 : 1;  // -inormal
+      lin = true;
+      break;
+    case CXP_BC_NEUMANN:
+      src[d] = edge;
+      break;
+    case CXP_BC_ROBIN:
+      src[d] = edge;
+      robin |= 1 << d;
+      break;
+    case CXP_BC_REFLECTION:
+      src[d] = r.refl[face][d] - dst[d];
+      parity *= (double)r.parity[d];
+      refl = true;
+      break;
+    default:
+      break;
+    }
+  }
+  if (!outside)
+    return false;
+  if (dirichlet) {
+    val = r.dirichlet;
+    return true;
+  }
+  val = fetch(src);
+  for (int d = 0; d < 3; ++d)
+    if (robin & (1 << d)) {
+      double xb[3], xi[3];
+      for (int e = 0; e < 3; ++e) {
+        xb[e] = r.xmin[e] + (double)dst[e] * r.dx[e];
+        xi[e] = r.xmin[e] + (double)src[e] * r.dx[e];
+      }
+      const double rb = sqrt(xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2]);
+      const double ri = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+      const double qr = ri / rb;
+      val = r.robin + qr * (val - r.robin);
+    }
+  if (lin) {
+    const int sp[3] = {src[0] + delta[0], src[1] + delta[1],
+                       src[2] + delta[2]};
+    const double grad = val - fetch(sp);
+    int num = 0, den = 0;
+    for (int e = 0; e < 3; ++e) {
+      num += (dst[e] - src[e]) * (dst[e] - src[e]);
+      den += delta[e] * delta[e];
+    }
+    // integer quotient, as the reference divides two int sums
+    val += sqrt((double)(num / den)) * grad;
+  }
+  if (refl)
+    val *= parity;
+  return true;
+}
+
+// One chunk of boundary points: sources from the box itself (k_bc, after
+// the gather) or from one coarse FAB (inside the gather launch).
+__device__ __forceinline__ void bc_chunk(const BcRun &r, const BcChunk &ch,
+                                         const BcDirect *dir, double *buf) {
   const int nx = r.hi[0] - r.lo[0] + 1, ny = r.hi[1] - r.lo[1] + 1;
   double *v = (r.p ? r.p : buf) + r.off;
   auto at = [&](const int p[3]) {
     return (long long)(p[0] - r.lo[0]) + (long long)(p[1] - r.lo[1]) * nx +
            (long long)(p[2] - r.lo[2]) * nx * ny;
   };
-  // the host sizes the patch to hold every source; a source outside it
-  // would be a host error, and gives NaN instead of a stray read
+  // the host sizes the box to hold every source; a source outside it would
+  // be a host error, and gives NaN instead of a stray read
   auto in_box = [&](const int p[3]) {
     return p[0] >= r.lo[0] && p[0] <= r.hi[0] && p[1] >= r.lo[1] &&
            p[1] <= r.hi[1] && p[2] >= r.lo[2] && p[2] <= r.hi[2];
+  };
+  auto from_box = [&](const int p[3]) {
+    return in_box(p) ? v[at(p)] : __builtin_nan("");
+  };
+  auto from_fab = [&](const int p[3]) {
+    return gld(dir->src, (long long)(p[0] + dir->shift[0] - dir->lo[0]) +
+                             (long long)(p[1] + dir->shift[1] - dir->lo[1]) *
+                                 dir->sy +
+                             (long long)(p[2] + dir->shift[2] - dir->lo[2]) *
+                                 dir->sz);
   };
   for (int q = ch.first + (int)threadIdx.x; q < ch.first + ch.count;
        q += 256) {
     const int dst[3] = {ch.blo[0] + q % ch.bnx, ch.blo[1] + (q / ch.bnx) % ch.bny,
                         ch.blo[2] + q / (ch.bnx * ch.bny)};
-    int src[3] = {dst[0], dst[1], dst[2]}, delta[3] = {0, 0, 0};
-    bool outside = false, dirichlet = false, lin = false, refl = false;
-    int robin = 0;  // bit d: Robin in direction d
-    double parity = 1.0;
-    for (int d = 0; d < 3; ++d) {
-      if (dst[d] >= r.imin[d] && dst[d] < r.imax[d])
-        continue;  // not a boundary direction of this point
-      const int face = dst[d] >= r.imax[d] ? 1 : 0;
-      if (r.type[face][d] == CXP_BC_NONE || r.type[face][d] == CXP_BC_PERIODIC)
-        continue;  // nor is a face without a condition, or a periodic one
-      outside = true;
-      const int edge = face ? r.imax[d] - 1 : r.imin[d];
-      switch (r.type[face][d]) {
-      case CXP_BC_DIRICHLET:
-        dirichlet = true;
-        break;
-      case CXP_BC_LINEAR_EXTRAPOLATION:
-        src[d] = edge;
-        delta[d] = face ? -1 : 1;  // -inormal
-        lin = true;
-        break;
-      case CXP_BC_NEUMANN:
-        src[d] = edge;
-        break;
-      case CXP_BC_ROBIN:
-        src[d] = edge;
-        robin |= 1 << d;
-        break;
-      case CXP_BC_REFLECTION:
-        src[d] = r.refl[face][d] - dst[d];
-        parity *= (double)r.parity[d];
-        refl = true;
-        break;
-      default:
-        break;
-      }
-    }
-    if (!outside)
-      continue;
-    const int sp[3] = {src[0] + delta[0], src[1] + delta[1],
-                       src[2] + delta[2]};
     double val;
-    if (dirichlet) {
-      val = r.dirichlet;
-    } else if (!in_box(src) || !in_box(sp)) {
-      val = __builtin_nan("");
-    } else {
-      val = v[at(src)];
-      for (int d = 0; d < 3; ++d)
-        if (robin & (1 << d)) {
-          double xb[3], xi[3];
-          for (int e = 0; e < 3; ++e) {
-            xb[e] = r.xmin[e] + (double)dst[e] * r.dx[e];
-            xi[e] = r.xmin[e] + (double)src[e] * r.dx[e];
-          }
-          const double rb = sqrt(xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2]);
-          const double ri = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
-          const double qr = ri / rb;
-          val = r.robin + qr * (val - r.robin);
-        }
-      if (lin) {
-        const double grad = val - v[at(sp)];
-        int num = 0, den = 0;
-        for (int e = 0; e < 3; ++e) {
-          num += (dst[e] - src[e]) * (dst[e] - src[e]);
-          den += delta[e] * delta[e];
-        }
-        // integer quotient, as the reference divides two int sums
-        val += sqrt((double)(num / den)) * grad;
-      }
-      if (refl)
-        val *= parity;
-    }
-    v[at(dst)] = val;
+    const bool b = dir ? bc_value(r, dst, from_fab, val)
+                       : bc_value(r, dst, from_box, val);
+    if (b)
+      v[at(dst)] = val;
   }
+}
+
+__global__ void __launch_bounds__(256)
+    k_bc(const BcRun *__restrict__ runs, const BcChunk *__restrict__ chunks,
+         double *__restrict__ buf) {
+  const BcChunk ch = chunks[blockIdx.x];
+  const BcRun r = runs[ch.run];
+  bc_chunk(r, ch, nullptr, buf);
+}
+
+// The gather launch: blocks below ngather copy (coarse patches, old-level
+// data), the others compute boundary values whose sources one coarse FAB
+// holds; neither reads what the other writes.
+__global__ void __launch_bounds__(256)
+    k_gather(const GatherRun *__restrict__ runs,
+             const GatherChunk *__restrict__ chunks, int ngather,
+             const BcRun *__restrict__ bruns, const BcChunk *__restrict__ bchunks,
+             const BcDirect *__restrict__ bdir, double *__restrict__ buf) {
+  if ((int)blockIdx.x < ngather) {
+    gather_chunk(runs, chunks[blockIdx.x], buf);
+    return;
+  }
+  const BcChunk ch = bchunks[blockIdx.x - ngather];
+  const BcRun r = bruns[ch.run];
+  bc_chunk(r, ch, bdir + ch.direct, buf);
 }
 
 // ---------------------------------------------------------------------------
