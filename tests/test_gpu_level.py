@@ -44,16 +44,16 @@ def domains(it):
     return cdom, fdom
 
 
-def coarse_level(it, g, cg=CG, poison_outside=False):
-    """8 coarse FABs (valid split at 12) from the global array g, whose
+def coarse_level(it, g, cg=CG, poison_outside=False, split=(12, 12, 12)):
+    """8 coarse FABs (valid split at `split`) from the global array g, whose
     box is the coarse domain grown by CG; with poison_outside, the points
     outside the domain hold 1e300 as well."""
     cdom, _ = domains(it)
     glo = tuple(cdom.lo[d] - CG for d in range(3))
     fabs, valid = [], []
     for c in range(8):
-        lo = tuple(0 if not (c >> d) & 1 else 12 for d in range(3))
-        hi = tuple(11 if not (c >> d) & 1 else cdom.hi[d] for d in range(3))
+        lo = tuple(0 if not (c >> d) & 1 else split[d] for d in range(3))
+        hi = tuple(split[d] - 1 if not (c >> d) & 1 else cdom.hi[d] for d in range(3))
         v = Box(lo, hi)
         b = v.grow(cg)
         sl = tuple(slice(b.lo[d] - glo[d], b.hi[d] - glo[d] + 1) for d in (2, 1, 0))
@@ -610,4 +610,28 @@ def test_level_fill_per_group_matches_oracle(op_args):
     _run_all_ways(wl, fine.fabs, originals, want_pg, ("per group", op_args))
     wl.set_per_group(False)
     _run_all_ways(wl, fine.fabs, originals, want_pv, ("per var", op_args))
+    wl.close()
+
+
+def test_boundary_sources_in_two_coarse_fabs():
+    """Coarse FABs split at x = 1: linear extrapolation at the lower x face
+    reads the edge (x = 0) and the point beside it (x = 1) from two FABs,
+    so those boundary values are computed from the gathered patch after the
+    gather; the others read the coarse FABs directly in the gather launch.
+    Both equal the oracle."""
+    op_args = ("natural", 3, (1, 1, 1))
+    faces = [["linear_extrapolation", "reflection", "neumann"],
+             ["dirichlet", "linear_extrapolation", "reflection"]]
+    it = op_args[2]
+    rng = np.random.default_rng(57)
+    cdom, _ = domains(it)
+    gshape = (NC_CRSE,) + tuple(cdom.size[d] + 2 * CG for d in (2, 1, 0))
+    g = rng.uniform(-1, 1, gshape)
+    crse = coarse_level(it, g, poison_outside=True, split=(1, 12, 12))
+    fine = fine_level(it, rng)
+    originals = [f.t.clone() for f in fine.fabs]
+    want = expected(op_args, it, _coarse_global(it, g, faces), fine)
+    wl = WorkList()
+    wl.append_level(get_interpolator(*op_args), crse, 1, fine, 0, NCOMP, bc=_bc(faces))
+    _run_all_ways(wl, fine.fabs, originals, want, ("two-FAB sources", op_args))
     wl.close()
