@@ -1,4 +1,5 @@
-"""ctypes binding of libcxp.so (the C ABI declared in include/cxp.h).
+"""ctypes binding of libcxp.so (the C ABI declared in include/cxp.h and
+include/cxp_tag.h).
 
 There is no fallback: importing the binding without the built library, or
 calling a device entry point without a GPU, raises.
@@ -10,6 +11,8 @@ import re
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcxp.so")
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "cxp.h")
+# every header whose functions libcxp.so exports
+HEADERS = [HEADER, os.path.join(os.path.dirname(_HERE), "include", "cxp_tag.h")]
 
 c_int = ctypes.c_int
 c_int64 = ctypes.c_int64
@@ -42,6 +45,11 @@ class CxpBc(ctypes.Structure):
     _fields_ = [("type", (c_int * 3) * 2), ("dirichlet_values", P_DOUBLE),
                 ("robin_values", P_DOUBLE), ("parities", P_INT),
                 ("xmin", ctypes.c_double * 3), ("dx", ctypes.c_double * 3)]
+
+
+class CxpBibRegion(ctypes.Structure):
+    _fields_ = [("active", c_int), ("shape", c_int), ("num_levels", c_int),
+                ("position", ctypes.c_double * 3), ("radius", ctypes.c_double * 3)]
 
 
 class CxpError(RuntimeError):
@@ -123,14 +131,21 @@ _SIGS = {
     "cxp_tune_load": (c_int, [c_char_p]),
     "cxp_tile_config_count": (c_int, []),
     "cxp_tile_config": (c_int, [c_int, P_INT, P_INT, P_INT]),
+    # include/cxp_tag.h
+    "cxp_boxinbox_tag": (c_int, [ctypes.POINTER(CxpBibRegion), c_int, c_int, P_DOUBLE,
+                                 P_DOUBLE, c_void_p, P_INT, P_INT, P_INT, P_INT, c_void_p]),
+    "cxp_error_est": (c_int, [c_void_p, P_INT, P_INT, P_INT, P_INT, ctypes.c_double,
+                              c_void_p, c_void_p, c_void_p]),
 }
 
 
 def header_symbols():
-    """Function names declared in include/cxp.h."""
-    with open(HEADER) as f:
-        text = f.read()
-    return sorted(set(re.findall(r"\b(cxp_[a-z0-9_]+)\s*\(", text)))
+    """Function names declared in include/cxp.h and include/cxp_tag.h."""
+    names = set()
+    for h in HEADERS:
+        with open(h) as f:
+            names |= set(re.findall(r"\b(cxp_[a-z0-9_]+)\s*\(", f.read()))
+    return sorted(names)
 
 
 def lib():

@@ -6,6 +6,8 @@ The scalar wave equation u_t = rho, rho_t = Laplace(u) is evolved on
 * a fine level of spacing h/2 covering the central half, n+1 points per
   direction plus 2 ghost points, whose ghost shell is prolongated from the
   coarse level (natural family, order 5, vertex-centred) at every RK4 stage.
+  Its place comes from box-in-box tags on the coarse level's cells (a cube
+  region of radius 1/4 about the centre; carpetx_amd.tagging).
 
 One coarse step is an RK4 step on the coarse level followed by two RK4 steps
 on the fine level (subcycling, dt_f = dt/2); the fine ghost values at a
@@ -37,6 +39,7 @@ from torch.utils.checkpoint import checkpoint
 
 from .autograd import fill_ghosts, region_view, restrict
 from .interp import Box, FArrayBox, get_interpolator
+from .tagging import BoxInBoxRegion, boxinbox_setup, error_est, refined_box
 
 GHOSTS = 2
 
@@ -141,9 +144,13 @@ def periodic_pad(u, g):
 
 class TwoLevelWave:
     def __init__(self, n=128, cfl=0.25, prolongation_order=5, device="cuda",
-                 native=False):
+                 native=False, regions=None):
         """native: RK4 stages through the native stage kernel and its
-        adjoint (same values as the torch arithmetic, bit for bit)."""
+        adjoint (same values as the torch arithmetic, bit for bit).
+
+        regions: the box-in-box regions (carpetx_amd.tagging.BoxInBoxRegion)
+        that place the fine level; the default, a cube of radius 1/4 about
+        the centre for level 1, places it over the central half."""
         assert n % 4 == 0
         self.native = native
         self.n = n
@@ -151,15 +158,38 @@ class TwoLevelWave:
         self.dt = cfl * self.h
         self.device = device
         self.P = get_interpolator("natural", prolongation_order, (0, 0, 0))
-        q = n // 4
-        # fine interior: coarse [q, 3q] -> fine [2q, 6q] (VC points)
-        self.fine_int = Box((2 * q,) * 3, (6 * q,) * 3)
+        if regions is None:
+            regions = [BoxInBoxRegion("cube", num_levels=2, radius=[0.5, 0.25])]
+        self.fine_int = self.fine_region(regions)
         self.fine_box = self.fine_int.grow(GHOSTS)
         shell_cb = self.P.CoarseBox(self.fine_box)
         # coarse sub-box read by the ghost fill (inside the periodic domain)
         self.crse_sub = shell_cb
-        assert all(0 <= shell_cb.lo[d] and shell_cb.hi[d] < n for d in range(3))
-        self.crse_under = Box((q,) * 3, (3 * q,) * 3)  # coarse points under fine
+        if not all(0 <= shell_cb.lo[d] and shell_cb.hi[d] < n for d in range(3)):
+            raise ValueError("the fine level's coarse box %r leaves the domain" % (shell_cb,))
+        # coarse points under the fine level
+        self.crse_under = Box([v // 2 for v in self.fine_int.lo],
+                              [v // 2 for v in self.fine_int.hi])
+
+    def fine_region(self, regions):
+        """Fine interior points (vertex-centred) of the level the regions
+        ask for: box-in-box tags on the coarse level's cells (BoxInBox_Setup,
+        then ErrorEst's threshold 1), their bounding box grown by one coarse
+        cell and refined, as AMReX clusters a box of tags.  Coarse vertex i
+        sits at i h - 1/2, so cell i's centre is -1/2 + h/2 + i h."""
+        n, h = self.n, self.h
+        err = FArrayBox(Box((0, 0, 0), (n - 1,) * 3), 1, device=self.device)
+        boxinbox_setup(regions, 1, (-0.5 + h / 2,) * 3, (h,) * 3, err)
+        count, tb = error_est(err, 1.0)
+        if count == 0:
+            raise ValueError("the regions tag no coarse cell")
+        if count != tb.numPts:
+            raise ValueError("the tags do not form a box (%d of %d cells)" % (count, tb.numPts))
+        cells = refined_box(tb)
+        fi = Box(cells.lo, [cells.hi[d] + 1 for d in range(3)])
+        if len(set(fi.size)) != 1 or len(set(fi.lo)) != 1:
+            raise ValueError("the harness's fine level must be a centred cube, got %r" % (fi,))
+        return fi
 
     # --- initial data -----------------------------------------------------
     def gaussian(self, width=0.05, amp=1.0):

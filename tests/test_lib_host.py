@@ -579,3 +579,74 @@ def test_level_append_edge_cases():
         want += _count(keep, blo, bhi)
     assert v[2].value == 2 * want
     assert L.cxp_worklist_destroy(wl) == 0
+
+
+def test_boxinbox_argument_errors_before_any_launch():
+    """cxp_boxinbox_tag / cxp_error_est reject bad arguments on the host,
+    with the reference's abort conditions as status codes."""
+    import ctypes
+    from carpetx_amd._lib import CxpBibRegion, P_DOUBLE
+    L = _lib.lib()
+    d3 = (ctypes.c_double * 3)
+    x0, dx = d3(0.5, 0.5, 0.5), d3(1.0, 1.0, 1.0)
+    flo, fhi = _lib.i3((0, 0, 0)), _lib.i3((7, 7, 7))
+
+    def reg(shape=1, active=1, num_levels=2, radius=(2.0, 2.0, 2.0)):
+        r = CxpBibRegion()
+        r.active, r.shape, r.num_levels = active, shape, num_levels
+        r.position = d3(4.0, 4.0, 4.0)
+        r.radius = d3(*radius)
+        return r
+
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: every call below fails first
+
+    def tag(regions, n, err=fake, lo=(0, 0, 0), hi=(7, 7, 7), level=1):
+        arr = (CxpBibRegion * max(1, len(regions)))(*regions)
+        return L.cxp_boxinbox_tag(arr, n, level, ctypes.cast(x0, P_DOUBLE),
+                                  ctypes.cast(dx, P_DOUBLE), err, flo, fhi,
+                                  _lib.i3(lo), _lib.i3(hi), None)
+
+    ARG, REGION = -3, -4
+    assert tag([reg()], 1, err=None) == ARG
+    assert tag([reg()] * 3, 4) == ARG
+    assert tag([reg()], -1) == ARG
+    assert tag([reg(shape=2)], 1) == ARG
+    assert tag([reg(shape=2, active=0)], 1) == ARG  # shapes are checked for every region
+    assert tag([reg(radius=(2.0, 0.0, 2.0))], 1) == ARG
+    assert tag([reg(radius=(2.0, -1.0, 2.0))], 1) == ARG
+    # valid regions, box outside the FAB or empty
+    assert tag([reg()], 1, lo=(-1, 0, 0)) == REGION
+    assert tag([reg()], 1, hi=(7, 8, 7)) == REGION
+    assert tag([reg()], 1, lo=(3, 0, 0), hi=(2, 7, 7)) == REGION
+    # a zero radius of a region that does not apply at this level is no
+    # error (the reference checks applying regions only): the call then
+    # fails on the box instead
+    assert tag([reg(radius=(0.0,) * 3, num_levels=1)], 1, lo=(-1, 0, 0)) == REGION
+    assert tag([reg(radius=(0.0,) * 3, active=0)], 1, lo=(-1, 0, 0)) == REGION
+
+    def est(tags=fake, box=fake, lo=(0, 0, 0), hi=(7, 7, 7), err=fake):
+        return L.cxp_error_est(err, flo, fhi, _lib.i3(lo), _lib.i3(hi), 0.5, tags, box, None)
+
+    assert est(tags=None, box=None) == ARG
+    assert est(err=None) == ARG
+    assert est(lo=(0, 0, -2)) == REGION
+    assert est(lo=(5, 5, 5), hi=(4, 7, 7)) == REGION
+
+
+def test_boxinbox_region_parameters_and_refined_box():
+    """Per-level radii as the parameters give them (a negative per-direction
+    entry defers to radius), and the fine box of a tag bounding box."""
+    from carpetx_amd.interp import Box
+    from carpetx_amd.tagging import BoxInBoxRegion, refined_box
+    r = BoxInBoxRegion("sphere", num_levels=3, radius=[4.0, 2.0, 1.0],
+                       radius_x=[-1.0, 3.0], radius_z=[-1.0, -1.0, 0.5])
+    assert r.radii(0) == (4.0, 4.0, 4.0)
+    assert r.radii(1) == (3.0, 2.0, 2.0)
+    assert r.radii(2) == (1.0, 1.0, 0.5)
+    assert r.radii(5) == (-1.0, -1.0, -1.0)  # unset: the parameters' default
+    c = r.c(1)
+    assert c.shape == 0 and c.num_levels == 3 and tuple(c.radius) == (3.0, 2.0, 2.0)
+    with pytest.raises(ValueError):
+        BoxInBoxRegion("torus")
+    # coarse tags [33, 94] grown by one cell -> [32, 95] -> fine cells [64, 191]
+    assert refined_box(Box((33, 40, 33), (94, 80, 94))) == Box((64, 78, 64), (191, 163, 191))
