@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tile shapes (skip the in-context autotuner)")
     ap.add_argument("--cpu-boxes", type=int, default=512)
+    ap.add_argument("--cpu-boxes-one-thread", type=int, default=48,
+                    help="boxes of the one-thread CPU sample")
     ap.add_argument("--no-wave", action="store_true")
     ap.add_argument("--no-per-box", action="store_true", help="skip timing the per-box path")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -470,6 +472,46 @@ def bench_wave(log, n=128, steps=16, ck=4):
     return rec
 
 
+def bench_tagging(stream, log, n=256):
+    """Box-in-box tags and the threshold test on a level of n^3 cells (two
+    regions, a cube and a sphere): one double written per cell, then one
+    read (and one tag byte written) per cell."""
+    import torch
+    from carpetx_amd.interp import Box, FArrayBox
+    from carpetx_amd.tagging import BoxInBoxRegion, boxinbox_setup, error_est
+    h = 1.0 / n
+    regions = [BoxInBoxRegion("cube", num_levels=2, radius=[1.0, 0.25]),
+               BoxInBoxRegion("sphere", num_levels=2, position=(0.2, -0.1, 0.05),
+                              radius=[1.0, 0.3])]
+    with torch.cuda.stream(stream):
+        err = FArrayBox(Box((0, 0, 0), (n - 1,) * 3), 1, device="cuda")
+        tags = torch.empty(tuple(err.t.shape), dtype=torch.int8, device="cuda")
+        box_out = torch.empty(7, dtype=torch.int32, device="cuda")
+
+        def tag():
+            boxinbox_setup(regions, 1, (-0.5 + h / 2,) * 3, (h,) * 3, err, stream=stream)
+
+        def est():
+            error_est(err, 1.0, box_out=box_out, stream=stream, read=False)
+
+        def est_tags():
+            error_est(err, 1.0, tags=tags, box_out=box_out, stream=stream, read=False)
+
+        for f in (tag, est, est_tags):
+            f()
+        torch.cuda.synchronize()
+        count = int(box_out[0])
+        out = {"cells": n ** 3, "tagged": count}
+        for name, f, nbytes in (("boxinbox_tag", tag, 8), ("error_est", est, 8),
+                                ("error_est_with_tags", est_tags, 9)):
+            ms = back_to_back(f, 20, stream)
+            gbs = nbytes * n ** 3 / (ms * 1e-3) / 1e9
+            out[name] = {"ms": round(ms, 4), "algo_MB": round(nbytes * n ** 3 / 1e6, 1),
+                         "GB_per_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 3)}
+    log("tagging %d^3: %s" % (n, out))
+    return out
+
+
 def load_traffic(name):
     """HBM traffic per launch from the committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", "k_fill_batched_uneven_512_pmc.json")
@@ -480,9 +522,9 @@ def load_traffic(name):
         return None
 
 
-def cpu_baseline(wlz, nboxes, log):
-    """The oracle's OpenMP build (reference CI flags) on the shells of the
-    first `nboxes` boxes of the same level, host copies of the same data."""
+def _cpu_items(wlz, nboxes):
+    """Host copies of the first `nboxes` boxes' coarse data under their
+    shells, and one oracle work item per shell slab."""
     import numpy as np
     import oracle
     from carpetx_amd.workloads import shell_slabs
@@ -510,15 +552,30 @@ def cpu_baseline(wlz, nboxes, log):
         for s in slabs:
             items.append((op, ch, fh, s.lo, s.hi, fab.ncomp))
             npts += s.numPts * fab.ncomp
-    keep = items  # noqa: F841 (host arrays must outlive the calls)
+    return items, npts
+
+
+def _cpu_median(items, budget_s, max_reps):
+    """Median seconds of prolongate_many over the items after one warm-up:
+    at least 3 runs, more while under budget_s and max_reps."""
+    import oracle
     oracle.prolongate_many(items, kind="fast")  # warm-up
     reps = []
     t_total = time.perf_counter()
-    while len(reps) < 3 or (time.perf_counter() - t_total < 15 and len(reps) < 25):
+    while len(reps) < 3 or (time.perf_counter() - t_total < budget_s and len(reps) < max_reps):
         t0 = time.perf_counter()
         oracle.prolongate_many(items, kind="fast")
         reps.append(time.perf_counter() - t0)
-    med = sorted(reps)[len(reps) // 2]
+    return sorted(reps)[len(reps) // 2], reps
+
+
+def cpu_baseline(wlz, nboxes, log):
+    """The oracle's OpenMP build (reference CI flags) on the shells of the
+    first `nboxes` boxes of the same level, host copies of the same data,
+    on every host thread OpenMP offers."""
+    import oracle
+    items, npts = _cpu_items(wlz, nboxes)
+    med, reps = _cpu_median(items, 15, 25)
     threads = oracle.num_threads("fast")
     log("cpu baseline: %d boxes, %.3e pts, median %.3fs over %d runs, %d threads"
         % (nboxes, npts, med, len(reps), threads))
@@ -527,6 +584,24 @@ def cpu_baseline(wlz, nboxes, log):
                        "oracle/liboracle_fast.so (-O3 -ffp-contract=fast -march=x86-64-v3 "
                        "-fopenmp), median of %d runs, %.1f s of CPU work in total"
                        % (nboxes, npts, len(reps), sum(reps)))
+
+
+def cpu_one_thread(wlz, nboxes, log):
+    """The same oracle build on one host thread, over the shells of the
+    first `nboxes` boxes (a bounded sample: a few seconds)."""
+    import oracle
+    items, npts = _cpu_items(wlz, nboxes)
+    threads = oracle.num_threads("fast")
+    oracle.set_num_threads(1)
+    try:
+        med, reps = _cpu_median(items, 5, 5)
+    finally:
+        oracle.set_num_threads(threads)
+    log("cpu one thread: %d boxes, %.3e pts, median %.3fs over %d runs"
+        % (nboxes, npts, med, len(reps)))
+    return dict(value=npts / med, unit="fine pts/s", cores=1,
+                sample="ghost shells of the first %d of the 512 boxes, 10 GF, %d fine "
+                       "pts per run, median of %d runs" % (nboxes, npts, len(reps)))
 
 
 def main():
@@ -592,10 +667,12 @@ def main():
         line["extra"]["level_fill_64"] = bench_level_fill(args, stream, log)
         line["extra"]["regrid_64"] = bench_regrid(args, stream, log)
         line["extra"]["adjoint_96_192"] = bench_adjoint(stream, log)
+        line["extra"]["boxinbox_tag_256"] = bench_tagging(stream, log)
         if not args.no_wave:
             line["extra"]["wave_two_level"] = bench_wave(log)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wlz, args.cpu_boxes, log)
+        line["extra"]["cpu_one_thread"] = cpu_one_thread(wlz, args.cpu_boxes_one_thread, log)
     else:
         line["cpu_baseline"] = None
     if args.copy_calibration:

@@ -75,6 +75,7 @@ def lib(kind="strict"):
                                   ctypes.POINTER(OrFab), ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.or_num_threads.restype = ctypes.c_int
+        L.or_set_num_threads.argtypes = [ctypes.c_int]
         # the C interface's own signatures (cxp_ref.h); cxp_fab structs are
         # laid out as in include/cxp.h
         P = ctypes.POINTER
@@ -210,3 +211,8 @@ def restrict(indextype, fine, crse, rlo, rhi, ncomp=None, kind="strict"):
 
 def num_threads(kind="fast"):
     return lib(kind).or_num_threads()
+
+
+def set_num_threads(n, kind="fast"):
+    """Threads of the later parallel regions started from this thread."""
+    lib(kind).or_set_num_threads(int(n))

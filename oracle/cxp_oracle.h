@@ -88,6 +88,8 @@ int or_restrict(const int indextype[3], const or_fab *fine, int fine_comp,
                 const int rhi[3]);
 
 int or_num_threads(void);
+/* Threads of the later parallel regions (n > 0). */
+void or_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -363,3 +363,35 @@ def test_eno_joint_max_keeps_centre():
         wy = rows[0] if offj == 0 else rows[0][::-1]
         ey = sum(wy[i] * cy[i] for i in range(3))
         assert fine.arr[0, 0, fj, f] == pytest.approx(ex + ey, abs=1e-12), f
+
+
+def test_fast_build_results_do_not_depend_on_the_thread_count():
+    """The CPU baseline's OpenMP build splits work items over threads, each
+    item computed by one thread: one thread and every thread give the same
+    bits (the single-thread timing measures the same computation)."""
+    rng = np.random.default_rng(11)
+    items, fines = [], []
+    for n, (typ, order, it) in enumerate([("natural", 3, (1, 0, 1)), ("natural", 5, (0, 0, 0)),
+                                          ("natural", 1, (1, 1, 1)), ("eno", 3, (1, 1, 1))]):
+        op = oracle.lookup(typ, order, it, kind="fast")
+        rlo, rhi = (-3 + n, 2, -1), (9, 14 + n, 6)
+        clo, chi = oracle.coarse_box(op, rlo, rhi)
+        shape = (2,) + tuple(chi[d] - clo[d] + 1 for d in (2, 1, 0))
+        crse = oracle.Fab(clo, chi, 2, rng.uniform(-1, 1, shape))
+        fine = oracle.Fab(rlo, rhi, 2)
+        items.append((op, crse, fine, rlo, rhi, 2))
+        fines.append(fine)
+    threads = oracle.num_threads("fast")
+    oracle.prolongate_many(items, kind="fast")
+    first = [f.arr.copy() for f in fines]
+    oracle.set_num_threads(1)
+    try:
+        for f in fines:
+            f.arr[...] = 0.0
+        oracle.prolongate_many(items, kind="fast")
+    finally:
+        oracle.set_num_threads(threads)
+    assert oracle.num_threads("fast") == threads
+    for a, f in zip(first, fines):
+        assert np.array_equal(a, f.arr)
+        assert np.isfinite(a).all() and np.abs(a).max() > 0
