@@ -8,6 +8,9 @@
 //   * restrict_fab                 Restrict(), CarpetX/src/schedule.cxx:2791-2816
 //   * WorkList / Graph             the FillPatchInterp loop over fine patches
 //                                  (CarpetX/src/fillpatch.cxx:116-119)
+//   * boxinbox_setup / error_est   BoxInBox_Setup (BoxInBox/src/boxinbox.cxx:
+//                                  102-174) and ErrorEst's threshold test
+//                                  (CarpetX/src/driver.cxx:1025-1080)
 // FABs are views of device memory in AMReX FArrayBox layout.  Where the
 // reference aborts (assert / CCTK_VERROR), these throw cxp::Error carrying
 // the library status.
@@ -24,6 +27,7 @@
 #include <vector>
 
 #include "../../include/cxp.h"
+#include "../../include/cxp_tag.h"
 
 namespace cxp {
 
@@ -391,6 +395,61 @@ public:
 
 private:
   cxp_worklist wl_ = nullptr;
+};
+
+// One box-in-box region at the level being decided (the parameters
+// active_N, shape_N, num_levels_N, position_*_N and, per direction,
+// radius_*_N[level] or radius_N[level]).
+struct BoxInBoxRegion {
+  bool active = true;
+  int shape = CXP_BIB_CUBE;
+  int num_levels = 1;
+  std::array<double, 3> position{0.0, 0.0, 0.0};
+  std::array<double, 3> radius{-1.0, -1.0, -1.0};
+};
+
+// regrid_error over `region` of a one-component FAB: 1 in the cells a
+// region refines, 0 elsewhere.  x0 is the coordinate of cell 0's centre.
+inline void boxinbox_setup(const std::vector<BoxInBoxRegion> &regions,
+                           int level, const std::array<double, 3> &x0,
+                           const std::array<double, 3> &dx, const Fab &err,
+                           const Box &region, void *stream = nullptr) {
+  std::vector<cxp_bib_region> r(regions.size() ? regions.size() : 1);
+  for (size_t i = 0; i < regions.size(); ++i) {
+    r[i].active = regions[i].active;
+    r[i].shape = regions[i].shape;
+    r[i].num_levels = regions[i].num_levels;
+    for (int d = 0; d < 3; ++d) {
+      r[i].position[d] = regions[i].position[d];
+      r[i].radius[d] = regions[i].radius[d];
+    }
+  }
+  check(cxp_boxinbox_tag(r.data(), int(regions.size()), level, x0.data(),
+                         dx.data(), err.data, err.box.lo.data(),
+                         err.box.hi.data(), region.lo.data(),
+                         region.hi.data(), stream),
+        "cxp_boxinbox_tag");
+}
+
+// ErrorEst's tags over `region` (err >= threshold): tags (device, one byte
+// per cell of err's box, or null) and box_out (device, 7 ints, or null).
+inline void error_est(const Fab &err, const Box &region, double threshold,
+                      signed char *tags, int *box_out, void *stream = nullptr) {
+  check(cxp_error_est(err.data, err.box.lo.data(), err.box.hi.data(),
+                      region.lo.data(), region.hi.data(), threshold, tags,
+                      box_out, stream),
+        "cxp_error_est");
+}
+
+// The 7 result words of error_est, copied to the host: the number of set
+// cells and their bounding box (empty when none).
+struct TagSummary {
+  int64_t count = 0;
+  Box box;
+  explicit TagSummary(const int w[7]) : count(w[0]) {
+    if (count > 0)
+      box = Box({w[1], w[2], w[3]}, {w[4], w[5], w[6]});
+  }
 };
 
 // A captured fill sequence (hipGraph).

@@ -23,6 +23,10 @@
 //      and then given its own reflection boundaries, and a remake of it
 //      onto a shifted box (old data where the old box covers, prolongation
 //      elsewhere), both reproduce the data at every point of the FABs.
+//   7. Box-in-box tags of 64^3 cells of [0, 1]^3 for a cube of radius 1/4
+//      about the centre, then the threshold test: exactly the cells
+//      [17, 46]^3 are tagged (|i + 1/2 - 32| <= 16 - 1 on this dyadic
+//      grid), counted and boxed on the device, and the tag bytes agree.
 //
 // Exit status 0 when every check passes.
 #include <hip/hip_runtime.h>
@@ -338,6 +342,46 @@ int main() {
     std::printf("      remake, %lld points: max error %.3e\n",
                 (long long)rfab.numPts() * 2, er);
     expect(er < 1e-12, "remade-level fill with boundaries reproduces the data everywhere");
+  }
+
+  // 7. box-in-box tags and the threshold test
+  {
+    const int nc = 64;
+    const cxp::Box cells({0, 0, 0}, {nc - 1, nc - 1, nc - 1});
+    DevFab err(cells);
+    cxp::BoxInBoxRegion cube;
+    cube.num_levels = 2;
+    cube.position = {0.5, 0.5, 0.5};
+    cube.radius = {0.25, 0.25, 0.25};
+    const double h = 1.0 / nc;
+    cxp::boxinbox_setup({cube}, 1, {h / 2, h / 2, h / 2}, {h, h, h}, err.fab,
+                        cells, s);
+    signed char *tags = nullptr;
+    int *words = nullptr;
+    (void)hipMalloc(&tags, cells.numPts());
+    (void)hipMalloc(&words, 7 * sizeof(int));
+    cxp::error_est(err.fab, cells, 1.0, tags, words, s);
+    int hw[7];
+    std::vector<signed char> ht(cells.numPts());
+    (void)hipMemcpyAsync(hw, words, sizeof hw, hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(ht.data(), tags, ht.size(), hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    const cxp::TagSummary sum(hw);
+    const auto e = err.get();
+    int64_t nt = 0, ne = 0;
+    bool agree = true;
+    for (size_t i = 0; i < ht.size(); ++i) {
+      nt += ht[i];
+      ne += e[i] == 1.0;
+      agree = agree && (ht[i] == 1) == (e[i] == 1.0) && (e[i] == 0.0 || e[i] == 1.0);
+    }
+    std::printf("      tags: %lld set, box [%d,%d]^3\n", (long long)sum.count,
+                sum.box.lo[0], sum.box.hi[0]);
+    expect(sum.count == 30 * 30 * 30 && nt == sum.count && ne == sum.count && agree &&
+               sum.box == cxp::Box({17, 17, 17}, {46, 46, 46}),
+           "box-in-box cube tags exactly [17,46]^3, counted and boxed");
+    (void)hipFree(tags);
+    (void)hipFree(words);
   }
 
   (void)hipStreamDestroy(s);
