@@ -200,11 +200,15 @@ int z_chunk(long long blocks_per_plane, int nz, long long target) {
 }
 
 // [lo, hi] nonempty and inside [flo, fhi]: the map of the box into the FAB.
+// The launches put y rows (4 per block) on grid.y, so the box's y extent is
+// at most 4 x 65535 rows.
 int box_map(const int flo[3], const int fhi[3], const int lo[3],
             const int hi[3], BoxMap &m) {
   for (int d = 0; d < 3; ++d)
     if (lo[d] > hi[d] || lo[d] < flo[d] || hi[d] > fhi[d] || flo[d] > fhi[d])
       return CXP_ERR_REGION;
+  if ((long long)hi[1] - lo[1] + 1 > 4LL * 65535)
+    return CXP_ERR_ARG;
   const long long nx = (long long)fhi[0] - flo[0] + 1;
   const long long ny = (long long)fhi[1] - flo[1] + 1;
   m.sy = nx;

@@ -57,8 +57,8 @@ typedef struct {
  * Errors: CXP_ERR_ARG for a null pointer, nregions outside [0, 3], an
  * unknown shape, or a non-positive radius of a region that applies
  * at this level (the reference's "Non-positive radius found" abort,
- * boxinbox.cxx:143-151); CXP_ERR_REGION if [lo, hi] is empty or not inside
- * the FAB. */
+ * boxinbox.cxx:143-151) or a box more than 262140 cells long in y;
+ * CXP_ERR_REGION if [lo, hi] is empty or not inside the FAB. */
 int cxp_boxinbox_tag(const cxp_bib_region *regions, int nregions, int level,
                      const double x0[3], const double dx[3], double *err,
                      const int err_lo[3], const int err_hi[3],

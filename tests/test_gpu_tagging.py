@@ -138,3 +138,29 @@ def test_wave_harness_places_its_fine_level_from_tags():
     sc, sf = w.gaussian()
     sc, sf = w.step(sc, sf)
     assert torch.isfinite(sf).all() and torch.isfinite(sc).all()
+
+
+def test_error_est_tags_without_result_words():
+    """cxp_error_est with tags only (box_out null), through the C ABI."""
+    import ctypes
+    from carpetx_amd import _lib
+    fab_box = Box((3, -2, 0), (40, 29, 17))
+    data = torch.rand((1,) + tuple(reversed(fab_box.size)), dtype=torch.float64,
+                      generator=torch.Generator().manual_seed(3))
+    err = FArrayBox(fab_box, 1, device="cuda", tensor=data.cuda())
+    tags = torch.full(tuple(err.t.shape), 7, dtype=torch.int8, device="cuda")
+    sub = Box((5, 0, 2), (33, 21, 15))
+    st = _lib.lib().cxp_error_est(
+        ctypes.c_void_p(err.t.data_ptr()), _lib.i3(fab_box.lo), _lib.i3(fab_box.hi),
+        _lib.i3(sub.lo), _lib.i3(sub.hi), 0.25, ctypes.c_void_p(tags.data_ptr()),
+        None, None)
+    assert st == 0
+    torch.cuda.synchronize()
+    t = tags[0].cpu().numpy()
+    sl = tuple(slice(sub.lo[d] - fab_box.lo[d], sub.hi[d] - fab_box.lo[d] + 1)
+               for d in (2, 1, 0))
+    want, _, _ = ot.error_est(data[0].numpy()[sl], sub.lo, 0.25)
+    assert np.array_equal(t[sl], want)
+    outside = np.ones(t.shape, dtype=bool)
+    outside[sl] = False
+    assert np.all(t[outside] == 7)
