@@ -29,7 +29,8 @@ namespace cxp {
 // compute unit progress more slowly than any fixed estimate says.  The
 // smallest tiles of each XCD class x therefore sit in a queue, tiles
 // [q.start[x], q.start[x + 1]), which the class's workgroups claim q.chunk
-// at a time through counter q.ctr[32 x] once their own range is done.  The
+// at a time through counter q.ctr[32 x] once their own range is done (the
+// next claim is issued while the current chunk runs).  The
 // last workgroup to finish (counter q.ctr[32 q.nq]) zeroes the counters for
 // the next launch, so launches of one list must not overlap.
 struct FillQueue {
@@ -67,16 +68,24 @@ __global__ void __launch_bounds__(CXP_NT, 4)
     return;
   const int x = blockIdx.x % q.nq;
   const int p0 = q.start[x], p1 = q.start[x + 1];
+  // Thread 0 claims the next chunk before the current one runs, so the
+  // atomic's round trip overlaps the tile instead of preceding it; the last
+  // claim of each workgroup goes unused.
+  int next = 0;
+  if (threadIdx.x == 0)
+    next = __hip_atomic_fetch_add(q.ctr + 32 * x, 1, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
   for (;;) {
     if (threadIdx.x == 0)
-      claim = p0 + q.chunk * __hip_atomic_fetch_add(q.ctr + 32 * x, 1,
-                                                    __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
+      claim = p0 + q.chunk * next;
     __syncthreads();
     // every later write of claim follows at least one tile's barrier
     const int c0 = claim;
     if (c0 >= p1)
       break;
+    if (threadIdx.x == 0)
+      next = __hip_atomic_fetch_add(q.ctr + 32 * x, 1, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
     const int c1 = min(c0 + q.chunk, p1);
     for (int n = c0; n < c1; ++n)
       fill_one<SU>(tasks, tiles, n, fill_smem);
