@@ -635,3 +635,43 @@ def test_boundary_sources_in_two_coarse_fabs():
     wl.append_level(get_interpolator(*op_args), crse, 1, fine, 0, NCOMP, bc=_bc(faces))
     _run_all_ways(wl, fine.fabs, originals, want, ("two-FAB sources", op_args))
     wl.close()
+
+
+def test_new_level_placed_by_boxinbox_tags():
+    """The regrid sequence end to end: box-in-box tags on each coarse FAB's
+    valid box (BoxInBox_Setup), ErrorEst's threshold test per box, the
+    union of the tagged cells' bounding boxes grown by one coarse cell and
+    refined (a box of tags clusters to its bounding box), then
+    FillPatch_NewLevel on that box, against the oracle.  Coarse cells of
+    [0, 1]^3 (N = 24, h = 1/24), a cube of radius 1/4 about the centre:
+    |i + 1/2 - 12| <= 6 - 1 tags cells 7..16, half a cell from either
+    decision, so the box is [7, 16]^3 whatever the rounding."""
+    from carpetx_amd.tagging import BoxInBoxRegion, boxinbox_setup, error_est, refined_box
+    op_args = ("natural", 3, (1, 1, 1))
+    it = op_args[2]
+    rng = np.random.default_rng(23)
+    cdom, fdom = domains(it)
+    g = rng.uniform(-1, 1, (NC_CRSE,) + tuple(cdom.size[d] + 2 * CG for d in (2, 1, 0)))
+    crse = coarse_level(it, g)
+    h = 1.0 / N
+    cube = BoxInBoxRegion("cube", num_levels=2, position=(0.5, 0.5, 0.5), radius=[1.0, 0.25])
+    total, lo, hi = 0, [1 << 30] * 3, [-(1 << 30)] * 3
+    for v in crse.valid:
+        err = FArrayBox(v, 1)
+        boxinbox_setup([cube], 1, (h / 2,) * 3, (h,) * 3, err)
+        count, tb = error_est(err, 1.0)
+        if count:
+            total += count
+            lo = [min(lo[d], tb.lo[d]) for d in range(3)]
+            hi = [max(hi[d], tb.hi[d]) for d in range(3)]
+    assert total == 10 ** 3 and Box(lo, hi) == Box((7,) * 3, (16,) * 3)
+    new = refined_box(Box(lo, hi))
+    assert new == Box((12,) * 3, (35,) * 3)
+    fine = fine_level(it, rng, boxes=[(new.lo, new.hi)], fill_valid=False)
+    originals = [f.t.clone() for f in fine.fabs]
+    want = [_prolongated(op_args, it, g, fab) for fab in fine.fabs]
+    assert fdom.contains(fine.fabs[0].box)
+    wl = WorkList()
+    wl.append_new_level(get_interpolator(*op_args), crse, 1, fine, 0, NCOMP)
+    _run_all_ways(wl, fine.fabs, originals, want, ("new level from tags", op_args))
+    wl.close()
