@@ -40,45 +40,59 @@ __device__ __forceinline__ double max1(double x, double y) {
   return x != x ? x : (y != y ? y : (x < y ? y : x));
 }
 
-// One thread per (x, y) column of cells, marching over a chunk of zc
-// planes: x on 64 lanes, 4 rows of y per block.  Each region's x and y
-// quotients, and the first two terms of its sum or maximum, are computed
-// once per column; the order of the operations is the reference's, so the
-// hoisting changes no result.
+// One thread per two (x, y) columns of cells, 64 apart in x (so each of
+// the wave's stores covers 64 consecutive cells), marching over a chunk of
+// zc planes: 4 rows of y per block.  Each region's x and y quotients, and
+// the first two terms of its sum or maximum, are computed once per column,
+// and its z quotient once per plane for both columns; the order of the
+// operations is the reference's, so the hoisting changes no result.
 __global__ void __launch_bounds__(256)
     k_bib_tag(BibArgs a, double *__restrict__ err, BoxMap m, int zc) {
-  const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int ix = blockIdx.x * 128 + (threadIdx.x & 63);
   const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
   const int z0 = blockIdx.z * zc;
   const int z1 = min(z0 + zc, m.n[2]);
   if (ix >= m.n[0] || iy >= m.n[1])
     return;
-  const double X = a.x0[0] + (double)(m.lo[0] + ix) * a.dx[0];
+  const bool two = ix + 64 < m.n[0];
   const double Y = a.x0[1] + (double)(m.lo[1] + iy) * a.dx[1];
-  double part[CXP_BIB_MAX_REGIONS];  // sphere: (0 + qx qx) + qy qy; cube: max1
+  double part[2][CXP_BIB_MAX_REGIONS];  // sphere: (0 + qx qx) + qy qy; cube: max1
 #pragma unroll
-  for (int r = 0; r < CXP_BIB_MAX_REGIONS; ++r) {
-    if (r < a.n) {
-      const double qx = (X - a.pos[r][0]) / a.rmax[r][0];
-      const double qy = (Y - a.pos[r][1]) / a.rmax[r][1];
-      part[r] = a.shape[r] == CXP_BIB_SPHERE
-                    ? (0.0 + qx * qx) + qy * qy
-                    : max1(max1(-INFINITY, fabs(qx)), fabs(qy));
+  for (int c = 0; c < 2; ++c) {
+    const double X = a.x0[0] + (double)(m.lo[0] + ix + 64 * c) * a.dx[0];
+#pragma unroll
+    for (int r = 0; r < CXP_BIB_MAX_REGIONS; ++r) {
+      if (r < a.n) {
+        const double qx = (X - a.pos[r][0]) / a.rmax[r][0];
+        const double qy = (Y - a.pos[r][1]) / a.rmax[r][1];
+        part[c][r] = a.shape[r] == CXP_BIB_SPHERE
+                         ? (0.0 + qx * qx) + qy * qy
+                         : max1(max1(-INFINITY, fabs(qx)), fabs(qy));
+      }
     }
   }
   double *p = err + m.base + ix + iy * m.sy + z0 * m.sz;
   for (int iz = z0; iz < z1; ++iz, p += m.sz) {
     const double Z = a.x0[2] + (double)(m.lo[2] + iz) * a.dx[2];
-    bool refine = false;
+    bool refine0 = false, refine1 = false;
 #pragma unroll
     for (int r = 0; r < CXP_BIB_MAX_REGIONS; ++r) {
       if (r < a.n) {
         const double qz = (Z - a.pos[r][2]) / a.rmax[r][2];
-        refine |= a.shape[r] == CXP_BIB_SPHERE ? part[r] + qz * qz <= 1.0
-                                               : max1(part[r], fabs(qz)) <= 1.0;
+        if (a.shape[r] == CXP_BIB_SPHERE) {
+          const double q2 = qz * qz;
+          refine0 |= part[0][r] + q2 <= 1.0;
+          refine1 |= part[1][r] + q2 <= 1.0;
+        } else {
+          const double aq = fabs(qz);
+          refine0 |= max1(part[0][r], aq) <= 1.0;
+          refine1 |= max1(part[1][r], aq) <= 1.0;
+        }
       }
     }
-    *p = refine ? 1.0 : 0.0;
+    p[0] = refine0 ? 1.0 : 0.0;
+    if (two)
+      p[64] = refine1 ? 1.0 : 0.0;
   }
 }
 
@@ -259,7 +273,7 @@ extern "C" int cxp_boxinbox_tag(const cxp_bib_region *regions, int nregions,
   const int e = box_map(err_lo, err_hi, lo, hi, m);
   if (e != CXP_OK)
     return e;
-  const int nbx = (m.n[0] + 63) / 64, nby = (m.n[1] + 3) / 4;
+  const int nbx = (m.n[0] + 127) / 128, nby = (m.n[1] + 3) / 4;
   const int zc = z_chunk((long long)nbx * nby, m.n[2], 8192);
   const dim3 grid(nbx, nby, (m.n[2] + zc - 1) / zc);
   hipLaunchKernelGGL(k_bib_tag, grid, dim3(256), 0, (hipStream_t)stream, a,
