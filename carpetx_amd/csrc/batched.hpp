@@ -32,7 +32,8 @@ namespace cxp {
 // at a time through counter q.ctr[32 x] once their own range is done (the
 // next claim is issued while the current chunk runs).  The
 // last workgroup to finish (counter q.ctr[32 q.nq]) zeroes the counters for
-// the next launch, so launches of one list must not overlap.
+// the next launch, so launches of one list must not overlap: the host orders
+// a launch on another stream after the list's previous launch.
 struct FillQueue {
   int *ctr;  // null: no queue
   int start[9];

@@ -408,28 +408,77 @@ int launch_adj_gather(const AdjPointTask &t, const unsigned char *codes,
   return hip_status(hipGetLastError());
 }
 
+// Bytes of branch codes the recomputing ENO / fallback adjoint needs: one
+// per fine point, per component in per-var mode.
+size_t adjoint_code_bytes(int ncomp, const int rlo[3], const int rhi[3],
+                          int per_group) {
+  size_t npts = 1;
+  for (int d = 0; d < 3; ++d)
+    npts *= (size_t)(rhi[d] - rlo[d] + 1);
+  return npts * (size_t)(per_group ? 1 : ncomp);
+}
+
+// Whether s is capturing a graph (an invalidated capture is an error).
+int stream_capturing(hipStream_t s, bool &capturing,
+                     unsigned long long *id = nullptr) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  unsigned long long cid = 0;
+  if (hipStreamGetCaptureInfo(s, &cs, &cid) != hipSuccess)
+    return CXP_ERR_HIP;
+  if (cs == hipStreamCaptureStatusInvalidated)
+    return CXP_ERR_STATE;
+  capturing = cs == hipStreamCaptureStatusActive;
+  if (id)
+    *id = cid;
+  return CXP_OK;
+}
+
+// Branch-code buffers of the recomputing adjoint: one per (device, stream),
+// grown on demand, stream-ordered (allocated and freed on that stream), and
+// reused by later calls on the same stream, which stream order keeps apart.
+struct CodePool {
+  unsigned char *p = nullptr;
+  size_t cap = 0;
+};
+std::mutex g_pool_mu;
+std::map<std::pair<int, hipStream_t>, CodePool> g_code_pool;
+
+int pooled_codes(hipStream_t s, size_t nbytes, unsigned char **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return CXP_ERR_HIP;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  CodePool &cp = g_code_pool[{dev, s}];
+  if (nbytes > cp.cap) {
+    if (cp.p && hipFreeAsync(cp.p, s) != hipSuccess)
+      return CXP_ERR_HIP;
+    cp.p = nullptr;
+    cp.cap = 0;
+    const size_t cap = std::max(nbytes + nbytes / 4, (size_t)1 << 20);
+    if (hipMallocAsync((void **)&cp.p, cap, s) != hipSuccess) {
+      cp.p = nullptr;
+      return CXP_ERR_HIP;
+    }
+    cp.cap = cap;
+  }
+  *out = cp.p;
+  return CXP_OK;
+}
+
 // Adjoint of an ENO or linear-fallback operator, linearised at crse_in:
-// record every fine point's branch, then gather per coarse point.
-int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
-                         const cxp_fab *crse_bar, int crse_comp,
-                         const cxp_fab *fine_bar, int fine_comp, int ncomp,
-                         const int rlo[3], const int rhi[3], int per_group,
-                         hipStream_t s) {
-  if (!crse_in)
-    return CXP_ERR_ARG;
-  if (const int e = check_prolong(op, crse_in, crse_comp, fine_bar, fine_comp,
-                                  ncomp, rlo, rhi, per_group))
-    return e;
+// record every fine point's branch into codes, then gather per coarse
+// point.  The caller has checked the arguments and sized codes.
+int adjoint_point_run(const cxp_op_s *op, const cxp_fab *crse_in,
+                      const cxp_fab *crse_bar, int crse_comp,
+                      const cxp_fab *fine_bar, int fine_comp, int ncomp,
+                      const int rlo[3], const int rhi[3], int per_group,
+                      unsigned char *codes, hipStream_t s) {
   const AdjPointTask t =
       make_adj_point_task(op, crse_in, crse_bar, crse_comp, fine_bar,
                           fine_comp, ncomp, rlo, rhi, per_group);
   long long npts = 1;
   for (int d = 0; d < 3; ++d)
     npts *= rhi[d] - rlo[d] + 1;
-  unsigned char *codes = nullptr;
-  const size_t nbytes = (size_t)npts * (per_group ? 1 : ncomp);
-  if (hipMallocAsync((void **)&codes, nbytes, s) != hipSuccess)
-    return CXP_ERR_HIP;
   // the branch record: LDS-tiled like the forward when a tile's coarse
   // block fits, else one thread per point
   const size_t lds =
@@ -453,9 +502,41 @@ int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
   int st = hip_status(hipGetLastError());
   if (st == CXP_OK)
     st = launch_adj_gather(t, codes, ncomp, s);
-  if (hipFreeAsync(codes, s) != hipSuccess && st == CXP_OK)
-    st = CXP_ERR_HIP;
   return st;
+}
+
+int adjoint_point_check(const cxp_op_s *op, const cxp_fab *crse_in,
+                        int crse_comp, const cxp_fab *fine_bar, int fine_comp,
+                        int ncomp, const int rlo[3], const int rhi[3],
+                        int per_group) {
+  if (!crse_in)
+    return CXP_ERR_ARG;
+  return check_prolong(op, crse_in, crse_comp, fine_bar, fine_comp, ncomp,
+                       rlo, rhi, per_group);
+}
+
+// The recomputing adjoint with its codes from the stream's pool.  A
+// captured graph would keep the pool's buffer, which a later call may free
+// or share, so it refuses capture (CXP_ERR_STATE).
+int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
+                         const cxp_fab *crse_bar, int crse_comp,
+                         const cxp_fab *fine_bar, int fine_comp, int ncomp,
+                         const int rlo[3], const int rhi[3], int per_group,
+                         hipStream_t s) {
+  if (const int e = adjoint_point_check(op, crse_in, crse_comp, fine_bar,
+                                        fine_comp, ncomp, rlo, rhi, per_group))
+    return e;
+  bool capturing = false;
+  if (const int e = stream_capturing(s, capturing))
+    return e;
+  if (capturing)
+    return CXP_ERR_STATE;
+  unsigned char *codes = nullptr;
+  if (const int e = pooled_codes(
+          s, adjoint_code_bytes(ncomp, rlo, rhi, per_group), &codes))
+    return e;
+  return adjoint_point_run(op, crse_in, crse_bar, crse_comp, fine_bar,
+                           fine_comp, ncomp, rlo, rhi, per_group, codes, s);
 }
 
 } // namespace
@@ -522,14 +603,104 @@ struct cxp_worklist_s {
   int lds = 0;  // bytes of dynamic LDS of k_fill_batched
   int uniform_sid = -1;  // the one stencil of every separable task, or -1
   int64_t fine_points = 0, algo_bytes = 0;
+  // Identity and version for captured graphs: every change that may
+  // reallocate the device buffers or change the tiles (append, clear,
+  // per-group setting, upload, autotune) bumps epoch, and a graph that
+  // launched the list at another epoch refuses to replay.
+  uint64_t id = 0;
+  uint64_t epoch = 0;
+  // Launch ordering: one list's claim counters and coarse-patch buffer serve
+  // one launch at a time, so a launch on another stream than the previous
+  // one waits for that launch's completion event.
+  std::mutex mu;
+  hipEvent_t done = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool launched = false;
 };
 
 struct cxp_graph_s {
   hipGraph_t graph;
   hipGraphExec_t exec;
+  // (list id, epoch) of every list the captured work launched
+  std::vector<std::pair<uint64_t, uint64_t>> lists;
 };
 
 namespace {
+
+// ---- list identity, capture records and launch ordering
+
+std::mutex g_lists_mu;  // guards the three maps and every list's epoch
+std::map<uint64_t, cxp_worklist_s *> g_lists;  // live lists by id
+uint64_t g_next_list_id = 1;
+// lists launched on a capturing stream, by capture id, until the capture
+// ends in cxp_graph_capture_end (captures ended elsewhere are dropped
+// oldest first beyond kMaxCaptureRecords)
+std::map<unsigned long long, std::vector<std::pair<uint64_t, uint64_t>>>
+    g_capture_lists;
+constexpr size_t kMaxCaptureRecords = 256;
+
+void touch(cxp_worklist_s *wl) {
+  std::lock_guard<std::mutex> lk(g_lists_mu);
+  ++wl->epoch;
+}
+
+// Waits on the host for the list's last launch (before its device buffers
+// are rewritten or freed).
+int list_drain(cxp_worklist_s *wl) {
+  std::lock_guard<std::mutex> lk(wl->mu);
+  if (wl->launched && hipEventSynchronize(wl->done) != hipSuccess)
+    return CXP_ERR_HIP;
+  return CXP_OK;
+}
+
+// Before launching wl's work on s: on a capturing stream, records the list
+// in the capture; otherwise orders s after the list's previous launch when
+// that went to another stream.  Called with wl->mu held.
+int list_enter(cxp_worklist_s *wl, hipStream_t s, bool &capturing) {
+  unsigned long long cid = 0;
+  if (const int e = stream_capturing(s, capturing, &cid))
+    return e;
+  if (capturing) {
+    std::lock_guard<std::mutex> lk(g_lists_mu);
+    g_capture_lists[cid].emplace_back(wl->id, wl->epoch);
+    while (g_capture_lists.size() > kMaxCaptureRecords)
+      g_capture_lists.erase(g_capture_lists.begin());
+    return CXP_OK;
+  }
+  if (wl->launched && wl->last_stream != s &&
+      hipStreamWaitEvent(s, wl->done, 0) != hipSuccess)
+    return CXP_ERR_HIP;
+  return CXP_OK;
+}
+
+// After wl's work went to s (not capturing): records its completion.
+// Called with wl->mu held.
+int list_leave(cxp_worklist_s *wl, hipStream_t s) {
+  if (!wl->done &&
+      hipEventCreateWithFlags(&wl->done, hipEventDisableTiming) != hipSuccess) {
+    wl->done = nullptr;
+    return CXP_ERR_HIP;
+  }
+  if (hipEventRecord(wl->done, s) != hipSuccess)
+    return CXP_ERR_HIP;
+  wl->last_stream = s;
+  wl->launched = true;
+  return CXP_OK;
+}
+
+// Runs fn (the launches of one entry point) between list_enter and
+// list_leave with the list's lock held.
+template <typename F> int list_launch(cxp_worklist_s *wl, hipStream_t s, F fn) {
+  std::lock_guard<std::mutex> lk(wl->mu);
+  bool capturing = false;
+  if (const int e = list_enter(wl, s, capturing))
+    return e;
+  const int st = fn();
+  if (capturing)
+    return st;
+  const int e = list_leave(wl, s);
+  return st ? st : e;
+}
 
 // Union of the coarse points read for one patch.
 int64_t patch_coarse_points(const cxp_worklist_s::Patch &p) {
@@ -813,19 +984,28 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
   schedule_tiles(wl, wl->grid);
 }
 
+// Grows a device array to hold n elements (twice that, for later growth);
+// cap is set only once the allocation succeeded.
+template <typename T> int grow_device(T *&d, size_t &cap, size_t n) {
+  if (n <= cap)
+    return CXP_OK;
+  if (d)
+    (void)hipFree(d);
+  d = nullptr;
+  cap = 0;
+  if (hipMalloc(&d, 2 * n * sizeof(T)) != hipSuccess) {
+    d = nullptr;
+    return CXP_ERR_HIP;
+  }
+  cap = 2 * n;
+  return CXP_OK;
+}
+
 // Grows a device array to hold v and copies v into it (asynchronously).
 template <typename T>
 int upload_vec(const std::vector<T> &v, T *&d, size_t &cap, hipStream_t s) {
-  if (v.size() > cap) {
-    if (d)
-      (void)hipFree(d);
-    d = nullptr;
-    cap = v.size() * 2;
-    if (hipMalloc(&d, cap * sizeof(T)) != hipSuccess) {
-      cap = 0;
-      return CXP_ERR_HIP;
-    }
-  }
+  if (int e = grow_device(d, cap, v.size()))
+    return e;
   if (!v.empty() && hipMemcpyAsync(d, v.data(), v.size() * sizeof(T),
                                    hipMemcpyHostToDevice, s) != hipSuccess)
     return CXP_ERR_HIP;
@@ -934,30 +1114,12 @@ int upload_lists(cxp_worklist_s *wl, hipStream_t s) {
   if (int e = upload_vec(wl->poison_chunks, wl->d_poison_chunks,
                          wl->cap_poison_chunks, s))
     return e;
-  if (wl->tasks.size() > wl->cap_tasks) {
-    if (wl->d_tasks)
-      (void)hipFree(wl->d_tasks);
-    wl->d_tasks = nullptr;
-    wl->cap_tasks = wl->tasks.size() * 2;
-    if (hipMalloc(&wl->d_tasks, wl->cap_tasks * sizeof(SepTask)) != hipSuccess)
-      return CXP_ERR_HIP;
-  }
-  if (wl->tiles.size() > wl->cap_tiles) {
-    if (wl->d_tiles)
-      (void)hipFree(wl->d_tiles);
-    wl->d_tiles = nullptr;
-    wl->cap_tiles = wl->tiles.size() * 2;
-    if (hipMalloc(&wl->d_tiles, wl->cap_tiles * sizeof(SepTile)) != hipSuccess)
-      return CXP_ERR_HIP;
-  }
-  if (wl->wg_start.size() > wl->cap_wg) {
-    if (wl->d_wg_start)
-      (void)hipFree(wl->d_wg_start);
-    wl->d_wg_start = nullptr;
-    wl->cap_wg = wl->wg_start.size() * 2;
-    if (hipMalloc(&wl->d_wg_start, wl->cap_wg * sizeof(int)) != hipSuccess)
-      return CXP_ERR_HIP;
-  }
+  if (int e = grow_device(wl->d_tasks, wl->cap_tasks, wl->tasks.size()))
+    return e;
+  if (int e = grow_device(wl->d_tiles, wl->cap_tiles, wl->tiles.size()))
+    return e;
+  if (int e = grow_device(wl->d_wg_start, wl->cap_wg, wl->wg_start.size()))
+    return e;
   if (!wl->d_ctr) {
     if (hipMalloc(&wl->d_ctr, 9 * 32 * sizeof(int)) != hipSuccess)
       return CXP_ERR_HIP;
@@ -1290,13 +1452,26 @@ int cxp_prolongate_adjoint_recorded(cxp_op op, const unsigned char *branches,
 int cxp_worklist_create(cxp_worklist *out) {
   if (!out)
     return CXP_ERR_ARG;
-  *out = new cxp_worklist_s();
+  cxp_worklist_s *wl = new cxp_worklist_s();
+  {
+    std::lock_guard<std::mutex> lk(g_lists_mu);
+    wl->id = g_next_list_id++;
+    g_lists[wl->id] = wl;
+  }
+  *out = wl;
   return CXP_OK;
 }
 
 int cxp_worklist_destroy(cxp_worklist wl) {
   if (!wl)
     return CXP_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> lk(g_lists_mu);
+    g_lists.erase(wl->id);
+  }
+  (void)list_drain(wl);
+  if (wl->done)
+    (void)hipEventDestroy(wl->done);
   if (wl->d_tasks)
     (void)hipFree(wl->d_tasks);
   if (wl->d_tiles)
@@ -1345,6 +1520,7 @@ int cxp_worklist_clear(cxp_worklist wl) {
   wl->post_chunks.clear();
   wl->gather_doubles = 0;
   wl->uploaded = false;
+  touch(wl);
   wl->fine_points = wl->algo_bytes = 0;
   return CXP_OK;
 }
@@ -1371,6 +1547,7 @@ int cxp_worklist_append(cxp_worklist wl, cxp_op op, const cxp_fab *crse,
   }
   wl->patches.push_back(p);
   wl->uploaded = false;
+  touch(wl);
   const int64_t npts = int64_t(rhi[0] - rlo[0] + 1) * (rhi[1] - rlo[1] + 1) *
                        (rhi[2] - rlo[2] + 1);
   wl->fine_points += npts * ncomp;
@@ -2213,6 +2390,7 @@ int append_level_impl(cxp_worklist wl, cxp_op op, const cxp_level *crse,
   wl->bc_direct.insert(wl->bc_direct.end(), bdirs.begin(), bdirs.end());
   wl->gather_doubles = gather;
   wl->uploaded = false;
+  touch(wl);
   wl->fine_points += fine_pts * ncomp;
   wl->algo_bytes += 8 * (int64_t)ncomp * (fine_pts + coarse_pts + 2 * copied);
   return CXP_OK;
@@ -2302,6 +2480,7 @@ int cxp_worklist_append_boundaries(cxp_worklist wl, const cxp_level *level,
   wl->post_runs.insert(wl->post_runs.end(), runs.begin(), runs.end());
   wl->post_chunks.insert(wl->post_chunks.end(), chunks.begin(), chunks.end());
   wl->uploaded = false;
+  touch(wl);
   return CXP_OK;
 }
 
@@ -2310,6 +2489,7 @@ int cxp_worklist_set_per_group(cxp_worklist wl, int per_group) {
     return CXP_ERR_ARG;
   wl->per_group = per_group;
   wl->uploaded = false;
+  touch(wl);
   return CXP_OK;
 }
 
@@ -2321,6 +2501,10 @@ int cxp_worklist_upload(cxp_worklist wl, void *stream) {
     for (const auto &p : wl->patches)
       if (p.ncomp > 16 && !op_is_separable(p.op->d, true))
         return CXP_ERR_NCOMP;
+  touch(wl);
+  // a pending launch may still read the buffers about to be rewritten
+  if (const int e = list_drain(wl))
+    return e;
   if (const int e = place_gathered(wl))
     return e;
   build_tiles(wl);
@@ -2353,10 +2537,13 @@ int cxp_worklist_poison(cxp_worklist wl, double value, void *stream) {
     return CXP_ERR_STATE;
   if (wl->poison_chunks.empty())
     return CXP_OK;
-  hipLaunchKernelGGL(k_poison, dim3((unsigned)wl->poison_chunks.size()),
-                     dim3(256), 0, (hipStream_t)stream, wl->d_poison_runs,
-                     wl->d_poison_chunks, value);
-  return hip_status(hipGetLastError());
+  hipStream_t s = (hipStream_t)stream;
+  return list_launch(wl, s, [&] {
+    hipLaunchKernelGGL(k_poison, dim3((unsigned)wl->poison_chunks.size()),
+                       dim3(256), 0, s, wl->d_poison_runs,
+                       wl->d_poison_chunks, value);
+    return hip_status(hipGetLastError());
+  });
 }
 
 int cxp_prolongate_batched(cxp_worklist wl, void *stream) {
@@ -2364,33 +2551,39 @@ int cxp_prolongate_batched(cxp_worklist wl, void *stream) {
     return CXP_ERR_ARG;
   if (!wl->uploaded)
     return CXP_ERR_STATE;
-  if (const int e = launch_gather(wl, (hipStream_t)stream))
-    return e;
-  if (const int e = launch_batched(wl, (hipStream_t)stream))
-    return e;
-  return launch_post_bc(wl, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  return list_launch(wl, s, [&] {
+    if (const int e = launch_gather(wl, s))
+      return e;
+    if (const int e = launch_batched(wl, s))
+      return e;
+    return launch_post_bc(wl, s);
+  });
 }
 
 int cxp_prolongate_per_patch(cxp_worklist wl, void *stream) {
   if (!wl)
     return CXP_ERR_ARG;
-  if (!wl->gather_runs.empty() || !wl->post_runs.empty() ||
-      !wl->bc_runs.empty()) {
-    // gathers, copies and boundary runs live on the device only after an
-    // upload
-    if (!wl->uploaded)
-      return CXP_ERR_STATE;
-    if (const int e = launch_gather(wl, (hipStream_t)stream))
-      return e;
-  }
-  for (const auto &p : wl->patches) {
-    const int e = prolongate_impl(p.op, &p.crse, p.crse_comp, &p.fine,
-                                  p.fine_comp, p.ncomp, p.rlo, p.rhi,
-                                  wl->per_group, (hipStream_t)stream);
-    if (e)
-      return e;
-  }
-  return launch_post_bc(wl, (hipStream_t)stream);
+  const bool staged = !wl->gather_runs.empty() || !wl->post_runs.empty() ||
+                      !wl->bc_runs.empty();
+  // gathers, copies and boundary runs live on the device only after an
+  // upload
+  if (staged && !wl->uploaded)
+    return CXP_ERR_STATE;
+  hipStream_t s = (hipStream_t)stream;
+  return list_launch(wl, s, [&] {
+    if (staged)
+      if (const int e = launch_gather(wl, s))
+        return e;
+    for (const auto &p : wl->patches) {
+      const int e = prolongate_impl(p.op, &p.crse, p.crse_comp, &p.fine,
+                                    p.fine_comp, p.ncomp, p.rlo, p.rhi,
+                                    wl->per_group, s);
+      if (e)
+        return e;
+    }
+    return launch_post_bc(wl, s);
+  });
 }
 
 // ---- graphs
@@ -2405,6 +2598,21 @@ int cxp_graph_capture_begin(void *stream) {
 int cxp_graph_capture_end(void *stream, cxp_graph *out) {
   if (!stream || !out)
     return CXP_ERR_ARG;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  unsigned long long cid = 0;
+  if (hipStreamGetCaptureInfo((hipStream_t)stream, &cs, &cid) != hipSuccess)
+    return CXP_ERR_HIP;
+  std::vector<std::pair<uint64_t, uint64_t>> lists;
+  {
+    std::lock_guard<std::mutex> lk(g_lists_mu);
+    auto it = g_capture_lists.find(cid);
+    if (it != g_capture_lists.end()) {
+      lists = std::move(it->second);
+      g_capture_lists.erase(it);
+    }
+  }
+  std::sort(lists.begin(), lists.end());
+  lists.erase(std::unique(lists.begin(), lists.end()), lists.end());
   hipGraph_t g;
   if (hipStreamEndCapture((hipStream_t)stream, &g) != hipSuccess)
     return CXP_ERR_HIP;
@@ -2413,14 +2621,38 @@ int cxp_graph_capture_end(void *stream, cxp_graph *out) {
     (void)hipGraphDestroy(g);
     return CXP_ERR_HIP;
   }
-  *out = new cxp_graph_s{g, ex};
+  *out = new cxp_graph_s{g, ex, std::move(lists)};
   return CXP_OK;
 }
 
 int cxp_graph_launch(cxp_graph g, void *stream) {
   if (!g)
     return CXP_ERR_ARG;
-  return hip_status(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  hipStream_t s = (hipStream_t)stream;
+  // every captured list must be alive and unchanged since the capture
+  std::vector<cxp_worklist_s *> wls;
+  {
+    std::lock_guard<std::mutex> lk(g_lists_mu);
+    for (const auto &r : g->lists) {
+      auto it = g_lists.find(r.first);
+      if (it == g_lists.end() || it->second->epoch != r.second)
+        return CXP_ERR_STATE;
+      wls.push_back(it->second);
+    }
+  }
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (cxp_worklist_s *wl : wls) {
+    locks.emplace_back(wl->mu);
+    if (wl->launched && wl->last_stream != s &&
+        hipStreamWaitEvent(s, wl->done, 0) != hipSuccess)
+      return CXP_ERR_HIP;
+  }
+  if (hipGraphLaunch(g->exec, s) != hipSuccess)
+    return CXP_ERR_HIP;
+  for (cxp_worklist_s *wl : wls)
+    if (const int e = list_leave(wl, s))
+      return e;
+  return CXP_OK;
 }
 
 int cxp_graph_destroy(cxp_graph g) {
@@ -2463,6 +2695,9 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
   if (!wl || reps < 1)
     return CXP_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
+  touch(wl);
+  if (const int e = list_drain(wl))
+    return e;
   if (int e = place_gathered(wl))
     return e;
   // separable, non-thin keys of the list, grouped by size class
@@ -2560,6 +2795,7 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   wl->uploaded = false;
+  touch(wl);
   if (status)
     return status;
   return cxp_worklist_upload(wl, stream);

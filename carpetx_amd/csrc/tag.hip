@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(64 * CXP_EST_WAVES)
           if (ix < m.n[0]) {
             const bool set = v[u] >= threshold;
             if (tags)
-              tags[row + ix] = set ? 1 : 0;
+              tags[row + ix] = set ? CXP_TAG_SET : CXP_TAG_CLEAR;
             if (set) {
               ++cnt;
               bl[0] = min(bl[0], ix);

@@ -126,6 +126,7 @@ extern "C" int cxp_wave_rk4_stage_adjoint(int stage, const double *dst_bar,
                                           void *stream) {
   if (stage < 1 || stage > 4 || !dst_bar || !y_bar ||
       (stage < 4 && !acc_bar) || ghosts < 0 || (periodic && ghosts != 0) ||
+      (!periodic && ghosts < 2) ||
       nx - 2 * ghosts < 1 || ny - 2 * ghosts < 1 || nz - 2 * ghosts < 1 ||
       (periodic && (nx < 3 || ny < 3 || nz < 3)))
     return CXP_ERR_ARG;
@@ -141,7 +142,8 @@ extern "C" int cxp_wave_rk4_stage(int stage, const double *y, double *dst,
                                   int ny, int nz, int ghosts, int periodic,
                                   double c, double a, void *stream) {
   if (stage < 1 || stage > 4 || !y || !dst || !base || !acc || ghosts < 0 ||
-      (periodic && ghosts != 0) || nx - 2 * ghosts < 1 ||
+      (periodic && ghosts != 0) || (!periodic && ghosts < 2) ||
+      nx - 2 * ghosts < 1 ||
       ny - 2 * ghosts < 1 || nz - 2 * ghosts < 1 ||
       (periodic && (nx < 3 || ny < 3 || nz < 3)) || y == dst)
     return CXP_ERR_ARG;

@@ -432,7 +432,8 @@ inline void boxinbox_setup(const std::vector<BoxInBoxRegion> &regions,
 }
 
 // ErrorEst's tags over `region` (err >= threshold): tags (device, one byte
-// per cell of err's box, or null) and box_out (device, 7 ints, or null).
+// per cell of err's box, CXP_TAG_SET = amrex::TagBox::SET or CXP_TAG_CLEAR,
+// or null) and box_out (device, 7 ints, or null).
 inline void error_est(const Fab &err, const Box &region, double threshold,
                       signed char *tags, int *box_out, void *stream = nullptr) {
   check(cxp_error_est(err.data, err.box.lo.data(), err.box.hi.data(),

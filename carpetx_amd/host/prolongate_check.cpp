@@ -371,9 +371,10 @@ int main() {
     int64_t nt = 0, ne = 0;
     bool agree = true;
     for (size_t i = 0; i < ht.size(); ++i) {
-      nt += ht[i];
+      nt += ht[i] == CXP_TAG_SET;
       ne += e[i] == 1.0;
-      agree = agree && (ht[i] == 1) == (e[i] == 1.0) && (e[i] == 0.0 || e[i] == 1.0);
+      agree = agree && (ht[i] == CXP_TAG_SET || ht[i] == CXP_TAG_CLEAR) &&
+              (ht[i] == CXP_TAG_SET) == (e[i] == 1.0) && (e[i] == 0.0 || e[i] == 1.0);
     }
     std::printf("      tags: %lld set, box [%d,%d]^3\n", (long long)sum.count,
                 sum.box.lo[0], sum.box.hi[0]);

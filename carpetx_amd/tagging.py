@@ -19,6 +19,7 @@ from .interp import Box, FArrayBox, _stream
 
 MAX_REGIONS = 3
 SHAPES = {"sphere": 0, "cube": 1}
+TAG_CLEAR, TAG_SET = 0, 2  # amrex::TagBox::CLEAR / SET (include/cxp_tag.h)
 MAX_LEVELS = 30  # max_num_levels, boxinbox.cxx:17
 
 
@@ -86,7 +87,8 @@ def error_est(err, threshold, region=None, tags=None, box_out=None, stream=None,
               read=True):
     """ErrorEst's tags over `region` (default: err's whole box).
 
-    tags: optional int8 tensor of err's shape, set to 1 / 0 in `region`.
+    tags: optional int8 tensor of err's shape, set to TAG_SET (2) / TAG_CLEAR
+    (0), amrex::TagBox::SET / CLEAR, in `region`.
     box_out: optional int32 device tensor of 7 entries; by default one is
     allocated.  Returns (count, Box of the set cells or None); reading them
     synchronises with the stream.  With read=False nothing is read back and

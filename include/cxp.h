@@ -99,8 +99,13 @@ int cxp_restrict(const int indextype[3], const cxp_fab *fine, int fine_comp,
  * forward input crse_in (required; CXP_ERR_ARG without it): each fine
  * point's branch (ENO shift, fallback decision) is recomputed from crse_in
  * with the forward's decision code, and the transpose of that branch's
- * stencil is applied; per_group selects the per-group decisions.  MINMOD
- * operators return CXP_ERR_UNSUPPORTED. */
+ * stencil is applied; per_group selects the per-group decisions.  Their
+ * branch codes live in a buffer of the calling stream (per device), grown
+ * on demand with stream-ordered allocation and reused by later calls on
+ * that stream; so these calls cannot be captured in a graph
+ * (CXP_ERR_STATE on a capturing stream): capture cxp_prolongate_adjoint_
+ * recorded with a caller-owned branch record instead.  MINMOD operators
+ * return CXP_ERR_UNSUPPORTED. */
 int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
                            const cxp_fab *crse_bar, int crse_comp,
                            const cxp_fab *fine_bar, int fine_comp, int ncomp,
@@ -285,8 +290,11 @@ int cxp_worklist_stats(cxp_worklist wl, int64_t *npatches, int64_t *ntiles,
  * point-kernel launch per ENO/fallback patch; first the gather of the level
  * fills' coarse patches and old-level copies, then the coarse patches'
  * boundary values; last the levels' own boundary conditions).  Launches of
- * one list must not overlap: the list's dynamic-tail claim counters and
- * gathered patches are its own. */
+ * one list never overlap: the list's dynamic-tail claim counters and
+ * gathered patches serve one launch at a time, so a launch (of this, of
+ * cxp_prolongate_per_patch or cxp_worklist_poison, or of a graph that
+ * captured them) on another stream than the list's previous launch first
+ * makes that stream wait for the previous launch (hipStreamWaitEvent). */
 int cxp_prolongate_batched(cxp_worklist wl, void *stream);
 /* The plain path over the same list: one cxp_prolongate launch per patch. */
 int cxp_prolongate_per_patch(cxp_worklist wl, void *stream);
@@ -298,7 +306,12 @@ int cxp_prolongate_per_patch(cxp_worklist wl, void *stream);
 #define CXP_POISON_BITS 0xfff80000deadbeefULL
 int cxp_worklist_poison(cxp_worklist wl, double value, void *stream);
 
-/* ---- hipGraph capture of a fill sequence. */
+/* ---- hipGraph capture of a fill sequence.  A graph keeps the device
+ * buffers and tiles of every list it launched.  Any append, clear,
+ * cxp_worklist_set_per_group, upload, autotune or destroy of such a list
+ * invalidates the graph: cxp_graph_launch then returns CXP_ERR_STATE and a
+ * new capture is needed (after a regrid, say).  Lists launched inside a
+ * capture begun by other means (torch.cuda.graph) are not tracked. */
 typedef struct cxp_graph_s *cxp_graph;
 int cxp_graph_capture_begin(void *stream);
 int cxp_graph_capture_end(void *stream, cxp_graph *out);

@@ -28,6 +28,8 @@ extern "C" {
 #define CXP_BIB_MAX_REGIONS 3 /* max_num_regions, boxinbox.cxx:16 */
 #define CXP_BIB_SPHERE 0      /* shape_t::sphere, boxinbox.cxx:19 */
 #define CXP_BIB_CUBE 1        /* shape_t::cube */
+#define CXP_TAG_CLEAR 0       /* amrex::TagBox::CLEAR */
+#define CXP_TAG_SET 2         /* amrex::TagBox::SET (TagVal {CLEAR, BUF, SET}) */
 
 /* One refined region (the parameters active_N, shape_N, num_levels_N,
  * position_{x,y,z}_N of BoxInBox/param.ccl).  radius[d] is the region's
@@ -66,8 +68,11 @@ int cxp_boxinbox_tag(const cxp_bib_region *regions, int nregions, int level,
 
 /* ErrorEst's tags over the interior [lo, hi] of one box: a cell is set
  * where err >= threshold (driver.cxx:1064-1066).
- *   tags     (optional) one byte per cell in err's FAB layout: 1 (set) or 0
- *            (clear), amrex::TagBox::SET / CLEAR; only [lo, hi] is written;
+ *   tags     (optional) one byte per cell in err's FAB layout:
+ *            CXP_TAG_SET (2) or CXP_TAG_CLEAR (0), the values of
+ *            amrex::TagBox::SET / CLEAR (enum TagVal {CLEAR = 0, BUF, SET}),
+ *            so AMReX's error-buffer step grows around them; only [lo, hi]
+ *            is written;
  *   box_out  (optional) 7 device ints: the number of set cells, then lo[3]
  *            and hi[3] of their bounding box (lo > hi when none is set).
  *            The call initialises them itself (with a kernel, no memset).

@@ -22,7 +22,8 @@ extern "C" {
  * [ghosts, n - ghosts) per direction (periodic: ghosts = 0, neighbours
  * wrap).  k = (rho, c L u) with L the 4th-order centred Laplacian
  * (-u[-2] + 16 u[-1] - 30 u + 16 u[+1] - u[+2] per direction, c =
- * 1 / (12 h^2)); then
+ * 1 / (12 h^2)); the Laplacian reads two points each way, so a
+ * non-periodic box needs ghosts >= 2 (CXP_ERR_ARG otherwise); then
  *   stage 1:    acc = k;          dst = base + a k
  *   stage 2, 3: acc = acc + 2 k;  dst = base + a k
  *   stage 4:                      dst = base + a (acc + k)
@@ -38,7 +39,7 @@ int cxp_wave_rk4_stage(int stage, const double *y, double *dst,
  * interior points (w = 1 for stage 1, 2 for stages 2 and 3; stage 4: k_bar
  * = a dst_bar, acc_bar unused).  y_bar covers the whole box (ghosts
  * included); the gradients of base and of the incoming acc are pointwise
- * and left to the caller. */
+ * and left to the caller.  The same argument rules as the forward. */
 int cxp_wave_rk4_stage_adjoint(int stage, const double *dst_bar,
                                const double *acc_bar, double *y_bar, int nx,
                                int ny, int nz, int ghosts, int periodic,

@@ -65,9 +65,10 @@ def boxinbox_tag(regions, level, x0, dx, lo, hi):
 
 
 def error_est(err, lo, threshold):
-    """Tags (int8, 1 = set) of an array over the box from lo, and the number
-    and bounding box (lo, hi; None when empty) of the set cells."""
-    tags = (err >= threshold).astype(np.int8)
+    """Tags (int8: amrex::TagBox::SET = 2, CLEAR = 0) of an array over the box
+    from lo, and the number and bounding box (lo, hi; None when empty) of the
+    set cells (driver.cxx:1064-1066)."""
+    tags = np.where(err >= threshold, 2, 0).astype(np.int8)
     idx = np.nonzero(tags)
     count = int(idx[0].size)
     if count == 0:

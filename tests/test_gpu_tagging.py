@@ -2,7 +2,8 @@
 (cxp_error_est) against the oracle (oracle/tagging.py), bit for bit.
 
 Expected values: the oracle evaluates the same IEEE expressions in numpy on
-the same parameters; the tags are 0 / 1, so any difference in a decision
+the same parameters; the tags are amrex::TagBox's CLEAR = 0 / SET = 2, so any
+difference in a decision
 shows.  FAB points outside the tagged box hold a sentinel the kernels must
 not touch."""
 import numpy as np
@@ -161,6 +162,8 @@ def test_error_est_tags_without_result_words():
                for d in (2, 1, 0))
     want, _, _ = ot.error_est(data[0].numpy()[sl], sub.lo, 0.25)
     assert np.array_equal(t[sl], want)
+    # the byte values are amrex::TagBox::SET (2) and CLEAR (0)
+    assert set(np.unique(t[sl]).tolist()) == {0, 2}
     outside = np.ones(t.shape, dtype=bool)
     outside[sl] = False
     assert np.all(t[outside] == 7)

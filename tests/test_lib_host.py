@@ -524,12 +524,17 @@ def test_wave_library_exports_its_header():
     assert W.cxp_wave_rk4_stage(1, fake, fake, fake, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
     assert W.cxp_wave_rk4_stage(1, fake, fake + 8, fake, fake, 8, 8, 8, 2, 1, 1.0, 1.0, None) == -3
     assert W.cxp_wave_rk4_stage(1, fake, fake + 8, fake, fake, 4, 8, 8, 2, 0, 1.0, 1.0, None) == -3
+    # the Laplacian reads two points each way: a non-periodic box needs 2 ghosts
+    for g in (0, 1):
+        assert W.cxp_wave_rk4_stage(1, fake, fake + 8, fake, fake, 8, 8, 8, g, 0, 1.0, 1.0, None) == -3
     adj = W.cxp_wave_rk4_stage_adjoint
     adj.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int] * 5 + \
         [ctypes.c_double] * 2 + [ctypes.c_void_p]
     assert adj(0, fake, fake, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
     assert adj(2, fake, None, fake, 8, 8, 8, 2, 0, 1.0, 1.0, None) == -3
     assert adj(4, fake, None, fake, 8, 8, 8, 1, 1, 1.0, 1.0, None) == -3
+    for g in (0, 1):
+        assert adj(4, fake, None, fake, 8, 8, 8, g, 0, 1.0, 1.0, None) == -3
 
 
 def test_level_append_edge_cases():

@@ -98,7 +98,8 @@ def test_error_est_threshold_count_and_bounding_box():
     err = rng.uniform(0, 1, (4, 5, 6))
     err[0, 0, 0] = 0.75  # exactly at the threshold: set (>=)
     tags, count, box = error_est(err, lo, 0.75)
-    assert tags.dtype == np.int8 and tags[0, 0, 0] == 1
+    assert tags.dtype == np.int8 and tags[0, 0, 0] == 2  # amrex::TagBox::SET
+    assert set(np.unique(tags)) <= {0, 2}
     assert count == int((err >= 0.75).sum())
     pts = [(i + lo[0], j + lo[1], k + lo[2]) for k, j, i in zip(*np.nonzero(err >= 0.75))]
     assert box == (tuple(min(p[d] for p in pts) for d in range(3)),
