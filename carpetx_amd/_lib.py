@@ -27,7 +27,7 @@ STATUS = {
     0: "CXP_OK", -1: "CXP_ERR_TYPE", -2: "CXP_ERR_ORDER", -3: "CXP_ERR_ARG",
     -4: "CXP_ERR_REGION", -5: "CXP_ERR_COARSE", -6: "CXP_ERR_NCOMP",
     -7: "CXP_ERR_RATIO", -8: "CXP_ERR_HIP", -9: "CXP_ERR_COMP",
-    -10: "CXP_ERR_STATE", -11: "CXP_ERR_UNSUPPORTED",
+    -10: "CXP_ERR_STATE", -11: "CXP_ERR_UNSUPPORTED", -12: "CXP_ERR_SCRATCH",
 }
 
 
@@ -77,6 +77,10 @@ _SIGS = {
                                        ctypes.POINTER(CxpFab), c_int,
                                        ctypes.POINTER(CxpFab), c_int, c_int,
                                        P_INT, P_INT, c_int, c_void_p]),
+    "cxp_prolongate_adjoint_noalloc": (c_int, [c_void_p, ctypes.POINTER(CxpFab),
+                                               ctypes.POINTER(CxpFab), c_int,
+                                               ctypes.POINTER(CxpFab), c_int, c_int,
+                                               P_INT, P_INT, c_int, c_void_p]),
     "cxp_branch_bytes": (c_int, [c_void_p, P_INT, P_INT, c_int, c_int, P_INT64]),
     "cxp_prolongate_record": (c_int, [c_void_p, ctypes.POINTER(CxpFab), c_int,
                                       ctypes.POINTER(CxpFab), c_int, c_int, P_INT,

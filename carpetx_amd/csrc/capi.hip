@@ -19,6 +19,14 @@
 
 using namespace cxp;
 
+// Scratch of cxp_prolongate_adjoint_noalloc: the branch codes of the
+// recomputing adjoint in a file-scope device array, one copy per device,
+// held until the process ends.  In host code `scratch` names a host-side
+// stand-in, never a device address: the address comes from
+// hipGetSymbolAddress (scratch_address below), and only that address is
+// passed to kernels.
+static __device__ __align__(256) unsigned char scratch[CXP_NOALLOC_SCRATCH_BYTES];
+
 struct cxp_op_s {
   OpDesc d;
 };
@@ -462,6 +470,27 @@ int pooled_codes(hipStream_t s, size_t nbytes, unsigned char **out) {
     cp.cap = cap;
   }
   *out = cp.p;
+  return CXP_OK;
+}
+
+// The address of the scratch array (top of this file) on the current
+// device, looked up once per device with that device current.
+std::mutex g_scratch_mu;
+std::map<int, unsigned char *> g_scratch_ptr;  // device -> address of scratch
+
+int scratch_address(unsigned char **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return CXP_ERR_HIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return CXP_ERR_HIP;
+    it = g_scratch_ptr.emplace(dev, (unsigned char *)p).first;
+  }
+  *out = it->second;
   return CXP_OK;
 }
 
@@ -1218,6 +1247,8 @@ const char *cxp_status_string(int st) {
   case CXP_ERR_COMP: return "component range outside the FAB";
   case CXP_ERR_STATE: return "object used in the wrong state";
   case CXP_ERR_UNSUPPORTED: return "operation not supported for this operator";
+  case CXP_ERR_SCRATCH:
+    return "call needs more than the no-allocation scratch array holds";
   }
   return "unknown status";
 }
@@ -1377,6 +1408,41 @@ int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
   return adjoint_launch(od, crse_in, crse_bar, crse_comp, fine_bar,
                         fine_comp, ncomp, rlo, rhi, per_group,
                         (hipStream_t)stream);
+}
+
+int cxp_prolongate_adjoint_noalloc(cxp_op op, const cxp_fab *crse_in,
+                                   const cxp_fab *crse_bar, int crse_comp,
+                                   const cxp_fab *fine_bar, int fine_comp,
+                                   int ncomp, const int rlo[3],
+                                   const int rhi[3], int per_group,
+                                   void *stream) {
+  const int e = check_prolong(op, crse_bar, crse_comp, fine_bar, fine_comp,
+                              ncomp, rlo, rhi, per_group);
+  if (e)
+    return e;
+  const OpDesc &od = op->d;
+  bool minmod = false;
+  for (int d = 0; d < 3; ++d)
+    minmod = minmod || od.sid[d] == SID_MINMOD;
+  if (minmod || !(op_has_eno(od) || op_fallback_per_var(od) ||
+                  op_fallback_per_group(od)))
+    // the linear adjoints use no scratch (MINMOD: CXP_ERR_UNSUPPORTED)
+    return adjoint_launch(od, crse_in, crse_bar, crse_comp, fine_bar,
+                          fine_comp, ncomp, rlo, rhi, per_group,
+                          (hipStream_t)stream);
+  if (const int e2 = adjoint_point_check(op, crse_in, crse_comp, fine_bar,
+                                         fine_comp, ncomp, rlo, rhi,
+                                         per_group))
+    return e2;
+  if (adjoint_code_bytes(ncomp, rlo, rhi, per_group) >
+      (size_t)CXP_NOALLOC_SCRATCH_BYTES)
+    return CXP_ERR_SCRATCH;
+  unsigned char *codes = nullptr;
+  if (const int e2 = scratch_address(&codes))
+    return e2;
+  return adjoint_point_run(op, crse_in, crse_bar, crse_comp, fine_bar,
+                           fine_comp, ncomp, rlo, rhi, per_group, codes,
+                           (hipStream_t)stream);
 }
 
 // Operators whose forward takes a branch per fine point (ENO shift, linear

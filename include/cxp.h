@@ -35,6 +35,7 @@ extern "C" {
 #define CXP_ERR_COMP -9        /* component range outside the FAB */
 #define CXP_ERR_STATE -10      /* worklist / graph used in the wrong state */
 #define CXP_ERR_UNSUPPORTED -11 /* operator has no adjoint of this kind */
+#define CXP_ERR_SCRATCH -12    /* exceeds the no-allocation scratch array */
 
 /* Centring and interpolation codes (prolongate_3d_rf2.hxx:15-31). */
 #define CXP_VC 0
@@ -111,6 +112,26 @@ int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
                            const cxp_fab *fine_bar, int fine_comp, int ncomp,
                            const int region_lo[3], const int region_hi[3],
                            int per_group, void *stream);
+
+/* cxp_prolongate_adjoint that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results (bit for bit).  The ENO and
+ * linear-fallback operators keep their branch codes (cxp_branch_bytes of
+ * the call) in a static device array of CXP_NOALLOC_SCRATCH_BYTES bytes,
+ * one per device, held until the process ends; a larger call returns
+ * CXP_ERR_SCRATCH before any device work.  The other operators use no
+ * scratch.  Every call on one device shares that device's array, so calls
+ * that overlap on one device (on two streams, or a graph replay beside
+ * another call) race; the array keeps the last call's codes, which every
+ * call rewrites in full before reading. */
+#define CXP_NOALLOC_SCRATCH_BYTES (16 << 20)
+int cxp_prolongate_adjoint_noalloc(cxp_op op, const cxp_fab *crse_in,
+                                   const cxp_fab *crse_bar, int crse_comp,
+                                   const cxp_fab *fine_bar, int fine_comp,
+                                   int ncomp, const int region_lo[3],
+                                   const int region_hi[3], int per_group,
+                                   void *stream);
 
 /* Bytes of the branch record of a forward over [region_lo, region_hi]
  * (no reference counterpart): one byte per fine point (per component in
