@@ -311,10 +311,9 @@ def bench_regrid(args, stream, log):
 
 def bench_adjoint(stream, log):
     """96^3 -> 192^3 VC 5th-order prolongation: forward vs adjoint (per-box
-    launches on one stream), plus the CCC minmod forward (its adjoint is not
-    available) and the CCC ENO forward and adjoint (linearised at the
-    forward input: recomputing the branches, and from the forward's branch
-    record)."""
+    launches on one stream), plus the CCC minmod and CCC ENO forwards and
+    adjoints (linearised at the forward input: recomputing the branches, and
+    from the forward's branch record)."""
     import torch
     from carpetx_amd.interp import Box, FArrayBox, get_interpolator
     out = {}

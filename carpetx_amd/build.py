@@ -78,6 +78,21 @@ def build_host_check():
     os.replace(tmp, HOST_CHECK)
 
 
+MINMOD_HOST = os.path.join(HERE, "host", "libminmod_adjoint_host.so")
+
+
+def build_minmod_host():
+    """The MINMOD adjoint's record and gather routines built for the host
+    with bounds-checked accessors (host/minmod_adjoint_host.cpp): the CPU
+    check of the kernels' indexing, loaded by the tests only."""
+    tmp = MINMOD_HOST + ".tmp"
+    run([HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17",
+         "-ffp-contract=off", "-fPIC", "-shared",
+         "-Wno-unused-command-line-argument",
+         os.path.join(HERE, "host", "minmod_adjoint_host.cpp"), "-o", tmp])
+    os.replace(tmp, MINMOD_HOST)
+
+
 def build_oracle():
     run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"])
 
@@ -87,6 +102,7 @@ def main():
     build_lib()
     build_wave_lib()
     build_host_check()
+    build_minmod_host()
     print("built %s in %.1fs" % (LIB, time.time() - t0))
 
 

@@ -568,6 +568,35 @@ int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
                            fine_comp, ncomp, rlo, rhi, per_group, codes, s);
 }
 
+// The MINMOD adjoint (record the branch codes from crse_in, then gather)
+// with its codes from the stream's pool (or from `codes`, when given, of
+// at least mm_code_bytes).  Refuses capture like adjoint_point_launch when
+// it takes the pool.
+int minmod_adjoint_launch(const cxp_op_s *op, const cxp_fab *crse_in,
+                          const cxp_fab *crse_bar, int crse_comp,
+                          const cxp_fab *fine_bar, int fine_comp, int ncomp,
+                          const int rlo[3], const int rhi[3], int per_group,
+                          hipStream_t s, unsigned char *codes = nullptr) {
+  if (const int e = adjoint_point_check(op, crse_in, crse_comp, fine_bar,
+                                        fine_comp, ncomp, rlo, rhi, per_group))
+    return e;
+  const long long nbytes = mm_code_bytes(op->d, rlo, rhi, ncomp);
+  if (!codes && nbytes > 0) {
+    bool capturing = false;
+    if (const int e = stream_capturing(s, capturing))
+      return e;
+    if (capturing)
+      return CXP_ERR_STATE;
+    if (const int e = pooled_codes(s, (size_t)nbytes, &codes))
+      return e;
+  }
+  if (const int e = mm_record_launch(op->d, crse_in, crse_comp, ncomp, rlo,
+                                     rhi, codes, s))
+    return e;
+  return mm_gather_launch(op->d, crse_bar, crse_comp, fine_bar, fine_comp,
+                          ncomp, rlo, rhi, codes, s);
+}
+
 } // namespace
 
 // ===========================================================================
@@ -1397,11 +1426,11 @@ int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
   if (e)
     return e;
   const OpDesc &od = op->d;
-  bool minmod = false;
-  for (int d = 0; d < 3; ++d)
-    minmod = minmod || od.sid[d] == SID_MINMOD;
-  if (!minmod && (op_has_eno(od) || op_fallback_per_var(od) ||
-                  op_fallback_per_group(od)))
+  if (op_has_minmod(od))
+    return minmod_adjoint_launch(op, crse_in, crse_bar, crse_comp, fine_bar,
+                                 fine_comp, ncomp, rlo, rhi, per_group,
+                                 (hipStream_t)stream);
+  if (op_has_eno(od) || op_fallback_per_var(od) || op_fallback_per_group(od))
     return adjoint_point_launch(op, crse_in, crse_bar, crse_comp, fine_bar,
                                 fine_comp, ncomp, rlo, rhi, per_group,
                                 (hipStream_t)stream);
@@ -1421,12 +1450,23 @@ int cxp_prolongate_adjoint_noalloc(cxp_op op, const cxp_fab *crse_in,
   if (e)
     return e;
   const OpDesc &od = op->d;
-  bool minmod = false;
-  for (int d = 0; d < 3; ++d)
-    minmod = minmod || od.sid[d] == SID_MINMOD;
-  if (minmod || !(op_has_eno(od) || op_fallback_per_var(od) ||
-                  op_fallback_per_group(od)))
-    // the linear adjoints use no scratch (MINMOD: CXP_ERR_UNSUPPORTED)
+  if (op_has_minmod(od)) {
+    if (const int e2 = adjoint_point_check(op, crse_in, crse_comp, fine_bar,
+                                           fine_comp, ncomp, rlo, rhi,
+                                           per_group))
+      return e2;
+    if (mm_code_bytes(od, rlo, rhi, ncomp) > CXP_NOALLOC_SCRATCH_BYTES)
+      return CXP_ERR_SCRATCH;
+    unsigned char *codes = nullptr;
+    if (const int e2 = scratch_address(&codes))
+      return e2;
+    return minmod_adjoint_launch(op, crse_in, crse_bar, crse_comp, fine_bar,
+                                 fine_comp, ncomp, rlo, rhi, per_group,
+                                 (hipStream_t)stream, codes);
+  }
+  if (!(op_has_eno(od) || op_fallback_per_var(od) ||
+        op_fallback_per_group(od)))
+    // the linear adjoints use no scratch
     return adjoint_launch(od, crse_in, crse_bar, crse_comp, fine_bar,
                           fine_comp, ncomp, rlo, rhi, per_group,
                           (hipStream_t)stream);
@@ -1446,11 +1486,10 @@ int cxp_prolongate_adjoint_noalloc(cxp_op op, const cxp_fab *crse_in,
 }
 
 // Operators whose forward takes a branch per fine point (ENO shift, linear
-// fallback) and has a device adjoint; MINMOD is excluded (no adjoint).
+// fallback); MINMOD operators record their per-pass codes instead.
 bool op_records_branches(const OpDesc &od, int per_group) {
-  for (int d = 0; d < 3; ++d)
-    if (od.sid[d] == SID_MINMOD)
-      return false;
+  if (op_has_minmod(od))
+    return false;
   return !op_is_separable(od, per_group != 0);
 }
 
@@ -1464,8 +1503,12 @@ int cxp_branch_bytes(cxp_op op, const int rlo[3], const int rhi[3], int ncomp,
       return CXP_ERR_ARG;
     n *= rhi[d] - rlo[d] + 1;
   }
-  *bytes = op_records_branches(op->d, per_group) ? n * (per_group ? 1 : ncomp)
-                                                 : 0;
+  if (op_has_minmod(op->d))
+    *bytes = mm_code_bytes(op->d, rlo, rhi, ncomp);
+  else
+    *bytes = op_records_branches(op->d, per_group)
+                 ? n * (per_group ? 1 : ncomp)
+                 : 0;
   return CXP_OK;
 }
 
@@ -1477,6 +1520,16 @@ int cxp_prolongate_record(cxp_op op, const cxp_fab *crse, int crse_comp,
                               rhi, per_group);
   if (e)
     return e;
+  if (op_has_minmod(op->d) && mm_code_bytes(op->d, rlo, rhi, ncomp) > 0) {
+    if (!branches)
+      return CXP_ERR_ARG;
+    if (const int e2 = prolongate_impl(op, crse, crse_comp, fine, fine_comp,
+                                       ncomp, rlo, rhi, per_group,
+                                       (hipStream_t)stream))
+      return e2;
+    return mm_record_launch(op->d, crse, crse_comp, ncomp, rlo, rhi, branches,
+                            (hipStream_t)stream);
+  }
   if (!op_records_branches(op->d, per_group))
     return prolongate_impl(op, crse, crse_comp, fine, fine_comp, ncomp, rlo,
                            rhi, per_group, (hipStream_t)stream);
@@ -1498,9 +1551,12 @@ int cxp_prolongate_adjoint_recorded(cxp_op op, const unsigned char *branches,
   if (e)
     return e;
   const OpDesc &od = op->d;
-  for (int d = 0; d < 3; ++d)
-    if (od.sid[d] == SID_MINMOD)
-      return CXP_ERR_UNSUPPORTED;
+  if (op_has_minmod(od) && mm_code_bytes(od, rlo, rhi, ncomp) > 0) {
+    if (!branches)
+      return CXP_ERR_ARG;
+    return mm_gather_launch(od, crse_bar, crse_comp, fine_bar, fine_comp,
+                            ncomp, rlo, rhi, branches, (hipStream_t)stream);
+  }
   if (!op_records_branches(od, per_group))
     return adjoint_launch(od, nullptr, crse_bar, crse_comp, fine_bar,
                           fine_comp, ncomp, rlo, rhi, per_group,

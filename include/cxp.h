@@ -100,13 +100,17 @@ int cxp_restrict(const int indextype[3], const cxp_fab *fine, int fine_comp,
  * forward input crse_in (required; CXP_ERR_ARG without it): each fine
  * point's branch (ENO shift, fallback decision) is recomputed from crse_in
  * with the forward's decision code, and the transpose of that branch's
- * stencil is applied; per_group selects the per-group decisions.  Their
- * branch codes live in a buffer of the calling stream (per device), grown
- * on demand with stream-ordered allocation and reused by later calls on
- * that stream; so these calls cannot be captured in a graph
- * (CXP_ERR_STATE on a capturing stream): capture cxp_prolongate_adjoint_
- * recorded with a caller-owned branch record instead.  MINMOD operators
- * return CXP_ERR_UNSUPPORTED. */
+ * stencil is applied; per_group selects the per-group decisions.  MINMOD
+ * operators (impl.hxx:846-878) are linearised at crse_in too (required):
+ * each MINMOD application of the x, y and z passes takes one of its three
+ * branches (no slope, the minus or the plus difference), recorded per point
+ * the pass reads, and the chain of the branches' weights is gathered per
+ * coarse point.  The branch codes of both kinds live in a buffer of the
+ * calling stream (per device), grown on demand with stream-ordered
+ * allocation and reused by later calls on that stream; so these calls
+ * cannot be captured in a graph (CXP_ERR_STATE on a capturing stream):
+ * capture cxp_prolongate_adjoint_recorded with a caller-owned branch
+ * record, or cxp_prolongate_adjoint_noalloc, instead. */
 int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
                            const cxp_fab *crse_bar, int crse_comp,
                            const cxp_fab *fine_bar, int fine_comp, int ncomp,
@@ -116,9 +120,10 @@ int cxp_prolongate_adjoint(cxp_op op, const cxp_fab *crse_in,
 /* cxp_prolongate_adjoint that allocates nothing.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under graph capture, tight serving loops, harnesses that forbid
- * hipMalloc).  Same arguments and results (bit for bit).  The ENO and
- * linear-fallback operators keep their branch codes (cxp_branch_bytes of
- * the call) in a static device array of CXP_NOALLOC_SCRATCH_BYTES bytes,
+ * hipMalloc).  Same arguments and results (bit for bit).  The ENO,
+ * linear-fallback and MINMOD operators keep their branch codes
+ * (cxp_branch_bytes of the call) in a static device array of
+ * CXP_NOALLOC_SCRATCH_BYTES bytes,
  * one per device, held until the process ends; a larger call returns
  * CXP_ERR_SCRATCH before any device work.  The other operators use no
  * scratch.  Every call on one device shares that device's array, so calls
@@ -135,7 +140,9 @@ int cxp_prolongate_adjoint_noalloc(cxp_op op, const cxp_fab *crse_in,
 
 /* Bytes of the branch record of a forward over [region_lo, region_hi]
  * (no reference counterpart): one byte per fine point (per component in
- * per-var mode) for ENO and linear-fallback operators, 0 for the others. */
+ * per-var mode) for ENO and linear-fallback operators; for MINMOD
+ * operators one byte per point each MINMOD pass reads (about 7/8 per fine
+ * point for CCC), per component; 0 for the linear operators. */
 int cxp_branch_bytes(cxp_op op, const int region_lo[3],
                      const int region_hi[3], int ncomp, int per_group,
                      int64_t *bytes);
@@ -151,7 +158,7 @@ int cxp_prolongate_record(cxp_op op, const cxp_fab *crse, int crse_comp,
 /* cxp_prolongate_adjoint from a branch record of cxp_prolongate_record over
  * the same region, components and mode (no reference counterpart): the
  * gather alone, without recomputing the branches from crse_in.  Linear
- * operators ignore branches; MINMOD returns CXP_ERR_UNSUPPORTED. */
+ * operators ignore branches. */
 int cxp_prolongate_adjoint_recorded(cxp_op op, const unsigned char *branches,
                                     const cxp_fab *crse_bar, int crse_comp,
                                     const cxp_fab *fine_bar, int fine_comp,

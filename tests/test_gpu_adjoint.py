@@ -11,7 +11,9 @@ forward operator):
   one branch: every fine point's value is linear in the coarse data for a
   fixed ENO shift and fallback decision), transposed; and the dot-product
   identity with the GPU forward's directional derivative;
-* MINMOD: switched off (see MINMOD_OFF).
+* MINMOD: the oracle's Jacobian by exact central differences, the
+  host build of the same record and gather routines bit for bit, the
+  dot-product identity with P(u) = J(u) u, and the recorded path.
 """
 import numpy as np
 import pytest
@@ -188,43 +190,62 @@ def test_branch_record_matches_recomputed_branches(fam, order, it, pg, nc):
     assert torch.equal(a1.t, a2.t)
 
 
-MINMOD_OFF = ("the MINMOD adjoint is switched off: its first implementation hung the "
-              "GPU twice in testing and was removed; the entry point returns "
-              "CXP_ERR_UNSUPPORTED until it is rewritten")
+def _dyadic(rng, shape):
+    return rng.integers(-(1 << 24), 1 << 24, size=shape).astype(np.float64) / float(1 << 24)
 
 
-@pytest.mark.skip(reason=MINMOD_OFF)
 @pytest.mark.parametrize("it", [(1, 1, 1), (1, 0, 1), (0, 1, 0)])
 def test_minmod_adjoint_against_dense_jacobian(it):
+    """The oracle's Jacobian at u by central differences, transposed.  The
+    data are dyadic (k / 2^24) and h = 2^-28, so the differences are exact
+    inside one limiter branch and no branch flips within +-h."""
+    from test_minmod_adjoint_host import oracle_jt
     rng = np.random.default_rng(4)
     rlo, rhi = (0, 1, -2), (7, 6, 4)
-    op_o = oracle.lookup("minmod", 3, it)
-    P, cb, fbox, u, v = _fabs(("minmod", 3, it), rlo, rhi, 1, rng)
-    ncoarse = int(np.prod(cb.size))
-    h = 1e-7
-
-    def fwd(x):
-        ch = oracle.Fab(cb.lo, cb.hi, 1, x.reshape((1,) + tuple(reversed(cb.size))))
-        fh = oracle.Fab(rlo, rhi, 1)
-        oracle.prolongate(op_o, ch, fh, rlo, rhi)
-        return fh.arr.reshape(-1)
-    u0 = u.reshape(-1)
-    J = np.zeros((int(np.prod(fbox.size)), ncoarse))
-    for n in range(ncoarse):
-        e = np.zeros(ncoarse)
-        e[n] = h
-        J[:, n] = (fwd(u0 + e) - fwd(u0 - e)) / (2 * h)
-    expect = (J.T @ v.reshape(-1)).reshape((1,) + tuple(reversed(cb.size)))
-    crse = FArrayBox(cb, 1, tensor=torch.from_numpy(u).cuda())
-    vb = FArrayBox(fbox, 1, tensor=torch.from_numpy(v).cuda())
+    P = get_interpolator("minmod", 3, it)
+    cb = P.CoarseBox(Box(rlo, rhi))
+    u = _dyadic(rng, tuple(reversed(cb.size)))
+    v = _dyadic(rng, tuple(reversed(Box(rlo, rhi).size)))
+    expect = oracle_jt(3, it, u, cb.lo, cb.hi, v, rlo, rhi)
+    crse = FArrayBox(cb, 1, tensor=torch.from_numpy(u[None]).cuda())
+    vb = FArrayBox(Box(rlo, rhi), 1, tensor=torch.from_numpy(v[None]).cuda())
     cbar = FArrayBox(cb, 1)
-    P.interp_adjoint(cbar, 0, vb, 0, 1, fbox, crse_in=crse)
+    P.interp_adjoint(cbar, 0, vb, 0, 1, Box(rlo, rhi), crse_in=crse)
     torch.cuda.synchronize()
-    got = cbar.t.cpu().numpy()
-    assert np.max(np.abs(got - expect)) <= 1e-7 * max(1.0, np.max(np.abs(expect)))
+    got = cbar.t.cpu().numpy()[0]
+    assert np.max(np.abs(got - expect)) <= 1e-12 * max(1.0, np.max(np.abs(expect)))
 
 
-@pytest.mark.skip(reason=MINMOD_OFF)
+@pytest.mark.parametrize("order", [1, 3, 5])
+def test_minmod_adjoint_equals_its_host_build(order):
+    """Every centring with a MINMOD direction: the device kernels give the
+    bits of the same record and gather routines built for the host with
+    bounds-checked accesses (carpetx_amd/host/minmod_adjoint_host.cpp),
+    which the CPU tests pin to the oracle's Jacobian.  VVV has no MINMOD
+    direction and runs the linear adjoint, which sums in another order."""
+    from test_minmod_adjoint_host import host_adjoint
+    rlo, rhi = (-5, 2, -3), (21, 15, 12)
+    for c in range(8):
+        it = ((c >> 2) & 1, (c >> 1) & 1, c & 1)
+        rng = np.random.default_rng(50 + 8 * order + c)
+        P = get_interpolator("minmod", order, it)
+        cb = P.CoarseBox(Box(rlo, rhi))
+        u = rng.uniform(-1, 1, tuple(reversed(cb.size)))
+        v = rng.uniform(-1, 1, tuple(reversed(Box(rlo, rhi).size)))
+        bad, want, _ = host_adjoint(order, it, u, cb.lo, cb.hi, v, rlo, rhi)
+        assert bad == 0
+        crse = FArrayBox(cb, 1, tensor=torch.from_numpy(u[None]).cuda())
+        vb = FArrayBox(Box(rlo, rhi), 1, tensor=torch.from_numpy(v[None]).cuda())
+        cbar = FArrayBox(cb, 1)
+        P.interp_adjoint(cbar, 0, vb, 0, 1, Box(rlo, rhi), crse_in=crse)
+        torch.cuda.synchronize()
+        got = cbar.t.cpu().numpy()[0]
+        if any(it):
+            assert np.array_equal(got, want), (order, it)
+        else:
+            assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want)), (order, it)
+
+
 def test_minmod_dot_product_with_linearisation():
     """<J u, v> == <u, J^T v> where J u = P(u) for a positively homogeneous
     limiter (minmod(a u) = a minmod(u) for a > 0, so P(u) = J(u) u)."""
@@ -241,6 +262,32 @@ def test_minmod_dot_product_with_linearisation():
     lhs = float((fine.t * vb.t).sum())
     rhs = float((crse.t * cbar.t).sum())
     assert abs(lhs - rhs) <= 1e-12 * float((fine.t.abs() * vb.t.abs()).sum())
+
+
+@pytest.mark.parametrize("it", [(1, 1, 1), (0, 1, 1), (1, 0, 0)])
+def test_minmod_record_and_recorded_adjoint(it):
+    """cxp_prolongate_record writes the plain forward's values and the
+    MINMOD branch codes; the adjoint from them equals the recomputing
+    adjoint bit for bit (2 components, codes per component)."""
+    rng = np.random.default_rng(12)
+    rlo, rhi = (-3, 0, 2), (26, 19, 17)
+    P, cb, fbox, u, v = _fabs(("minmod", 5, it), rlo, rhi, 2, rng)
+    crse = FArrayBox(cb, 2, tensor=torch.from_numpy(u).cuda())
+    f1, f2 = FArrayBox(fbox, 2), FArrayBox(fbox, 2)
+    P.interp(crse, 0, f1, 0, 2, fbox)
+    nb = P.branch_bytes(fbox, 2)
+    assert nb > 0
+    br = torch.full((nb,), 255, dtype=torch.uint8, device="cuda")
+    P.interp_record(crse, 0, f2, 0, 2, fbox, br)
+    torch.cuda.synchronize()
+    assert torch.equal(f1.t, f2.t)
+    assert int(br.max()) <= 2   # every code written, each one of 3 branches
+    vb = FArrayBox(fbox, 2, tensor=torch.from_numpy(v).cuda())
+    a1, a2 = FArrayBox(cb, 2), FArrayBox(cb, 2)
+    P.interp_adjoint(a1, 0, vb, 0, 2, fbox, crse_in=crse)
+    P.interp_adjoint_recorded(br, a2, 0, vb, 0, 2, fbox)
+    torch.cuda.synchronize()
+    assert torch.equal(a1.t, a2.t)
 
 
 def test_restrict_adjoint_dot_product():

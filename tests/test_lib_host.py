@@ -121,9 +121,8 @@ def test_validation_errors_without_device_work():
 
 
 def test_adjoint_argument_checks_before_device_work():
-    """MINMOD operators have no device adjoint (CXP_ERR_UNSUPPORTED); ENO
-    and fallback operators linearise at crse_in, so a call without it is
-    rejected (CXP_ERR_ARG).  Both before any device work."""
+    """ENO, fallback and MINMOD operators linearise at crse_in, so a call
+    without it is rejected (CXP_ERR_ARG) before any device work."""
     L = _lib.lib()
     fake = ctypes.c_void_p(0x1000)
 
@@ -136,7 +135,7 @@ def test_adjoint_argument_checks_before_device_work():
         return f
     crse = fab((-4, -4, -4), (20, 20, 20))
     fine = fab((0, 0, 0), (15, 15, 15))
-    for fam, order, it, want in [("minmod", 3, (1, 1, 1), -11), ("eno", 3, (1, 0, 0), -3),
+    for fam, order, it, want in [("minmod", 3, (1, 1, 1), -3), ("eno", 3, (1, 0, 0), -3),
                                  ("poly-cons3lfb", 3, (1, 1, 1), -3)]:
         h = _lookup(fam, order, it)[1]
         st = L.cxp_prolongate_adjoint(h, None, ctypes.byref(crse), 0,
@@ -147,9 +146,11 @@ def test_adjoint_argument_checks_before_device_work():
 
 def test_branch_record_sizes_and_checks_without_device_work():
     """cxp_branch_bytes: one byte per fine point (per component in per-var
-    mode) for ENO and fallback operators, 0 for linear ones and MINMOD; the
-    record and the recorded adjoint reject a missing record, and MINMOD has
-    no recorded adjoint, all before any device work."""
+    mode) for ENO and fallback operators, 0 for linear ones; for MINMOD one
+    byte per point each MINMOD pass reads (Bx over x parents and the coarse
+    y, z the later passes read, By over fine x, y parents and coarse z, Bz
+    over fine x, y and z parents), per component.  The record and the
+    recorded adjoint reject a missing record, all before any device work."""
     L = _lib.lib()
     fake = ctypes.c_void_p(0x1000)
 
@@ -168,7 +169,13 @@ def test_branch_record_sizes_and_checks_without_device_work():
                                      ("eno", 3, (1, 1, 1), 1, npts),
                                      ("poly-cons3lfb", 3, (1, 0, 1), 0, 3 * npts),
                                      ("natural", 5, (0, 0, 0), 0, 0),
-                                     ("minmod", 3, (1, 1, 1), 0, 0)]:
+                                     ("minmod", 3, (0, 0, 0), 0, 0),
+                                     # x parents 0..7, y -1..5, z -1..4; fine x
+                                     # 0..15, y parents 0..4; fine x, y, z parents
+                                     ("minmod", 3, (1, 1, 1), 0,
+                                      3 * (8 * 7 * 6 + 16 * 5 * 6 + 16 * 10 * 4)),
+                                     ("minmod", 1, (1, 1, 1), 1,
+                                      3 * (8 * 7 * 6 + 16 * 5 * 6 + 16 * 10 * 4))]:
         h = _lookup(fam, order, it)[1]
         assert L.cxp_branch_bytes(h, i3(rlo), i3(rhi), 3, pg, ctypes.byref(n)) == 0
         assert n.value == want, (fam, pg, n.value)
@@ -179,9 +186,11 @@ def test_branch_record_sizes_and_checks_without_device_work():
                                              ctypes.byref(fine), 0, 3, i3(rlo),
                                              i3(rhi), 0, None) == -3
     hm = _lookup("minmod", 3, (1, 1, 1))[1]
+    assert L.cxp_prolongate_record(hm, ctypes.byref(crse), 0, ctypes.byref(fine), 0, 3,
+                                   i3(rlo), i3(rhi), 0, None, None) == -3
     assert L.cxp_prolongate_adjoint_recorded(hm, None, ctypes.byref(crse), 0,
                                              ctypes.byref(fine), 0, 3, i3(rlo),
-                                             i3(rhi), 0, None) == -11
+                                             i3(rhi), 0, None) == -3
 
 
 def _level(boxes, domain, ncomp, ghosts):

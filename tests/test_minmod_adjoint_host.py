@@ -1,0 +1,110 @@
+"""The MINMOD adjoint's record and gather routines, built for the host with
+every access bounds-checked (carpetx_amd/host/minmod_adjoint_host.cpp runs
+the per-point functions of carpetx_amd/csrc/minmod_adjoint.hpp that the
+device kernels run), against the transpose of the oracle's Jacobian.
+
+The forward is piecewise linear in the coarse data, so central differences
+give the Jacobian exactly inside one limiter branch.  The data are dyadic
+(k / 2^24), the step h = 2^-28, so the differences are exact and no
+branch flips within +-h; ties, where the Jacobian is undefined, do not
+occur for these seeds."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "carpetx_amd", "host", "libminmod_adjoint_host.so")
+
+I3 = ctypes.c_int * 3
+PD = ctypes.POINTER(ctypes.c_double)
+
+
+def host_lib():
+    if not os.path.exists(LIB):
+        pytest.fail("%s is not built (run __graft_entry__.build())" % LIB)
+    L = ctypes.CDLL(LIB)
+    L.cxp_host_minmod_adjoint.restype = ctypes.c_longlong
+    L.cxp_host_minmod_adjoint.argtypes = [ctypes.c_int, I3, PD, I3, I3, PD, I3, I3, PD,
+                                          ctypes.POINTER(ctypes.c_longlong)]
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(PD)
+
+
+def host_adjoint(order, it, crse, clo, chi, fbar, rlo, rhi):
+    cbar = np.zeros_like(crse)
+    n = ctypes.c_longlong()
+    bad = host_lib().cxp_host_minmod_adjoint(order, I3(*it), _ptr(crse), I3(*clo), I3(*chi),
+                                             _ptr(fbar), I3(*rlo), I3(*rhi), _ptr(cbar),
+                                             ctypes.byref(n))
+    return bad, cbar, n.value
+
+
+def oracle_jt(order, it, crse, clo, chi, fbar, rlo, rhi, h=2.0 ** -28):
+    op = oracle.lookup("minmod", order, it)
+    shape = crse.shape
+
+    def fwd(x):
+        c = oracle.Fab(clo, chi, 1, x.reshape((1,) + shape))
+        f = oracle.Fab(rlo, rhi, 1)
+        oracle.prolongate(op, c, f, rlo, rhi)
+        return f.arr.reshape(-1)
+    u = crse.reshape(-1)
+    v = fbar.reshape(-1)
+    out = np.empty(u.size)
+    for n in range(u.size):
+        e = np.zeros(u.size)
+        e[n] = h
+        out[n] = np.dot((fwd(u + e) - fwd(u - e)) / (2 * h), v)
+    return out.reshape(shape)
+
+
+def dyadic(rng, shape):
+    return rng.integers(-(1 << 24), 1 << 24, size=shape).astype(np.float64) / float(1 << 24)
+
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+@pytest.mark.parametrize("c", range(8))
+def test_host_minmod_adjoint_equals_the_jacobian_transpose(order, c):
+    it = ((c >> 2) & 1, (c >> 1) & 1, c & 1)
+    rlo, rhi = (-3, 1, -2), (8, 6, 5)
+    op = oracle.lookup("minmod", order, it)
+    clo, chi = oracle.coarse_box(op, rlo, rhi)
+    rng = np.random.default_rng(1000 + 10 * order + c)
+    crse = dyadic(rng, tuple(chi[d] - clo[d] + 1 for d in (2, 1, 0)))
+    fbar = dyadic(rng, tuple(rhi[d] - rlo[d] + 1 for d in (2, 1, 0)))
+    bad, got, ncodes = host_adjoint(order, it, crse, clo, chi, fbar, rlo, rhi)
+    assert bad == 0
+    # codes only for the MINMOD (cell-centred) directions
+    assert (ncodes > 0) == any(it)
+    want = oracle_jt(order, it, crse, clo, chi, fbar, rlo, rhi)
+    err = np.max(np.abs(got - want))
+    assert err <= 1e-12 * max(1.0, np.max(np.abs(want))), (order, it, err)
+
+
+def test_host_minmod_adjoint_larger_box_with_a_coarse_margin():
+    """A coarse FAB larger than CoarseBox(region): the adjoint touches only
+    the output window and leaves the margin at zero."""
+    order, it = 3, (1, 0, 1)
+    rlo, rhi = (-5, 2, 1), (13, 9, 10)
+    op = oracle.lookup("minmod", order, it)
+    cb_lo, cb_hi = oracle.coarse_box(op, rlo, rhi)
+    clo = tuple(x - 2 for x in cb_lo)
+    chi = tuple(x + 1 for x in cb_hi)
+    rng = np.random.default_rng(77)
+    crse = dyadic(rng, tuple(chi[d] - clo[d] + 1 for d in (2, 1, 0)))
+    fbar = dyadic(rng, tuple(rhi[d] - rlo[d] + 1 for d in (2, 1, 0)))
+    bad, got, _ = host_adjoint(order, it, crse, clo, chi, fbar, rlo, rhi)
+    assert bad == 0
+    want = oracle_jt(order, it, crse, clo, chi, fbar, rlo, rhi)
+    assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+    inner = tuple(slice(cb_lo[d] - clo[d], cb_hi[d] - clo[d] + 1) for d in (2, 1, 0))
+    margin = np.ones(got.shape, dtype=bool)
+    margin[inner] = False
+    assert np.all(got[margin] == 0.0)

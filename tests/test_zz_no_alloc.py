@@ -70,7 +70,7 @@ def _adjoint(fn, P, crse, fbar, region, ncomp, per_group):
 @pytest.mark.gpu
 @pytest.mark.parametrize("fam,order,it", [
     ("eno", 3, (1, 1, 1)), ("poly-cons3lfb", 5, (1, 0, 1)), ("poly-eno3lfb", 3, (0, 1, 1)),
-    ("natural", 5, (0, 0, 0)),
+    ("natural", 5, (0, 0, 0)), ("minmod", 3, (1, 1, 1)), ("minmod", 5, (0, 1, 1)),
 ])
 def test_noalloc_equals_the_allocating_adjoint(fam, order, it):
     L = _lib.lib()
