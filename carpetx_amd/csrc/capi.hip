@@ -669,7 +669,9 @@ struct cxp_worklist_s {
   uint64_t epoch = 0;
   // Launch ordering: one list's claim counters and coarse-patch buffer serve
   // one launch at a time, so a launch on another stream than the previous
-  // one waits for that launch's completion event.
+  // one first waits for what that stream holds so far (an event recorded
+  // there at that moment; none is recorded on the same-stream path, where
+  // each event between kernels would cost microseconds of GPU time).
   std::mutex mu;
   hipEvent_t done = nullptr;
   hipStream_t last_stream = nullptr;
@@ -706,7 +708,23 @@ void touch(cxp_worklist_s *wl) {
 // are rewritten or freed).
 int list_drain(cxp_worklist_s *wl) {
   std::lock_guard<std::mutex> lk(wl->mu);
-  if (wl->launched && hipEventSynchronize(wl->done) != hipSuccess)
+  if (wl->launched && hipStreamSynchronize(wl->last_stream) != hipSuccess)
+    return CXP_ERR_HIP;
+  return CXP_OK;
+}
+
+// Makes s wait for everything issued so far on the list's previous launch
+// stream, when that is another stream.  Called with wl->mu held.
+int list_order_after_last(cxp_worklist_s *wl, hipStream_t s) {
+  if (!wl->launched || wl->last_stream == s)
+    return CXP_OK;
+  if (!wl->done &&
+      hipEventCreateWithFlags(&wl->done, hipEventDisableTiming) != hipSuccess) {
+    wl->done = nullptr;
+    return CXP_ERR_HIP;
+  }
+  if (hipEventRecord(wl->done, wl->last_stream) != hipSuccess ||
+      hipStreamWaitEvent(s, wl->done, 0) != hipSuccess)
     return CXP_ERR_HIP;
   return CXP_OK;
 }
@@ -725,22 +743,11 @@ int list_enter(cxp_worklist_s *wl, hipStream_t s, bool &capturing) {
       g_capture_lists.erase(g_capture_lists.begin());
     return CXP_OK;
   }
-  if (wl->launched && wl->last_stream != s &&
-      hipStreamWaitEvent(s, wl->done, 0) != hipSuccess)
-    return CXP_ERR_HIP;
-  return CXP_OK;
+  return list_order_after_last(wl, s);
 }
 
-// After wl's work went to s (not capturing): records its completion.
-// Called with wl->mu held.
+// After wl's work went to s (not capturing).  Called with wl->mu held.
 int list_leave(cxp_worklist_s *wl, hipStream_t s) {
-  if (!wl->done &&
-      hipEventCreateWithFlags(&wl->done, hipEventDisableTiming) != hipSuccess) {
-    wl->done = nullptr;
-    return CXP_ERR_HIP;
-  }
-  if (hipEventRecord(wl->done, s) != hipSuccess)
-    return CXP_ERR_HIP;
   wl->last_stream = s;
   wl->launched = true;
   return CXP_OK;
@@ -2765,9 +2772,8 @@ int cxp_graph_launch(cxp_graph g, void *stream) {
   std::vector<std::unique_lock<std::mutex>> locks;
   for (cxp_worklist_s *wl : wls) {
     locks.emplace_back(wl->mu);
-    if (wl->launched && wl->last_stream != s &&
-        hipStreamWaitEvent(s, wl->done, 0) != hipSuccess)
-      return CXP_ERR_HIP;
+    if (const int e = list_order_after_last(wl, s))
+      return e;
   }
   if (hipGraphLaunch(g->exec, s) != hipSuccess)
     return CXP_ERR_HIP;

@@ -322,7 +322,9 @@ int cxp_worklist_stats(cxp_worklist wl, int64_t *npatches, int64_t *ntiles,
  * gathered patches serve one launch at a time, so a launch (of this, of
  * cxp_prolongate_per_patch or cxp_worklist_poison, or of a graph that
  * captured them) on another stream than the list's previous launch first
- * makes that stream wait for the previous launch (hipStreamWaitEvent). */
+ * makes its stream wait for everything issued so far on that previous
+ * stream (an event recorded there, then hipStreamWaitEvent), which must
+ * therefore still exist.  Launches on one stream add no event. */
 int cxp_prolongate_batched(cxp_worklist wl, void *stream);
 /* The plain path over the same list: one cxp_prolongate launch per patch. */
 int cxp_prolongate_per_patch(cxp_worklist wl, void *stream);
