@@ -471,6 +471,46 @@ def bench_wave(log, n=128, steps=16, ck=4):
     return rec
 
 
+def bench_reflux(stream, log):
+    """Reflux of the 64-box level (128^3 coarse cells in 8 FABs, 64 fine
+    boxes of 32^3 cells, 4 grid functions): one plan run (6 launches, one
+    per face orientation), back to back; bytes per run from the plan."""
+    import torch
+    from carpetx_amd import workloads
+    from carpetx_amd.interp import Box, FArrayBox
+    from carpetx_amd.reflux import FluxRegister
+    lv = workloads.level_fill_64(ncomp=4)["cons2_ccc"]
+    crse, fine = lv["crse"], lv["fine"]
+    g = torch.Generator(device="cuda").manual_seed(13)
+
+    def faces(level):
+        out = []
+        for d in range(3):
+            fl = []
+            for v in level.valid:
+                hi = list(v.hi)
+                hi[d] += 1
+                b = Box(v.lo, hi)
+                fl.append(FArrayBox(b, 4, tensor=torch.rand((4,) + tuple(reversed(b.size)),
+                                                             dtype=torch.float64, device="cuda",
+                                                             generator=g)))
+            out.append(fl)
+        return out
+    cf, ff = faces(crse), faces(fine)
+    fr = FluxRegister(crse, cf, fine, ff, (1.0 / 128,) * 3)
+    st = fr.stats()
+    with torch.cuda.stream(stream):
+        for _ in range(3):
+            fr.reflux(stream)
+        ms = back_to_back(lambda: fr.reflux(stream), 20, stream)
+    fr.close()
+    rec = dict(register_faces=st["faces"], MB=round(st["bytes"] / 1e6, 2), ms=round(ms, 4),
+               GB_per_s=round(st["bytes"] / (ms * 1e-3) / 1e9, 1),
+               hbm_frac=round(st["bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+    log("reflux: %s" % rec)
+    return rec
+
+
 def bench_tagging(stream, log, n=256):
     """Box-in-box tags and the threshold test on a level of n^3 cells (two
     regions, a cube and a sphere): one double written per cell, then one
@@ -667,6 +707,7 @@ def main():
         line["extra"]["regrid_64"] = bench_regrid(args, stream, log)
         line["extra"]["adjoint_96_192"] = bench_adjoint(stream, log)
         line["extra"]["boxinbox_tag_256"] = bench_tagging(stream, log)
+        line["extra"]["reflux_64"] = bench_reflux(stream, log)
         if not args.no_wave:
             line["extra"]["wave_two_level"] = bench_wave(log)
     if not args.no_cpu:

@@ -140,6 +140,15 @@ _SIGS = {
                                  P_DOUBLE, c_void_p, P_INT, P_INT, P_INT, P_INT, c_void_p]),
     "cxp_error_est": (c_int, [c_void_p, P_INT, P_INT, P_INT, P_INT, ctypes.c_double,
                               c_void_p, c_void_p, c_void_p]),
+    "cxp_reflux_plan_create": (c_int, [ctypes.POINTER(CxpLevel),
+                                       ctypes.POINTER(ctypes.POINTER(CxpFab)),
+                                       ctypes.POINTER(CxpLevel),
+                                       ctypes.POINTER(ctypes.POINTER(CxpFab)), c_int, c_int,
+                                       P_DOUBLE, P_INT, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.POINTER(c_void_p)]),
+    "cxp_reflux_run": (c_int, [c_void_p, c_void_p]),
+    "cxp_reflux_plan_stats": (c_int, [c_void_p, P_INT64, P_INT64]),
+    "cxp_reflux_plan_destroy": (c_int, [c_void_p]),
 }
 
 

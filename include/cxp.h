@@ -336,6 +336,53 @@ int cxp_prolongate_per_patch(cxp_worklist wl, void *stream);
 #define CXP_POISON_BITS 0xfff80000deadbeefULL
 int cxp_worklist_poison(cxp_worklist wl, double value, void *stream);
 
+/* ---- Reflux (CarpetX/src/schedule.cxx:2632-2716): the flux-register
+ * correction of a coarse level where a finer level covers part of it, run
+ * before restriction.  The reference calls AMReX's FluxRegister, made for
+ * the fine level's cell-centred boxes with ratio 2 (driver.cxx:918-924):
+ * CrseInit(coarse flux_d, d, ..., -1) and FineAdd(fine flux_d, d, ..., +1)
+ * for d = 0, 1, 2, then Reflux(coarse state, 1.0, ..., coarse geometry)
+ * (:2694-2704).  Here, for every register face F (a coarse face on the low
+ * or high side of a coarsened fine valid box) and component:
+ *   reg = crse_mult * Fc(F) + fine_mult * (each of the 4 fine faces of F);
+ *   the coarse cell beyond F (index - 1 for a low face, the index for a high
+ *   face, and its image across a periodic direction) gets
+ *   -scale * reg / V (low) or +scale * reg / V (high), V = dx[0] dx[1] dx[2],
+ *   if it is a valid cell of the coarse level (covered cells included, as in
+ *   the reference; restriction overwrites them),
+ * orientation by orientation in the order x-lo, y-lo, z-lo, x-hi, y-hi, z-hi.
+ * The reference's multipliers are crse_mult = -1, fine_mult = 1, scale = 1;
+ * AMReX's usual ones are -dt*A_coarse, dt*A_fine, 1.
+ *   crse       the coarse level: cell-centred state FABs, valid boxes,
+ *              domain (its extent is the period of periodic directions);
+ *   crse_flux  per direction d, crse->nfabs FABs face-centred in d, each
+ *              holding the faces of its state FAB's valid box;
+ *   fine       the fine level: only its valid boxes are used (lo even, hi
+ *              odd: coarsenable by 2);
+ *   fine_flux  per direction d, fine->nfabs FABs holding the faces of their
+ *              fine box in d.
+ * Components comp .. comp + ncomp - 1 of every FAB.  Errors: CXP_ERR_ARG,
+ * CXP_ERR_REGION (a FAB without its faces, cells or components),
+ * CXP_ERR_COARSE (a register face no coarse flux FAB holds).  The plan holds
+ * device copies of its pieces; cxp_reflux_run allocates nothing, so it can
+ * be captured in a graph.  Destroy a plan after its launches completed. */
+typedef struct cxp_reflux_s *cxp_reflux_plan;
+int cxp_reflux_plan_create(const cxp_level *crse,
+                           const cxp_fab *const crse_flux[3],
+                           const cxp_level *fine,
+                           const cxp_fab *const fine_flux[3], int comp,
+                           int ncomp, const double dx[3],
+                           const int periodic[3], double crse_mult,
+                           double fine_mult, double scale,
+                           cxp_reflux_plan *out);
+int cxp_reflux_run(cxp_reflux_plan plan, void *stream);
+/* Register faces of the plan, and the bytes one run moves (per face and
+ * component: the coarse flux and 4 fine fluxes read, the target cell read
+ * and written). */
+int cxp_reflux_plan_stats(cxp_reflux_plan plan, int64_t *faces,
+                          int64_t *bytes);
+int cxp_reflux_plan_destroy(cxp_reflux_plan plan);
+
 /* ---- hipGraph capture of a fill sequence.  A graph keeps the device
  * buffers and tiles of every list it launched.  Any append, clear,
  * cxp_worklist_set_per_group, upload, autotune or destroy of such a list

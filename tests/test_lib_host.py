@@ -664,3 +664,60 @@ def test_boxinbox_region_parameters_and_refined_box():
         BoxInBoxRegion("torus")
     # coarse tags [33, 94] grown by one cell -> [32, 95] -> fine cells [64, 191]
     assert refined_box(Box((33, 40, 33), (94, 80, 94))) == Box((64, 78, 64), (191, 163, 191))
+
+
+def test_reflux_plan_argument_errors_before_device_work():
+    """cxp_reflux_plan_create checks its levels and FABs before allocating:
+    fine boxes not coarsenable by 2 (CXP_ERR_ARG), a flux FAB without its
+    faces (CXP_ERR_REGION), a register face that no coarse flux FAB holds
+    (CXP_ERR_COARSE)."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+
+    def fab(lo, hi, nc=1):
+        f = _lib.CxpFab()
+        f.data = fake
+        f.lo, f.hi = i3(lo), i3(hi)
+        f.comp_stride = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+        f.ncomp = nc
+        return f
+
+    def level(boxes, domain):
+        fabs = (_lib.CxpFab * len(boxes))(*[fab(lo, hi) for lo, hi in boxes])
+        val = (ctypes.c_int * (6 * len(boxes)))(*[x for lo, hi in boxes for x in lo + hi])
+        lv = _lib.CxpLevel()
+        lv.nfabs = len(boxes)
+        lv.fabs = ctypes.cast(fabs, ctypes.POINTER(_lib.CxpFab))
+        lv.valid = ctypes.cast(val, ctypes.POINTER(ctypes.c_int))
+        lv.domain_lo, lv.domain_hi = i3(domain[0]), i3(domain[1])
+        return lv, (fabs, val)
+
+    def fluxes(boxes, grow=0):
+        out = (ctypes.POINTER(_lib.CxpFab) * 3)()
+        keep = []
+        for d in range(3):
+            fs = []
+            for lo, hi in boxes:
+                h = list(hi)
+                h[d] += 1 - grow
+                fs.append(fab(lo, tuple(h)))
+            arr = (_lib.CxpFab * len(fs))(*fs)
+            keep.append(arr)
+            out[d] = ctypes.cast(arr, ctypes.POINTER(_lib.CxpFab))
+        return out, keep
+
+    def create(cboxes, fboxes, fgrow=0, cgrow=0):
+        cl, k1 = level(cboxes, ((0, 0, 0), (7, 7, 7)))
+        fl, k2 = level(fboxes, ((0, 0, 0), (15, 15, 15)))
+        cf, k3 = fluxes(cboxes, cgrow)
+        ff, k4 = fluxes(fboxes, fgrow)
+        h = ctypes.c_void_p()
+        st = L.cxp_reflux_plan_create(ctypes.byref(cl), cf, ctypes.byref(fl), ff, 0, 1,
+                                      (ctypes.c_double * 3)(1, 1, 1), (ctypes.c_int * 3)(0, 0, 0),
+                                      -1.0, 1.0, 1.0, ctypes.byref(h))
+        return st
+    cb = [((0, 0, 0), (7, 7, 7))]
+    assert create(cb, [((1, 2, 2), (6, 5, 5))]) == -3          # odd fine lo
+    assert create(cb, [((2, 2, 2), (5, 5, 5))], fgrow=1) == -4  # fine flux misses a face
+    assert create(cb, [((2, 2, 2), (5, 5, 5))], cgrow=1) == -4  # coarse flux misses a face
+    assert create([((0, 0, 0), (3, 3, 3))], [((8, 2, 2), (11, 5, 5))]) == -5  # uncovered face
