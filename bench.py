@@ -594,18 +594,47 @@ def _cpu_items(wlz, nboxes):
     return items, npts
 
 
-def _cpu_median(items, budget_s, max_reps):
-    """Median seconds of prolongate_many over the items after one warm-up:
-    at least 3 runs, more while under budget_s and max_reps."""
+def _cpu_median(items, warmups=2, runs=10):
+    """Median seconds of prolongate_many over the items: `warmups` untimed
+    runs, then the median of `runs` (the protocol of the baseline plan)."""
     import oracle
-    oracle.prolongate_many(items, kind="fast")  # warm-up
+    for _ in range(warmups):
+        oracle.prolongate_many(items, kind="fast")
     reps = []
-    t_total = time.perf_counter()
-    while len(reps) < 3 or (time.perf_counter() - t_total < budget_s and len(reps) < max_reps):
+    for _ in range(runs):
         t0 = time.perf_counter()
         oracle.prolongate_many(items, kind="fast")
         reps.append(time.perf_counter() - t0)
-    return sorted(reps)[len(reps) // 2], reps
+    s = sorted(reps)
+    return 0.5 * (s[(runs - 1) // 2] + s[runs // 2]), reps
+
+
+HOST_CPU = None  # taken at start, before the OpenMP runtime binds this thread
+
+
+def host_cpu():
+    """CPU model, physical cores and the CPUs this process may run on."""
+    model, cores = "?", set()
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    return {"model": model, "physical_cores": len(cores) or None,
+            "allowed_cpus": len(os.sched_getaffinity(0)),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
 
 
 def cpu_baseline(wlz, nboxes, log):
@@ -614,15 +643,19 @@ def cpu_baseline(wlz, nboxes, log):
     on every host thread OpenMP offers."""
     import oracle
     items, npts = _cpu_items(wlz, nboxes)
-    med, reps = _cpu_median(items, 15, 25)
+    med, reps = _cpu_median(items)
     threads = oracle.num_threads("fast")
-    log("cpu baseline: %d boxes, %.3e pts, median %.3fs over %d runs, %d threads"
-        % (nboxes, npts, med, len(reps), threads))
+    cpu = HOST_CPU or host_cpu()
+    log("cpu baseline: %d boxes, %.3e pts, median %.3fs over %d runs, %d threads, %s"
+        % (nboxes, npts, med, len(reps), threads, cpu))
     return dict(value=npts / med, unit="fine pts/s", cores=threads, kind="port",
+                host=cpu,
                 sample="ghost shells of %d of the 512 boxes, 10 GF, %d fine pts per run; "
                        "oracle/liboracle_fast.so (-O3 -ffp-contract=fast -march=x86-64-v3 "
-                       "-fopenmp), median of %d runs, %.1f s of CPU work in total"
-                       % (nboxes, npts, len(reps), sum(reps)))
+                       "-fopenmp), %d threads, OMP_PROC_BIND=%s OMP_PLACES=%s, 2 warm-ups "
+                       "then the median of %d runs, %.1f s of CPU work in total"
+                       % (nboxes, npts, threads, cpu["omp"]["OMP_PROC_BIND"],
+                          cpu["omp"]["OMP_PLACES"], len(reps), sum(reps)))
 
 
 def cpu_one_thread(wlz, nboxes, log):
@@ -633,17 +666,24 @@ def cpu_one_thread(wlz, nboxes, log):
     threads = oracle.num_threads("fast")
     oracle.set_num_threads(1)
     try:
-        med, reps = _cpu_median(items, 5, 5)
+        med, reps = _cpu_median(items)
     finally:
         oracle.set_num_threads(threads)
     log("cpu one thread: %d boxes, %.3e pts, median %.3fs over %d runs"
         % (nboxes, npts, med, len(reps)))
     return dict(value=npts / med, unit="fine pts/s", cores=1,
                 sample="ghost shells of the first %d of the 512 boxes, 10 GF, %d fine "
-                       "pts per run, median of %d runs" % (nboxes, npts, len(reps)))
+                       "pts per run, 2 warm-ups then the median of %d runs"
+                       % (nboxes, npts, len(reps)))
 
 
 def main():
+    # the CPU baseline's OpenMP threads: one per core, packed (read by the
+    # OpenMP runtime when it starts, so before anything loads it)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    global HOST_CPU
+    HOST_CPU = host_cpu()
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: the transfer path has no multi-GPU "
