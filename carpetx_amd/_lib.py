@@ -65,6 +65,10 @@ _SIGS = {
     "cxp_version": (c_int, []),
     "cxp_status_string": (c_char_p, [c_int]),
     "cxp_op_lookup": (c_int, [c_char_p, c_int, P_INT, ctypes.POINTER(c_void_p)]),
+    "cxp_op_lookup_fallback": (c_int, [c_char_p, c_int, P_INT, c_int,
+                                       ctypes.POINTER(c_void_p)]),
+    "cxp_group_op_lookup": (c_int, [c_char_p, c_char_p, c_int, P_INT,
+                                    ctypes.POINTER(c_void_p)]),
     "cxp_op_describe": (c_int, [c_void_p, P_INT, P_INT, P_INT, P_INT, P_INT]),
     "cxp_coarse_box": (c_int, [c_void_p, P_INT, P_INT, P_INT, P_INT]),
     "cxp_prolongate": (c_int, [c_void_p, ctypes.POINTER(CxpFab), c_int,

@@ -78,6 +78,22 @@ def build_host_check():
     os.replace(tmp, HOST_CHECK)
 
 
+INTERP_CHECK = os.path.join(ROOT, "tests", "amrex_shim", "interpolater_check")
+
+
+def build_interpolater_check():
+    """The Interpolater adapter (host/cxp_interpolater.hxx) compiled against
+    the AMReX / Cactus shim of tests/amrex_shim, linked to libcxp.so."""
+    tmp = INTERP_CHECK + ".tmp"
+    shim = os.path.dirname(INTERP_CHECK)
+    run([HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950",
+         "-Wno-unused-command-line-argument",
+         "-I" + shim, "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "host"),
+         os.path.join(shim, "interpolater_check.cpp"),
+         "-L" + HERE, "-lcxp", "-Wl,-rpath,$ORIGIN/../../carpetx_amd", "-o", tmp])
+    os.replace(tmp, INTERP_CHECK)
+
+
 MINMOD_HOST = os.path.join(HERE, "host", "libminmod_adjoint_host.so")
 
 
@@ -103,6 +119,7 @@ def main():
     build_wave_lib()
     build_host_check()
     build_minmod_host()
+    build_interpolater_check()
     print("built %s in %.1fs" % (LIB, time.time() - t0))
 
 

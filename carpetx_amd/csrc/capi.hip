@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <array>
+#include <cctype>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -39,6 +41,8 @@ namespace {
 std::once_flag g_ops_once;
 cxp_op_s g_ops[9][8][8];
 bool g_ops_valid[9][8][8];
+// the same operators with the fallback set by the caller: [fb]
+cxp_op_s g_ops_fb[9][8][8][2];
 
 void init_ops() {
   for (int f = 0; f < 9; ++f)
@@ -47,9 +51,82 @@ void init_ops() {
         const int it[3] = {(c >> 2) & 1, (c >> 1) & 1, c & 1};
         OpDesc d;
         g_ops_valid[f][o][c] = op_lookup(kFamilies[f], o, it, &d) == 0;
-        if (g_ops_valid[f][o][c])
+        if (g_ops_valid[f][o][c]) {
           g_ops[f][o][c].d = d;
+          for (int fb = 0; fb < 2; ++fb) {
+            g_ops_fb[f][o][c][fb].d = d;
+            g_ops_fb[f][o][c][fb].d.fb = fb;
+          }
+        }
       }
+}
+
+// Value of key in a Cactus tags string (key=value pairs separated by
+// blanks; a value is a "quoted string", a {braced list} or a bare word;
+// *quoted tells the first kind).  Keys compare case-insensitively.  Returns
+// 1 if found, 0 if absent, -1 if the string does not parse.
+int tags_value(const char *tags, const char *key, std::string &out,
+               bool *quoted) {
+  const char *p = tags;
+  auto blank = [](char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; };
+  while (true) {
+    while (*p && blank(*p))
+      ++p;
+    if (!*p)
+      return 0;
+    const char *k0 = p;
+    while (*p && (std::isalnum((unsigned char)*p) || *p == '_'))
+      ++p;
+    if (p == k0)
+      return -1;
+    const std::string k(k0, p);
+    while (*p && blank(*p))
+      ++p;
+    if (*p != '=')
+      return -1;
+    ++p;
+    while (*p && blank(*p))
+      ++p;
+    std::string v;
+    const bool q = *p == '"';
+    if (*p == '"') {
+      const char *v0 = ++p;
+      while (*p && *p != '"')
+        ++p;
+      if (!*p)
+        return -1;
+      v.assign(v0, p);
+      ++p;
+    } else if (*p == '{') {
+      int depth = 0;
+      const char *v0 = p;
+      do {
+        if (*p == '{')
+          ++depth;
+        else if (*p == '}')
+          --depth;
+        ++p;
+      } while (*p && depth > 0);
+      if (depth)
+        return -1;
+      v.assign(v0, p);
+    } else {
+      const char *v0 = p;
+      while (*p && !blank(*p))
+        ++p;
+      if (p == v0)
+        return -1;
+      v.assign(v0, p);
+    }
+    if (k.size() == std::strlen(key) &&
+        std::equal(k.begin(), k.end(), key, [](char a, char b) {
+          return std::tolower((unsigned char)a) == std::tolower((unsigned char)b);
+        })) {
+      out = v;
+      *quoted = q;
+      return 1;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1305,6 +1382,54 @@ int cxp_op_lookup(const char *type, int order, const int indextype[3],
   const int c = (indextype[0] << 2) | (indextype[1] << 1) | indextype[2];
   *out = &g_ops[d.family][order][c];
   return CXP_OK;
+}
+
+int cxp_op_lookup_fallback(const char *type, int order,
+                           const int indextype[3], int fallback,
+                           cxp_op *out) {
+  if (fallback != CXP_FB_KEEP && fallback != CXP_FB_NONE &&
+      fallback != CXP_FB_LINEAR)
+    return CXP_ERR_ARG;
+  cxp_op op = nullptr;
+  const int e = cxp_op_lookup(type, order, indextype, &op);
+  if (e)
+    return e;
+  if (fallback == CXP_FB_KEEP) {
+    *out = op;
+    return CXP_OK;
+  }
+  const int c = (indextype[0] << 2) | (indextype[1] << 1) | indextype[2];
+  *out = &g_ops_fb[op->d.family][order][c][fallback];
+  return CXP_OK;
+}
+
+int cxp_group_op_lookup(const char *tags, const char *default_type,
+                        int default_order, const int indextype[3],
+                        cxp_op *out) {
+  if (!default_type || !indextype || !out)
+    return CXP_ERR_ARG;
+  std::string type = default_type;
+  int order = default_order;
+  if (tags) {
+    std::string v;
+    bool quoted = false;
+    const int t = tags_value(tags, "prolongation_type", v, &quoted);
+    if (t < 0)
+      return CXP_ERR_ARG;
+    if (t == 1)
+      type = v;
+    const int o = tags_value(tags, "prolongation_order", v, &quoted);
+    if (o < 0)
+      return CXP_ERR_ARG;
+    if (o == 1) {
+      char *end = nullptr;
+      const long n = std::strtol(v.c_str(), &end, 10);
+      if (quoted || v.empty() || *end || n < INT32_MIN || n > INT32_MAX)
+        return CXP_ERR_ARG;  // the reference asserts an integer value
+      order = (int)n;
+    }
+  }
+  return cxp_op_lookup(type.c_str(), order, indextype, out);
 }
 
 int cxp_op_describe(cxp_op op, int cent[3], int intp[3], int order[3],

@@ -199,13 +199,37 @@ class Prolongate3DRF2:
             self.prolongation_type, self.prolongation_order, self.indextype)
 
 
-def get_interpolator(prolongation_type, prolongation_order, indextype):
-    """get_interpolator (CarpetX/src/driver.cxx:593-642)."""
+FB_KEEP, FB_NONE, FB_LINEAR = -1, 0, 1
+
+
+def get_interpolator(prolongation_type, prolongation_order, indextype, fallback=FB_KEEP):
+    """get_interpolator (CarpetX/src/driver.cxx:593-642).  fallback other
+    than FB_KEEP replaces the family's linear fallback (FB_NONE / FB_LINEAR;
+    no reference counterpart, cxp_op_lookup_fallback)."""
     h = ctypes.c_void_p()
-    check("cxp_op_lookup", _lib.lib().cxp_op_lookup(
-        prolongation_type.encode(), int(prolongation_order), i3(indextype),
-        ctypes.byref(h)))
+    if fallback == FB_KEEP:
+        check("cxp_op_lookup", _lib.lib().cxp_op_lookup(
+            prolongation_type.encode(), int(prolongation_order), i3(indextype),
+            ctypes.byref(h)))
+    else:
+        check("cxp_op_lookup_fallback", _lib.lib().cxp_op_lookup_fallback(
+            prolongation_type.encode(), int(prolongation_order), i3(indextype),
+            int(fallback), ctypes.byref(h)))
     return Prolongate3DRF2(h, prolongation_type, prolongation_order, indextype)
+
+
+def get_group_interpolator(tags, prolongation_type, prolongation_order, indextype):
+    """get_interpolator(groupdata) (CarpetX/src/driver.cxx:558-591, 628-642):
+    the group's tags string may set prolongation_type / prolongation_order;
+    the driver parameters prolongation_type / prolongation_order fill in the
+    rest."""
+    h = ctypes.c_void_p()
+    check("cxp_group_op_lookup", _lib.lib().cxp_group_op_lookup(
+        tags.encode() if tags is not None else None, prolongation_type.encode(),
+        int(prolongation_order), i3(indextype), ctypes.byref(h)))
+    P = Prolongate3DRF2(h, prolongation_type, prolongation_order, indextype)
+    P.tags = tags
+    return P
 
 
 def restrict(fine, crse, indextype, ncomp=None, crse_region=None, fine_comp=0,

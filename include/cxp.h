@@ -64,6 +64,32 @@ const char *cxp_status_string(int status);
 int cxp_op_lookup(const char *type, int order, const int indextype[3],
                   cxp_op *out);
 
+/* cxp_op_lookup with the linear fallback chosen by the caller.  No
+ * reference counterpart: there the fallback is part of each family's
+ * operator type (prolongate_3d_rf2.hxx:33-62; FB_LINEAR for poly-cons3lfb
+ * and poly-eno3lfb).  fallback CXP_FB_KEEP keeps the family's, CXP_FB_NONE
+ * or CXP_FB_LINEAR replaces it; the fallback is active where the reference
+ * makes it active (CONS / ENO directions above order 1 per variable, CONS
+ * directions per group, impl.hxx:1535-1538, 1990-1992). */
+#define CXP_FB_KEEP -1
+#define CXP_FB_NONE 0
+#define CXP_FB_LINEAR 1
+int cxp_op_lookup_fallback(const char *type, int order,
+                           const int indextype[3], int fallback,
+                           cxp_op *out);
+
+/* get_interpolator(groupdata) -- CarpetX/src/driver.cxx:558-591, 628-642:
+ * the operator of a grid-function group whose tags (the TAGS string of its
+ * interface.ccl declaration, key=value pairs; may be NULL) may set
+ * prolongation_type="..." and prolongation_order=N; unset keys take the
+ * driver parameters default_type and default_order.  Keys compare
+ * case-insensitively.  Errors: as cxp_op_lookup; CXP_ERR_ARG for tags that
+ * do not parse or a prolongation_order that is not an integer (the
+ * reference asserts). */
+int cxp_group_op_lookup(const char *tags, const char *default_type,
+                        int default_order, const int indextype[3],
+                        cxp_op *out);
+
 /* Per-direction description: centring, interpolation, order, required
  * ghosts; fb = 1 for the FB_LINEAR families
  * (prolongate_3d_rf2.hxx:64-75 indextype()/interpolation()/order()). */

@@ -204,3 +204,41 @@ def test_same_call_through_the_device_and_oracle_libraries(fam, order, it, pg):
     torch.cuda.synchronize()
     assert sd == sr == 0
     np.testing.assert_allclose(q_d.cpu().numpy(), q_h, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("fam,order,it,fb", [
+    ("natural", 3, (1, 1, 1), 1), ("natural", 5, (1, 0, 1), 1), ("eno", 3, (1, 1, 0), 1),
+    ("poly-cons3lfb", 3, (1, 1, 1), 0), ("poly-eno3lfb", 5, (0, 1, 1), 0),
+])
+def test_fallback_override_matches_the_oracle(fam, order, it, fb):
+    """cxp_op_lookup_fallback: the family's stencils with the linear fallback
+    switched on or off, against the oracle's operator with the same
+    fallback, per-var and per-group, bit for bit."""
+    rng = np.random.default_rng(31)
+    rlo, rhi = (-3, 1, -2), (17, 12, 9)
+    op_o = oracle.lookup(fam, order, it)
+    op_o.fb = fb
+    P = get_interpolator(fam, order, it, fallback=fb)
+    clo, chi = oracle.coarse_box(op_o, rlo, rhi)
+    cbox = Box(clo, chi)
+    # a step in x so that the fallback fires somewhere
+    x = np.arange(clo[0], chi[0] + 1)
+    base = rng.uniform(-1, 1, (2,) + tuple(reversed(cbox.size)))
+    cdata = base + 3.0 * (x >= (clo[0] + chi[0]) // 2)[None, None, None, :]
+    for pg in (False, True):
+        crse_h = oracle.Fab(cbox.lo, cbox.hi, 2, cdata.copy())
+        fine_h = oracle.Fab(rlo, rhi, 2)
+        oracle.prolongate(op_o, crse_h, fine_h, rlo, rhi, ncomp=2, per_group=pg)
+        crse = FArrayBox(cbox, 2, tensor=torch.from_numpy(cdata.copy()).cuda())
+        fine = FArrayBox(Box(rlo, rhi), 2)
+        P.interp(crse, 0, fine, 0, 2, Box(rlo, rhi), per_group=pg)
+        torch.cuda.synchronize()
+        assert_parity(fine.t.cpu().numpy(), fine_h.arr, "%s o%d %s fb=%d pg=%d" % (fam, order, it, fb, pg))
+    # the override changes the result where the family's fallback differs
+    keep = get_interpolator(fam, order, it)
+    fine2 = FArrayBox(Box(rlo, rhi), 2)
+    keep.interp(crse, 0, fine2, 0, 2, Box(rlo, rhi))
+    f1 = FArrayBox(Box(rlo, rhi), 2)
+    P.interp(crse, 0, f1, 0, 2, Box(rlo, rhi))
+    torch.cuda.synchronize()
+    assert not torch.equal(fine2.t, f1.t)

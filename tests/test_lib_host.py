@@ -721,3 +721,49 @@ def test_reflux_plan_argument_errors_before_device_work():
     assert create(cb, [((2, 2, 2), (5, 5, 5))], fgrow=1) == -4  # fine flux misses a face
     assert create(cb, [((2, 2, 2), (5, 5, 5))], cgrow=1) == -4  # coarse flux misses a face
     assert create([((0, 0, 0), (3, 3, 3))], [((8, 2, 2), (11, 5, 5))]) == -5  # uncovered face
+
+
+def test_fallback_override_and_group_tags_lookup():
+    """cxp_op_lookup_fallback: the family's operator with its fallback
+    replaced (or kept); cxp_group_op_lookup: a group's tags override the
+    driver parameters prolongation_type / prolongation_order
+    (driver.cxx:558-591), keys case-insensitive, other keys ignored."""
+    L = _lib.lib()
+    it = (1, 0, 1)
+
+    def fb_of(h):
+        fb = ctypes.c_int()
+        assert L.cxp_op_describe(h, None, None, None, None, ctypes.byref(fb)) == 0
+        return fb.value
+    base = _lookup("natural", 3, it)[1]
+    for want, fb in ((None, -1), (0, 0), (1, 1)):
+        h = ctypes.c_void_p()
+        assert L.cxp_op_lookup_fallback(b"natural", 3, i3(it), fb, ctypes.byref(h)) == 0
+        assert fb_of(h) == (fb_of(base) if want is None else want)
+        if fb == -1:
+            assert h.value == base.value
+    h = ctypes.c_void_p()
+    assert L.cxp_op_lookup_fallback(b"natural", 3, i3(it), 2, ctypes.byref(h)) == -3
+    assert L.cxp_op_lookup_fallback(b"natural", 4, i3(it), 1, ctypes.byref(h)) == -2
+    lfb = _lookup("poly-cons3lfb", 3, it)[1]
+    assert fb_of(lfb) == 1
+    assert L.cxp_op_lookup_fallback(b"poly-cons3lfb", 3, i3(it), 0, ctypes.byref(h)) == 0
+    assert fb_of(h) == 0
+
+    def group(tags, t="natural", o=1):
+        h = ctypes.c_void_p()
+        st = L.cxp_group_op_lookup(tags.encode() if tags is not None else None, t.encode(), o,
+                                   i3(it), ctypes.byref(h))
+        return st, h.value
+    assert group(None) == (0, _lookup("natural", 1, it)[1].value)
+    assert group('prolongation_type="eno" prolongation_order=3') == \
+        (0, _lookup("eno", 3, it)[1].value)
+    assert group("tensortypealias=\"scalar\" PROLONGATION_ORDER=5 parities={+1 -1 +1}") == \
+        (0, _lookup("natural", 5, it)[1].value)
+    assert group('prolongation_type = "minmod"', "natural", 3) == \
+        (0, _lookup("minmod", 3, it)[1].value)
+    assert group('prolongation_type="bogus"')[0] == -1
+    assert group('prolongation_order=2', "eno")[0] == -2
+    assert group('prolongation_order="3"')[0] == -3       # not an integer
+    assert group('prolongation_type="eno')[0] == -3       # unterminated string
+    assert group('parities={+1 +1')[0] == -3              # unterminated list
