@@ -264,4 +264,192 @@ __host__ __device__ inline double mm_gather_one(const MmGeom &g, const B &codes,
   return acc;
 }
 
+
+// ---------------------------------------------------------------------------
+// Faster record: one thread per column of codes, marching in z.
+//
+// Columns: Bx(px, j, .), By(fx, py, .), Bz(fx, fy, .) in that order; a Bz
+// column keeps XY of three consecutive parents, so each code costs one new
+// XY (9 coarse loads) instead of three.  The values and branches are those
+// of mm_record_one.
+
+// Column pieces of array w: (x, y) columns times z-chunks of zc codes.
+__host__ __device__ inline long long mm_columns(const MmGeom &g, int w,
+                                                int zc) {
+  return (long long)g.bn[w][0] * g.bn[w][1] * ((g.bn[w][2] + zc - 1) / zc);
+}
+
+// Piece n of all arrays (Bx pieces first, x fastest within a chunk);
+// store(w, x, y, z, code) stores one code.
+template <class C, class S>
+__host__ __device__ inline void mm_record_column(const MmGeom &g, const C &c,
+                                                 const S &store, long long n,
+                                                 int zc) {
+  int w = 0;
+  while (w < 2 && n >= mm_columns(g, w, zc)) {
+    n -= mm_columns(g, w, zc);
+    ++w;
+  }
+  if (n >= mm_columns(g, w, zc))
+    return;
+  const long long cols = (long long)g.bn[w][0] * g.bn[w][1];
+  const long long col = n % cols;
+  const int ch = (int)(n / cols);
+  const int x = g.blo[w][0] + (int)(col % g.bn[w][0]);
+  const int y = g.blo[w][1] + (int)(col / g.bn[w][0]);
+  const int z0 = g.blo[w][2] + ch * zc;
+  const int zend = g.blo[w][2] + g.bn[w][2] - 1;
+  const int z1 = z0 + zc - 1 < zend ? z0 + zc - 1 : zend;
+  if (w == 0) {
+    for (int z = z0; z <= z1; ++z)
+      store(0, x, y, z, mm_code(c(x - 1, y, z), c(x, y, z), c(x + 1, y, z)));
+  } else if (w == 1) {
+    for (int z = z0; z <= z1; ++z)
+      store(1, x, y, z,
+            mm_code(mm_xval(g, c, x, y - 1, z), mm_xval(g, c, x, y, z),
+                    mm_xval(g, c, x, y + 1, z)));
+  } else {
+    double a = mm_xyval(g, c, x, y, z0 - 1), b = mm_xyval(g, c, x, y, z0);
+    for (int z = z0; z <= z1; ++z) {
+      const double n1 = mm_xyval(g, c, x, y, z + 1);
+      store(2, x, y, z, mm_code(a, b, n1));
+      a = b;
+      b = n1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled gather for operators MINMOD in all three directions (CCC).
+//
+// A workgroup of MM_T threads owns the coarse outputs i in [i0, i0 + 16),
+// j in [j0, j0 + 16) of planes k in [k0, k1], and marches the parents pz
+// from k0 - 1 to k1 + 1.  Its fine footprint is fx in [2 i0 - 2, 2 i0 + 34)
+// and fy likewise (36 x 36, the fine points whose parents are within one
+// of the tile); thread t holds footprint positions t + MM_T m.
+//   z^T: each parent pz adds, for both of its fine planes, w(oz, e; Bz) *
+//        fbar to XYbar(fx, fy, pz + e), e = -1, 0, 1, kept in a 3-slot
+//        register window; after pz, plane pz - 1 is complete;
+//   y^T: Xbar(fx, j, k) = sum over the 6 fy whose parents are within one of
+//        j of w(oy, j - py; By(fx, py, k)) * XYbar(fx, fy, k)   (LDS);
+//   x^T: cbar(i, j, k) += sum over the 6 fx of w(ox, i - px; Bx(px, j, k))
+//        * Xbar(fx, j, k)                                       (LDS).
+// The steps are host-and-device functions of (thread, state), so the host
+// build runs the same indexing with checked accessors (one thread after the
+// other between the barriers).
+
+#define MM_T 256
+#define MM_TILE 16
+#define MM_FP (2 * MM_TILE + 4)         // footprint extent (36)
+#define MM_NPOS ((MM_FP * MM_FP + MM_T - 1) / MM_T)  // positions per thread
+
+struct MmTile {
+  int i0, j0, k0, k1;
+};
+
+struct MmWin {
+  double s[MM_NPOS][3];  // XYbar of planes pz - 1, pz, pz + 1
+};
+
+__host__ __device__ inline void mm_win_zero(MmWin &w) {
+  for (int m = 0; m < MM_NPOS; ++m)
+    w.s[m][0] = w.s[m][1] = w.s[m][2] = 0.0;
+}
+
+// z^T contributions of parent pz.  codes(w, x, y, z), fbar(x, y, z).
+template <class B, class F>
+__host__ __device__ inline void mm_zt_add(const MmGeom &g, const MmTile &t,
+                                          int tid, int pz, MmWin &win,
+                                          const B &codes, const F &fbar) {
+  const int plo = g.rlo[2] >> 1, phi = g.rhi[2] >> 1;
+  if (pz < plo || pz > phi)
+    return;
+  for (int m = 0; m < MM_NPOS; ++m) {
+    const int q = tid + MM_T * m;
+    if (q >= MM_FP * MM_FP)
+      break;
+    const int fx = 2 * t.i0 - 2 + q % MM_FP, fy = 2 * t.j0 - 2 + q / MM_FP;
+    if (fx < g.rlo[0] || fx > g.rhi[0] || fy < g.rlo[1] || fy > g.rhi[1])
+      continue;
+    const unsigned char code = codes(2, fx, fy, pz);
+    for (int oz = 0; oz < 2; ++oz) {
+      const int fz = 2 * pz + oz;
+      if (fz < g.rlo[2] || fz > g.rhi[2])
+        continue;
+      const double f = fbar(fx, fy, fz);
+      for (int e = -1; e <= 1; ++e)
+        win.s[m][e + 1] += mm_weight(code, oz, e) * f;
+    }
+  }
+}
+
+// Stores the completed plane (slot 0) with xy_st(q, v), q < MM_FP^2, when
+// store is set, and rotates the window.
+template <class XS>
+__host__ __device__ inline void mm_zt_flush(int tid, MmWin &win, bool store,
+                                            const XS &xy_st) {
+  for (int m = 0; m < MM_NPOS; ++m) {
+    const int q = tid + MM_T * m;
+    if (store && q < MM_FP * MM_FP)
+      xy_st(q, win.s[m][0]);
+    win.s[m][0] = win.s[m][1];
+    win.s[m][1] = win.s[m][2];
+    win.s[m][2] = 0.0;
+  }
+}
+
+// y^T of plane k into xb (MM_FP * MM_TILE values, fx fastest, then j - j0):
+// xy_ld(idx) reads the plane, xb_st(q, v) writes.
+template <class B, class XL, class BS>
+__host__ __device__ inline void mm_yt(const MmGeom &g, const MmTile &t,
+                                      int tid, int k, const XL &xy_ld,
+                                      const BS &xb_st, const B &codes) {
+  for (int q = tid; q < MM_FP * MM_TILE; q += MM_T) {
+    const int lx = q % MM_FP, lj = q / MM_FP;
+    const int fx = 2 * t.i0 - 2 + lx, j = t.j0 + lj;
+    double s = 0.0;
+    if (fx >= g.rlo[0] && fx <= g.rhi[0] && j <= g.ohi[1]) {
+      const int flo = 2 * j - 2 > g.rlo[1] ? 2 * j - 2 : g.rlo[1];
+      const int fhi = 2 * j + 3 < g.rhi[1] ? 2 * j + 3 : g.rhi[1];
+      for (int fy = flo; fy <= fhi; ++fy) {
+        const int py = fy >> 1;
+        s += mm_weight(codes(1, fx, py, k), fy & 1, j - py) *
+             xy_ld((fy - (2 * t.j0 - 2)) * MM_FP + lx);
+      }
+    }
+    xb_st(q, s);
+  }
+}
+
+// x^T of plane k: cbar_add(i, j, k, value) for the thread's output;
+// xb_ld(idx) reads y^T's values.
+template <class B, class BL, class A>
+__host__ __device__ inline void mm_xt(const MmGeom &g, const MmTile &t,
+                                      int tid, int k, const BL &xb_ld,
+                                      const B &codes, const A &cbar_add) {
+  const int i = t.i0 + tid % MM_TILE, j = t.j0 + tid / MM_TILE;
+  if (i > g.ohi[0] || j > g.ohi[1])
+    return;
+  const int flo = 2 * i - 2 > g.rlo[0] ? 2 * i - 2 : g.rlo[0];
+  const int fhi = 2 * i + 3 < g.rhi[0] ? 2 * i + 3 : g.rhi[0];
+  double s = 0.0;
+  for (int fx = flo; fx <= fhi; ++fx) {
+    const int px = fx >> 1;
+    s += mm_weight(codes(0, px, j, k), fx & 1, i - px) *
+         xb_ld((j - t.j0) * MM_FP + (fx - (2 * t.i0 - 2)));
+  }
+  cbar_add(i, j, k, s);
+}
+
+// Tiles of the output window: x, y tiles of MM_TILE and z-chunks of kc.
+__host__ __device__ inline MmTile mm_tile(const MmGeom &g, int b, int ntx,
+                                          int nty, int kc) {
+  MmTile t;
+  t.i0 = g.olo[0] + MM_TILE * (b % ntx);
+  t.j0 = g.olo[1] + MM_TILE * ((b / ntx) % nty);
+  t.k0 = g.olo[2] + kc * (b / (ntx * nty));
+  t.k1 = t.k0 + kc - 1 < g.ohi[2] ? t.k0 + kc - 1 : g.ohi[2];
+  return t;
+}
+
 } // namespace cxp

@@ -108,3 +108,54 @@ def test_host_minmod_adjoint_larger_box_with_a_coarse_margin():
     margin = np.ones(got.shape, dtype=bool)
     margin[inner] = False
     assert np.all(got[margin] == 0.0)
+
+
+def host_adjoint_tiled(order, it, crse, clo, chi, fbar, rlo, rhi, zc, kc):
+    L = host_lib()
+    fn = L.cxp_host_minmod_adjoint_tiled
+    fn.restype = ctypes.c_longlong
+    fn.argtypes = [ctypes.c_int, I3, PD, I3, I3, PD, I3, I3, PD, ctypes.c_int, ctypes.c_int,
+                   ctypes.POINTER(ctypes.c_longlong)]
+    cbar = np.zeros_like(crse)
+    n = ctypes.c_longlong()
+    bad = fn(order, I3(*it), _ptr(crse), I3(*clo), I3(*chi), _ptr(fbar), I3(*rlo), I3(*rhi),
+             _ptr(cbar), zc, kc, ctypes.byref(n))
+    return bad, cbar
+
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+@pytest.mark.parametrize("c", range(8))
+def test_host_tiled_routines_equal_the_per_point_gather(order, c):
+    """The kernels' column record (z-chunks of 3) and, for CCC, the tiled
+    gather (several 16 x 16 tiles, z-chunks of 2), run on the host with
+    every access checked: no out-of-range access, every code written once,
+    and the result of the per-point gather (pinned to the oracle's
+    Jacobian above) to rounding."""
+    it = ((c >> 2) & 1, (c >> 1) & 1, c & 1)
+    rlo, rhi = (-5, 2, -3), (40, 37, 8)
+    op = oracle.lookup("minmod", order, it)
+    clo, chi = oracle.coarse_box(op, rlo, rhi)
+    rng = np.random.default_rng(300 + 10 * order + c)
+    crse = rng.uniform(-1, 1, tuple(chi[d] - clo[d] + 1 for d in (2, 1, 0)))
+    fbar = rng.uniform(-1, 1, tuple(rhi[d] - rlo[d] + 1 for d in (2, 1, 0)))
+    bad0, want, _ = host_adjoint(order, it, crse, clo, chi, fbar, rlo, rhi)
+    assert bad0 == 0
+    for zc, kc in ((3, 2), (64, 16)):
+        bad, got = host_adjoint_tiled(order, it, crse, clo, chi, fbar, rlo, rhi, zc, kc)
+        assert bad == 0, (zc, kc)
+        err = np.max(np.abs(got - want))
+        assert err <= 1e-13 * max(1.0, np.max(np.abs(want))), (order, it, zc, kc, err)
+
+
+def test_host_tiled_gather_against_the_jacobian():
+    rlo, rhi = (1, -2, 0), (34, 9, 7)
+    it = (1, 1, 1)
+    op = oracle.lookup("minmod", 3, it)
+    clo, chi = oracle.coarse_box(op, rlo, rhi)
+    rng = np.random.default_rng(77)
+    crse = dyadic(rng, tuple(chi[d] - clo[d] + 1 for d in (2, 1, 0)))
+    fbar = dyadic(rng, tuple(rhi[d] - rlo[d] + 1 for d in (2, 1, 0)))
+    bad, got = host_adjoint_tiled(3, it, crse, clo, chi, fbar, rlo, rhi, 2, 3)
+    assert bad == 0
+    want = oracle_jt(3, it, crse, clo, chi, fbar, rlo, rhi)
+    assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
