@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcxp.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["capi.hip", "tag.hip", "reflux.hip"]
+SOURCES = ["capi.hip", "tag.hip", "reflux.hip", "minmod_adjoint.hip"]
 
 # FMA contraction stays off so the tensor passes reproduce the reference's
 # per-point arithmetic bit for bit; packed FP32 instructions are disabled.

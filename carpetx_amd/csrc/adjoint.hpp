@@ -18,8 +18,8 @@
 //    one thread per coarse point, gathers over the fine points whose
 //    stencils can reach it.
 //  * MINMOD operators: the Jacobian is a chain of per-pass branches
-//    (minmod_adjoint.hpp); one pass records each MINMOD application's branch,
-//    one thread per coarse point gathers.
+//    (minmod_adjoint.hpp); one pass records each MINMOD application's branch
+//    by columns, a second gathers (tiled for CCC, per coarse point else).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,7 +28,7 @@
 
 #include "../../include/cxp.h"
 #include "kernels.hpp"
-#include "minmod_adjoint.hpp"
+#include "minmod_launch.hpp"
 #include "ops.hpp"
 
 namespace cxp {
@@ -539,53 +539,6 @@ __global__ void __launch_bounds__(256)
   gst(cb.p, ci, gld(cb.p, ci) + sum);
 }
 
-// --------------------------------------------------------------------------
-// MINMOD adjoint kernels (routines in minmod_adjoint.hpp); one component per
-// grid row y
-
-struct MmData {
-  DFabC crse;  // forward input (record only)
-  DFabC fbar;
-  DFabC cbar;
-  unsigned char *codes;
-  long long ccs_in, fcs, ccs_bar;  // component strides
-};
-
-__global__ void __launch_bounds__(256)
-    k_mm_record(MmGeom g, MmData t) {
-  const long long n = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (n >= g.ncodes)
-    return;
-  const int comp = blockIdx.y;
-  const double *cp = t.crse.p + comp * t.ccs_in;
-  const DFabC c = t.crse;
-  auto cv = [&](int i, int j, int k) { return cp[dcidx(c, i, j, k)]; };
-  t.codes[comp * g.ncodes + n] = mm_record_one(g, cv, n);
-}
-
-__global__ void __launch_bounds__(256)
-    k_mm_gather(MmGeom g, MmData t) {
-  const int ni = g.ohi[0] - g.olo[0] + 1, nj = g.ohi[1] - g.olo[1] + 1,
-            nk = g.ohi[2] - g.olo[2] + 1;
-  const long long n = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (n >= (long long)ni * nj * nk)
-    return;
-  const int comp = blockIdx.y;
-  const int i = g.olo[0] + (int)(n % ni);
-  const int j = g.olo[1] + (int)((n / ni) % nj);
-  const int k = g.olo[2] + (int)(n / ((long long)ni * nj));
-  const unsigned char *cd = t.codes + comp * g.ncodes;
-  const double *fp = t.fbar.p + comp * t.fcs;
-  const DFabC fb = t.fbar;
-  auto codes = [&](int w, int x, int y, int z) {
-    return cd[mm_code_index(g, w, x, y, z)];
-  };
-  auto fv = [&](int x, int y, int z) { return fp[dcidx(fb, x, y, z)]; };
-  const double s = mm_gather_one(g, codes, fv, i, j, k);
-  double *cb = t.cbar.p + comp * t.ccs_bar;
-  cb[dcidx(t.cbar, i, j, k)] += s;
-}
-
 inline DFab adj_dfab(const cxp_fab *f, int comp) {
   DFab v;
   v.p = f->data + (int64_t)comp * f->comp_stride;
@@ -599,76 +552,6 @@ inline DFab adj_dfab(const cxp_fab *f, int comp) {
 inline long long adj_component_bytes(const cxp_fab *f) {
   return 8LL * (f->hi[0] - f->lo[0] + 1) * (f->hi[1] - f->lo[1] + 1) *
          (f->hi[2] - f->lo[2] + 1);
-}
-
-// ---- MINMOD adjoint launches (host)
-
-inline bool op_has_minmod(const OpDesc &op) {
-  for (int d = 0; d < 3; ++d)
-    if (op.sid[d] == SID_MINMOD)
-      return true;
-  return false;
-}
-
-inline DFabC mm_dfab(const cxp_fab *f, int comp) {
-  DFabC v;
-  v.p = f->data + (int64_t)comp * f->comp_stride;
-  for (int d = 0; d < 3; ++d)
-    v.lo[d] = f->lo[d];
-  v.sy = f->hi[0] - f->lo[0] + 1;
-  v.sz = v.sy * (f->hi[1] - f->lo[1] + 1);
-  return v;
-}
-
-// Bytes of branch codes of a MINMOD operator over the region: one per point
-// each MINMOD pass reads along its direction, per component.
-inline long long mm_code_bytes(const OpDesc &op, const int rlo[3],
-                               const int rhi[3], int ncomp) {
-  MmGeom g;
-  mm_setup(g, op.sid, rlo, rhi);
-  return g.ncodes * ncomp;
-}
-
-// Records the branch codes of every component from the forward input.
-inline int mm_record_launch(const OpDesc &op, const cxp_fab *crse_in,
-                            int crse_comp, int ncomp, const int rlo[3],
-                            const int rhi[3], unsigned char *codes,
-                            hipStream_t s) {
-  MmGeom g;
-  mm_setup(g, op.sid, rlo, rhi);
-  if (g.ncodes == 0)
-    return CXP_OK;
-  MmData t{};
-  t.crse = mm_dfab(crse_in, crse_comp);
-  t.ccs_in = crse_in->comp_stride;
-  t.codes = codes;
-  hipLaunchKernelGGL(k_mm_record,
-                     dim3((unsigned)((g.ncodes + 255) / 256), ncomp),
-                     dim3(256), 0, s, g, t);
-  return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
-}
-
-// crse_bar += P^T fine_bar from recorded codes, one thread per coarse point
-// of the output window.
-inline int mm_gather_launch(const OpDesc &op, const cxp_fab *crse_bar,
-                            int crse_comp, const cxp_fab *fine_bar,
-                            int fine_comp, int ncomp, const int rlo[3],
-                            const int rhi[3], const unsigned char *codes,
-                            hipStream_t s) {
-  MmGeom g;
-  mm_setup(g, op.sid, rlo, rhi);
-  long long nout = 1;
-  for (int d = 0; d < 3; ++d)
-    nout *= g.ohi[d] - g.olo[d] + 1;
-  MmData t{};
-  t.fbar = mm_dfab(fine_bar, fine_comp);
-  t.cbar = mm_dfab(crse_bar, crse_comp);
-  t.fcs = fine_bar->comp_stride;
-  t.ccs_bar = crse_bar->comp_stride;
-  t.codes = const_cast<unsigned char *>(codes);
-  hipLaunchKernelGGL(k_mm_gather, dim3((unsigned)((nout + 255) / 256), ncomp),
-                     dim3(256), 0, s, g, t);
-  return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
 }
 
 inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,

@@ -645,10 +645,11 @@ int adjoint_point_launch(const cxp_op_s *op, const cxp_fab *crse_in,
                            fine_comp, ncomp, rlo, rhi, per_group, codes, s);
 }
 
-// The MINMOD adjoint (record the branch codes from crse_in, then gather)
-// with its codes from the stream's pool (or from `codes`, when given, of
-// at least mm_code_bytes).  Refuses capture like adjoint_point_launch when
-// it takes the pool.
+// The MINMOD adjoint: for CCC operators one fused launch that recomputes
+// the branches; else record the branch codes from crse_in, then gather,
+// with the codes from the stream's pool (or from `codes`, when given, of at
+// least mm_code_bytes).  Refuses capture like adjoint_point_launch when it
+// takes the pool.
 int minmod_adjoint_launch(const cxp_op_s *op, const cxp_fab *crse_in,
                           const cxp_fab *crse_bar, int crse_comp,
                           const cxp_fab *fine_bar, int fine_comp, int ncomp,
@@ -657,6 +658,9 @@ int minmod_adjoint_launch(const cxp_op_s *op, const cxp_fab *crse_in,
   if (const int e = adjoint_point_check(op, crse_in, crse_comp, fine_bar,
                                         fine_comp, ncomp, rlo, rhi, per_group))
     return e;
+  if (op_all_minmod(op->d))
+    return mm_fused_launch(op->d, crse_in, crse_bar, crse_comp, fine_bar,
+                           fine_comp, ncomp, rlo, rhi, s);
   const long long nbytes = mm_code_bytes(op->d, rlo, rhi, ncomp);
   if (!codes && nbytes > 0) {
     bool capturing = false;
@@ -1587,6 +1591,10 @@ int cxp_prolongate_adjoint_noalloc(cxp_op op, const cxp_fab *crse_in,
                                            fine_comp, ncomp, rlo, rhi,
                                            per_group))
       return e2;
+    if (op_all_minmod(od))  // the fused CCC adjoint keeps no codes
+      return minmod_adjoint_launch(op, crse_in, crse_bar, crse_comp, fine_bar,
+                                   fine_comp, ncomp, rlo, rhi, per_group,
+                                   (hipStream_t)stream);
     if (mm_code_bytes(od, rlo, rhi, ncomp) > CXP_NOALLOC_SCRATCH_BYTES)
       return CXP_ERR_SCRATCH;
     unsigned char *codes = nullptr;

@@ -15,19 +15,23 @@
 //   Bx(px, j, k)   from c(px - 1 .. px + 1, j, k),
 //   By(fx, py, k)  from X(fx, py - 1 .. py + 1, k),
 //   Bz(fx, fy, pz) from XY(fx, fy, pz - 1 .. pz + 1).
-// The record pass writes these codes (one byte each); the gather pass runs
-// one thread per coarse point (i, j, k) and sums, over the fine points F
-// whose stencils reach it,
+// The record pass writes these codes (one byte each); the gather sums, for
+// each coarse point (i, j, k), over the fine points F whose stencils reach
+// it,
 //   wz(oz, k - pz; Bz(F)) * wy(oy, j - py; By(fx, py, k))
 //     * wx(ox, i - px; Bx(px, j, k)) * fbar(F),
 // the Jacobian entry of f(F) by the chain rule: f(F) depends on plane k
 // through XY(fx, fy, k) alone, which depends on row j through X(fx, j, k)
 // alone.  Linear directions take their fixed weights.  No atomics.
 //
-// The per-point routines are host-and-device functions over accessor
+// The kernels (k_mm_record_col, k_mm_gather_tiled for CCC operators and
+// k_mm_gather otherwise, in adjoint.hpp) record by columns and gather a
+// tile at a time, the transposes applied as z^T, y^T, x^T (below); the
+// per-point routines (mm_record_one, mm_gather_one) state the same sums one
+// entry at a time.  All are host-and-device functions over accessor
 // functors, so a host build runs them with bounds-checked accessors
-// (carpetx_amd/host/minmod_adjoint_host.cpp) and the kernels (k_mm_record,
-// k_mm_gather in adjoint.hpp) with plain ones.
+// (carpetx_amd/host/minmod_adjoint_host.cpp) and the kernels with plain
+// ones.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -320,136 +324,226 @@ __host__ __device__ inline void mm_record_column(const MmGeom &g, const C &c,
 }
 
 // ---------------------------------------------------------------------------
-// Tiled gather for operators MINMOD in all three directions (CCC).
+// Tiled adjoint of the operators MINMOD in all three directions (CCC).
 //
-// A workgroup of MM_T threads owns the coarse outputs i in [i0, i0 + 16),
-// j in [j0, j0 + 16) of planes k in [k0, k1], and marches the parents pz
-// from k0 - 1 to k1 + 1.  Its fine footprint is fx in [2 i0 - 2, 2 i0 + 34)
-// and fy likewise (36 x 36, the fine points whose parents are within one
-// of the tile); thread t holds footprint positions t + MM_T m.
-//   z^T: each parent pz adds, for both of its fine planes, w(oz, e; Bz) *
-//        fbar to XYbar(fx, fy, pz + e), e = -1, 0, 1, kept in a 3-slot
-//        register window; after pz, plane pz - 1 is complete;
-//   y^T: Xbar(fx, j, k) = sum over the 6 fy whose parents are within one of
-//        j of w(oy, j - py; By(fx, py, k)) * XYbar(fx, fy, k)   (LDS);
-//   x^T: cbar(i, j, k) += sum over the 6 fx of w(ox, i - px; Bx(px, j, k))
-//        * Xbar(fx, j, k)                                       (LDS).
-// The steps are host-and-device functions of (thread, state), so the host
-// build runs the same indexing with checked accessors (one thread after the
-// other between the barriers).
+// One MINMOD pass, f(2p + o) = c(p) + (2o - 1) delta(p) with delta(p) one of
+// 0, (c(p) - c(p - 1)) / 4, (c(p + 1) - c(p)) / 4 by the branch at p, has
+// for the fine adjoints g0 = fbar(2p), g1 = fbar(2p + 1) the transpose
+//   d = (g1 - g0) / 4,  a = d on MINUS else 0,  b = d on PLUS else 0,
+//   cbar(p - 1) -= a,  cbar(p) += ((g0 + g1) + a) - b,  cbar(p + 1) += b,
+// so out(i) = (b(i - 1) + c(i)) - a(i + 1) with c = ((g0 + g1) + a) - b
+// (mt_pair).  The tiles apply it as z^T, y^T, x^T:
+//   z^T: parent pz adds -a, c, b to the XYbar planes pz - 1, pz, pz + 1 in a
+//        3-slot register window per footprint position; after pz, plane
+//        pz - 1 is complete and goes to LDS (the order of the adds is that
+//        of the out(i) formula);
+//   y^T: Xbar(fx, j, k) from the XYbar rows of parents j - 1 .. j + 1;
+//   x^T: crse_bar(i, j, k) += from the Xbar columns of parents i - 1 .. i + 1.
+// Fine points outside the region have a zero adjoint, so a parent outside
+// the region gives a = b = c = 0 whatever its branch: only the fine loads
+// and the output stores are masked, and branches of such parents may be
+// anything.
+//
+// A workgroup of MT_T threads owns the outputs [i0, i0 + 14) x [j0, j0 + 14)
+// of the planes [k0, k1].  Its fine footprint, the fine points whose parents
+// are within one of the tile, is 32 x 32 (fx = 2 i0 - 2 + lx, fy likewise):
+// four positions per thread, lx = tid & 31, ly = (tid >> 5) + 8 m.  LDS
+// index conventions: xy[ly * 32 + lx]; xb[lj * 32 + lx] (j = j0 + lj); By of
+// parent py = j0 - 1 + lp at [lp * 32 + lx]; Bx of parent px = i0 - 1 + lp
+// on row j0 + lj at [lj * 16 + lp].
+//
+// Two kernels use these routines: the gather from recorded codes (parents
+// marched pz = k0 - 1 .. k1 + 1) and the fused adjoint at crse_in (below),
+// which recomputes the codes.  Both give the same bits.  The steps are
+// host-and-device functions, so a host build runs the same indexing with
+// checked accessors (the threads of a workgroup one after the other between
+// barriers).
 
-#define MM_T 256
-#define MM_TILE 16
-#define MM_FP (2 * MM_TILE + 4)         // footprint extent (36)
-#define MM_NPOS ((MM_FP * MM_FP + MM_T - 1) / MM_T)  // positions per thread
+#define MT_T 256
+#define MT_O 14                  // outputs per tile edge
+#define MT_FP 32                 // footprint edge, 2 MT_O + 4
+#define MT_NPOS 4                // footprint positions per thread
+#define MT_PB 16                 // parents per tile edge, MT_O + 2
+#define MT_CW 18                 // coarse window edge and X rows, MT_O + 4
+#define MT_NY (MT_O * MT_FP)     // y^T outputs (448)
+#define MT_NX (MT_O * MT_O)      // x^T outputs (196)
+#define MT_RING 4
 
-struct MmTile {
+struct MtTile {
   int i0, j0, k0, k1;
 };
 
-struct MmWin {
-  double s[MM_NPOS][3];  // XYbar of planes pz - 1, pz, pz + 1
+// Tile b of the output window [olo, ohi]: x, y tiles of MT_O, z-chunks of kc.
+__host__ __device__ inline MtTile mt_tile(const int olo[3], const int ohi[3],
+                                          int b, int ntx, int nty, int kc) {
+  MtTile t;
+  t.i0 = olo[0] + MT_O * (b % ntx);
+  t.j0 = olo[1] + MT_O * ((b / ntx) % nty);
+  t.k0 = olo[2] + kc * (b / (ntx * nty));
+  t.k1 = t.k0 + kc - 1 < ohi[2] ? t.k0 + kc - 1 : ohi[2];
+  return t;
+}
+
+__host__ __device__ inline void mt_pair(double g0, double g1,
+                                        unsigned char code, double &a,
+                                        double &b, double &c) {
+  const double d = 0.25 * (g1 - g0);
+  a = code == MM_MINUS ? d : 0.0;
+  b = code == MM_PLUS ? d : 0.0;
+  c = ((g0 + g1) + a) - b;
+}
+
+struct MtWin {
+  double s[MT_NPOS][3];  // XYbar of planes pz - 1, pz, pz + 1
 };
 
-__host__ __device__ inline void mm_win_zero(MmWin &w) {
-  for (int m = 0; m < MM_NPOS; ++m)
+__host__ __device__ inline void mt_win_zero(MtWin &w) {
+  for (int m = 0; m < MT_NPOS; ++m)
     w.s[m][0] = w.s[m][1] = w.s[m][2] = 0.0;
 }
 
-// z^T contributions of parent pz.  codes(w, x, y, z), fbar(x, y, z).
-template <class B, class F>
-__host__ __device__ inline void mm_zt_add(const MmGeom &g, const MmTile &t,
-                                          int tid, int pz, MmWin &win,
-                                          const B &codes, const F &fbar) {
-  const int plo = g.rlo[2] >> 1, phi = g.rhi[2] >> 1;
-  if (pz < plo || pz > phi)
-    return;
-  for (int m = 0; m < MM_NPOS; ++m) {
-    const int q = tid + MM_T * m;
-    if (q >= MM_FP * MM_FP)
-      break;
-    const int fx = 2 * t.i0 - 2 + q % MM_FP, fy = 2 * t.j0 - 2 + q / MM_FP;
-    if (fx < g.rlo[0] || fx > g.rhi[0] || fy < g.rlo[1] || fy > g.rhi[1])
-      continue;
-    const unsigned char code = codes(2, fx, fy, pz);
-    for (int oz = 0; oz < 2; ++oz) {
-      const int fz = 2 * pz + oz;
-      if (fz < g.rlo[2] || fz > g.rhi[2])
-        continue;
-      const double f = fbar(fx, fy, fz);
-      for (int e = -1; e <= 1; ++e)
-        win.s[m][e + 1] += mm_weight(code, oz, e) * f;
-    }
-  }
+// z^T of one parent at position m: fine adjoints g0 (plane 2 pz), g1.
+__host__ __device__ inline void mt_zt(MtWin &w, int m, unsigned char code,
+                                      double g0, double g1) {
+  double a, b, c;
+  mt_pair(g0, g1, code, a, b, c);
+  w.s[m][0] -= a;
+  w.s[m][1] += c;
+  w.s[m][2] += b;
 }
 
-// Stores the completed plane (slot 0) with xy_st(q, v), q < MM_FP^2, when
-// store is set, and rotates the window.
+// Stores plane slot 0 of every position with xy_st(q, v) when store is set
+// (q = ly * 32 + lx), and rotates the window.
 template <class XS>
-__host__ __device__ inline void mm_zt_flush(int tid, MmWin &win, bool store,
-                                            const XS &xy_st) {
-  for (int m = 0; m < MM_NPOS; ++m) {
-    const int q = tid + MM_T * m;
-    if (store && q < MM_FP * MM_FP)
-      xy_st(q, win.s[m][0]);
-    win.s[m][0] = win.s[m][1];
-    win.s[m][1] = win.s[m][2];
-    win.s[m][2] = 0.0;
+__host__ __device__ inline void mt_flush(int tid, MtWin &w, bool store,
+                                         const XS &xy_st) {
+  for (int m = 0; m < MT_NPOS; ++m) {
+    if (store)
+      xy_st(tid + MT_T * m, w.s[m][0]);
+    w.s[m][0] = w.s[m][1];
+    w.s[m][1] = w.s[m][2];
+    w.s[m][2] = 0.0;
   }
 }
 
-// y^T of plane k into xb (MM_FP * MM_TILE values, fx fastest, then j - j0):
-// xy_ld(idx) reads the plane, xb_st(q, v) writes.
-template <class B, class XL, class BS>
-__host__ __device__ inline void mm_yt(const MmGeom &g, const MmTile &t,
-                                      int tid, int k, const XL &xy_ld,
-                                      const BS &xb_st, const B &codes) {
-  for (int q = tid; q < MM_FP * MM_TILE; q += MM_T) {
-    const int lx = q % MM_FP, lj = q / MM_FP;
-    const int fx = 2 * t.i0 - 2 + lx, j = t.j0 + lj;
-    double s = 0.0;
-    if (fx >= g.rlo[0] && fx <= g.rhi[0] && j <= g.ohi[1]) {
-      const int flo = 2 * j - 2 > g.rlo[1] ? 2 * j - 2 : g.rlo[1];
-      const int fhi = 2 * j + 3 < g.rhi[1] ? 2 * j + 3 : g.rhi[1];
-      for (int fy = flo; fy <= fhi; ++fy) {
-        const int py = fy >> 1;
-        s += mm_weight(codes(1, fx, py, k), fy & 1, j - py) *
-             xy_ld((fy - (2 * t.j0 - 2)) * MM_FP + lx);
-      }
-    }
-    xb_st(q, s);
+// y^T output e < MT_NY (lx = e & 31, j = j0 + (e >> 5)): xy_ld(q) reads
+// XYbar, byc(lx, lp) the By code of parent j0 - 1 + lp.
+template <class XL, class BC>
+__host__ __device__ inline double mt_yt(int e, const XL &xy_ld,
+                                        const BC &byc) {
+  const int lx = e & (MT_FP - 1), lj = e / MT_FP;
+  double a[3], b[3], c[3];
+  for (int t = 0; t < 3; ++t) {  // parent j - 1 + t: fine rows 2 (lj + t) + o
+    const int r = 2 * (lj + t);
+    mt_pair(xy_ld(r * MT_FP + lx), xy_ld((r + 1) * MT_FP + lx), byc(lx, lj + t),
+            a[t], b[t], c[t]);
+  }
+  return (b[0] + c[1]) - a[2];
+}
+
+// x^T output (i0 + li, j0 + lj): xb_ld(q) reads Xbar, bxc(lp, lj) the Bx
+// code of parent i0 - 1 + lp on row j0 + lj.
+template <class BL, class BC>
+__host__ __device__ inline double mt_xt(int li, int lj, const BL &xb_ld,
+                                        const BC &bxc) {
+  double a[3], b[3], c[3];
+  for (int t = 0; t < 3; ++t) {
+    const int x = lj * MT_FP + 2 * (li + t);
+    mt_pair(xb_ld(x), xb_ld(x + 1), bxc(li + t, lj), a[t], b[t], c[t]);
+  }
+  return (b[0] + c[1]) - a[2];
+}
+
+// ---------------------------------------------------------------------------
+// Fused CCC adjoint at crse_in: the tiled adjoint with its codes recomputed
+// inside the workgroup from the coarse data (no code array).  Step s of the
+// march (coarse plane s, from k0 - 2 to k1 + 3; "produce" steps s <= k1 + 2,
+// "own" steps k0 <= s <= k1) runs
+//   A  coarse plane s -> cpl (x in [i0 - 2, i0 + 16), y likewise; zero
+//      outside the output window, which holds every coarse point the
+//      region's values read);
+//      barrier
+//   B  X(., ., s) -> xpl (the footprint's fine x, rows j0 - 2 .. j0 + 15);
+//      own: Bx(., ., s) -> bx ring slot s % 4;
+//      y^T of plane s - 3 (xy, by ring) -> xb;
+//      barrier
+//   C  own: By(., ., s) -> by ring slot s % 4;
+//      XY(., ., s) -> the thread's window of XY planes s - 2 .. s;
+//      x^T of plane s - 3 (xb, bx ring) -> crse_bar;
+//      z^T of parent s - 1 with Bz from the XY window; XYbar plane s - 2
+//      -> xy.
+// X and XY are the forward's minmod_apply on the same values and the codes
+// mm_code's, so they are the recorded codes bit for bit wherever a fine
+// point of the region depends on them.  Ring slots are written in one phase
+// and read three phases of their kind later, with barriers between.
+
+struct MtXY {
+  double v[MT_NPOS][3];  // XY of planes s - 2, s - 1, s
+};
+
+__host__ __device__ inline void mt_xy_zero(MtXY &x) {
+  for (int m = 0; m < MT_NPOS; ++m)
+    x.v[m][0] = x.v[m][1] = x.v[m][2] = 0.0;
+}
+
+// A: c(x, y, s) for points of the output window [olo, ohi], else 0.
+template <class C, class CS>
+__host__ __device__ inline void mt_load(const int olo[3], const int ohi[3],
+                                        const MtTile &t, int tid, int s,
+                                        const C &c, const CS &cpl_st) {
+  const bool zin = s >= olo[2] && s <= ohi[2];
+  for (int q = tid; q < MT_CW * MT_CW; q += MT_T) {
+    const int x = t.i0 - 2 + q % MT_CW, y = t.j0 - 2 + q / MT_CW;
+    const bool in = zin && x >= olo[0] && x <= ohi[0] && y >= olo[1] &&
+                    y <= ohi[1];
+    cpl_st(q, in ? c(x, y, s) : 0.0);
   }
 }
 
-// x^T of plane k: cbar_add(i, j, k, value) for the thread's output;
-// xb_ld(idx) reads y^T's values.
-template <class B, class BL, class A>
-__host__ __device__ inline void mm_xt(const MmGeom &g, const MmTile &t,
-                                      int tid, int k, const BL &xb_ld,
-                                      const B &codes, const A &cbar_add) {
-  const int i = t.i0 + tid % MM_TILE, j = t.j0 + tid / MM_TILE;
-  if (i > g.ohi[0] || j > g.ohi[1])
+// B: X of the plane (xpl_st(q, v), q = r * 32 + lx, row j0 - 2 + r) and,
+// when bx is set, its Bx codes (bx_st(q, code)).
+template <class CL, class XS, class BS>
+__host__ __device__ inline void mt_x(int tid, const CL &cpl_ld,
+                                     const XS &xpl_st, bool bx,
+                                     const BS &bx_st) {
+  for (int e = tid; e < MT_FP * MT_CW; e += MT_T) {
+    const int lx = e & (MT_FP - 1), r = e / MT_FP;
+    const int q = r * MT_CW + (lx >> 1);  // coarse x of parent - 1
+    xpl_st(e, minmod_apply(cpl_ld(q), cpl_ld(q + 1), cpl_ld(q + 2), lx & 1));
+  }
+  if (!bx)
     return;
-  const int flo = 2 * i - 2 > g.rlo[0] ? 2 * i - 2 : g.rlo[0];
-  const int fhi = 2 * i + 3 < g.rhi[0] ? 2 * i + 3 : g.rhi[0];
-  double s = 0.0;
-  for (int fx = flo; fx <= fhi; ++fx) {
-    const int px = fx >> 1;
-    s += mm_weight(codes(0, px, j, k), fx & 1, i - px) *
-         xb_ld((j - t.j0) * MM_FP + (fx - (2 * t.i0 - 2)));
+  for (int e = tid; e < MT_PB * MT_O; e += MT_T) {
+    const int lp = e & (MT_PB - 1), lj = e / MT_PB;
+    const int q = (lj + 2) * MT_CW + lp;
+    bx_st(e, mm_code(cpl_ld(q), cpl_ld(q + 1), cpl_ld(q + 2)));
   }
-  cbar_add(i, j, k, s);
 }
 
-// Tiles of the output window: x, y tiles of MM_TILE and z-chunks of kc.
-__host__ __device__ inline MmTile mm_tile(const MmGeom &g, int b, int ntx,
-                                          int nty, int kc) {
-  MmTile t;
-  t.i0 = g.olo[0] + MM_TILE * (b % ntx);
-  t.j0 = g.olo[1] + MM_TILE * ((b / ntx) % nty);
-  t.k0 = g.olo[2] + kc * (b / (ntx * nty));
-  t.k1 = t.k0 + kc - 1 < g.ohi[2] ? t.k0 + kc - 1 : g.ohi[2];
-  return t;
+// C: the By codes of the plane (by_st(q, code)).
+template <class XL, class BS>
+__host__ __device__ inline void mt_by(int tid, const XL &xpl_ld,
+                                      const BS &by_st) {
+  for (int e = tid; e < MT_FP * MT_PB; e += MT_T) {
+    const int lx = e & (MT_FP - 1), lp = e / MT_FP;
+    by_st(e, mm_code(xpl_ld(lp * MT_FP + lx), xpl_ld((lp + 1) * MT_FP + lx),
+                     xpl_ld((lp + 2) * MT_FP + lx)));
+  }
+}
+
+// C: XY of the plane into the window, shifting out plane s - 3.
+template <class XL>
+__host__ __device__ inline void mt_xy(int tid, const XL &xpl_ld, MtXY &xy) {
+  const int lx = tid & (MT_FP - 1);
+  for (int m = 0; m < MT_NPOS; ++m) {
+    const int ly = tid / MT_FP + (MT_T / MT_FP) * m;
+    const int r = ly >> 1;
+    xy.v[m][0] = xy.v[m][1];
+    xy.v[m][1] = xy.v[m][2];
+    xy.v[m][2] = minmod_apply(xpl_ld(r * MT_FP + lx),
+                              xpl_ld((r + 1) * MT_FP + lx),
+                              xpl_ld((r + 2) * MT_FP + lx), ly & 1);
+  }
 }
 
 } // namespace cxp

@@ -24,6 +24,52 @@ long long at(const int lo[3], const int hi[3], int i, int j, int k) {
   return (i - lo[0]) + nx * ((j - lo[1]) + ny * (long long)(k - lo[2]));
 }
 
+// An LDS array of a host-emulated workgroup: every element keeps the plane
+// it was written for and the phase (barrier interval) of its last write and
+// read.  A read must see a value of the expected plane written in an
+// earlier phase; no element may be written in a phase that reads it, or
+// twice in one phase.
+struct Lds {
+  const char *name;
+  std::vector<double> v;
+  std::vector<long long> tag;
+  std::vector<int> wph, rph;
+  Lds(const char *n, size_t size)
+      : name(n), v(size), tag(size, -(1LL << 40)), wph(size, -1),
+        rph(size, -1) {}
+};
+
+struct LdsCheck {
+  int phase = 0;
+  long long bad = 0;
+  void report(const char *what, const char *name, long long q, long long tag) {
+    if (bad++ < 8)
+      std::fprintf(stderr, "minmod tiled check: %s %s [%lld] plane %lld phase %d\n",
+                   what, name, q, tag, phase);
+  }
+  void st(Lds &b, long long q, double v, long long tag) {
+    if (q < 0 || q >= (long long)b.v.size()) {
+      report("store out of range", b.name, q, tag);
+      return;
+    }
+    if (b.rph[q] == phase || b.wph[q] == phase)
+      report("store races a read or store of its phase", b.name, q, tag);
+    b.v[q] = v;
+    b.tag[q] = tag;
+    b.wph[q] = phase;
+  }
+  double ld(Lds &b, long long q, long long tag) {
+    if (q < 0 || q >= (long long)b.v.size()) {
+      report("load out of range", b.name, q, tag);
+      return 0.0;
+    }
+    if (b.wph[q] >= phase || b.tag[q] != tag)
+      report("load of a stale or unsynchronised value", b.name, q, tag);
+    b.rph[q] = phase;
+    return b.v[q];
+  }
+};
+
 } // namespace
 
 extern "C" {
@@ -196,64 +242,218 @@ long long cxp_host_minmod_adjoint_tiled(int order, const int it[3],
           add(i, j, k, mm_gather_one(g, cd, fv, i, j, k));
     return bad;
   }
-  const int ntx = (g.ohi[0] - g.olo[0] + MM_TILE) / MM_TILE;
-  const int nty = (g.ohi[1] - g.olo[1] + MM_TILE) / MM_TILE;
+  // k_mm_gather_tiled
+  LdsCheck ck;
+  const int ntx = (g.ohi[0] - g.olo[0] + MT_O) / MT_O;
+  const int nty = (g.ohi[1] - g.olo[1] + MT_O) / MT_O;
   const int nkc = (g.ohi[2] - g.olo[2] + kc) / kc;
-  std::vector<double> xy(MM_FP * MM_FP), xb(MM_FP * MM_TILE);
-  std::vector<char> xy_set(xy.size()), xb_set(xb.size());
-  auto xy_st = [&](int q, double v) {
-    if (q < 0 || q >= (int)xy.size()) {
-      report("xy store out of range", q, 0, 0, 0);
-      return;
-    }
-    xy[q] = v;
-    xy_set[q] = 1;
+  const int plo[3] = {rlo[0] >> 1, rlo[1] >> 1, rlo[2] >> 1};
+  const int phi[3] = {rhi[0] >> 1, rhi[1] >> 1, rhi[2] >> 1};
+  auto in_region = [&](int fx, int fy) {
+    return fx >= rlo[0] && fx <= rhi[0] && fy >= rlo[1] && fy <= rhi[1];
   };
-  auto xy_ld = [&](int q) {
-    if (q < 0 || q >= (int)xy.size() || !xy_set[q]) {
-      report("xy load out of range or unset", q, 0, 0, 0);
-      return 0.0;
-    }
-    return xy[q];
-  };
-  auto xb_st = [&](int q, double v) {
-    if (q < 0 || q >= (int)xb.size()) {
-      report("xb store out of range", q, 0, 0, 0);
-      return;
-    }
-    xb[q] = v;
-    xb_set[q] = 1;
-  };
-  auto xb_ld = [&](int q) {
-    if (q < 0 || q >= (int)xb.size() || !xb_set[q]) {
-      report("xb load out of range or unset", q, 0, 0, 0);
-      return 0.0;
-    }
-    return xb[q];
-  };
-  std::vector<MmWin> win(MM_T);
   for (int b = 0; b < ntx * nty * nkc; ++b) {
-    const MmTile t = mm_tile(g, b, ntx, nty, kc);
+    const MtTile t = mt_tile(g.olo, g.ohi, b, ntx, nty, kc);
+    Lds xy("xy", MT_FP * MT_FP), xb("xb", MT_NY);
+    std::vector<MtWin> win(MT_T);
     for (auto &w : win)
-      mm_win_zero(w);
+      mt_win_zero(w);
     for (int pz = t.k0 - 1; pz <= t.k1 + 1; ++pz) {
-      for (int tid = 0; tid < MM_T; ++tid)
-        mm_zt_add(g, t, tid, pz, win[tid], cd, fv);
+      if (pz >= plo[2] && pz <= phi[2]) {
+        const int fz0 = 2 * pz;
+        for (int tid = 0; tid < MT_T; ++tid)
+          for (int m = 0; m < MT_NPOS; ++m) {
+            const int fx = 2 * t.i0 - 2 + (tid & (MT_FP - 1));
+            const int fy = 2 * t.j0 - 2 + tid / MT_FP + (MT_T / MT_FP) * m;
+            double g0 = 0.0, g1 = 0.0;
+            unsigned char code = MM_ZERO;
+            if (in_region(fx, fy)) {
+              code = cd(2, fx, fy, pz);
+              if (fz0 >= rlo[2])
+                g0 = fv(fx, fy, fz0);
+              if (fz0 + 1 <= rhi[2])
+                g1 = fv(fx, fy, fz0 + 1);
+            }
+            mt_zt(win[tid], m, code, g0, g1);
+          }
+      }
       const int k = pz - 1;
       const bool out = k >= t.k0;
-      std::fill(xy_set.begin(), xy_set.end(), 0);
-      for (int tid = 0; tid < MM_T; ++tid)
-        mm_zt_flush(tid, win[tid], out, xy_st);
+      for (int tid = 0; tid < MT_T; ++tid)
+        mt_flush(tid, win[tid], out,
+                 [&](int q, double v) { ck.st(xy, q, v, k); });
       if (!out)
         continue;
-      std::fill(xb_set.begin(), xb_set.end(), 0);
-      for (int tid = 0; tid < MM_T; ++tid)
-        mm_yt(g, t, tid, k, xy_ld, xb_st, cd);
-      for (int tid = 0; tid < MM_T; ++tid)
-        mm_xt(g, t, tid, k, xb_ld, cd, add);
+      ++ck.phase;
+      auto byc = [&](int lx, int lp) -> unsigned char {
+        const int fx = 2 * t.i0 - 2 + lx, py = t.j0 - 1 + lp;
+        if (fx < rlo[0] || fx > rhi[0] || py < plo[1] || py > phi[1])
+          return MM_ZERO;
+        return cd(1, fx, py, k);
+      };
+      for (int tid = 0; tid < MT_T; ++tid)
+        for (int e = tid; e < MT_NY; e += MT_T)
+          ck.st(xb, e,
+                mt_yt(e, [&](int q) { return ck.ld(xy, q, k); }, byc), k);
+      ++ck.phase;
+      auto bxc = [&](int lp, int lj) -> unsigned char {
+        const int px = t.i0 - 1 + lp, j = t.j0 + lj;
+        if (px < plo[0] || px > phi[0] || j > g.ohi[1])
+          return MM_ZERO;
+        return cd(0, px, j, k);
+      };
+      for (int tid = 0; tid < MT_NX; ++tid) {
+        const int li = tid % MT_O, lj = tid / MT_O;
+        const double v =
+            mt_xt(li, lj, [&](int q) { return ck.ld(xb, q, k); }, bxc);
+        if (t.i0 + li <= g.ohi[0] && t.j0 + lj <= g.ohi[1])
+          add(t.i0 + li, t.j0 + lj, k, v);
+      }
     }
   }
-  return bad;
+  return bad + ck.bad;
+}
+
+// The fused CCC adjoint (k_mm_fused) on the host: the workgroups of z-chunks
+// of kc planes one after the other, the threads of a phase one after the
+// other, barriers as phase boundaries; LDS accesses checked as in the tiled
+// gather, fine and coarse accesses as above.  Returns the number of failed
+// checks, or -1 for an operator that is not MINMOD in all three directions.
+long long cxp_host_minmod_adjoint_fused(int order, const int it[3],
+                                        const double *crse, const int clo[3],
+                                        const int chi[3], const double *fbar,
+                                        const int rlo[3], const int rhi[3],
+                                        double *cbar, int kc) {
+  OpDesc od;
+  if (op_lookup("minmod", order, it, &od) || kc < 1)
+    return -1;
+  if (!(od.sid[0] == SID_MINMOD && od.sid[1] == SID_MINMOD &&
+        od.sid[2] == SID_MINMOD))
+    return -1;
+  MmGeom g;
+  mm_setup(g, od.sid, rlo, rhi);
+  LdsCheck ck;
+  auto cv = [&](int i, int j, int k) {
+    if (!inside(clo, chi, i, j, k)) {
+      ck.report("coarse read outside crse", "crse", i, k);
+      return 0.0;
+    }
+    return crse[at(clo, chi, i, j, k)];
+  };
+  auto fv = [&](int x, int y, int z) {
+    if (!inside(rlo, rhi, x, y, z)) {
+      ck.report("fine read outside the region", "fbar", x, z);
+      return 0.0;
+    }
+    return fbar[at(rlo, rhi, x, y, z)];
+  };
+  auto add = [&](int i, int j, int k, double v) {
+    if (!inside(clo, chi, i, j, k) || i < g.olo[0] || j < g.olo[1] ||
+        k < g.olo[2] || i > g.ohi[0] || j > g.ohi[1] || k > g.ohi[2]) {
+      ck.report("coarse write outside the output window", "cbar", i, k);
+      return;
+    }
+    cbar[at(clo, chi, i, j, k)] += v;
+  };
+  const int ntx = (g.ohi[0] - g.olo[0] + MT_O) / MT_O;
+  const int nty = (g.ohi[1] - g.olo[1] + MT_O) / MT_O;
+  const int nkc = (g.ohi[2] - g.olo[2] + kc) / kc;
+  const int plo2 = rlo[2] >> 1, phi2 = rhi[2] >> 1;
+  const int nbx = MT_PB * MT_O, nby = MT_FP * MT_PB;
+  for (int b = 0; b < ntx * nty * nkc; ++b) {
+    const MtTile t = mt_tile(g.olo, g.ohi, b, ntx, nty, kc);
+    Lds cpl("cpl", MT_CW * MT_CW), xpl("xpl", MT_FP * MT_CW),
+        xy("xy", MT_FP * MT_FP), xb("xb", MT_NY), bxr("bx", MT_RING * nbx),
+        byr("by", MT_RING * nby);
+    std::vector<MtWin> win(MT_T);
+    std::vector<MtXY> xyv(MT_T);
+    for (int tid = 0; tid < MT_T; ++tid) {
+      mt_win_zero(win[tid]);
+      mt_xy_zero(xyv[tid]);
+    }
+    for (int s = t.k0 - 2; s <= t.k1 + 3; ++s) {
+      const bool produce = s <= t.k1 + 2;
+      const bool own = s >= t.k0 && s <= t.k1;
+      const int k = s - 3;
+      const long long bxs = (long long)(s & (MT_RING - 1)) * nbx;
+      const long long bys = (long long)(s & (MT_RING - 1)) * nby;
+      const long long bxk = (long long)(k & (MT_RING - 1)) * nbx;
+      const long long byk = (long long)(k & (MT_RING - 1)) * nby;
+      // A
+      if (produce)
+        for (int tid = 0; tid < MT_T; ++tid)
+          mt_load(g.olo, g.ohi, t, tid, s, cv,
+                  [&](int q, double v) { ck.st(cpl, q, v, s); });
+      ++ck.phase;
+      // B
+      for (int tid = 0; tid < MT_T; ++tid) {
+        if (produce)
+          mt_x(tid, [&](int q) { return ck.ld(cpl, q, s); },
+               [&](int q, double v) { ck.st(xpl, q, v, s); }, own,
+               [&](int q, unsigned char c) {
+                 if (q < 0 || q >= nbx)
+                   ck.report("bx store outside its slot", "bx", q, s);
+                 ck.st(bxr, bxs + q, c, s);
+               });
+        if (k >= t.k0)
+          for (int e = tid; e < MT_NY; e += MT_T)
+            ck.st(xb, e,
+                  mt_yt(e, [&](int q) { return ck.ld(xy, q, k); },
+                        [&](int lx, int lp) {
+                          const long long q = (long long)lp * MT_FP + lx;
+                          if (q < 0 || q >= nby)
+                            ck.report("by load outside its slot", "by", q, k);
+                          return (unsigned char)ck.ld(byr, byk + q, k);
+                        }),
+                  k);
+      }
+      ++ck.phase;
+      // C
+      for (int tid = 0; tid < MT_T; ++tid) {
+        if (produce) {
+          if (own)
+            mt_by(tid, [&](int q) { return ck.ld(xpl, q, s); },
+                  [&](int q, unsigned char c) {
+                    if (q < 0 || q >= nby)
+                      ck.report("by store outside its slot", "by", q, s);
+                    ck.st(byr, bys + q, c, s);
+                  });
+          mt_xy(tid, [&](int q) { return ck.ld(xpl, q, s); }, xyv[tid]);
+        }
+        if (k >= t.k0 && tid < MT_NX) {
+          const int li = tid % MT_O, lj = tid / MT_O;
+          const double v = mt_xt(
+              li, lj, [&](int q) { return ck.ld(xb, q, k); },
+              [&](int lp, int ljj) {
+                const long long q = (long long)ljj * MT_PB + lp;
+                if (q < 0 || q >= nbx)
+                  ck.report("bx load outside its slot", "bx", q, k);
+                return (unsigned char)ck.ld(bxr, bxk + q, k);
+              });
+          if (t.i0 + li <= g.ohi[0] && t.j0 + lj <= g.ohi[1])
+            add(t.i0 + li, t.j0 + lj, k, v);
+        }
+        const int pz = s - 1;
+        if (pz >= t.k0 - 1 && pz <= t.k1 + 1 && pz >= plo2 && pz <= phi2) {
+          const int fz0 = 2 * pz;
+          for (int m = 0; m < MT_NPOS; ++m) {
+            const int fx = 2 * t.i0 - 2 + (tid & (MT_FP - 1));
+            const int fy = 2 * t.j0 - 2 + tid / MT_FP + (MT_T / MT_FP) * m;
+            const bool in = fx >= rlo[0] && fx <= rhi[0] && fy >= rlo[1] &&
+                            fy <= rhi[1];
+            const double g0 = in && fz0 >= rlo[2] ? fv(fx, fy, fz0) : 0.0;
+            const double g1 = in && fz0 + 1 <= rhi[2] ? fv(fx, fy, fz0 + 1) : 0.0;
+            mt_zt(win[tid], m,
+                  mm_code(xyv[tid].v[m][0], xyv[tid].v[m][1], xyv[tid].v[m][2]),
+                  g0, g1);
+          }
+        }
+        mt_flush(tid, win[tid], s - 2 >= t.k0 && s - 2 <= t.k1,
+                 [&](int q, double v) { ck.st(xy, q, v, s - 2); });
+      }
+    }
+  }
+  return ck.bad;
 }
 
 } // extern "C"

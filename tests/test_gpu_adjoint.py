@@ -221,10 +221,12 @@ def test_minmod_adjoint_equals_its_host_build(order):
     """Every centring with a MINMOD direction: the device kernels give the
     bits of the same record and gather routines built for the host with
     bounds-checked accesses (carpetx_amd/host/minmod_adjoint_host.cpp),
-    which the CPU tests pin to the oracle's Jacobian.  VVV has no MINMOD
-    direction and runs the linear adjoint, which sums in another order."""
-    from test_minmod_adjoint_host import host_adjoint
-    rlo, rhi = (-5, 2, -3), (21, 15, 12)
+    which the CPU tests pin to the oracle's Jacobian; CCC against the tiled
+    gather's host build (several tiles and z-chunks), the others against
+    the per-point gather.  VVV has no MINMOD direction and runs the linear
+    adjoint, which sums in another order."""
+    from test_minmod_adjoint_host import host_adjoint, host_adjoint_tiled
+    rlo, rhi = (-5, 2, -3), (40, 37, 12)
     for c in range(8):
         it = ((c >> 2) & 1, (c >> 1) & 1, c & 1)
         rng = np.random.default_rng(50 + 8 * order + c)
@@ -232,7 +234,10 @@ def test_minmod_adjoint_equals_its_host_build(order):
         cb = P.CoarseBox(Box(rlo, rhi))
         u = rng.uniform(-1, 1, tuple(reversed(cb.size)))
         v = rng.uniform(-1, 1, tuple(reversed(Box(rlo, rhi).size)))
-        bad, want, _ = host_adjoint(order, it, u, cb.lo, cb.hi, v, rlo, rhi)
+        if all(it):
+            bad, want = host_adjoint_tiled(order, it, u, cb.lo, cb.hi, v, rlo, rhi, 5, 3)
+        else:
+            bad, want, _ = host_adjoint(order, it, u, cb.lo, cb.hi, v, rlo, rhi)
         assert bad == 0
         crse = FArrayBox(cb, 1, tensor=torch.from_numpy(u[None]).cuda())
         vb = FArrayBox(Box(rlo, rhi), 1, tensor=torch.from_numpy(v[None]).cuda())

@@ -159,3 +159,53 @@ def test_host_tiled_gather_against_the_jacobian():
     assert bad == 0
     want = oracle_jt(3, it, crse, clo, chi, fbar, rlo, rhi)
     assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+
+
+def host_adjoint_fused(order, it, crse, clo, chi, fbar, rlo, rhi, kc):
+    L = host_lib()
+    fn = L.cxp_host_minmod_adjoint_fused
+    fn.restype = ctypes.c_longlong
+    fn.argtypes = [ctypes.c_int, I3, PD, I3, I3, PD, I3, I3, PD, ctypes.c_int]
+    cbar = np.zeros_like(crse)
+    bad = fn(order, I3(*it), _ptr(crse), I3(*clo), I3(*chi), _ptr(fbar), I3(*rlo), I3(*rhi),
+             _ptr(cbar), kc)
+    return bad, cbar
+
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+@pytest.mark.parametrize("rlo,rhi", [((-5, 2, -3), (40, 37, 8)),      # ragged, odd corners
+                                      ((0, 0, 0), (31, 31, 15)),      # tile-aligned
+                                      ((3, -7, 1), (4, -6, 2))])      # 2 x 2 x 2 fine points
+def test_host_fused_adjoint_equals_record_and_tiled_gather(order, rlo, rhi):
+    """The fused CCC march (branches recomputed per workgroup, rings of
+    codes in LDS): no failed check (every LDS read sees the value of its
+    plane written in an earlier phase, no element written where it is read
+    or twice in a phase, no access out of range) and the bits of record +
+    tiled gather, for z-chunks of 1, 2, 5 and 16 planes."""
+    it = (1, 1, 1)
+    op = oracle.lookup("minmod", order, it)
+    clo, chi = oracle.coarse_box(op, rlo, rhi)
+    rng = np.random.default_rng(900 + order)
+    crse = rng.uniform(-1, 1, tuple(chi[d] - clo[d] + 1 for d in (2, 1, 0)))
+    fbar = rng.uniform(-1, 1, tuple(rhi[d] - rlo[d] + 1 for d in (2, 1, 0)))
+    bad0, want = host_adjoint_tiled(order, it, crse, clo, chi, fbar, rlo, rhi, 4, 3)
+    assert bad0 == 0
+    for kc in (1, 2, 5, 16):
+        bad, got = host_adjoint_fused(order, it, crse, clo, chi, fbar, rlo, rhi, kc)
+        assert bad == 0, kc
+        assert np.array_equal(got, want), kc
+
+
+def test_host_fused_adjoint_against_the_jacobian():
+    rlo, rhi = (1, -2, 0), (34, 9, 7)
+    it = (1, 1, 1)
+    op = oracle.lookup("minmod", 3, it)
+    clo, chi = oracle.coarse_box(op, rlo, rhi)
+    rng = np.random.default_rng(78)
+    crse = dyadic(rng, tuple(chi[d] - clo[d] + 1 for d in (2, 1, 0)))
+    fbar = dyadic(rng, tuple(rhi[d] - rlo[d] + 1 for d in (2, 1, 0)))
+    bad, got = host_adjoint_fused(3, it, crse, clo, chi, fbar, rlo, rhi, 3)
+    assert bad == 0
+    want = oracle_jt(3, it, crse, clo, chi, fbar, rlo, rhi)
+    assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+    assert host_adjoint_fused(3, (1, 0, 1), crse, clo, chi, fbar, rlo, rhi, 3)[0] == -1
