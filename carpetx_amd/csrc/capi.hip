@@ -658,7 +658,7 @@ int minmod_adjoint_launch(const cxp_op_s *op, const cxp_fab *crse_in,
   if (const int e = adjoint_point_check(op, crse_in, crse_comp, fine_bar,
                                         fine_comp, ncomp, rlo, rhi, per_group))
     return e;
-  if (op_all_minmod(op->d))
+  if (op_all_minmod(op->d) && mm_tiled_fits(fine_bar))
     return mm_fused_launch(op->d, crse_in, crse_bar, crse_comp, fine_bar,
                            fine_comp, ncomp, rlo, rhi, s);
   const long long nbytes = mm_code_bytes(op->d, rlo, rhi, ncomp);
@@ -1591,7 +1591,7 @@ int cxp_prolongate_adjoint_noalloc(cxp_op op, const cxp_fab *crse_in,
                                            fine_comp, ncomp, rlo, rhi,
                                            per_group))
       return e2;
-    if (op_all_minmod(od))  // the fused CCC adjoint keeps no codes
+    if (op_all_minmod(od) && mm_tiled_fits(fine_bar))  // keeps no codes
       return minmod_adjoint_launch(op, crse_in, crse_bar, crse_comp, fine_bar,
                                    fine_comp, ncomp, rlo, rhi, per_group,
                                    (hipStream_t)stream);

@@ -9,6 +9,10 @@
 // One component per grid row y.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "minmod_adjoint.hpp"
 #include "minmod_launch.hpp"
 
@@ -18,11 +22,17 @@ namespace {
 
 constexpr int kMmRecordChunk = 12;  // codes per column piece
 
-// Workgroups the tiled CCC kernels aim for; their z-chunk follows.
-#ifndef CXP_MM_TILED_BLOCKS
-#define CXP_MM_TILED_BLOCKS 1024
-#endif
-constexpr int kMtMinDepth = 2, kMtMaxDepth = 32;
+// z-chunks of the tiled CCC kernels: at least this many planes (a chunk
+// recomputes 4 planes of X and XY and re-reads 2 parents of fine data).
+constexpr int kMtMinDepth = 2;
+
+// Blocks are handed to the 8 XCDs round robin (block i to XCD i % 8); the
+// tiled kernels renumber them so that each XCD runs a contiguous range of
+// tiles, whose footprints overlap in its L2.
+__device__ inline int mt_block(int i, int nb) {
+  const int q = nb / 8, r = nb % 8, x = i % 8;
+  return x * q + (x < r ? x : r) + i / 8;
+}
 
 struct MmData {
   DFabC crse;  // forward input (record only)
@@ -93,7 +103,7 @@ __global__ void __launch_bounds__(256)
 // (0 for positions outside the region, whose loads are masked) and the
 // in-region flags.
 struct MtPos {
-  long long off[MT_NPOS];
+  int off[MT_NPOS];  // the launch checks that a plane has < 2^31 points
   bool in[MT_NPOS];
 };
 
@@ -106,7 +116,8 @@ __device__ inline MtPos mt_positions(const MtArgs &a, const MtTile &t,
   for (int m = 0; m < MT_NPOS; ++m) {
     const int fy = 2 * t.j0 - 2 + tid / MT_FP + (MT_T / MT_FP) * m;
     p.in[m] = xin && fy >= a.rlo[1] && fy <= a.rhi[1];
-    p.off[m] = p.in[m] ? (fx - a.f_lo[0]) + (fy - a.f_lo[1]) * a.f_sy : 0;
+    p.off[m] = p.in[m] ? (int)((fx - a.f_lo[0]) + (fy - a.f_lo[1]) * a.f_sy)
+                       : 0;
   }
   return p;
 }
@@ -135,7 +146,8 @@ __global__ void __launch_bounds__(MT_T) k_mm_gather_tiled(MtArgs a) {
   __shared__ double xb[MT_NY];
   const int tid = threadIdx.x;
   const int comp = blockIdx.y;
-  const MtTile t = mt_tile(a.olo, a.ohi, blockIdx.x, a.ntx, a.nty, a.kc);
+  const MtTile t = mt_tile(a.olo, a.ohi, mt_block(blockIdx.x, gridDim.x),
+                           a.ntx, a.nty, a.kc);
   const double *fp = a.fbar + comp * a.fcs;
   double *bp = a.cbar + comp * a.bcs;
   const unsigned char *cd = a.codes + comp * a.ncodes;
@@ -144,39 +156,44 @@ __global__ void __launch_bounds__(MT_T) k_mm_gather_tiled(MtArgs a) {
   const bool xin = fx >= a.rlo[0] && fx <= a.rhi[0];
   const int plo[3] = {a.rlo[0] >> 1, a.rlo[1] >> 1, a.rlo[2] >> 1};
   const int phi[3] = {a.rhi[0] >> 1, a.rhi[1] >> 1, a.rhi[2] >> 1};
-  auto xy_st = [&](int q, double v) { xy[q] = v; };
+  // in-plane part of each position's Bz index (its plane has < 2^31 codes:
+  // fewer than the fine plane's points)
+  int bzo[MT_NPOS];
+#pragma unroll
+  for (int m = 0; m < MT_NPOS; ++m) {
+    const int fy = 2 * t.j0 - 2 + tid / MT_FP + (MT_T / MT_FP) * m;
+    bzo[m] = pos.in[m] ? (fy - a.blo[2][1]) * a.bn[2][0] + (fx - a.blo[2][0])
+                       : 0;
+  }
+  const long long bz_plane = (long long)a.bn[2][1] * a.bn[2][0];
   auto xy_ld = [&](int q) { return xy[q]; };
   auto xb_ld = [&](int q) { return xb[q]; };
   MtWin win;
   mt_win_zero(win);
   for (int pz = t.k0 - 1; pz <= t.k1 + 1; ++pz) {
-    if (pz >= plo[2] && pz <= phi[2]) {
-      const int fz0 = 2 * pz;
-      const bool in0 = fz0 >= a.rlo[2], in1 = fz0 + 1 <= a.rhi[2];
-      const double *pl0 = fp + (fz0 - a.f_lo[2]) * a.f_sz;
-      const double *pl1 = pl0 + a.f_sz;
-#pragma unroll
-      for (int m = 0; m < MT_NPOS; ++m) {
-        double g0 = 0.0, g1 = 0.0;
-        unsigned char code = MM_ZERO;
-        if (pos.in[m]) {
-          const int fy = 2 * t.j0 - 2 + tid / MT_FP + (MT_T / MT_FP) * m;
-          code = cd[a.boff[2] +
-                    ((long long)(pz - a.blo[2][2]) * a.bn[2][1] +
-                     (fy - a.blo[2][1])) *
-                        a.bn[2][0] +
-                    (fx - a.blo[2][0])];
-          if (in0)
-            g0 = pl0[pos.off[m]];
-          if (in1)
-            g1 = pl1[pos.off[m]];
-        }
-        mt_zt(win, m, code, g0, g1);
-      }
-    }
+    const bool zin = pz >= plo[2] && pz <= phi[2];
+    const int fz0 = 2 * pz;
+    const bool in0 = zin && fz0 >= a.rlo[2], in1 = zin && fz0 + 1 <= a.rhi[2];
+    const double *pl0 = fp + (fz0 - a.f_lo[2]) * a.f_sz;
+    const double *pl1 = pl0 + a.f_sz;
     const int k = pz - 1;
     const bool out = k >= t.k0;
-    mt_flush(tid, win, out, xy_st);
+    const unsigned char *bzp = cd + a.boff[2] + (pz - a.blo[2][2]) * bz_plane;
+#pragma unroll
+    for (int m = 0; m < MT_NPOS; ++m) {
+      double g0 = 0.0, g1 = 0.0;
+      unsigned char code = MM_ZERO;
+      if (zin && pos.in[m]) {
+        code = bzp[bzo[m]];
+        if (in0)
+          g0 = pl0[pos.off[m]];
+        if (in1)
+          g1 = pl1[pos.off[m]];
+      }
+      const double done = mt_zt(win, m, code, g0, g1);
+      if (out)
+        xy[tid + MT_T * m] = done;
+    }
     if (!out)
       continue;
     __syncthreads();
@@ -190,8 +207,7 @@ __global__ void __launch_bounds__(MT_T) k_mm_gather_tiled(MtArgs a) {
                     a.bn[1][0] +
                 (2 * t.i0 - 2 + lx - a.blo[1][0])];
     };
-    for (int e = tid; e < MT_NY; e += MT_T)
-      xb[e] = mt_yt(e, xy_ld, byc);
+    mt_yt_all(tid, xy_ld, byc, [&](int e, double v) { xb[e] = v; });
     __syncthreads();
     auto bxc = [&](int lp, int lj) -> unsigned char {
       const int px = t.i0 - 1 + lp, j = t.j0 + lj;
@@ -207,10 +223,72 @@ __global__ void __launch_bounds__(MT_T) k_mm_gather_tiled(MtArgs a) {
   }
 }
 
+// Tiled record of a CCC operator's codes (minmod_adjoint.hpp): each tile
+// stores the codes it owns.  g carries the code arrays' boxes.
+__global__ void __launch_bounds__(MT_T) k_mm_record_tiled(MtArgs a, MmGeom g) {
+  __shared__ double cpl[MT_CW * MT_CW];
+  __shared__ double xpl[MT_FP * MT_CW];
+  const int tid = threadIdx.x;
+  const int comp = blockIdx.y;
+  const MtTile t = mt_tile(a.olo, a.ohi, mt_block(blockIdx.x, gridDim.x),
+                           a.ntx, a.nty, a.kc);
+  const double *cp = a.crse + comp * a.ccs;
+  unsigned char *cd = const_cast<unsigned char *>(a.codes) + comp * a.ncodes;
+  const int plo2 = a.rlo[2] >> 1, phi2 = a.rhi[2] >> 1;
+  auto cv = [&](int x, int y, int z) {
+    return cp[(x - a.c_lo[0]) + (y - a.c_lo[1]) * a.c_sy +
+              (z - a.c_lo[2]) * a.c_sz];
+  };
+  auto cpl_ld = [&](int q) { return cpl[q]; };
+  auto xpl_st = [&](int q, double v) { xpl[q] = v; };
+  auto xpl_ld = [&](int q) { return xpl[q]; };
+  MtXY xyv;
+  mt_xy_zero(xyv);
+  for (int s = t.k0 - 1; s <= t.k1 + 1; ++s) {
+    const bool own = s >= t.k0 && s <= t.k1;
+    double cl[MT_NCL];
+#pragma unroll
+    for (int n = 0; n < MT_NCL; ++n) {
+      const int q = tid + MT_T * n;
+      cl[n] = q < MT_CW * MT_CW ? mt_coarse(a.olo, a.ohi, t, q, s, cv) : 0.0;
+    }
+#pragma unroll
+    for (int n = 0; n < MT_NCL; ++n) {
+      const int q = tid + MT_T * n;
+      if (q < MT_CW * MT_CW)
+        cpl[q] = cl[n];
+    }
+    __syncthreads();
+    mt_x(tid, cpl_ld, xpl_st, own, [&](int q, unsigned char c) {
+      int px, j;
+      if (mt_owns_bx(g, t, q, px, j))
+        cd[mm_code_index(g, 0, px, j, s)] = c;
+    });
+    __syncthreads();
+    if (own)
+      mt_by(tid, xpl_ld, [&](int q, unsigned char c) {
+        int fx, py;
+        if (mt_owns_by(g, t, q, fx, py))
+          cd[mm_code_index(g, 1, fx, py, s)] = c;
+      });
+    unsigned char bz[MT_NPOS];
+    mt_xy(tid, xpl_ld, xyv, bz);
+    const int pz = s - 1;
+    if (pz >= t.k0 && pz <= t.k1 && pz >= plo2 && pz <= phi2)
+#pragma unroll
+      for (int m = 0; m < MT_NPOS; ++m) {
+        int fx, fy;
+        if (mt_owns_bz(g, t, tid, m, fx, fy))
+          cd[mm_code_index(g, 2, fx, fy, pz)] = bz[m];
+      }
+  }
+}
+
 // Fused CCC adjoint at crse_in (the march of minmod_adjoint.hpp: phases A,
 // B, C, two barriers per coarse plane).  Block-uniform control flow: every
 // thread reaches every barrier.
-__global__ void __launch_bounds__(MT_T) k_mm_fused(MtArgs a) {
+__global__ void __launch_bounds__(MT_T)
+k_mm_fused(MtArgs a) {
   __shared__ double cpl[MT_CW * MT_CW];
   __shared__ double xpl[MT_FP * MT_CW];
   __shared__ double xy[MT_FP * MT_FP];
@@ -219,7 +297,8 @@ __global__ void __launch_bounds__(MT_T) k_mm_fused(MtArgs a) {
   __shared__ unsigned char byr[MT_RING][MT_FP * MT_PB];
   const int tid = threadIdx.x;
   const int comp = blockIdx.y;
-  const MtTile t = mt_tile(a.olo, a.ohi, blockIdx.x, a.ntx, a.nty, a.kc);
+  const MtTile t = mt_tile(a.olo, a.ohi, mt_block(blockIdx.x, gridDim.x),
+                           a.ntx, a.nty, a.kc);
   const double *cp = a.crse + comp * a.ccs;
   const double *fp = a.fbar + comp * a.fcs;
   double *bp = a.cbar + comp * a.bcs;
@@ -229,11 +308,9 @@ __global__ void __launch_bounds__(MT_T) k_mm_fused(MtArgs a) {
     return cp[(x - a.c_lo[0]) + (y - a.c_lo[1]) * a.c_sy +
               (z - a.c_lo[2]) * a.c_sz];
   };
-  auto cpl_st = [&](int q, double v) { cpl[q] = v; };
   auto cpl_ld = [&](int q) { return cpl[q]; };
   auto xpl_st = [&](int q, double v) { xpl[q] = v; };
   auto xpl_ld = [&](int q) { return xpl[q]; };
-  auto xy_st = [&](int q, double v) { xy[q] = v; };
   auto xy_ld = [&](int q) { return xy[q]; };
   auto xb_ld = [&](int q) { return xb[q]; };
   MtWin win;
@@ -248,41 +325,62 @@ __global__ void __launch_bounds__(MT_T) k_mm_fused(MtArgs a) {
     unsigned char *bys = byr[s & (MT_RING - 1)];
     const unsigned char *bxk = bxr[k & (MT_RING - 1)];
     const unsigned char *byk = byr[k & (MT_RING - 1)];
+    // A: this plane's coarse values and the fine adjoints of parent s - 1
+    // (used in C; the barriers wait for LDS traffic only)
+    double cl[MT_NCL];
     if (produce)
-      mt_load(a.olo, a.ohi, t, tid, s, cv, cpl_st);
+#pragma unroll
+      for (int n = 0; n < MT_NCL; ++n) {
+        const int q = tid + MT_T * n;
+        cl[n] = q < MT_CW * MT_CW ? mt_coarse(a.olo, a.ohi, t, q, s, cv) : 0.0;
+      }
+    const int pz = s - 1;
+    const bool zt =
+        pz >= t.k0 - 1 && pz <= t.k1 + 1 && pz >= plo2 && pz <= phi2;
+    double g0[MT_NPOS], g1[MT_NPOS];
+    {
+      const int fz0 = 2 * pz;
+      const bool in0 = zt && fz0 >= a.rlo[2], in1 = zt && fz0 + 1 <= a.rhi[2];
+      const double *pl0 = fp + (fz0 - a.f_lo[2]) * a.f_sz;
+      const double *pl1 = pl0 + a.f_sz;
+#pragma unroll
+      for (int m = 0; m < MT_NPOS; ++m) {
+        g0[m] = pos.in[m] && in0 ? pl0[pos.off[m]] : 0.0;
+        g1[m] = pos.in[m] && in1 ? pl1[pos.off[m]] : 0.0;
+      }
+    }
+    if (produce)
+#pragma unroll
+      for (int n = 0; n < MT_NCL; ++n) {
+        const int q = tid + MT_T * n;
+        if (q < MT_CW * MT_CW)
+          cpl[q] = cl[n];
+      }
     __syncthreads();
     if (produce)
       mt_x(tid, cpl_ld, xpl_st, own,
            [&](int q, unsigned char c) { bxs[q] = c; });
     if (k >= t.k0) {
       auto byc = [&](int lx, int lp) { return byk[lp * MT_FP + lx]; };
-      for (int e = tid; e < MT_NY; e += MT_T)
-        xb[e] = mt_yt(e, xy_ld, byc);
+      mt_yt_all(tid, xy_ld, byc, [&](int e, double v) { xb[e] = v; });
     }
     __syncthreads();
+    unsigned char bz[MT_NPOS] = {};
     if (produce) {
       if (own)
         mt_by(tid, xpl_ld, [&](int q, unsigned char c) { bys[q] = c; });
-      mt_xy(tid, xpl_ld, xyv);
+      mt_xy(tid, xpl_ld, xyv, bz);
     }
     if (k >= t.k0)
       mt_store_xt(a, t, bp, tid, k, xb_ld,
                   [&](int lp, int lj) { return bxk[lj * MT_PB + lp]; });
-    const int pz = s - 1;
-    if (pz >= t.k0 - 1 && pz <= t.k1 + 1 && pz >= plo2 && pz <= phi2) {
-      const int fz0 = 2 * pz;
-      const bool in0 = fz0 >= a.rlo[2], in1 = fz0 + 1 <= a.rhi[2];
-      const double *pl0 = fp + (fz0 - a.f_lo[2]) * a.f_sz;
-      const double *pl1 = pl0 + a.f_sz;
+    const bool store = s - 2 >= t.k0 && s - 2 <= t.k1;
 #pragma unroll
-      for (int m = 0; m < MT_NPOS; ++m) {
-        const double g0 = pos.in[m] && in0 ? pl0[pos.off[m]] : 0.0;
-        const double g1 = pos.in[m] && in1 ? pl1[pos.off[m]] : 0.0;
-        mt_zt(win, m, mm_code(xyv.v[m][0], xyv.v[m][1], xyv.v[m][2]), g0,
-              g1);
-      }
+    for (int m = 0; m < MT_NPOS; ++m) {
+      const double done = mt_zt(win, m, bz[m], g0[m], g1[m]);
+      if (store)
+        xy[tid + MT_T * m] = done;
     }
-    mt_flush(tid, win, s - 2 >= t.k0 && s - 2 <= t.k1, xy_st);
   }
 }
 
@@ -296,11 +394,35 @@ DFabC mm_dfab(const cxp_fab *f, int comp) {
   return v;
 }
 
+// Workgroups of a tiled kernel resident at once on the current device
+// (CUs x blocks per CU), cached per device and kernel.
+template <class K> long long mt_slots(K kernel, int which) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, long long> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return 1024;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({dev, which});
+  if (it != cache.end())
+    return it->second;
+  int cus = 0, per_cu = 0;
+  long long n = 1024;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, MT_T, 0) ==
+          hipSuccess &&
+      cus > 0 && per_cu > 0)
+    n = (long long)cus * per_cu;
+  cache[{dev, which}] = n;
+  return n;
+}
+
 // The tiled kernels' arguments and grid for a CCC operator.
 MtArgs mt_args(const MmGeom &g, const cxp_fab *crse_in, int crse_comp,
                const cxp_fab *crse_bar, int bar_comp, const cxp_fab *fine_bar,
                int fine_comp, int ncomp, const unsigned char *codes,
-               dim3 &grid) {
+               long long slots, dim3 &grid) {
   MtArgs a{};
   if (crse_in) {
     const DFabC c = mm_dfab(crse_in, crse_comp);
@@ -337,9 +459,15 @@ MtArgs mt_args(const MmGeom &g, const cxp_fab *crse_in, int crse_comp,
   }
   a.ntx = (g.ohi[0] - g.olo[0] + MT_O) / MT_O;
   a.nty = (g.ohi[1] - g.olo[1] + MT_O) / MT_O;
+  // As many z-chunks as fill the device's resident workgroups once: longer
+  // marches are cheaper per plane, a second partial round costs a whole
+  // chunk's time.
   const long long nk = g.ohi[2] - g.olo[2] + 1;
-  long long kc = nk * a.ntx * a.nty * ncomp / CXP_MM_TILED_BLOCKS;
-  kc = kc < kMtMinDepth ? kMtMinDepth : (kc > kMtMaxDepth ? kMtMaxDepth : kc);
+  const long long cols = (long long)a.ntx * a.nty * ncomp;
+  long long nkc = slots / cols;
+  nkc = nkc < 1 ? 1 : (nkc > nk ? nk : nkc);
+  long long kc = (nk + nkc - 1) / nkc;
+  kc = kc < kMtMinDepth ? kMtMinDepth : kc;
   a.kc = (int)kc;
   grid = dim3((unsigned)(a.ntx * a.nty * ((nk + kc - 1) / kc)), ncomp);
   return a;
@@ -361,6 +489,21 @@ int mm_record_launch(const OpDesc &op, const cxp_fab *crse_in, int crse_comp,
   mm_setup(g, op.sid, rlo, rhi);
   if (g.ncodes == 0)
     return CXP_OK;
+  if (op_all_minmod(op)) {
+    dim3 grid;
+    // the record reads no fine data; the fine box is the region's
+    cxp_fab region{};
+    for (int d = 0; d < 3; ++d) {
+      region.lo[d] = rlo[d];
+      region.hi[d] = rhi[d];
+    }
+    region.comp_stride = 1;
+    const MtArgs a =
+        mt_args(g, crse_in, crse_comp, &region, 0, &region, 0, ncomp, codes,
+                mt_slots(k_mm_record_tiled, 2), grid);
+    hipLaunchKernelGGL(k_mm_record_tiled, grid, dim3(MT_T), 0, s, a, g);
+    return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
+  }
   MmData t{};
   t.crse = mm_dfab(crse_in, crse_comp);
   t.ccs_in = crse_in->comp_stride;
@@ -380,10 +523,11 @@ int mm_gather_launch(const OpDesc &op, const cxp_fab *crse_bar, int crse_comp,
                      const unsigned char *codes, hipStream_t s) {
   MmGeom g;
   mm_setup(g, op.sid, rlo, rhi);
-  if (op_all_minmod(op)) {
+  if (op_all_minmod(op) && mm_tiled_fits(fine_bar)) {
     dim3 grid;
-    const MtArgs a = mt_args(g, nullptr, 0, crse_bar, crse_comp, fine_bar,
-                             fine_comp, ncomp, codes, grid);
+    const MtArgs a =
+        mt_args(g, nullptr, 0, crse_bar, crse_comp, fine_bar, fine_comp, ncomp,
+                codes, mt_slots(k_mm_gather_tiled, 0), grid);
     hipLaunchKernelGGL(k_mm_gather_tiled, grid, dim3(MT_T), 0, s, a);
     return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
   }
@@ -405,13 +549,14 @@ int mm_fused_launch(const OpDesc &op, const cxp_fab *crse_in,
                     const cxp_fab *crse_bar, int crse_comp,
                     const cxp_fab *fine_bar, int fine_comp, int ncomp,
                     const int rlo[3], const int rhi[3], hipStream_t s) {
-  if (!op_all_minmod(op))
+  if (!op_all_minmod(op) || !mm_tiled_fits(fine_bar))
     return CXP_ERR_UNSUPPORTED;
   MmGeom g;
   mm_setup(g, op.sid, rlo, rhi);
   dim3 grid;
-  const MtArgs a = mt_args(g, crse_in, crse_comp, crse_bar, crse_comp,
-                           fine_bar, fine_comp, ncomp, nullptr, grid);
+  const MtArgs a =
+      mt_args(g, crse_in, crse_comp, crse_bar, crse_comp, fine_bar, fine_comp,
+              ncomp, nullptr, mt_slots(k_mm_fused, 1), grid);
   hipLaunchKernelGGL(k_mm_fused, grid, dim3(MT_T), 0, s, a);
   return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
 }

@@ -57,14 +57,16 @@ __host__ __device__ inline long long dcidx(const DFabC &f, int i, int j,
 // The branch of minmod on (c-, c, c+), as the forward's arithmetic takes it
 // (stencil1d.hpp minmod_apply): no slope when the one-sided differences'
 // signs differ, else the smaller difference (the minus side on a tie).
-__host__ __device__ inline unsigned char mm_code(double cm, double c0,
-                                                 double cp) {
-  const double dm = c0 - cm;
-  const double dp = cp - c0;
+// ... from the one-sided differences dm = c - c-, dp = c+ - c
+__host__ __device__ inline unsigned char mm_code_d(double dm, double dp) {
   const double k = copysign(0.125, dm) + copysign(0.125, dp);
   if (k == 0.0)
     return MM_ZERO;
   return fabs(dm) <= fabs(dp) ? MM_MINUS : MM_PLUS;
+}
+__host__ __device__ inline unsigned char mm_code(double cm, double c0,
+                                                 double cp) {
+  return mm_code_d(c0 - cm, cp - c0);
 }
 
 // d (c + s delta) / d c(a), a in {-1, 0, +1}, for branch code at offset off.
@@ -333,8 +335,8 @@ __host__ __device__ inline void mm_record_column(const MmGeom &g, const C &c,
 //   cbar(p - 1) -= a,  cbar(p) += ((g0 + g1) + a) - b,  cbar(p + 1) += b,
 // so out(i) = (b(i - 1) + c(i)) - a(i + 1) with c = ((g0 + g1) + a) - b
 // (mt_pair).  The tiles apply it as z^T, y^T, x^T:
-//   z^T: parent pz adds -a, c, b to the XYbar planes pz - 1, pz, pz + 1 in a
-//        3-slot register window per footprint position; after pz, plane
+//   z^T: parent pz adds -a, c, b to the XYbar planes pz - 1, pz, pz + 1,
+//        kept per footprint position in registers (mt_zt); after pz, plane
 //        pz - 1 is complete and goes to LDS (the order of the adds is that
 //        of the out(i) formula);
 //   y^T: Xbar(fx, j, k) from the XYbar rows of parents j - 1 .. j + 1;
@@ -347,7 +349,7 @@ __host__ __device__ inline void mm_record_column(const MmGeom &g, const C &c,
 // A workgroup of MT_T threads owns the outputs [i0, i0 + 14) x [j0, j0 + 14)
 // of the planes [k0, k1].  Its fine footprint, the fine points whose parents
 // are within one of the tile, is 32 x 32 (fx = 2 i0 - 2 + lx, fy likewise):
-// four positions per thread, lx = tid & 31, ly = (tid >> 5) + 8 m.  LDS
+// two positions per thread, lx = tid & 31, ly = (tid >> 5) + 16 m.  LDS
 // index conventions: xy[ly * 32 + lx]; xb[lj * 32 + lx] (j = j0 + lj); By of
 // parent py = j0 - 1 + lp at [lp * 32 + lx]; Bx of parent px = i0 - 1 + lp
 // on row j0 + lj at [lj * 16 + lp].
@@ -359,10 +361,10 @@ __host__ __device__ inline void mm_record_column(const MmGeom &g, const C &c,
 // checked accessors (the threads of a workgroup one after the other between
 // barriers).
 
-#define MT_T 256
+#define MT_T 512
 #define MT_O 14                  // outputs per tile edge
 #define MT_FP 32                 // footprint edge, 2 MT_O + 4
-#define MT_NPOS 4                // footprint positions per thread
+#define MT_NPOS (MT_FP * MT_FP / MT_T)  // footprint positions per thread
 #define MT_PB 16                 // parents per tile edge, MT_O + 2
 #define MT_CW 18                 // coarse window edge and X rows, MT_O + 4
 #define MT_NY (MT_O * MT_FP)     // y^T outputs (448)
@@ -393,37 +395,29 @@ __host__ __device__ inline void mt_pair(double g0, double g1,
   c = ((g0 + g1) + a) - b;
 }
 
+// z^T state of one footprint position before parent pz: s[0] the sum so
+// far of plane pz - 1 (b(pz - 2) + c(pz - 1)), s[1] that of plane pz
+// (b(pz - 1)); plane pz + 1 has none yet.
 struct MtWin {
-  double s[MT_NPOS][3];  // XYbar of planes pz - 1, pz, pz + 1
+  double s[MT_NPOS][2];
 };
 
 __host__ __device__ inline void mt_win_zero(MtWin &w) {
   for (int m = 0; m < MT_NPOS; ++m)
-    w.s[m][0] = w.s[m][1] = w.s[m][2] = 0.0;
+    w.s[m][0] = w.s[m][1] = 0.0;
 }
 
-// z^T of one parent at position m: fine adjoints g0 (plane 2 pz), g1.
-__host__ __device__ inline void mt_zt(MtWin &w, int m, unsigned char code,
-                                      double g0, double g1) {
+// z^T of parent pz at position m (fine adjoints g0 of plane 2 pz, g1 of
+// 2 pz + 1; zeros for a parent outside the region): returns plane pz - 1,
+// now complete, and moves the window on by one plane.
+__host__ __device__ inline double mt_zt(MtWin &w, int m, unsigned char code,
+                                        double g0, double g1) {
   double a, b, c;
   mt_pair(g0, g1, code, a, b, c);
-  w.s[m][0] -= a;
-  w.s[m][1] += c;
-  w.s[m][2] += b;
-}
-
-// Stores plane slot 0 of every position with xy_st(q, v) when store is set
-// (q = ly * 32 + lx), and rotates the window.
-template <class XS>
-__host__ __device__ inline void mt_flush(int tid, MtWin &w, bool store,
-                                         const XS &xy_st) {
-  for (int m = 0; m < MT_NPOS; ++m) {
-    if (store)
-      xy_st(tid + MT_T * m, w.s[m][0]);
-    w.s[m][0] = w.s[m][1];
-    w.s[m][1] = w.s[m][2];
-    w.s[m][2] = 0.0;
-  }
+  const double done = w.s[m][0] - a;
+  w.s[m][0] = w.s[m][1] + c;
+  w.s[m][1] = b;
+  return done;
 }
 
 // y^T output e < MT_NY (lx = e & 31, j = j0 + (e >> 5)): xy_ld(q) reads
@@ -439,6 +433,19 @@ __host__ __device__ inline double mt_yt(int e, const XL &xy_ld,
             a[t], b[t], c[t]);
   }
   return (b[0] + c[1]) - a[2];
+}
+
+// All y^T outputs of thread tid: xb_st(e, mt_yt(e, ...)).
+template <class XL, class BC, class BS>
+__host__ __device__ inline void mt_yt_all(int tid, const XL &xy_ld,
+                                          const BC &byc, const BS &xb_st) {
+#pragma unroll
+  for (int n = 0; n < (MT_NY + MT_T - 1) / MT_T; ++n) {
+    const int e = tid + MT_T * n;
+    if (e >= MT_NY)
+      break;
+    xb_st(e, mt_yt(e, xy_ld, byc));
+  }
 }
 
 // x^T output (i0 + li, j0 + lj): xb_ld(q) reads Xbar, bxc(lp, lj) the Bx
@@ -468,36 +475,40 @@ __host__ __device__ inline double mt_xt(int li, int lj, const BL &xb_ld,
 //      y^T of plane s - 3 (xy, by ring) -> xb;
 //      barrier
 //   C  own: By(., ., s) -> by ring slot s % 4;
-//      XY(., ., s) -> the thread's window of XY planes s - 2 .. s;
+//      XY(., ., s) -> the thread's XY window (last value and difference),
+//      giving Bz of parent s - 1;
 //      x^T of plane s - 3 (xb, bx ring) -> crse_bar;
-//      z^T of parent s - 1 with Bz from the XY window; XYbar plane s - 2
-//      -> xy.
+//      z^T of parent s - 1 (fine loads issued in A); XYbar plane s - 2,
+//      now complete, -> xy.
 // X and XY are the forward's minmod_apply on the same values and the codes
 // mm_code's, so they are the recorded codes bit for bit wherever a fine
 // point of the region depends on them.  Ring slots are written in one phase
 // and read three phases of their kind later, with barriers between.
 
+// XY of the planes before s at one footprint position, as the Bz code
+// needs them: v the last plane's value, dm its difference to the one
+// before.
 struct MtXY {
-  double v[MT_NPOS][3];  // XY of planes s - 2, s - 1, s
+  double v[MT_NPOS], dm[MT_NPOS];
 };
 
 __host__ __device__ inline void mt_xy_zero(MtXY &x) {
   for (int m = 0; m < MT_NPOS; ++m)
-    x.v[m][0] = x.v[m][1] = x.v[m][2] = 0.0;
+    x.v[m] = x.dm[m] = 0.0;
 }
 
-// A: c(x, y, s) for points of the output window [olo, ohi], else 0.
-template <class C, class CS>
-__host__ __device__ inline void mt_load(const int olo[3], const int ohi[3],
-                                        const MtTile &t, int tid, int s,
-                                        const C &c, const CS &cpl_st) {
-  const bool zin = s >= olo[2] && s <= ohi[2];
-  for (int q = tid; q < MT_CW * MT_CW; q += MT_T) {
-    const int x = t.i0 - 2 + q % MT_CW, y = t.j0 - 2 + q / MT_CW;
-    const bool in = zin && x >= olo[0] && x <= ohi[0] && y >= olo[1] &&
-                    y <= ohi[1];
-    cpl_st(q, in ? c(x, y, s) : 0.0);
-  }
+// A: the value cpl[q] (q < MT_CW^2) of coarse plane s: c(x, y, s) for
+// points of the output window [olo, ohi], else 0.  Each thread loads
+// q = tid + MT_T n, n < MT_NCL.
+#define MT_NCL ((MT_CW * MT_CW + MT_T - 1) / MT_T)
+template <class C>
+__host__ __device__ inline double mt_coarse(const int olo[3], const int ohi[3],
+                                            const MtTile &t, int q, int s,
+                                            const C &c) {
+  const int x = t.i0 - 2 + q % MT_CW, y = t.j0 - 2 + q / MT_CW;
+  const bool in = s >= olo[2] && s <= ohi[2] && x >= olo[0] && x <= ohi[0] &&
+                  y >= olo[1] && y <= ohi[1];
+  return in ? c(x, y, s) : 0.0;
 }
 
 // B: X of the plane (xpl_st(q, v), q = r * 32 + lx, row j0 - 2 + r) and,
@@ -506,14 +517,22 @@ template <class CL, class XS, class BS>
 __host__ __device__ inline void mt_x(int tid, const CL &cpl_ld,
                                      const XS &xpl_st, bool bx,
                                      const BS &bx_st) {
-  for (int e = tid; e < MT_FP * MT_CW; e += MT_T) {
+#pragma unroll
+  for (int n = 0; n < (MT_FP * MT_CW + MT_T - 1) / MT_T; ++n) {
+    const int e = tid + MT_T * n;
+    if (e >= MT_FP * MT_CW)
+      break;
     const int lx = e & (MT_FP - 1), r = e / MT_FP;
     const int q = r * MT_CW + (lx >> 1);  // coarse x of parent - 1
     xpl_st(e, minmod_apply(cpl_ld(q), cpl_ld(q + 1), cpl_ld(q + 2), lx & 1));
   }
   if (!bx)
     return;
-  for (int e = tid; e < MT_PB * MT_O; e += MT_T) {
+#pragma unroll
+  for (int n = 0; n < (MT_PB * MT_O + MT_T - 1) / MT_T; ++n) {
+    const int e = tid + MT_T * n;
+    if (e >= MT_PB * MT_O)
+      break;
     const int lp = e & (MT_PB - 1), lj = e / MT_PB;
     const int q = (lj + 2) * MT_CW + lp;
     bx_st(e, mm_code(cpl_ld(q), cpl_ld(q + 1), cpl_ld(q + 2)));
@@ -524,26 +543,71 @@ __host__ __device__ inline void mt_x(int tid, const CL &cpl_ld,
 template <class XL, class BS>
 __host__ __device__ inline void mt_by(int tid, const XL &xpl_ld,
                                       const BS &by_st) {
-  for (int e = tid; e < MT_FP * MT_PB; e += MT_T) {
+#pragma unroll
+  for (int n = 0; n < (MT_FP * MT_PB + MT_T - 1) / MT_T; ++n) {
+    const int e = tid + MT_T * n;
+    if (e >= MT_FP * MT_PB)
+      break;
     const int lx = e & (MT_FP - 1), lp = e / MT_FP;
     by_st(e, mm_code(xpl_ld(lp * MT_FP + lx), xpl_ld((lp + 1) * MT_FP + lx),
                      xpl_ld((lp + 2) * MT_FP + lx)));
   }
 }
 
-// C: XY of the plane into the window, shifting out plane s - 3.
+// C: XY of plane s at the thread's positions; code[m] gets Bz of parent
+// s - 1 (mm_code of XY at s - 2, s - 1, s) and the window moves on.
 template <class XL>
-__host__ __device__ inline void mt_xy(int tid, const XL &xpl_ld, MtXY &xy) {
+__host__ __device__ inline void mt_xy(int tid, const XL &xpl_ld, MtXY &xy,
+                                      unsigned char (&code)[MT_NPOS]) {
   const int lx = tid & (MT_FP - 1);
   for (int m = 0; m < MT_NPOS; ++m) {
     const int ly = tid / MT_FP + (MT_T / MT_FP) * m;
     const int r = ly >> 1;
-    xy.v[m][0] = xy.v[m][1];
-    xy.v[m][1] = xy.v[m][2];
-    xy.v[m][2] = minmod_apply(xpl_ld(r * MT_FP + lx),
-                              xpl_ld((r + 1) * MT_FP + lx),
-                              xpl_ld((r + 2) * MT_FP + lx), ly & 1);
+    const double v = minmod_apply(xpl_ld(r * MT_FP + lx),
+                                  xpl_ld((r + 1) * MT_FP + lx),
+                                  xpl_ld((r + 2) * MT_FP + lx), ly & 1);
+    const double dp = v - xy.v[m];
+    code[m] = mm_code_d(xy.dm[m], dp);
+    xy.dm[m] = dp;
+    xy.v[m] = v;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled record of a CCC operator's codes: the fused march's phases A, B and
+// C for steps s = k0 - 1 .. k1 + 1, storing the codes a tile owns instead
+// of keeping them in LDS: Bx of parents i0 .. i0 + 13 on its rows, By of
+// fine x 2 i0 .. 2 i0 + 27 and parents j0 .. j0 + 13, Bz of fine x, y in
+// those ranges and parents k0 .. k1 (each code of the arrays belongs to
+// one tile).  The predicates below say which entry of mt_x's / mt_by's
+// index space, or which footprint position, is stored.
+
+__host__ __device__ inline bool mt_owns_bx(const MmGeom &g, const MtTile &t,
+                                           int q, int &px, int &j) {
+  const int lp = q % MT_PB, lj = q / MT_PB;
+  px = t.i0 - 1 + lp;
+  j = t.j0 + lj;
+  return lp >= 1 && lp <= MT_O && px >= (g.rlo[0] >> 1) &&
+         px <= (g.rhi[0] >> 1) && j <= g.ohi[1];
+}
+
+__host__ __device__ inline bool mt_owns_by(const MmGeom &g, const MtTile &t,
+                                           int q, int &fx, int &py) {
+  const int lx = q % MT_FP, lp = q / MT_FP;
+  fx = 2 * t.i0 - 2 + lx;
+  py = t.j0 - 1 + lp;
+  return lx >= 2 && lx < 2 + 2 * MT_O && lp >= 1 && lp <= MT_O &&
+         fx >= g.rlo[0] && fx <= g.rhi[0] && py >= (g.rlo[1] >> 1) &&
+         py <= (g.rhi[1] >> 1);
+}
+
+__host__ __device__ inline bool mt_owns_bz(const MmGeom &g, const MtTile &t,
+                                           int tid, int m, int &fx, int &fy) {
+  const int lx = tid % MT_FP, ly = tid / MT_FP + (MT_T / MT_FP) * m;
+  fx = 2 * t.i0 - 2 + lx;
+  fy = 2 * t.j0 - 2 + ly;
+  return lx >= 2 && lx < 2 + 2 * MT_O && ly >= 2 && ly < 2 + 2 * MT_O &&
+         fx >= g.rlo[0] && fx <= g.rhi[0] && fy >= g.rlo[1] && fy <= g.rhi[1];
 }
 
 } // namespace cxp

@@ -23,6 +23,13 @@ inline bool op_all_minmod(const OpDesc &op) {
          op.sid[2] == SID_MINMOD;
 }
 
+// The tiled CCC kernels address a plane of fine_bar with 32-bit offsets.
+inline bool mm_tiled_fits(const cxp_fab *fine_bar) {
+  return (long long)(fine_bar->hi[0] - fine_bar->lo[0] + 1) *
+             (fine_bar->hi[1] - fine_bar->lo[1] + 1) <
+         (1LL << 31);
+}
+
 // Bytes of branch codes of a MINMOD operator over the region: one per point
 // each MINMOD pass reads along its direction, per component.
 long long mm_code_bytes(const OpDesc &op, const int rlo[3], const int rhi[3],
@@ -41,7 +48,8 @@ int mm_gather_launch(const OpDesc &op, const cxp_fab *crse_bar, int crse_comp,
                      const unsigned char *codes, hipStream_t s);
 
 // crse_bar += J(crse_in)^T fine_bar for a CCC MINMOD operator in one
-// launch that recomputes the branches (no code array).
+// launch that recomputes the branches (no code array); needs
+// mm_tiled_fits(fine_bar).
 int mm_fused_launch(const OpDesc &op, const cxp_fab *crse_in,
                     const cxp_fab *crse_bar, int crse_comp,
                     const cxp_fab *fine_bar, int fine_comp, int ncomp,

@@ -259,29 +259,27 @@ long long cxp_host_minmod_adjoint_tiled(int order, const int it[3],
     for (auto &w : win)
       mt_win_zero(w);
     for (int pz = t.k0 - 1; pz <= t.k1 + 1; ++pz) {
-      if (pz >= plo[2] && pz <= phi[2]) {
-        const int fz0 = 2 * pz;
-        for (int tid = 0; tid < MT_T; ++tid)
-          for (int m = 0; m < MT_NPOS; ++m) {
-            const int fx = 2 * t.i0 - 2 + (tid & (MT_FP - 1));
-            const int fy = 2 * t.j0 - 2 + tid / MT_FP + (MT_T / MT_FP) * m;
-            double g0 = 0.0, g1 = 0.0;
-            unsigned char code = MM_ZERO;
-            if (in_region(fx, fy)) {
-              code = cd(2, fx, fy, pz);
-              if (fz0 >= rlo[2])
-                g0 = fv(fx, fy, fz0);
-              if (fz0 + 1 <= rhi[2])
-                g1 = fv(fx, fy, fz0 + 1);
-            }
-            mt_zt(win[tid], m, code, g0, g1);
-          }
-      }
+      const bool zin = pz >= plo[2] && pz <= phi[2];
+      const int fz0 = 2 * pz;
       const int k = pz - 1;
       const bool out = k >= t.k0;
       for (int tid = 0; tid < MT_T; ++tid)
-        mt_flush(tid, win[tid], out,
-                 [&](int q, double v) { ck.st(xy, q, v, k); });
+        for (int m = 0; m < MT_NPOS; ++m) {
+          const int fx = 2 * t.i0 - 2 + (tid & (MT_FP - 1));
+          const int fy = 2 * t.j0 - 2 + tid / MT_FP + (MT_T / MT_FP) * m;
+          double g0 = 0.0, g1 = 0.0;
+          unsigned char code = MM_ZERO;
+          if (zin && in_region(fx, fy)) {
+            code = cd(2, fx, fy, pz);
+            if (fz0 >= rlo[2])
+              g0 = fv(fx, fy, fz0);
+            if (fz0 + 1 <= rhi[2])
+              g1 = fv(fx, fy, fz0 + 1);
+          }
+          const double done = mt_zt(win[tid], m, code, g0, g1);
+          if (out)
+            ck.st(xy, tid + MT_T * m, done, k);
+        }
       if (!out)
         continue;
       ++ck.phase;
@@ -382,8 +380,11 @@ long long cxp_host_minmod_adjoint_fused(int order, const int it[3],
       // A
       if (produce)
         for (int tid = 0; tid < MT_T; ++tid)
-          mt_load(g.olo, g.ohi, t, tid, s, cv,
-                  [&](int q, double v) { ck.st(cpl, q, v, s); });
+          for (int n = 0; n < MT_NCL; ++n) {
+            const int q = tid + MT_T * n;
+            if (q < MT_CW * MT_CW)
+              ck.st(cpl, q, mt_coarse(g.olo, g.ohi, t, q, s, cv), s);
+          }
       ++ck.phase;
       // B
       for (int tid = 0; tid < MT_T; ++tid) {
@@ -410,6 +411,7 @@ long long cxp_host_minmod_adjoint_fused(int order, const int it[3],
       ++ck.phase;
       // C
       for (int tid = 0; tid < MT_T; ++tid) {
+        unsigned char bz[MT_NPOS] = {};
         if (produce) {
           if (own)
             mt_by(tid, [&](int q) { return ck.ld(xpl, q, s); },
@@ -418,7 +420,7 @@ long long cxp_host_minmod_adjoint_fused(int order, const int it[3],
                       ck.report("by store outside its slot", "by", q, s);
                     ck.st(byr, bys + q, c, s);
                   });
-          mt_xy(tid, [&](int q) { return ck.ld(xpl, q, s); }, xyv[tid]);
+          mt_xy(tid, [&](int q) { return ck.ld(xpl, q, s); }, xyv[tid], bz);
         }
         if (k >= t.k0 && tid < MT_NX) {
           const int li = tid % MT_O, lj = tid / MT_O;
@@ -434,25 +436,140 @@ long long cxp_host_minmod_adjoint_fused(int order, const int it[3],
             add(t.i0 + li, t.j0 + lj, k, v);
         }
         const int pz = s - 1;
-        if (pz >= t.k0 - 1 && pz <= t.k1 + 1 && pz >= plo2 && pz <= phi2) {
-          const int fz0 = 2 * pz;
-          for (int m = 0; m < MT_NPOS; ++m) {
-            const int fx = 2 * t.i0 - 2 + (tid & (MT_FP - 1));
-            const int fy = 2 * t.j0 - 2 + tid / MT_FP + (MT_T / MT_FP) * m;
-            const bool in = fx >= rlo[0] && fx <= rhi[0] && fy >= rlo[1] &&
-                            fy <= rhi[1];
-            const double g0 = in && fz0 >= rlo[2] ? fv(fx, fy, fz0) : 0.0;
-            const double g1 = in && fz0 + 1 <= rhi[2] ? fv(fx, fy, fz0 + 1) : 0.0;
-            mt_zt(win[tid], m,
-                  mm_code(xyv[tid].v[m][0], xyv[tid].v[m][1], xyv[tid].v[m][2]),
-                  g0, g1);
-          }
+        const bool zt =
+            pz >= t.k0 - 1 && pz <= t.k1 + 1 && pz >= plo2 && pz <= phi2;
+        const int fz0 = 2 * pz;
+        const bool store = s - 2 >= t.k0 && s - 2 <= t.k1;
+        for (int m = 0; m < MT_NPOS; ++m) {
+          const int fx = 2 * t.i0 - 2 + (tid & (MT_FP - 1));
+          const int fy = 2 * t.j0 - 2 + tid / MT_FP + (MT_T / MT_FP) * m;
+          const bool in = zt && fx >= rlo[0] && fx <= rhi[0] && fy >= rlo[1] &&
+                          fy <= rhi[1];
+          const double g0 = in && fz0 >= rlo[2] ? fv(fx, fy, fz0) : 0.0;
+          const double g1 = in && fz0 + 1 <= rhi[2] ? fv(fx, fy, fz0 + 1) : 0.0;
+          const double done = mt_zt(win[tid], m, bz[m], g0, g1);
+          if (store)
+            ck.st(xy, tid + MT_T * m, done, s - 2);
         }
-        mt_flush(tid, win[tid], s - 2 >= t.k0 && s - 2 <= t.k1,
-                 [&](int q, double v) { ck.st(xy, q, v, s - 2); });
       }
     }
   }
+  return ck.bad;
+}
+
+// The branch codes of one component recorded three ways: mode 0 per point
+// (mm_record_one), 1 by columns in z-chunks of kc (k_mm_record_col), 2 by
+// tiles with z-chunks of kc (k_mm_record_tiled, CCC only), into out
+// (ncodes bytes, *ncodes set).  Modes 1 and 2 check that every code is
+// written exactly once and mode 2 its LDS accesses.  Returns the number of
+// failed checks, -1 for an unknown operator or mode.
+long long cxp_host_minmod_codes(int order, const int it[3], const double *crse,
+                                const int clo[3], const int chi[3],
+                                const int rlo[3], const int rhi[3], int mode,
+                                int kc, unsigned char *out, long long cap,
+                                long long *ncodes) {
+  OpDesc od;
+  if (op_lookup("minmod", order, it, &od) || kc < 1 || mode < 0 || mode > 2)
+    return -1;
+  for (int d = 0; d < 3; ++d)
+    if (!mm_sid_ok(od.sid[d]))
+      return -1;
+  const bool all_minmod = od.sid[0] == SID_MINMOD &&
+                          od.sid[1] == SID_MINMOD && od.sid[2] == SID_MINMOD;
+  if (mode == 2 && !all_minmod)
+    return -1;
+  MmGeom g;
+  mm_setup(g, od.sid, rlo, rhi);
+  *ncodes = g.ncodes;
+  if (g.ncodes > cap)
+    return -1;
+  LdsCheck ck;
+  auto cv = [&](int i, int j, int k) {
+    if (!inside(clo, chi, i, j, k)) {
+      ck.report("coarse read outside crse", "crse", i, k);
+      return 0.0;
+    }
+    return crse[at(clo, chi, i, j, k)];
+  };
+  std::vector<int> writes((size_t)g.ncodes, 0);
+  auto store = [&](int w, int x, int y, int z, unsigned char v) {
+    int blo[3], bhi[3];
+    for (int d = 0; d < 3; ++d) {
+      blo[d] = g.blo[w][d];
+      bhi[d] = g.blo[w][d] + g.bn[w][d] - 1;
+    }
+    if (!inside(blo, bhi, x, y, z)) {
+      ck.report("code store outside its array", "codes", x, w);
+      return;
+    }
+    const long long m = mm_code_index(g, w, x, y, z);
+    ++writes[(size_t)m];
+    out[m] = v;
+  };
+  if (mode == 0) {
+    for (long long n = 0; n < g.ncodes; ++n)
+      out[n] = mm_record_one(g, cv, n);
+    return ck.bad;
+  }
+  if (mode == 1) {
+    long long pieces = 0;
+    for (int w = 0; w < 3; ++w)
+      pieces += mm_columns(g, w, kc);
+    for (long long n = 0; n < pieces; ++n)
+      mm_record_column(g, cv, store, n, kc);
+  } else {
+    const int ntx = (g.ohi[0] - g.olo[0] + MT_O) / MT_O;
+    const int nty = (g.ohi[1] - g.olo[1] + MT_O) / MT_O;
+    const int nkc = (g.ohi[2] - g.olo[2] + kc) / kc;
+    const int plo2 = rlo[2] >> 1, phi2 = rhi[2] >> 1;
+    for (int b = 0; b < ntx * nty * nkc; ++b) {
+      const MtTile t = mt_tile(g.olo, g.ohi, b, ntx, nty, kc);
+      Lds cpl("cpl", MT_CW * MT_CW), xpl("xpl", MT_FP * MT_CW);
+      std::vector<MtXY> xyv(MT_T);
+      for (auto &x : xyv)
+        mt_xy_zero(x);
+      for (int s = t.k0 - 1; s <= t.k1 + 1; ++s) {
+        const bool own = s >= t.k0 && s <= t.k1;
+        for (int tid = 0; tid < MT_T; ++tid)
+          for (int n = 0; n < MT_NCL; ++n) {
+            const int q = tid + MT_T * n;
+            if (q < MT_CW * MT_CW)
+              ck.st(cpl, q, mt_coarse(g.olo, g.ohi, t, q, s, cv), s);
+          }
+        ++ck.phase;
+        for (int tid = 0; tid < MT_T; ++tid)
+          mt_x(tid, [&](int q) { return ck.ld(cpl, q, s); },
+               [&](int q, double v) { ck.st(xpl, q, v, s); }, own,
+               [&](int q, unsigned char c) {
+                 int px, j;
+                 if (mt_owns_bx(g, t, q, px, j))
+                   store(0, px, j, s, c);
+               });
+        ++ck.phase;
+        for (int tid = 0; tid < MT_T; ++tid) {
+          if (own)
+            mt_by(tid, [&](int q) { return ck.ld(xpl, q, s); },
+                  [&](int q, unsigned char c) {
+                    int fx, py;
+                    if (mt_owns_by(g, t, q, fx, py))
+                      store(1, fx, py, s, c);
+                  });
+          unsigned char bz[MT_NPOS];
+          mt_xy(tid, [&](int q) { return ck.ld(xpl, q, s); }, xyv[tid], bz);
+          const int pz = s - 1;
+          if (pz >= t.k0 && pz <= t.k1 && pz >= plo2 && pz <= phi2)
+            for (int m = 0; m < MT_NPOS; ++m) {
+              int fx, fy;
+              if (mt_owns_bz(g, t, tid, m, fx, fy))
+                store(2, fx, fy, pz, bz[m]);
+            }
+        }
+      }
+    }
+  }
+  for (long long m = 0; m < g.ncodes; ++m)
+    if (writes[(size_t)m] != 1)
+      ck.report("code not written exactly once", "codes", m, writes[(size_t)m]);
   return ck.bad;
 }
 

@@ -295,6 +295,29 @@ def test_minmod_record_and_recorded_adjoint(it):
     assert torch.equal(a1.t, a2.t)
 
 
+@pytest.mark.parametrize("it", [(1, 1, 1), (1, 0, 1), (0, 1, 1)])
+def test_minmod_device_codes_equal_the_host_record(it):
+    """The device record (tiled for CCC, by columns else) writes, per
+    component, exactly the codes of the per-point host record
+    (cxp_host_minmod_codes mode 0), byte for byte."""
+    from test_minmod_adjoint_host import host_codes
+    rng = np.random.default_rng(31)
+    rlo, rhi = (-5, 2, -3), (40, 37, 12)
+    P, cb, fbox, u, v = _fabs(("minmod", 3, it), rlo, rhi, 2, rng)
+    crse = FArrayBox(cb, 2, tensor=torch.from_numpy(u).cuda())
+    fine = FArrayBox(fbox, 2)
+    nb = P.branch_bytes(fbox, 2)
+    br = torch.full((nb,), 255, dtype=torch.uint8, device="cuda")
+    P.interp_record(crse, 0, fine, 0, 2, fbox, br)
+    torch.cuda.synchronize()
+    got = br.cpu().numpy()
+    for comp in range(2):
+        bad, want = host_codes(3, it, np.ascontiguousarray(u[comp]), cb.lo, cb.hi, rlo, rhi,
+                               0, 1)
+        assert bad == 0 and 2 * len(want) == nb
+        assert np.array_equal(got[comp * len(want):(comp + 1) * len(want)], want), comp
+
+
 def test_restrict_adjoint_dot_product():
     from carpetx_amd.interp import restrict, restrict_adjoint
     rng = np.random.default_rng(2)

@@ -209,3 +209,41 @@ def test_host_fused_adjoint_against_the_jacobian():
     want = oracle_jt(3, it, crse, clo, chi, fbar, rlo, rhi)
     assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
     assert host_adjoint_fused(3, (1, 0, 1), crse, clo, chi, fbar, rlo, rhi, 3)[0] == -1
+
+
+def host_codes(order, it, crse, clo, chi, rlo, rhi, mode, kc):
+    L = host_lib()
+    fn = L.cxp_host_minmod_codes
+    fn.restype = ctypes.c_longlong
+    fn.argtypes = [ctypes.c_int, I3, PD, I3, I3, I3, I3, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]
+    cap = 8 * np.prod([rhi[d] - rlo[d] + 8 for d in range(3)])
+    out = np.full(int(cap), 255, np.uint8)
+    n = ctypes.c_longlong()
+    bad = fn(order, I3(*it), _ptr(crse), I3(*clo), I3(*chi), I3(*rlo), I3(*rhi), mode, kc,
+             out.ctypes.data, int(cap), ctypes.byref(n))
+    return bad, out[:n.value]
+
+
+@pytest.mark.parametrize("rlo,rhi", [((-5, 2, -3), (40, 37, 8)), ((0, 0, 0), (27, 27, 13)),
+                                      ((3, -7, 1), (4, -6, 2))])
+@pytest.mark.parametrize("c", range(8))
+def test_host_code_records_agree(rlo, rhi, c):
+    """The column record (every centring) and the tiled record (CCC) write
+    every code exactly once, with no out-of-range or unsynchronised access,
+    and give the per-point record's codes, for several z-chunks."""
+    it = ((c >> 2) & 1, (c >> 1) & 1, c & 1)
+    op = oracle.lookup("minmod", 3, it)
+    clo, chi = oracle.coarse_box(op, rlo, rhi)
+    rng = np.random.default_rng(40 + c)
+    crse = rng.uniform(-1, 1, tuple(chi[d] - clo[d] + 1 for d in (2, 1, 0)))
+    bad, want = host_codes(3, it, crse, clo, chi, rlo, rhi, 0, 1)
+    assert bad == 0 and (len(want) > 0) == any(it)
+    assert np.all(want <= 2)
+    for mode, kcs in ((1, (1, 5, 12)), (2, (1, 2, 5, 16))):
+        if mode == 2 and not all(it):
+            continue
+        for kc in kcs:
+            bad, got = host_codes(3, it, crse, clo, chi, rlo, rhi, mode, kc)
+            assert bad == 0, (mode, kc)
+            assert np.array_equal(got, want), (mode, kc)
