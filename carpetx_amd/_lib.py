@@ -137,6 +137,8 @@ _SIGS = {
     "cxp_autotune": (c_int, [c_void_p, c_int, c_void_p]),
     "cxp_tune_save": (c_int, [c_char_p]),
     "cxp_tune_load": (c_int, [c_char_p]),
+    "cxp_tune_set": (c_int, [c_void_p, P_INT, P_INT, c_int, c_int]),
+    "cxp_tune_clear": (c_int, []),
     "cxp_tile_config_count": (c_int, []),
     "cxp_tile_config": (c_int, [c_int, P_INT, P_INT, P_INT]),
     # include/cxp_tag.h

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcxp.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["capi.hip", "tag.hip", "reflux.hip", "minmod_adjoint.hip"]
+SOURCES = ["capi.hip", "tag.hip", "reflux.hip", "minmod_adjoint.hip", "eno_ccc.hip"]
 
 # FMA contraction stays off so the tensor passes reproduce the reference's
 # per-point arithmetic bit for bit; packed FP32 instructions are disabled.
@@ -109,6 +109,21 @@ def build_minmod_host():
     os.replace(tmp, MINMOD_HOST)
 
 
+ENO_HOST = os.path.join(HERE, "host", "libeno_ccc_host.so")
+
+
+def build_eno_host():
+    """The CCC ENO routines built for the host with checked accesses
+    (host/eno_ccc_host.cpp): the CPU check of the kernels' indexing, loaded
+    by the tests only."""
+    tmp = ENO_HOST + ".tmp"
+    run([HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17",
+         "-ffp-contract=off", "-fPIC", "-shared",
+         "-Wno-unused-command-line-argument",
+         os.path.join(HERE, "host", "eno_ccc_host.cpp"), "-o", tmp])
+    os.replace(tmp, ENO_HOST)
+
+
 def build_oracle():
     run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"])
 
@@ -119,6 +134,7 @@ def main():
     build_wave_lib()
     build_host_check()
     build_minmod_host()
+    build_eno_host()
     build_interpolater_check()
     print("built %s in %.1fs" % (LIB, time.time() - t0))
 

@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(256)
   __shared__ double Fs[2 * CXP_ADJ_WMAX * CXP_ADJ_HALF]; // [parity][row][x/2]
   __shared__ double Qs[CXP_ADJ_WMAX * CXP_ADJ_T];        // [row][cx]
   const int tid = threadIdx.x;
-  const int b = blockIdx.x;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
   const long long comp = blockIdx.y;
   const int ci0 = t.olo[0] + tx * CXP_ADJ_T;
@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(256)
     }
     T[e] = v;
   }
-  const int b = blockIdx.x;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int tv[3] = {b % ntx, (b / ntx) % nty, b / (ntx * nty)};
   const int tsz[3] = {CXP_ADJ_TX, CXP_ADJ_TY, CXP_ADJ_TZ};
   int c0[3], c1[3], w0[3], w1[3], wn[3];

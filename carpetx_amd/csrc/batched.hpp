@@ -47,11 +47,18 @@ __device__ __forceinline__ void fill_one(const SepTask *__restrict__ tasks,
                                          int n, double *smem) {
   const SepTile tl = tiles[n];
   const SepTask t = tasks[tl.task];
-  if (tl.cfg < 0)
+  if (tl.cfg < 0) {
     thin_tile(t, tl.comp, tl.x0, tl.y0, smem);
-  else
-    sep_tile<SU>(make_geom(tl.cfg), t, tl.comp, tl.x0, tl.y0, tl.z0, tl.z1,
-                 smem);
+  } else if (tl.cfg < 16) {
+    sep_tile<SU, 1>(make_geom(tl.cfg), t, tl.comp, tl.x0, tl.y0, tl.z0,
+                    tl.z1, smem);
+  } else if constexpr (SU >= 0) {
+    // two row pairs per thread: lists of one stencil only (the planner
+    // keeps mixed lists to one row pair, whose registers the dispatching
+    // kernel could not hold for both)
+    sep_tile<SU, 2>(make_geom(tl.cfg - 16, 2), t, tl.comp, tl.x0, tl.y0,
+                    tl.z0, tl.z1, smem);
+  }
   __syncthreads();  // the next tile rewrites the LDS buffers
 }
 

@@ -18,6 +18,7 @@
 #include "batched.hpp"
 #include "ops.hpp"
 #include "adjoint.hpp"
+#include "eno_launch.hpp"
 
 using namespace cxp;
 
@@ -200,25 +201,31 @@ int hip_status(hipError_t e) { return e == hipSuccess ? CXP_OK : CXP_ERR_HIP; }
 // Tile-shape selection
 
 struct TileCfg {
-  int tx, ty, lx;
+  int tx, ty, lx, rp;
 };
-// Tile shapes: TX x TY fine points per plane, TX * TY = 256 (two fine y per
-// thread).  Every one fits the kernel's LDS budget (checked at first use).
+// Tile shapes: TX x TY fine points per plane, TX * TY = 256 RP (2 RP fine y
+// per thread).  Every one fits the kernel's LDS budget (checked at first
+// use).
 const TileCfg kCfgs[] = {
-    {64, 4, 6}, {32, 8, 5}, {16, 16, 4}, {8, 32, 3}, {4, 64, 2},
+    {64, 4, 6, 1},  {32, 8, 5, 1},  {16, 16, 4, 1}, {8, 32, 3, 1},
+    {4, 64, 2, 1},  {64, 8, 6, 2},  {32, 16, 5, 2}, {16, 32, 4, 2},
+    {8, 64, 3, 2},
 };
 constexpr int kNumTileConfigs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-int cfg_code(int cfg) { return kCfgs[cfg].lx; }
+// A list tile's shape code (SepTile::cfg): log2 TX, plus 16 for RP = 2.
+int cfg_code(int cfg) { return kCfgs[cfg].lx + (kCfgs[cfg].rp == 2 ? 16 : 0); }
 
 bool cfgs_checked() {
   for (int c = 0; c < kNumTileConfigs; ++c) {
-    const Geom g = make_geom(kCfgs[c].lx);
-    const int max_coarse = (kCfgs[c].tx / 2 + 9) * (kCfgs[c].ty / 2 + 9);
-    if (geom_lds_doubles(g) > kLdsDoubles || g.tx * g.ty != 2 * CXP_NT ||
-        max_coarse > CXP_PRE * CXP_NT || g.tx != kCfgs[c].tx ||
-        g.ty != kCfgs[c].ty || (kCfgs[c].tx / 2 + 9) > g.cxs ||
-        (kCfgs[c].ty / 2 + 9) > g.cys)
+    const TileCfg &k = kCfgs[c];
+    const Geom g = make_geom(k.lx, k.rp);
+    const int max_coarse = (k.tx / 2 + 9) * (k.ty / 2 + 9);
+    if ((k.rp != 1 && k.rp != 2) || geom_lds_doubles(g) > kLdsDoubles ||
+        g.tx * g.ty != 2 * k.rp * CXP_NT ||
+        max_coarse > sep_pre(k.rp) * CXP_NT ||
+        g.tx != k.tx || g.ty != k.ty || (k.tx / 2 + 9) > g.cxs ||
+        (k.ty / 2 + 9) > g.cys || k.tx < 4 || (k.rp == 2 && k.tx < 8))
       return false;
   }
   return true;
@@ -394,15 +401,23 @@ int launch_sep_box(const SepTask &t, int ncomp, hipStream_t s) {
     switch (t.sid[0]) {
 #define CXP_SB(S)                                                              \
   case S:                                                                      \
-    hipLaunchKernelGGL(k_sep_box<S>, grid, dim3(CXP_NT), 0, s, t, c.lx, ntx,  \
-                       nty, zchunk);                                           \
+    if (c.rp == 2)                                                             \
+      hipLaunchKernelGGL((k_sep_box<S, 2>), grid, dim3(CXP_NT), 0, s, t, c.lx, \
+                         ntx, nty, zchunk);                                    \
+    else                                                                       \
+      hipLaunchKernelGGL((k_sep_box<S, 1>), grid, dim3(CXP_NT), 0, s, t, c.lx, \
+                         ntx, nty, zchunk);                                    \
     return hip_status(hipGetLastError());
       CXP_LINEAR_SIDS(CXP_SB)
 #undef CXP_SB
     }
   }
-  hipLaunchKernelGGL(k_sep_box<-1>, grid, dim3(CXP_NT), 0, s, t, c.lx, ntx,
-                     nty, zchunk);
+  if (c.rp == 2)
+    hipLaunchKernelGGL((k_sep_box<-1, 2>), grid, dim3(CXP_NT), 0, s, t, c.lx,
+                       ntx, nty, zchunk);
+  else
+    hipLaunchKernelGGL((k_sep_box<-1, 1>), grid, dim3(CXP_NT), 0, s, t, c.lx,
+                       ntx, nty, zchunk);
   return hip_status(hipGetLastError());
 }
 
@@ -416,6 +431,9 @@ int prolongate_impl(const cxp_op_s *op, const cxp_fab *crse, int crse_comp,
         make_sep_task(op->d, crse, crse_comp, fine, fine_comp, rlo, rhi);
     return launch_sep_box(t, ncomp, s);
   }
+  if (eno_ccc_fits(op->d, per_group))
+    return eno_ccc_launch(crse, crse_comp, fine, fine_comp, ncomp, rlo, rhi,
+                          nullptr, false, s);
   const PointTask t = make_point_task(op->d, crse, crse_comp, fine, fine_comp,
                                       ncomp, rlo, rhi);
   return launch_point(t, ncomp, per_group, s);
@@ -579,6 +597,13 @@ int adjoint_point_run(const cxp_op_s *op, const cxp_fab *crse_in,
                       const cxp_fab *fine_bar, int fine_comp, int ncomp,
                       const int rlo[3], const int rhi[3], int per_group,
                       unsigned char *codes, hipStream_t s) {
+  if (eno_ccc_fits(op->d, per_group)) {
+    if (const int e = eno_ccc_launch(crse_in, crse_comp, nullptr, 0, ncomp,
+                                     rlo, rhi, codes, true, s))
+      return e;
+    return eno_ccc_adj_launch(crse_bar, crse_comp, fine_bar, fine_comp, ncomp,
+                              rlo, rhi, codes, s);
+  }
   const AdjPointTask t =
       make_adj_point_task(op, crse_in, crse_bar, crse_comp, fine_bar,
                           fine_comp, ncomp, rlo, rhi, per_group);
@@ -913,9 +938,12 @@ void schedule_tiles(cxp_worklist_s *wl, int grid) {
   for (size_t i = 0; i < nt; ++i) {
     const SepTile &tl = wl->tiles[i];
     const SepTask &t = wl->tasks[tl.task];
+    // marching steps, weighted by the points a step covers
     cost[i] = tl.cfg < 0 ? 2
                          : (long long)((tl.z1 >> 1) + t.whi[2] - (tl.z0 >> 1) -
-                                       t.wlo[2] + 1) + 4;
+                                       t.wlo[2] + 1) *
+                                   (tl.cfg >= 16 ? 2 : 1) +
+                               4;
     if (groups.empty() || wl->tiles[groups.back().first].task != tl.task ||
         wl->tiles[groups.back().first].comp != tl.comp)
       groups.push_back({i, 0, 0});
@@ -1101,6 +1129,14 @@ void build_tiles(cxp_worklist_s *wl, int only_class_key_cfg = -1,
     }
     wl->uniform_sid = t.sid[0];
   }
+  // the dispatching kernel runs one row pair per thread: a mixed list takes
+  // the one-pair shape of the same width
+  if (wl->uniform_sid < 0)
+    for (Plan &pl : plans)
+      if (kCfgs[pl.cfg].rp != 1)
+        for (int c = 0; c < kNumTileConfigs; ++c)
+          if (kCfgs[c].rp == 1 && kCfgs[c].tx == kCfgs[pl.cfg].tx)
+            pl.cfg = c;
   const long long slots =
       4LL * persistent_grid(1 << 30, wl->lds, wl->uniform_sid);
   int zdef = 64;
@@ -1334,6 +1370,13 @@ int launch_batched(cxp_worklist_s *wl, hipStream_t s) {
 
   for (int n : wl->point_patches) {
     const auto &p = wl->patches[n];
+    if (eno_ccc_fits(p.op->d, wl->per_group)) {
+      if (const int e = eno_ccc_launch(&p.crse, p.crse_comp, &p.fine,
+                                       p.fine_comp, p.ncomp, p.rlo, p.rhi,
+                                       nullptr, false, s))
+        return e;
+      continue;
+    }
     const PointTask t = make_point_task(p.op->d, &p.crse, p.crse_comp,
                                         &p.fine, p.fine_comp, p.ncomp, p.rlo,
                                         p.rhi);
@@ -1675,6 +1718,9 @@ int cxp_prolongate_record(cxp_op op, const cxp_fab *crse, int crse_comp,
                            rhi, per_group, (hipStream_t)stream);
   if (!branches)
     return CXP_ERR_ARG;
+  if (eno_ccc_fits(op->d, per_group))
+    return eno_ccc_launch(crse, crse_comp, fine, fine_comp, ncomp, rlo, rhi,
+                          branches, false, (hipStream_t)stream);
   const PointTask t = make_point_task(op->d, crse, crse_comp, fine, fine_comp,
                                       ncomp, rlo, rhi);
   return launch_point(t, ncomp, per_group, (hipStream_t)stream, branches);
@@ -1703,6 +1749,9 @@ int cxp_prolongate_adjoint_recorded(cxp_op op, const unsigned char *branches,
                           (hipStream_t)stream);
   if (!branches)
     return CXP_ERR_ARG;
+  if (eno_ccc_fits(od, per_group))
+    return eno_ccc_adj_launch(crse_bar, crse_comp, fine_bar, fine_comp, ncomp,
+                              rlo, rhi, branches, (hipStream_t)stream);
   const AdjPointTask t =
       make_adj_point_task(op, crse_bar, crse_bar, crse_comp, fine_bar,
                           fine_comp, ncomp, rlo, rhi, per_group);
@@ -3036,6 +3085,8 @@ int cxp_autotune(cxp_worklist wl, int reps, void *stream) {
         for (int zi = 0; zi < 4 && !status; ++zi) {
           if (cfg == ch.first && zchunks[zi] == ch.second)
             continue;
+          if (kCfgs[cfg].rp != 1 && wl->uniform_sid < 0)
+            continue;  // a mixed list runs one-pair shapes only
           set_class(cl.second, cfg, zchunks[zi]);
           float ms = 0;
           if (!time_list(ms))
@@ -3075,6 +3126,24 @@ int cxp_tune_save(const char *path) {
                  std::get<1>(kv.first), std::get<2>(kv.first),
                  std::get<3>(kv.first), kv.second.first, kv.second.second);
   std::fclose(f);
+  return CXP_OK;
+}
+
+int cxp_tune_set(cxp_op op, const int region_lo[3], const int region_hi[3],
+                 int cfg, int zchunk) {
+  if (!op || !region_lo || !region_hi || cfg < 0 || cfg >= kNumTileConfigs ||
+      zchunk < 0 || zchunk == 1)
+    return CXP_ERR_ARG;
+  const TuneKey k(op->d.sid[0], op->d.sid[1], op->d.sid[2],
+                  size_class(region_lo, region_hi));
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[k] = {cfg, zchunk};
+  return CXP_OK;
+}
+
+int cxp_tune_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
   return CXP_OK;
 }
 

@@ -1,0 +1,239 @@
+// CCC ENO (CC ENO order 3 in all three directions, per-var, no fallback):
+// prolongate_3d_rf2_impl.hxx:1417-1517 (shift search) and :1519-1530 (the
+// shifted tensor stencil).
+//
+// The search for a fine point with coarse parent p scores each shift
+// s in {-1, 0, 1}^3 by
+//   (1 + 2^-26 (|sx| + |sy| + |sz|)) * max(Mx, My, Mz)(p + s),
+//   Mx(q) = max over dj, dk in [-1, 1] of |D2x(q + (0, dj, dk))|
+// (D2x the second undivided difference along x; My, Mz alike): with ENO in
+// every direction the stencil lines are those of the shifted centre for
+// either fine offset, so the score reads one coarse field, Mall, at p + s,
+// and all eight children of p take the same shift.  A workgroup of EC_T
+// threads owns 8 x 8 x 4 parents (one per thread):
+//   A  the coarse window, parents - 2 .. parents + 2 (12 x 12 x 8)   -> LDS
+//   B  |D2| along x, y, z where Mall needs them                      -> LDS
+//   C  Mall on parents - 1 .. parents + 1 (10 x 10 x 6)              -> LDS
+//   D  per parent: the scan over the 27 shifts in the reference's
+//      order (sz outermost, sx innermost; the first strictly smallest
+//      score wins, none finite leaves shift 0);
+//   E  per parent: the 27 coarse values around p + s and the 8 children,
+//      each by the reference's nested x, y, z passes (eno2_apply, the
+//      forward's own routine), so the values are the generic kernel's bit
+//      for bit.
+// |D2| is fmax(0, |undiv2|) and the maxima are fmax chains from 0, as the
+// generic search takes them (fmax is exact and ignores NaN).
+//
+// The adjoint from per-fine-point codes (the shift of each point's parent)
+// gathers per coarse point q over the parents p within 2 of it whose
+// shifted stencil covers q (t = q - p - s(p) in [-1, 1]^3):
+//   crse_bar(q) += sum_p sum_o wz(sz, oz, tz) wy(sy, oy, ty) wx(sx, ox, tx)
+//                  fine_bar(2 p + o),
+// parents in ascending z, y, x order and, per parent, oz outermost, ox
+// innermost.
+//
+// The steps are host-and-device functions over accessors, so a host build
+// runs them with checked accesses (carpetx_amd/host/eno_ccc_host.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "stencil1d.hpp"
+
+namespace cxp {
+
+#define EC_T 256
+#define EC_PX 8
+#define EC_PY 8
+#define EC_PZ 4
+#define EC_CX (EC_PX + 4)  // coarse window
+#define EC_CY (EC_PY + 4)
+#define EC_CZ (EC_PZ + 4)
+#define EC_MX (EC_PX + 2)  // Mall window
+#define EC_MY (EC_PY + 2)
+#define EC_MZ (EC_PZ + 2)
+#define EC_NC (EC_CX * EC_CY * EC_CZ)              // 1152
+#define EC_NDX (EC_MX * EC_CY * EC_CZ)             // 960
+#define EC_NDY (EC_CX * EC_MY * EC_CZ)             // 960
+#define EC_NDZ (EC_CX * EC_CY * EC_MZ)             // 864
+#define EC_ND (EC_NDX + EC_NDY + EC_NDZ)
+#define EC_NM (EC_MX * EC_MY * EC_MZ)              // 600
+
+// Parent tile b of the parents [plo, phi]: origin p0.
+__host__ __device__ inline void ec_tile(const int plo[3], int b, int ntx,
+                                        int nty, int p0[3]) {
+  p0[0] = plo[0] + EC_PX * (b % ntx);
+  p0[1] = plo[1] + EC_PY * ((b / ntx) % nty);
+  p0[2] = plo[2] + EC_PZ * (b / (ntx * nty));
+}
+
+__host__ __device__ inline int ec_cidx(int x, int y, int z) {
+  return (z * EC_CY + y) * EC_CX + x;
+}
+
+// A: window value q < EC_NC: c(x, y, z) inside the coarse window
+// [olo, ohi] (which holds every value the region's points read), else 0.
+template <class C>
+__host__ __device__ inline double ec_coarse(const int olo[3], const int ohi[3],
+                                            const int p0[3], int q,
+                                            const C &c) {
+  const int x = p0[0] - 2 + q % EC_CX, y = p0[1] - 2 + (q / EC_CX) % EC_CY,
+            z = p0[2] - 2 + q / (EC_CX * EC_CY);
+  const bool in = x >= olo[0] && x <= ohi[0] && y >= olo[1] && y <= ohi[1] &&
+                  z >= olo[2] && z <= ohi[2];
+  return in ? c(x, y, z) : 0.0;
+}
+
+__host__ __device__ inline double ec_undiv2(double m, double c, double p) {
+  double dd = -2.0 * c;
+  dd += 1.0 * (m + p);
+  return dd;
+}
+
+// B: |D2| entry e < EC_ND (x array, then y, then z), from the window.
+template <class CL>
+__host__ __device__ inline double ec_d2(int e, const CL &cl) {
+  int a, b, c, s;
+  if (e < EC_NDX) {  // (x', y', z') = window (x' + 1, y', z')
+    a = e % EC_MX;
+    b = (e / EC_MX) % EC_CY;
+    c = e / (EC_MX * EC_CY);
+    s = 1;
+    const int q = ec_cidx(a, b, c);
+    return fmax(0.0, fabs(ec_undiv2(cl(q), cl(q + s), cl(q + 2 * s))));
+  }
+  if (e < EC_NDX + EC_NDY) {  // window (x', y' + 1, z')
+    e -= EC_NDX;
+    a = e % EC_CX;
+    b = (e / EC_CX) % EC_MY;
+    c = e / (EC_CX * EC_MY);
+    s = EC_CX;
+  } else {  // window (x', y', z' + 1)
+    e -= EC_NDX + EC_NDY;
+    a = e % EC_CX;
+    b = (e / EC_CX) % EC_CY;
+    c = e / (EC_CX * EC_CY);
+    s = EC_CX * EC_CY;
+  }
+  const int q = ec_cidx(a, b, c);
+  return fmax(0.0, fabs(ec_undiv2(cl(q), cl(q + s), cl(q + 2 * s))));
+}
+
+// C: Mall at window point m = (mx, my, mz) of the Mall box (parents - 1
+// + m), from the |D2| arrays (dl(e) reads entry e).
+template <class DL>
+__host__ __device__ inline double ec_mall(int q, const DL &dl) {
+  const int mx = q % EC_MX, my = (q / EC_MX) % EC_MY, mz = q / (EC_MX * EC_MY);
+  double ddx = 0.0, ddy = 0.0, ddz = 0.0;
+  for (int dk = -1; dk <= 1; ++dk)
+    for (int dj = -1; dj <= 1; ++dj)
+      ddx = fmax(ddx, dl(((mz + 1 + dk) * EC_CY + (my + 1 + dj)) * EC_MX + mx));
+  for (int dk = -1; dk <= 1; ++dk)
+    for (int di = -1; di <= 1; ++di)
+      ddy = fmax(ddy, dl(EC_NDX + ((mz + 1 + dk) * EC_MY + my) * EC_CX +
+                         (mx + 1 + di)));
+  for (int dj = -1; dj <= 1; ++dj)
+    for (int di = -1; di <= 1; ++di)
+      ddz = fmax(ddz, dl(EC_NDX + EC_NDY + (mz * EC_CY + (my + 1 + dj)) * EC_CX +
+                         (mx + 1 + di)));
+  return fmax(fmax(ddx, ddy), ddz);
+}
+
+// D: the shift of local parent l (ml(q) reads Mall), as the code
+// (sx + 1) + 3 (sy + 1) + 9 (sz + 1).
+template <class ML>
+__host__ __device__ inline int ec_search(const int l[3], const ML &ml) {
+  double best = __builtin_inf();
+  int code = 13;  // shift 0
+  for (int sk = -1; sk <= 1; ++sk)
+    for (int sj = -1; sj <= 1; ++sj)
+      for (int si = -1; si <= 1; ++si) {
+        const double penalty =
+            1.0 + 0x1p-26 * (double)(abs(si) + abs(sj) + abs(sk));
+        const double dd =
+            penalty * ml(((l[2] + 1 + sk) * EC_MY + (l[1] + 1 + sj)) * EC_MX +
+                         (l[0] + 1 + si));
+        if (dd < best) {
+          best = dd;
+          code = (si + 1) + 3 * (sj + 1) + 9 * (sk + 1);
+        }
+      }
+  return code;
+}
+
+// E: the eight children of local parent l with shift code `code`:
+// v[oz][oy][ox] (cl(q) reads the window).
+template <class CL>
+__host__ __device__ inline void ec_children(const int l[3], int code,
+                                            const CL &cl, double (&v)[2][2][2]) {
+  const int s[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+  double X[2][3][3];  // [ox][dk - sz + 1][dj - sy + 1]
+  for (int ox = 0; ox < 2; ++ox)
+    for (int kk = 0; kk < 3; ++kk)
+      for (int jj = 0; jj < 3; ++jj) {
+        const int row = ec_cidx(l[0] + 2, l[1] + 2 + s[1] - 1 + jj,
+                                l[2] + 2 + s[2] - 1 + kk);
+        X[ox][kk][jj] =
+            eno2_apply([&](int o) { return cl(row + o); }, s[0], ox);
+      }
+  for (int ox = 0; ox < 2; ++ox)
+    for (int oy = 0; oy < 2; ++oy) {
+      double Y[3];
+      for (int kk = 0; kk < 3; ++kk)
+        Y[kk] = eno2_apply([&](int o) { return X[ox][kk][o - s[1] + 1]; }, s[1],
+                           oy);
+      for (int oz = 0; oz < 2; ++oz)
+        v[oz][oy][ox] =
+            eno2_apply([&](int o) { return Y[o - s[2] + 1]; }, s[2], oz);
+    }
+}
+
+// Weight of coarse offset s + t (t in [-1, 1]) in eno2_apply(., s, o).
+__host__ __device__ inline double ec_w(int s, int o, int t) {
+  return o == 0 ? eno2_w(s + 1, t + 1) : eno2_w(1 - s, 1 - t);
+}
+
+// The adjoint's sum for coarse point q: code(p) returns the shift code of
+// parent p (p within the region's parents), fb(x, y, z) the fine adjoint
+// (0 outside the region).
+template <class CD, class FB>
+__host__ __device__ inline double ec_adj_one(const int q[3], const int plo[3],
+                                             const int phi[3], const CD &code,
+                                             const FB &fb) {
+  double acc = 0.0;
+  for (int pz = q[2] - 2; pz <= q[2] + 2; ++pz) {
+    if (pz < plo[2] || pz > phi[2])
+      continue;
+    for (int py = q[1] - 2; py <= q[1] + 2; ++py) {
+      if (py < plo[1] || py > phi[1])
+        continue;
+      for (int px = q[0] - 2; px <= q[0] + 2; ++px) {
+        if (px < plo[0] || px > phi[0])
+          continue;
+        const int c = code(px, py, pz);
+        const int t[3] = {q[0] - px - (c % 3 - 1), q[1] - py - ((c / 3) % 3 - 1),
+                          q[2] - pz - (c / 9 - 1)};
+        if (t[0] < -1 || t[0] > 1 || t[1] < -1 || t[1] > 1 || t[2] < -1 ||
+            t[2] > 1)
+          continue;
+        const int s[3] = {c % 3 - 1, (c / 3) % 3 - 1, c / 9 - 1};
+        double g = 0.0;
+        for (int oz = 0; oz < 2; ++oz) {
+          double gy = 0.0;
+          for (int oy = 0; oy < 2; ++oy) {
+            double gx = 0.0;
+            for (int ox = 0; ox < 2; ++ox)
+              gx += ec_w(s[0], ox, t[0]) *
+                    fb(2 * px + ox, 2 * py + oy, 2 * pz + oz);
+            gy += ec_w(s[1], oy, t[1]) * gx;
+          }
+          g += ec_w(s[2], oz, t[2]) * gy;
+        }
+        acc += g;
+      }
+    }
+  }
+  return acc;
+}
+
+} // namespace cxp

@@ -23,6 +23,7 @@
 #include <cstdint>
 
 #include "stencil1d.hpp"
+#include "xcd.hpp"
 #include "../../include/cxp.h"
 
 namespace cxp {
@@ -102,7 +103,8 @@ struct SepTile {
   int comp;  // component offset applied with ccs/fcs
   int x0, y0;
   int z0, z1;
-  int cfg;  // tile shape: log2 of the fine tile width; -1: thin tile
+  int cfg;  // tile shape: log2 of the fine tile width (+ 16: two row pairs
+            // per thread); -1: thin tile
   int pad;
 };
 
@@ -114,9 +116,11 @@ struct SepTile {
   X(SID_CCONS6) X(SID_CCONS7) X(SID_MINMOD)
 
 // Runtime tile geometry.  A workgroup of 128 threads owns a TX x TY fine
-// (x, y) tile with TX * TY = 256: thread t owns fine x = x0 + (t % TX) and
-// the fine y pair y0 + 2 (t / TX) + {0, 1}.  Tiles start at even x and y, so
-// every pair shares one coarse index.
+// (x, y) tile with TX * TY = 256 RP: thread t owns fine x = x0 + (t % TX)
+// and the 2 RP fine rows y0 + 2 RP (t / TX) + {0 .. 2 RP - 1} (RP pairs of
+// rows, each pair sharing one coarse y).  Tiles start at even x and y.
+// RP = 2 gives each thread twice the points per coarse plane for the same
+// barriers and loop overhead.
 struct Geom {
   int lx;        // log2 TX
   int tx, ty;    // fine tile extents
@@ -124,11 +128,11 @@ struct Geom {
   int xs_off;    // start of the x-pass buffer (16-byte aligned)
 };
 
-__host__ __device__ inline Geom make_geom(int lx) {
+__host__ __device__ inline Geom make_geom(int lx, int rp = 1) {
   Geom g;
   g.lx = lx;
   g.tx = 1 << lx;
-  g.ty = 256 >> lx;
+  g.ty = (256 * rp) >> lx;
   g.cxs = g.tx / 2 + 10;  // tile / 2 + 1 + 8 halo, padded
   g.cys = g.ty / 2 + 10;
   g.xs_off = (g.cys * g.cxs + 1) & ~1;
@@ -138,19 +142,21 @@ __host__ __device__ inline int geom_lds_doubles(const Geom &g) {
   return g.xs_off + g.cys * g.tx;
 }
 
-// LDS budget per workgroup (doubles): the largest shape in use, TX = 64,
-// TY = 4, needs 12 x 42 + 12 x 64 = 1272.
-constexpr int kLdsDoubles = 1280;
+// LDS budget per workgroup (doubles): the largest shape in use, TX = 64
+// with RP = 2 (TY = 8), needs 14 x 42 + 14 x 64 = 1484.
+constexpr int kLdsDoubles = 1488;
 #define CXP_NT 128
-// Coarse values per thread and plane: at most (TX/2 + 9) x (TY/2 + 9)
-// <= 41 x 11 = 451 <= 4 * 128 for every shape with 4 <= TX <= 64.
-#define CXP_PRE 4
+// Coarse values per thread and plane: at most (TX/2 + 9) x (TY/2 + 9),
+// <= 41 x 11 = 451 <= 4 * 128 for the RP = 1 shapes (4 <= TX <= 64) and
+// <= 41 x 13 = 533 <= 5 * 128 for the RP = 2 ones (8 <= TX <= 64).
+#define CXP_PRE 5
+__host__ __device__ constexpr int sep_pre(int rp) { return rp == 1 ? 4 : 5; }
 // Register window of coarse z-planes: the widest z stencil spans 9 planes.
 #define CXP_ZW 9
 
 // x-pass: X(fine x pair, coarse y row) from the coarse plane tile.  One work
 // item computes both fine offsets of a coarse x and writes them as a pair;
-// lanes run over the TX/2 coarse x of a row, rows step by TY.
+// lanes run over the TX/2 coarse x of a row, rows step by 256 / TX.
 template <int SID>
 __device__ __forceinline__ void sep_xpass(const Geom &g,
                                           const double *__restrict__ Cs,
@@ -160,7 +166,7 @@ __device__ __forceinline__ void sep_xpass(const Geom &g,
   if (q >= npx)
     return;
   const int ic0 = x0 >> 1;
-  for (int cy = tid >> (g.lx - 1); cy < CY; cy += g.ty) {
+  for (int cy = tid >> (g.lx - 1); cy < CY; cy += (256 >> g.lx)) {
     const double *row = Cs + cy * g.cxs + (ic0 + q - ci0);
     const auto c = [&](int o) { return row[o]; };
     double2 v;
@@ -170,7 +176,8 @@ __device__ __forceinline__ void sep_xpass(const Geom &g,
   }
 }
 
-// y-pass: both fine offsets of the owned coarse y pair.
+// y-pass: both fine offsets of the coarse y row0 (local row of the x-pass
+// buffer).
 template <int SID>
 __device__ __forceinline__ void sep_ypass(const Geom &g,
                                           const double *__restrict__ Xs,
@@ -194,11 +201,13 @@ __device__ __forceinline__ double sep_zval(const double (&w)[CXP_ZW],
 // Process one tile [x0, x0+TX) x [y0, y0+TY) x [z0, z1] of a task, clipped
 // to the task's fine region (x0, y0 even).  All threads of the block call it.
 // SU >= 0: every direction uses stencil SU, compiled without dispatch;
-// SU < 0: per-pass runtime dispatch on the task's stencils.
-template <int SU = -1>
+// SU < 0: per-pass runtime dispatch on the task's stencils.  RP: row pairs
+// per thread (the geometry's).
+template <int SU = -1, int RP = 1>
 __device__ inline void sep_tile(const Geom &g, const SepTask &t,
                                 long long comp, int x0, int y0, int z0, int z1,
                                 double *smem) {
+  constexpr int NR = 2 * RP;  // fine rows per thread
   double *Cs = smem;
   double *Xs = Cs + g.xs_off;
   const int tid = threadIdx.x;
@@ -224,25 +233,28 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
   const int sx = t.sid[0], sy = t.sid[1], sz = t.sid[2];
   const int ncl = CX * CY;
 
-  // owned fine points: x = x0 + fi, y = y0 + 2 fjp + {0, 1}
+  // owned fine points: x = x0 + fi, y = y0 + NR fjq + r, r < NR
   const int fi = tid & (g.tx - 1);
-  const int fjp = tid >> g.lx;
+  const int fjq = tid >> g.lx;
   const int fx = x0 + fi;
-  const int fy = y0 + 2 * fjp;
+  const int fy = y0 + NR * fjq;
   const bool xin = fx >= t.rlo[0] && fx <= xe;
-  const bool pair_in = fy <= ye;  // the pair overlaps the tile's rows
-  const bool yin0 = pair_in && fy >= t.rlo[1];
-  const bool yin1 = pair_in && fy + 1 <= ye;
+  bool yin[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+    yin[r] = fy + r >= t.rlo[1] && fy + r <= ye;
+  const bool own = xin && fy <= ye;  // the rows overlap the tile's rows
   const int row0 = (fy >> 1) - cj0;
 
   // Each thread moves up to CXP_PRE coarse values per plane; the next
   // plane's loads are issued before the current plane is processed so their
   // latency overlaps the passes.
+  constexpr int PRE = sep_pre(RP);
   const __amdgpu_buffer_rsrc_t rc = make_rsrc(crse.p, t.cbytes);
   const __amdgpu_buffer_rsrc_t rf = make_rsrc(fine.p, t.fbytes);
-  int soff[CXP_PRE], loff[CXP_PRE];
+  int soff[PRE], loff[PRE];
 #pragma unroll
-  for (int r = 0; r < CXP_PRE; ++r) {
+  for (int r = 0; r < PRE; ++r) {
     const int idx = tid + r * CXP_NT;
     const int cy = idx / CX;
     const int cx = idx - cy * CX;
@@ -251,38 +263,37 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
   }
   const int pstride = 8 * (int)crse.sz;  // bytes per coarse plane
   int pbase = 8 * (int)didx(crse, ci0, cj0, kf);
-  double pre[CXP_PRE];
+  double pre[PRE];
 #pragma unroll
-  for (int r = 0; r < CXP_PRE; ++r)
+  for (int r = 0; r < PRE; ++r)
     pre[r] = bld(rc, soff[r] >= 0 ? pbase + loff[r] : CXP_OOB);
   // dummy stores: every path into the plane loop then has the same pending
   // loads-then-stores pattern
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int r = 0; r < 2 * NR; ++r)
     bst(rf, CXP_OOB, 0.0);
 
-  double w0[CXP_ZW], w1[CXP_ZW];
+  double w[NR][CXP_ZW];
 #pragma unroll
-  for (int q = 0; q < CXP_ZW; ++q) {
-    w0[q] = 0.0;
-    w1[q] = 0.0;
-  }
-  // byte offsets of the owned points (x/y part) in the fine component
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int q = 0; q < CXP_ZW; ++q)
+      w[r][q] = 0.0;
+  // byte offset of the first owned row (x/y part) in the fine component
   const int fo0 = 8 * ((fx - fine.lo[0]) + (fy - fine.lo[1]) * fine.sy);
-  const int fo1 = fo0 + 8 * fine.sy;
-  const bool own = xin && pair_in;
+  const int frow = 8 * fine.sy;
 
   for (int kp = kf; kp <= kl; ++kp) {
     // coarse plane tile -> LDS, then prefetch the next plane (out of range
     // after the last plane)
 #pragma unroll
-    for (int r = 0; r < CXP_PRE; ++r)
+    for (int r = 0; r < PRE; ++r)
       if (soff[r] >= 0)
         Cs[soff[r]] = pre[r];
     pbase += pstride;
     const bool more = kp < kl;
 #pragma unroll
-    for (int r = 0; r < CXP_PRE; ++r)
+    for (int r = 0; r < PRE; ++r)
       pre[r] = bld(rc, (more && soff[r] >= 0) ? pbase + loff[r] : CXP_OOB);
     __syncthreads();
     if constexpr (SU >= 0) {
@@ -300,46 +311,57 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     __syncthreads();
     // y-pass into the register window (rotated by one coarse plane)
 #pragma unroll
-    for (int q = 0; q < CXP_ZW - 1; ++q) {
-      w0[q] = w0[q + 1];
-      w1[q] = w1[q + 1];
-    }
-    double ya = 0.0, yb = 0.0;
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int q = 0; q < CXP_ZW - 1; ++q)
+        w[r][q] = w[r][q + 1];
+    double y[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      y[r] = 0.0;
     if (own) {
-      if constexpr (SU >= 0) {
-        sep_ypass<SU>(g, Xs, row0, fi, ya, yb);
-      } else {
-        switch (sy) {
+#pragma unroll
+      for (int p = 0; p < RP; ++p) {
+        if constexpr (SU >= 0) {
+          sep_ypass<SU>(g, Xs, row0 + p, fi, y[2 * p], y[2 * p + 1]);
+        } else {
+          switch (sy) {
 #define CXP_Y(S)                                                               \
   case S:                                                                      \
-    sep_ypass<S>(g, Xs, row0, fi, ya, yb);                                     \
+    sep_ypass<S>(g, Xs, row0 + p, fi, y[2 * p], y[2 * p + 1]);                 \
     break;
-          CXP_LINEAR_SIDS(CXP_Y)
+            CXP_LINEAR_SIDS(CXP_Y)
 #undef CXP_Y
+          }
         }
       }
     }
-    w0[CXP_ZW - 1] = ya;
-    w1[CXP_ZW - 1] = yb;
-    // z-pass: both fine z of coarse plane kc = kp - whi; always four stores,
-    // the ones outside the tile go out of range
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      w[r][CXP_ZW - 1] = y[r];
+    // z-pass: both fine z of coarse plane kc = kp - whi; always 2 NR
+    // stores, the ones outside the tile go out of range
     const int kc = kp - t.whi[2];
     const bool zact = kc >= kc0 && kc <= kc1;
-    double v00 = 0.0, v01 = 0.0, v10 = 0.0, v11 = 0.0;
+    double v[NR][2];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      v[r][0] = v[r][1] = 0.0;
     if (zact) {
       if constexpr (SU >= 0) {
-        v00 = sep_zval<SU>(w0, 0);
-        v01 = sep_zval<SU>(w0, 1);
-        v10 = sep_zval<SU>(w1, 0);
-        v11 = sep_zval<SU>(w1, 1);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          v[r][0] = sep_zval<SU>(w[r], 0);
+          v[r][1] = sep_zval<SU>(w[r], 1);
+        }
       } else {
         switch (sz) {
 #define CXP_Z(S)                                                               \
   case S:                                                                      \
-    v00 = sep_zval<S>(w0, 0);                                                  \
-    v01 = sep_zval<S>(w0, 1);                                                  \
-    v10 = sep_zval<S>(w1, 0);                                                  \
-    v11 = sep_zval<S>(w1, 1);                                                  \
+    for (int r = 0; r < NR; ++r) {                                             \
+      v[r][0] = sep_zval<S>(w[r], 0);                                          \
+      v[r][1] = sep_zval<S>(w[r], 1);                                          \
+    }                                                                          \
     break;
           CXP_LINEAR_SIDS(CXP_Z)
 #undef CXP_Z
@@ -351,21 +373,24 @@ __device__ inline void sep_tile(const Geom &g, const SepTask &t,
     const bool z1ok = zact && fz1 >= z0 && fz1 <= z1;
     const int zb0 = 8 * (int)((long long)(fz0 - fine.lo[2]) * fine.sz);
     const int zb1 = zb0 + 8 * (int)fine.sz;
-    bst(rf, (own && yin0 && z0ok) ? fo0 + zb0 : CXP_OOB, v00);
-    bst(rf, (own && yin0 && z1ok) ? fo0 + zb1 : CXP_OOB, v01);
-    bst(rf, (own && yin1 && z0ok) ? fo1 + zb0 : CXP_OOB, v10);
-    bst(rf, (own && yin1 && z1ok) ? fo1 + zb1 : CXP_OOB, v11);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      bst(rf, (own && yin[r] && z0ok) ? fo0 + r * frow + zb0 : CXP_OOB,
+          v[r][0]);
+      bst(rf, (own && yin[r] && z1ok) ? fo0 + r * frow + zb1 : CXP_OOB,
+          v[r][1]);
+    }
   }
 }
 
 // Per-box launch: grid.x = tiles (x fastest, then y, then z-chunks),
 // grid.y = component.  The tile shape is uniform over the launch.
-template <int SU>
+template <int SU, int RP>
 __global__ void __launch_bounds__(CXP_NT)
     k_sep_box(SepTask t, int lx, int ntx, int nty, int zchunk) {
   __shared__ double smem[kLdsDoubles];
-  const Geom g = make_geom(lx);
-  const int b = blockIdx.x;
+  const Geom g = make_geom(lx, RP);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int tx = b % ntx;
   const int ty = (b / ntx) % nty;
   const int tz = b / (ntx * nty);
@@ -373,7 +398,7 @@ __global__ void __launch_bounds__(CXP_NT)
   const int y0 = (t.rlo[1] & ~1) + ty * g.ty;
   const int z0 = t.rlo[2] + tz * zchunk;
   const int z1 = min(t.rhi[2], z0 + zchunk - 1);
-  sep_tile<SU>(g, t, blockIdx.y, x0, y0, z0, z1, smem);
+  sep_tile<SU, RP>(g, t, blockIdx.y, x0, y0, z0, z1, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -1032,7 +1057,7 @@ __global__ void __launch_bounds__(256)
     k_point_tiled(PointTask t, int ntx, int nty,
                   unsigned char *__restrict__ codes = nullptr) {
   extern __shared__ double ptile[];
-  const int b = blockIdx.x;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int tv[3] = {b % ntx, (b / ntx) % nty, b / (ntx * nty)};
   int f0[3], f1[3], c0[3], c1[3], n[3];
   for (int d = 0; d < 3; ++d) {

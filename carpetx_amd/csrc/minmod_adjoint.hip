@@ -15,6 +15,7 @@
 
 #include "minmod_adjoint.hpp"
 #include "minmod_launch.hpp"
+#include "xcd.hpp"
 
 namespace cxp {
 
@@ -25,14 +26,6 @@ constexpr int kMmRecordChunk = 12;  // codes per column piece
 // z-chunks of the tiled CCC kernels: at least this many planes (a chunk
 // recomputes 4 planes of X and XY and re-reads 2 parents of fine data).
 constexpr int kMtMinDepth = 2;
-
-// Blocks are handed to the 8 XCDs round robin (block i to XCD i % 8); the
-// tiled kernels renumber them so that each XCD runs a contiguous range of
-// tiles, whose footprints overlap in its L2.
-__device__ inline int mt_block(int i, int nb) {
-  const int q = nb / 8, r = nb % 8, x = i % 8;
-  return x * q + (x < r ? x : r) + i / 8;
-}
 
 struct MmData {
   DFabC crse;  // forward input (record only)
@@ -146,7 +139,7 @@ __global__ void __launch_bounds__(MT_T) k_mm_gather_tiled(MtArgs a) {
   __shared__ double xb[MT_NY];
   const int tid = threadIdx.x;
   const int comp = blockIdx.y;
-  const MtTile t = mt_tile(a.olo, a.ohi, mt_block(blockIdx.x, gridDim.x),
+  const MtTile t = mt_tile(a.olo, a.ohi, xcd_block(blockIdx.x, gridDim.x),
                            a.ntx, a.nty, a.kc);
   const double *fp = a.fbar + comp * a.fcs;
   double *bp = a.cbar + comp * a.bcs;
@@ -230,7 +223,7 @@ __global__ void __launch_bounds__(MT_T) k_mm_record_tiled(MtArgs a, MmGeom g) {
   __shared__ double xpl[MT_FP * MT_CW];
   const int tid = threadIdx.x;
   const int comp = blockIdx.y;
-  const MtTile t = mt_tile(a.olo, a.ohi, mt_block(blockIdx.x, gridDim.x),
+  const MtTile t = mt_tile(a.olo, a.ohi, xcd_block(blockIdx.x, gridDim.x),
                            a.ntx, a.nty, a.kc);
   const double *cp = a.crse + comp * a.ccs;
   unsigned char *cd = const_cast<unsigned char *>(a.codes) + comp * a.ncodes;
@@ -297,7 +290,7 @@ k_mm_fused(MtArgs a) {
   __shared__ unsigned char byr[MT_RING][MT_FP * MT_PB];
   const int tid = threadIdx.x;
   const int comp = blockIdx.y;
-  const MtTile t = mt_tile(a.olo, a.ohi, mt_block(blockIdx.x, gridDim.x),
+  const MtTile t = mt_tile(a.olo, a.ohi, xcd_block(blockIdx.x, gridDim.x),
                            a.ntx, a.nty, a.kc);
   const double *cp = a.crse + comp * a.ccs;
   const double *fp = a.fbar + comp * a.fcs;

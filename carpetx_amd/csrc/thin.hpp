@@ -227,7 +227,7 @@ __device__ __forceinline__ void thin_tile(const SepTask &t, long long comp,
 __global__ void __launch_bounds__(CXP_NT)
     k_thin_box(SepTask t, int ntx) {
   extern __shared__ double thin_smem[];
-  const int b = blockIdx.x;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int x0 = (t.rlo[0] & ~1) + 16 * (b % ntx);
   const int y0 = (t.rlo[1] & ~1) + 16 * (b / ntx);
   thin_tile(t, blockIdx.y, x0, y0, thin_smem);

@@ -434,6 +434,16 @@ int cxp_graph_destroy(cxp_graph g);
 int cxp_autotune(cxp_worklist wl, int reps, void *stream);
 int cxp_tune_save(const char *path);
 int cxp_tune_load(const char *path);
+/* Sets the table's entry for op's stencils and the size class of the
+ * region [lo, hi]: tile shape cfg (0 .. cxp_tile_config_count() - 1) and
+ * z-chunk zchunk (fine planes, >= 2; 0: the launch's default).  Applies to
+ * per-patch launches and to lists uploaded afterwards.  Errors: CXP_ERR_ARG
+ * for a null op or region, a shape out of range or a chunk of 1 or less
+ * than 0.  No device work. */
+int cxp_tune_set(cxp_op op, const int region_lo[3], const int region_hi[3],
+                 int cfg, int zchunk);
+/* Empties the table: every launch takes the default shapes again. */
+int cxp_tune_clear(void);
 int cxp_tile_config_count(void);
 int cxp_tile_config(int cfg, int *tx, int *ty, int *zwindow);
 

@@ -767,3 +767,22 @@ def test_fallback_override_and_group_tags_lookup():
     assert group('prolongation_order="3"')[0] == -3       # not an integer
     assert group('prolongation_type="eno')[0] == -3       # unterminated string
     assert group('parities={+1 +1')[0] == -3              # unterminated list
+
+
+def test_tune_set_checks_its_arguments():
+    """cxp_tune_set / cxp_tune_clear: argument errors before any table
+    change, no device work."""
+    import ctypes
+    from carpetx_amd.interp import get_interpolator
+    L = _lib.lib()
+    h = get_interpolator("interpolate", 3, (0, 0, 0)).handle
+    lo, hi = (ctypes.c_int * 3)(0, 0, 0), (ctypes.c_int * 3)(15, 15, 15)
+    n = L.cxp_tile_config_count()
+    assert L.cxp_tune_set(None, lo, hi, 0, 8) == -3
+    assert L.cxp_tune_set(h, None, hi, 0, 8) == -3
+    assert L.cxp_tune_set(h, lo, hi, n, 8) == -3
+    assert L.cxp_tune_set(h, lo, hi, -1, 8) == -3
+    assert L.cxp_tune_set(h, lo, hi, 0, 1) == -3
+    assert L.cxp_tune_set(h, lo, hi, 0, -2) == -3
+    assert L.cxp_tune_set(h, lo, hi, n - 1, 0) == 0
+    assert L.cxp_tune_clear() == 0
