@@ -3,9 +3,12 @@
 //  * k_eno_ccc<CODES>  forward: values (and, given a code array, each fine
 //                      point's branch code beside them), or with CODES the
 //                      branch codes alone;
-//  * k_eno_ccc_adj     crse_bar += J^T fine_bar from the branch codes.
+//  * k_eno_ccc_adj_tiled  crse_bar += J^T fine_bar from the branch codes.
 // One component per grid row y.
 #include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
 
 #include "eno_ccc.hpp"
 #include "eno_launch.hpp"
@@ -96,17 +99,28 @@ __global__ void __launch_bounds__(EC_T)
       }
 }
 
-// One thread per coarse point of the output window [olo, ohi], x fastest.
-__global__ void __launch_bounds__(256)
-    k_eno_ccc_adj(EcArgs a, const unsigned char *__restrict__ codes) {
-  const int nx = a.ohi[0] - a.olo[0] + 1, ny = a.ohi[1] - a.olo[1] + 1,
-            nz = a.ohi[2] - a.olo[2] + 1;
-  const long long n = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (n >= (long long)nx * ny * nz)
-    return;
+// The tiled adjoint (eno_ccc.hpp, ea_*): crse_bar += J^T fine_bar over the
+// output window [olo, ohi], tiles numbered so each XCD runs a contiguous
+// range; zc output planes per workgroup.  Thread t loads parent slots t and
+// t + EA_NT (EA_NP <= 2 EA_NT); the next plane's loads are issued before
+// the gather of the current one, so their latency overlaps it.
+__global__ void __launch_bounds__(EA_NT, 2)
+    k_eno_ccc_adj_tiled(EcArgs a, const unsigned char *__restrict__ codes,
+                        int ntx, int nty, int zc) {
+  static_assert(EA_NP <= 2 * EA_NT, "two parent slots per thread");
+  // H pairs: pair k (EA_NP + 1) + n = (H[0][k], H[1][k]) of slot n; pair
+  // EA_NP, zeros, is read for the classes a slot does not add to
+  __shared__ double hs[EA_NH * (EA_NP + 1)];
+  __shared__ unsigned cs[EA_NP];
+  __shared__ double ws[18];
+  const int tid = threadIdx.x;
   const int comp = blockIdx.y;
-  const int q[3] = {a.olo[0] + (int)(n % nx), a.olo[1] + (int)((n / nx) % ny),
-                    a.olo[2] + (int)(n / ((long long)nx * ny))};
+  int q0[3];
+  ea_tile(a.olo, xcd_block(blockIdx.x, gridDim.x), ntx, nty, zc, q0);
+  if (tid < 18)
+    ws[tid] = ec_w(tid / 6 - 1, (tid / 3) % 2, tid % 3 - 1);
+  if (tid < 2)
+    hs[2 * EA_NP + tid] = 0.0;
   const unsigned char *cd = codes + comp * a.npts;
   const double *fp = a.fine + comp * a.fcs;
   const long long rnx = a.rhi[0] - a.rlo[0] + 1;
@@ -119,17 +133,85 @@ __global__ void __launch_bounds__(256)
     return cd[(fx - a.rlo[0]) + (fy - a.rlo[1]) * rnx +
               (fz - a.rlo[2]) * rnx * rny];
   };
-  auto fb = [&](int x, int y, int z) {
-    if (x < a.rlo[0] || x > a.rhi[0] || y < a.rlo[1] || y > a.rhi[1] ||
-        z < a.rlo[2] || z > a.rhi[2])
-      return 0.0;
-    return fp[(x - a.f_lo[0]) + (y - a.f_lo[1]) * a.f_sy +
-              (z - a.f_lo[2]) * a.f_sz];
+  // the eight children of a parent in the region: one base offset, the
+  // children outside the region masked to 0
+  auto kids = [&](int px, int py, int pz, double (&f)[2][2][2]) {
+    const int x = 2 * px, y = 2 * py, z = 2 * pz;
+    const bool xo[2] = {x >= a.rlo[0], x + 1 <= a.rhi[0]};
+    const bool yo[2] = {y >= a.rlo[1], y + 1 <= a.rhi[1]};
+    const bool zo[2] = {z >= a.rlo[2], z + 1 <= a.rhi[2]};
+    const double *b = fp + (x - a.f_lo[0]) + (y - a.f_lo[1]) * a.f_sy +
+                      (z - a.f_lo[2]) * a.f_sz;
+#pragma unroll
+    for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+      for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox)
+          f[oz][oy][ox] = (xo[ox] && yo[oy] && zo[oz])
+                              ? b[ox + oy * a.f_sy + oz * a.f_sz]
+                              : 0.0;
   };
-  const double s = ec_adj_one(q, a.plo, a.phi, code, fb);
-  double *bp = const_cast<double *>(a.crse) + comp * a.ccs;
-  bp[(q[0] - a.c_lo[0]) + (q[1] - a.c_lo[1]) * a.c_sy +
-     (q[2] - a.c_lo[2]) * a.c_sz] += s;
+  auto w = [&](int sh, int o, int t) { return ws[(sh + 1) * 6 + o * 3 + t + 1]; };
+  const int lx = tid % EA_TX, ly = tid / EA_TX;
+  const int qx = q0[0] + lx, qy = q0[1] + ly;
+  const int qz1 = min(q0[2] + zc - 1, a.ohi[2]);
+  const bool own = qx <= a.ohi[0] && qy <= a.ohi[1];
+  const bool two = tid + EA_NT < EA_NP;
+  double *bp = const_cast<double *>(a.crse) + comp * a.ccs +
+               (qx - a.c_lo[0]) + (qy - a.c_lo[1]) * a.c_sy;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double f0[2][2][2], f1[2][2][2];
+  int c0 = ea_load(q0, tid, q0[2] - 2, a.plo, a.phi, code, kids, f0);
+  int c1 = two ? ea_load(q0, tid + EA_NT, q0[2] - 2, a.plo, a.phi, code, kids, f1)
+               : -1;
+  for (int pz = q0[2] - 2; pz <= qz1 + 2; ++pz) {
+    {
+      double h[EA_NH];
+      unsigned info = ea_sums(c0, f0, w, h);
+      cs[tid] = info;
+      if (info) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          hs[2 * (k * (EA_NP + 1) + tid)] = h[k];
+          hs[2 * (k * (EA_NP + 1) + tid) + 1] = h[9 + k];
+        }
+      }
+      if (two) {
+        info = ea_sums(c1, f1, w, h);
+        cs[tid + EA_NT] = info;
+        if (info) {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            hs[2 * (k * (EA_NP + 1) + tid + EA_NT)] = h[k];
+            hs[2 * (k * (EA_NP + 1) + tid + EA_NT) + 1] = h[9 + k];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (pz < qz1 + 2) {
+      c0 = ea_load(q0, tid, pz + 1, a.plo, a.phi, code, kids, f0);
+      c1 = two ? ea_load(q0, tid + EA_NT, pz + 1, a.plo, a.phi, code, kids, f1)
+               : -1;
+    }
+    ea_gather(
+        lx, ly, [&](int n) { return cs[n]; },
+        [&](int i, double &h0, double &h1) {
+          h0 = hs[2 * i];
+          h1 = hs[2 * i + 1];
+        },
+        acc);
+    const int qz = pz - 2;
+    if (own && qz >= q0[2])
+      bp[(long long)(qz - a.c_lo[2]) * a.c_sz] += acc[0];
+    acc[0] = acc[1];
+    acc[1] = acc[2];
+    acc[2] = acc[3];
+    acc[3] = acc[4];
+    acc[4] = 0.0;
+    __syncthreads();  // the next plane rewrites cs and hs
+  }
 }
 
 EcArgs ec_args(const cxp_fab *crse, int crse_comp, const cxp_fab *fine,
@@ -160,6 +242,30 @@ EcArgs ec_args(const cxp_fab *crse, int crse_comp, const cxp_fab *fine,
   return a;
 }
 
+// Workgroups of k_eno_ccc_adj_tiled resident at once on the current
+// device (CUs x workgroups per CU), cached per device.
+long long ea_slots() {
+  static std::mutex mu;
+  static std::map<int, long long> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return 512;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end())
+    return it->second;
+  int cus = 0, per_cu = 0;
+  long long n = 512;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eno_ccc_adj_tiled,
+                                                   EA_NT, 0) == hipSuccess &&
+      cus > 0 && per_cu > 0)
+    n = (long long)cus * per_cu;
+  cache[dev] = n;
+  return n;
+}
+
 } // namespace
 
 bool eno_ccc_fits(const OpDesc &op, int per_group) {
@@ -188,12 +294,21 @@ int eno_ccc_adj_launch(const cxp_fab *crse_bar, int crse_comp,
                        const unsigned char *codes, hipStream_t s) {
   const EcArgs a =
       ec_args(crse_bar, crse_comp, fine_bar, fine_comp, rlo, rhi);
-  long long nout = 1;
-  for (int d = 0; d < 3; ++d)
-    nout *= a.ohi[d] - a.olo[d] + 1;
-  hipLaunchKernelGGL(k_eno_ccc_adj,
-                     dim3((unsigned)((nout + 255) / 256), ncomp), dim3(256), 0,
-                     s, a, codes);
+  const int ntx = (a.ohi[0] - a.olo[0] + EA_TX) / EA_TX;
+  const int nty = (a.ohi[1] - a.olo[1] + EA_TY) / EA_TY;
+  // As many z-chunks as fill the device's resident workgroups once (each
+  // chunk marches 4 parent planes more than its outputs), at least 4 planes
+  // per chunk.
+  const long long nk = a.ohi[2] - a.olo[2] + 1;
+  const long long cols = (long long)ntx * nty * ncomp;
+  long long nkc = ea_slots() / cols;
+  nkc = nkc < 1 ? 1 : (nkc > nk ? nk : nkc);
+  long long zc = (nk + nkc - 1) / nkc;
+  zc = zc < 4 ? (nk < 4 ? nk : 4) : zc;
+  const long long ntz = (nk + zc - 1) / zc;
+  hipLaunchKernelGGL(k_eno_ccc_adj_tiled,
+                     dim3((unsigned)(ntx * nty * ntz), ncomp), dim3(EA_NT), 0,
+                     s, a, codes, ntx, nty, (int)zc);
   return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
 }
 

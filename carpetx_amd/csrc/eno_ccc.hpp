@@ -236,4 +236,154 @@ __host__ __device__ inline double ec_adj_one(const int q[3], const int plo[3],
   return acc;
 }
 
+// ---- the tiled adjoint
+//
+// A workgroup of EA_NT threads owns EA_TX x EA_TY coarse output columns
+// (one per thread) of a z-chunk of the output window [plo - 2, phi + 2]
+// and marches the parent planes pz = qz0 - 2 .. qz1 + 2.  Per plane:
+//   L  each parent slot of the (EA_TX + 4) x (EA_TY + 4) parents around the
+//      tile: from its eight fine adjoints, the 18 partial sums
+//        H[oz][ty][tx] = sum_oy wy(sy, oy, ty) sum_ox wx(sx, ox, tx) fine_bar
+//      and an info word: which of the 25 columns within 2 of the parent its
+//      shifted stencil covers, the H index base and sz          -> LDS
+//   G  per output column: the 25 parent slots within 2 in x and y; a slot
+//      covering the column adds its H[0], H[1] at (tx, ty) to the sums B of
+//      its z-shift sz; then, for each sz and tz, output plane pz + sz + tz
+//      of a five-plane register window gains wz(sz, 0, tz) B[sz][0] +
+//      wz(sz, 1, tz) B[sz][1] (the z weights are compile-time constants);
+//      plane pz - 2 is then complete.
+// The sums are the Jacobian transpose of ec_children's values for the
+// recorded shifts (ec_adj_one), grouped by parent plane and z-shift.
+#define EA_TX 16
+#ifndef EA_TY
+#define EA_TY 16
+#endif
+#define EA_NT (EA_TX * EA_TY)
+#define EA_PX (EA_TX + 4)
+#define EA_PY (EA_TY + 4)
+#define EA_NP (EA_PX * EA_PY)  // parent slots per plane
+#define EA_NH 18               // H values per parent
+
+// Tile b of the output window [olo, ...] with ntx x nty column tiles and
+// z-chunks of zc planes: origin q0.
+__host__ __device__ inline void ea_tile(const int olo[3], int b, int ntx,
+                                        int nty, int zc, int q0[3]) {
+  q0[0] = olo[0] + EA_TX * (b % ntx);
+  q0[1] = olo[1] + EA_TY * ((b / ntx) % nty);
+  q0[2] = olo[2] + zc * (b / (ntx * nty));
+}
+
+// The info word of a parent with shift code c: bit (dx + 2) + 5 (dy + 2)
+// is set when the parent's shifted stencil covers the column at offset
+// (dx, dy) from it (|dx - sx| <= 1 and |dy - sy| <= 1); bits 25-28 hold
+// kb = 4 - 3 sy - sx, so H index k(tx, ty) = kb + 3 dy + dx; bits 29-30
+// hold sz + 1.  0: no parent.
+__host__ __device__ inline unsigned ea_info(int c) {
+  const int sx = c % 3 - 1, sy = (c / 3) % 3 - 1, sz = c / 9 - 1;
+  const unsigned mask = 0x1CE7u << ((sx + 1) + 5 * (sy + 1));  // 3 x 3 block
+  return mask | (unsigned)(4 - 3 * sy - sx) << 25 | (unsigned)(sz + 1) << 29;
+}
+
+// L, loads: parent slot n (< EA_NP) of plane pz around tile origin q0.
+// Returns its shift code and its eight fine adjoints in f[oz][oy][ox], or
+// -1 when the parent is outside [plo, phi] (f untouched).
+// code(px, py, pz) reads a parent's shift code, kids(px, py, pz, f) its
+// children's fine adjoints (0 outside the region).
+template <class CD, class KD>
+__host__ __device__ inline int ea_load(const int q0[3], int n, int pz,
+                                       const int plo[3], const int phi[3],
+                                       const CD &code, const KD &kids,
+                                       double (&f)[2][2][2]) {
+  const int px = q0[0] - 2 + n % EA_PX, py = q0[1] - 2 + n / EA_PX;
+  if (px < plo[0] || px > phi[0] || py < plo[1] || py > phi[1] ||
+      pz < plo[2] || pz > phi[2])
+    return -1;
+  kids(px, py, pz, f);
+  return code(px, py, pz);
+}
+
+// L, sums: the info word of a parent with code c and fine adjoints f, and
+// its 18 partial sums into h (w(s, o, t) = ec_w(s, o, t)).  c < 0: info 0,
+// h untouched.
+template <class W>
+__host__ __device__ inline unsigned ea_sums(int c, const double (&f)[2][2][2],
+                                            const W &w, double (&h)[EA_NH]) {
+  if (c < 0)
+    return 0u;
+  const int sx = c % 3 - 1, sy = (c / 3) % 3 - 1;
+#pragma unroll
+  for (int tx = -1; tx <= 1; ++tx) {
+    const double wx0 = w(sx, 0, tx), wx1 = w(sx, 1, tx);
+#pragma unroll
+    for (int oz = 0; oz < 2; ++oz) {
+      const double gx0 = wx0 * f[oz][0][0] + wx1 * f[oz][0][1];
+      const double gx1 = wx0 * f[oz][1][0] + wx1 * f[oz][1][1];
+#pragma unroll
+      for (int ty = -1; ty <= 1; ++ty)
+        h[oz * 9 + (ty + 1) * 3 + (tx + 1)] =
+            w(sy, 0, ty) * gx0 + w(sy, 1, ty) * gx1;
+    }
+  }
+  return ea_info(c);
+}
+
+// G: output column (lx, ly) of the tile.  cl(n) reads slot n's info word;
+// hp(i, h0, h1) reads H pair i (H[0][k], H[1][k] of slot n at i = k
+// (EA_NP + 1) + n; pair EA_NP holds zeros); acc[u] is output plane
+// pz - 2 + u.  A row of five slots at a time: their info words, then for
+// each slot and each z-shift class the pair it adds to that class's sums:
+// its own where the slot covers the column and has that class, else the
+// zero pair.  So every sum takes one read and one add per slot, with no
+// branches and no selects of values.
+template <class CL, class HP>
+__host__ __device__ inline void ea_gather(int lx, int ly, const CL &cl,
+                                          const HP &hp, double (&acc)[5]) {
+  // slot of offset (dx, dy): nb + (2 - dy) EA_PX + (2 - dx), offsets >= 0
+  const int nb = ly * EA_PX + lx;
+  double b[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};  // [sz + 1][oz]
+  // rows one after another (not unrolled: an unrolled body keeps every
+  // row's reads live at once)
+#pragma unroll 1
+  for (int dy = 2; dy >= -2; --dy) {
+    unsigned inf[5];
+#pragma unroll
+    for (int dx = 2; dx >= -2; --dx)
+      inf[2 - dx] = cl(nb + (2 - dy) * EA_PX + (2 - dx));
+    int idx[5][3];
+#pragma unroll
+    for (int dx = 2; dx >= -2; --dx) {
+      const int n = nb + (2 - dy) * EA_PX + (2 - dx);
+      const unsigned info = inf[2 - dx];
+      // all ones when the slot covers the column, else 0
+      const unsigned vm =
+          0u - ((info >> ((dx + 2) + 5 * (dy + 2))) & 1u);
+      const int own = ((int)((info >> 25) & 15u) + 3 * dy + dx) * (EA_NP + 1) + n;
+      // byte c of sel is 0xff for the slot's class c (none when it does
+      // not cover the column)
+      const unsigned sel = (0xffu << (8u * (info >> 29))) & vm;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const unsigned m = (unsigned)((int)(sel << (24 - 8 * c)) >> 24);
+        idx[2 - dx][c] = EA_NP + (int)((unsigned)(own - EA_NP) & m);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double h0, h1;
+        hp(idx[i][c], h0, h1);
+        b[c][0] += h0;
+        b[c][1] += h1;
+      }
+    }
+  }
+#pragma unroll
+  for (int sz = -1; sz <= 1; ++sz)
+#pragma unroll
+    for (int tz = -1; tz <= 1; ++tz)
+      acc[sz + tz + 2] += ec_w(sz, 0, tz) * b[sz + 1][0] +
+                          ec_w(sz, 1, tz) * b[sz + 1][1];
+}
+
 } // namespace cxp

@@ -4,9 +4,10 @@
 // checked against its array and against the phase that wrote it, every
 // coarse read against the coarse box and the window [parents - 2,
 // parents + 2], every fine write against the region (each fine point
-// written exactly once); and the adjoint's per-coarse-point sum with its
-// reads checked.  The CPU check of the kernels' indexing and arithmetic
-// (tests/test_eno_ccc_host.py), not a product path.
+// written exactly once); the adjoint's per-coarse-point sum with its reads
+// checked; and the tiled adjoint's workgroups (k_eno_ccc_adj_tiled) with
+// every LDS, code and output access checked.  The CPU check of the kernels'
+// indexing and arithmetic (tests/test_eno_ccc_host.py), not a product path.
 #include <cstdio>
 #include <vector>
 
@@ -185,6 +186,135 @@ long long cxp_host_eno_ccc_adjoint(const unsigned char *codes,
         const int q[3] = {i, j, k};
         cbar[at(clo, chi, i, j, k)] += ec_adj_one(q, plo, phi, code, fb);
       }
+  return ck.bad;
+}
+
+// The tiled adjoint's workgroups (k_eno_ccc_adj_tiled) one after another,
+// z-chunks of zc planes: cbar (box [clo, chi]) += J^T fbar from codes.
+// Every LDS read is checked against the plane whose phase L wrote it, every
+// code read against the region, and every output point is written exactly
+// once.  Returns the number of failed checks.
+long long cxp_host_eno_ccc_adjoint_tiled(const unsigned char *codes,
+                                         const double *fbar, const int rlo[3],
+                                         const int rhi[3], double *cbar,
+                                         const int clo[3], const int chi[3],
+                                         int zc) {
+  Check ck;
+  const int kNone = -(1 << 30);  // plane stamp of a slot never written
+  int plo[3], phi[3], olo[3], ohi[3];
+  for (int d = 0; d < 3; ++d) {
+    plo[d] = rlo[d] >> 1;
+    phi[d] = rhi[d] >> 1;
+    olo[d] = plo[d] - 2;
+    ohi[d] = phi[d] + 2;
+  }
+  const int ntx = (ohi[0] - olo[0] + EA_TX) / EA_TX;
+  const int nty = (ohi[1] - olo[1] + EA_TY) / EA_TY;
+  const int ntz = (ohi[2] - olo[2] + zc) / zc;
+  std::vector<int> writes((size_t)at(olo, ohi, ohi[0], ohi[1], ohi[2]) + 1, 0);
+  auto code = [&](int px, int py, int pz) -> int {
+    if (px < plo[0] || px > phi[0] || py < plo[1] || py > phi[1] ||
+        pz < plo[2] || pz > phi[2]) {
+      ck.report("code of a parent outside the region", px, pz);
+      return 13;
+    }
+    const int fx = 2 * px > rlo[0] ? 2 * px : rlo[0];
+    const int fy = 2 * py > rlo[1] ? 2 * py : rlo[1];
+    const int fz = 2 * pz > rlo[2] ? 2 * pz : rlo[2];
+    if (!inside(rlo, rhi, fx, fy, fz)) {
+      ck.report("code read outside the region", fx, fz);
+      return 13;
+    }
+    return codes[at(rlo, rhi, fx, fy, fz)];
+  };
+  auto kids = [&](int px, int py, int pz, double (&f)[2][2][2]) {
+    if (px < plo[0] || px > phi[0] || py < plo[1] || py > phi[1] ||
+        pz < plo[2] || pz > phi[2])
+      ck.report("children of a parent outside the region", px, pz);
+    for (int oz = 0; oz < 2; ++oz)
+      for (int oy = 0; oy < 2; ++oy)
+        for (int ox = 0; ox < 2; ++ox) {
+          const int x = 2 * px + ox, y = 2 * py + oy, z = 2 * pz + oz;
+          f[oz][oy][ox] =
+              inside(rlo, rhi, x, y, z) ? fbar[at(rlo, rhi, x, y, z)] : 0.0;
+        }
+  };
+  for (int b = 0; b < ntx * nty * ntz; ++b) {
+    int q0[3];
+    ea_tile(olo, b, ntx, nty, zc, q0);
+    const int qz1 = q0[2] + zc - 1 < ohi[2] ? q0[2] + zc - 1 : ohi[2];
+    std::vector<unsigned> cs(EA_NP, 0x5A5A5A5Au);
+    std::vector<int> cs_ph(EA_NP, kNone);
+    std::vector<double> hs(EA_NH * (EA_NP + 1), 0.0);
+    std::vector<int> hs_ph(9 * (EA_NP + 1), kNone);
+    std::vector<double> acc((size_t)EA_NT * 5, 0.0);
+    for (int pz = q0[2] - 2; pz <= qz1 + 2; ++pz) {
+      for (int tid = 0; tid < EA_NT; ++tid)
+        for (int n = tid; n < EA_NP; n += EA_NT) {
+          double h[EA_NH], f[2][2][2];
+          const int code_n = ea_load(q0, n, pz, plo, phi, code, kids, f);
+          const unsigned c = ea_sums(code_n, f, ec_w, h);
+          if (cs_ph[n] == pz)
+            ck.report("parent slot written twice in a plane", n, pz);
+          cs[n] = c;
+          cs_ph[n] = pz;
+          if (c != 0u)
+            for (int k = 0; k < 9; ++k) {
+              hs[2 * (k * (EA_NP + 1) + n)] = h[k];
+              hs[2 * (k * (EA_NP + 1) + n) + 1] = h[9 + k];
+              hs_ph[k * (EA_NP + 1) + n] = pz;
+            }
+        }
+      for (int tid = 0; tid < EA_NT; ++tid) {
+        const int lx = tid % EA_TX, ly = tid / EA_TX;
+        double a5[5];
+        for (int u = 0; u < 5; ++u)
+          a5[u] = acc[tid * 5 + u];
+        ea_gather(
+            lx, ly,
+            [&](int n) {
+              if (n < 0 || n >= EA_NP || cs_ph[n] != pz) {
+                ck.report("info slot read outside the tile or stale", n, pz);
+                return 0u;
+              }
+              return cs[n];
+            },
+            [&](int i, double &h0, double &h1) {
+              h0 = h1 = 0.0;
+              if (i < 0 || i >= 9 * (EA_NP + 1)) {
+                ck.report("H pair read outside the array", i, pz);
+                return;
+              }
+              // pair EA_NP: the zero pair, never written
+              if (i != EA_NP && hs_ph[i] != pz) {
+                ck.report("stale H pair read", i, pz);
+                return;
+              }
+              h0 = hs[2 * i];
+              h1 = hs[2 * i + 1];
+            },
+            a5);
+        const int q[3] = {q0[0] + lx, q0[1] + ly, pz - 2};
+        if (q[0] <= ohi[0] && q[1] <= ohi[1] && q[2] >= q0[2]) {
+          if (!inside(olo, ohi, q[0], q[1], q[2]) ||
+              !inside(clo, chi, q[0], q[1], q[2])) {
+            ck.report("coarse write outside the window or cbar", q[0], q[2]);
+          } else {
+            ++writes[(size_t)at(olo, ohi, q[0], q[1], q[2])];
+            cbar[at(clo, chi, q[0], q[1], q[2])] += a5[0];
+          }
+        }
+        acc[tid * 5 + 0] = a5[1];
+        acc[tid * 5 + 1] = a5[2];
+        acc[tid * 5 + 2] = a5[3];
+        acc[tid * 5 + 3] = a5[4];
+        acc[tid * 5 + 4] = 0.0;
+      }
+    }
+  }
+  for (size_t m = 0; m < writes.size(); ++m)
+    if (writes[m] != 1)
+      ck.report("output point not written exactly once", (long long)m, writes[m]);
   return ck.bad;
 }
 
