@@ -109,6 +109,21 @@ def build_minmod_host():
     os.replace(tmp, MINMOD_HOST)
 
 
+ADJ_ZF_HOST = os.path.join(HERE, "host", "libadj_zfirst_host.so")
+
+
+def build_adj_zfirst_host():
+    """The z-first linear adjoint's routines built for the host with checked
+    accesses (host/adj_zfirst_host.cpp): the CPU check of the kernel's
+    indexing, loaded by the tests only."""
+    tmp = ADJ_ZF_HOST + ".tmp"
+    run([HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17",
+         "-ffp-contract=off", "-fPIC", "-shared",
+         "-Wno-unused-command-line-argument",
+         os.path.join(HERE, "host", "adj_zfirst_host.cpp"), "-o", tmp])
+    os.replace(tmp, ADJ_ZF_HOST)
+
+
 ENO_HOST = os.path.join(HERE, "host", "libeno_ccc_host.so")
 
 
@@ -135,6 +150,7 @@ def main():
     build_host_check()
     build_minmod_host()
     build_eno_host()
+    build_adj_zfirst_host()
     build_interpolater_check()
     print("built %s in %.1fs" % (LIB, time.time() - t0))
 

@@ -27,61 +27,15 @@
 #include <type_traits>
 
 #include "../../include/cxp.h"
+#include "adj_zfirst.hpp"
 #include "kernels.hpp"
 #include "minmod_launch.hpp"
 #include "ops.hpp"
 
 namespace cxp {
 
-// --------------------------------------------------------------------------
-// Weight tables of the linear operators: w[sid][off][q] is the weight of
-// coarse offset q + wlo(sid) in the forward formula for fine offset off.
-
-struct DeltaAcc {
-  int o;
-  __host__ __device__ constexpr double operator()(int k) const {
-    return k == o ? 1.0 : 0.0;
-  }
-};
-
-template <int S> constexpr double lin_weight(int off, int o) {
-  return Op1<S>::apply(DeltaAcc{o}, off);
-}
-
-constexpr double lin_weight_rt(int sid, int off, int o) {
-  switch (sid) {
-#define CXP_LW(S)                                                              \
-  case S:                                                                      \
-    return lin_weight<S>(off, o);
-    CXP_LW(SID_VPOLY1) CXP_LW(SID_VPOLY3) CXP_LW(SID_VPOLY5) CXP_LW(SID_VPOLY7)
-    CXP_LW(SID_CPOLY0) CXP_LW(SID_CPOLY1) CXP_LW(SID_CPOLY2) CXP_LW(SID_CPOLY3)
-    CXP_LW(SID_CPOLY4) CXP_LW(SID_CPOLY5) CXP_LW(SID_CPOLY6) CXP_LW(SID_CPOLY7)
-    CXP_LW(SID_VCONS0) CXP_LW(SID_VCONS1) CXP_LW(SID_CCONS0) CXP_LW(SID_CCONS1)
-    CXP_LW(SID_CCONS2) CXP_LW(SID_CCONS3) CXP_LW(SID_CCONS4) CXP_LW(SID_CCONS5)
-    CXP_LW(SID_CCONS6) CXP_LW(SID_CCONS7)
-#undef CXP_LW
-  }
-  return 0.0;
-}
-
-struct AdjW {
-  double w[SID_COUNT][2][9];
-};
-
-constexpr AdjW make_adjw() {
-  AdjW t{};
-  for (int s = 0; s < SID_COUNT; ++s)
-    for (int off = 0; off < 2; ++off)
-      for (int q = 0; q < 9; ++q) {
-        const SidInfo si = sid_info(s);
-        const int o = q + sid_wlo(s);
-        const bool in = o >= si.lo[off] && o <= si.hi[off];
-        t.w[s][off][q] = in ? lin_weight_rt(s, off, o) : 0.0;
-      }
-  return t;
-}
-
-__constant__ AdjW kAdjW = make_adjw();
+// The weight tables of the linear operators (kAdjWc) and the z-first
+// adjoint routines are in adj_zfirst.hpp.
 
 // --------------------------------------------------------------------------
 // Linear adjoint
@@ -97,8 +51,6 @@ __constant__ AdjW kAdjW = make_adjw();
 //   z^T: accumulate into a register window of coarse planes, one plane
 //        completed (and added to crse_bar) per pair of fine planes.
 // Each transpose is compiled per stencil with its nonzero weights only.
-
-constexpr AdjW kAdjWc = make_adjw();
 
 template <int S> constexpr int adj_span() { return sid_whi(S) - sid_wlo(S); }
 
@@ -322,6 +274,108 @@ __global__ void __launch_bounds__(256)
     for (int q = 0; q < 8; ++q)
       acc[q] = acc[q + 1];
     acc[8] = 0.0;
+  }
+}
+
+// Uniform-stencil linear adjoint, z-transpose first (routines and their
+// description in adj_zfirst.hpp).  Fine loads are raw buffer loads one
+// parent plane ahead (points outside the fine region read as 0); the old
+// crse_bar value of a complete plane is loaded before that plane's LDS
+// passes.
+template <int S>
+__global__ void __launch_bounds__(AZ_NT)
+    k_adj_zf(AdjTask t, int ntx, int nty, int kchunk) {
+  using G = AzGeom<S>;
+  __shared__ double Zs[G::NZ];
+  __shared__ double Ys[G::NY];
+  const int tid = threadIdx.x;
+  const long long comp = blockIdx.y;
+  int i0, j0, kca;
+  az_tile(t.olo, xcd_block(blockIdx.x, gridDim.x), ntx, nty, kchunk, i0, j0,
+          kca);
+  const int kcb = min(t.ohi[2], kca + kchunk - 1);
+  const __amdgpu_buffer_rsrc_t rf =
+      make_rsrc(t.fbar.p + comp * t.fcs, t.fbytes);
+  const __amdgpu_buffer_rsrc_t rc =
+      make_rsrc(t.cbar.p + comp * t.ccs, t.cbytes);
+  const int fxa = 2 * (i0 - G::whi), fya = 2 * (j0 - G::whi);
+  // owned columns: byte offset inside a fine plane, -1 outside the window
+  // or the fine region
+  int soff[G::R];
+#pragma unroll
+  for (int r = 0; r < G::R; ++r) {
+    const int c = tid + AZ_NT * r;
+    const int fy = c / G::W, fx = c - fy * G::W;
+    const int gx = fxa + fx, gy = fya + fy;
+    const bool in = c < G::NZ && gx >= t.rlo[0] && gx <= t.rhi[0] &&
+                    gy >= t.rlo[1] && gy <= t.rhi[1];
+    soff[r] = in ? 8 * (int)((gx - t.fbar.lo[0]) +
+                             (long long)(gy - t.fbar.lo[1]) * t.fbar.sy)
+                 : -1;
+  }
+  auto plane = [&](int fz) -> int {
+    return (fz >= t.rlo[2] && fz <= t.rhi[2])
+               ? 8 * (int)((long long)(fz - t.fbar.lo[2]) * t.fbar.sz)
+               : -1;
+  };
+  const int pA = kca - G::whi, pB = kcb - G::wlo;
+  double f[2][G::R];
+  auto load = [&](int pz) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int pb = plane(2 * pz + h);
+#pragma unroll
+      for (int r = 0; r < G::R; ++r)
+        f[h][r] = bld(rf, (pb >= 0 && soff[r] >= 0) ? pb + soff[r] : CXP_OOB);
+    }
+  };
+  load(pA);
+  double z[G::R][G::span + 1];
+#pragma unroll
+  for (int r = 0; r < G::R; ++r)
+#pragma unroll
+    for (int q = 0; q <= G::span; ++q)
+      z[r][q] = 0.0;
+  const int i = tid % AZ_T, j = tid / AZ_T;
+  const bool own = i0 + i <= t.ohi[0] && j0 + j <= t.ohi[1];
+  for (int pz = pA; pz <= pB; ++pz) {
+    double g[2][G::R];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int r = 0; r < G::R; ++r)
+        g[h][r] = f[h][r];
+    if (pz < pB)
+      load(pz + 1);
+#pragma unroll
+    for (int r = 0; r < G::R; ++r)
+      az_zstep<S>(z[r], g[0][r], g[1][r]);
+    const int k = pz + G::wlo;  // complete now
+    if (k >= kca) {             // uniform over the workgroup
+      const int co = own ? 8 * (int)didx(t.cbar, i0 + i, j0 + j, k) : CXP_OOB;
+      const double cold = bld(rc, co);
+#pragma unroll
+      for (int r = 0; r < G::R; ++r) {
+        const int c = tid + AZ_NT * r;
+        if (c < G::NZ)
+          Zs[c] = z[r][0];
+      }
+      __syncthreads();
+      for (int e = tid; e < G::NY; e += AZ_NT)
+        Ys[e] = az_y<S>(e, [&](int o) { return Zs[o]; });
+      __syncthreads();
+      // Zs is rewritten only after the next plane's z^T, Ys after the next
+      // barrier: every read of both precedes the barrier above or this one
+      const double x = az_x<S>(i, j, [&](int o) { return Ys[o]; });
+      bst(rc, co, cold + x);
+    }
+#pragma unroll
+    for (int r = 0; r < G::R; ++r) {
+#pragma unroll
+      for (int q = 0; q < G::span; ++q)
+        z[r][q] = z[r][q + 1];
+      z[r][G::span] = 0.0;
+    }
   }
 }
 
@@ -602,12 +656,18 @@ inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
     const dim3 grid(ntx * nty * nkc, ncomp);
     bool done = false;
     if (op.sid[0] == op.sid[1] && op.sid[1] == op.sid[2]) {
+      // one stencil in every direction: the z-first kernel, with z-chunks
+      // of at least 8 planes (a chunk re-reads span_z parent planes; at
+      // 96^3 -> 192^3 VVV order 5, chunks of 8 took 30 us, of 13 32 us, of
+      // 26 43 us: parallelism outweighs the re-reads)
+      int kz = (int)((nk * cols + 1023) / 1024);
+      kz = kz < 8 ? 8 : (kz > nk ? nk : kz);
+      const dim3 gz(ntx * nty * ((nk + kz - 1) / kz), ncomp);
       done = true;
       switch (op.sid[0]) {
 #define CXP_AL(S)                                                              \
   case S:                                                                      \
-    hipLaunchKernelGGL(k_adj_linear<S>, grid, dim3(256), 0, s, t, ntx, nty,     \
-                       kchunk);                                                \
+    hipLaunchKernelGGL(k_adj_zf<S>, gz, dim3(AZ_NT), 0, s, t, ntx, nty, kz);   \
     break;
         CXP_ADJ_LINEAR_SIDS(CXP_AL)
 #undef CXP_AL
