@@ -934,21 +934,44 @@ void schedule_tiles(cxp_worklist_s *wl, int grid) {
   };
   const size_t nt = wl->tiles.size();
   std::vector<long long> cost(nt);
+  // A group is every tile that writes one component of one fine FAB: all
+  // patches of a FAB (the six slabs of a ghost shell) go to one XCD, in
+  // patch order.  Their coarse reads overlap at the slabs' edges, and the
+  // low-x ghosts of one row and the high-x ghosts of the previous row share
+  // 128-byte lines, whose partial writes merge in that XCD's L2 when both
+  // slabs run at about the same time (equal-cost tiles keep this order in
+  // the longest-first sort below).
   std::vector<Group> groups;
-  for (size_t i = 0; i < nt; ++i) {
-    const SepTile &tl = wl->tiles[i];
-    const SepTask &t = wl->tasks[tl.task];
-    // marching steps, weighted by the points a step covers
-    cost[i] = tl.cfg < 0 ? 2
-                         : (long long)((tl.z1 >> 1) + t.whi[2] - (tl.z0 >> 1) -
-                                       t.wlo[2] + 1) *
-                                   (tl.cfg >= 16 ? 2 : 1) +
-                               4;
-    if (groups.empty() || wl->tiles[groups.back().first].task != tl.task ||
-        wl->tiles[groups.back().first].comp != tl.comp)
-      groups.push_back({i, 0, 0});
-    ++groups.back().count;
-    groups.back().cost += cost[i];
+  std::vector<size_t> member;  // tiles in group order
+  {
+    std::map<std::pair<const double *, int>, size_t> gid;
+    std::vector<std::vector<size_t>> gt;
+    for (size_t i = 0; i < nt; ++i) {
+      const SepTile &tl = wl->tiles[i];
+      const SepTask &t = wl->tasks[tl.task];
+      // marching steps, weighted by the points a step covers
+      cost[i] = tl.cfg < 0 ? 2
+                           : (long long)((tl.z1 >> 1) + t.whi[2] - (tl.z0 >> 1) -
+                                         t.wlo[2] + 1) *
+                                     (tl.cfg >= 16 ? 2 : 1) +
+                                 4;
+      const auto key = std::make_pair((const double *)t.fine.p, tl.comp);
+      auto it = gid.find(key);
+      if (it == gid.end()) {
+        it = gid.emplace(key, gt.size()).first;
+        gt.emplace_back();
+      }
+      gt[it->second].push_back(i);
+    }
+    member.reserve(nt);
+    for (const auto &v : gt) {
+      Group g{member.size(), v.size(), 0};
+      for (size_t i : v) {
+        member.push_back(i);
+        g.cost += cost[i];
+      }
+      groups.push_back(g);
+    }
   }
   const int nx = grid >= 8 ? 8 : 1;  // XCD classes of workgroups
   const int cus = device_cus();
@@ -967,7 +990,7 @@ void schedule_tiles(cxp_worklist_s *wl, int grid) {
       if (xload[x] < xload[best])
         best = x;
     for (size_t k = 0; k < groups[gi].count; ++k)
-      xcd_tiles[best].push_back(groups[gi].first + k);
+      xcd_tiles[best].push_back(member[groups[gi].first + k]);
     xload[best] += groups[gi].cost;
   }
   // within an XCD: the smallest tiles, kQueuePct percent of the XCD's cost,
