@@ -310,18 +310,11 @@ k_mm_fused(MtArgs a) {
   MtXY xyv;
   mt_win_zero(win);
   mt_xy_zero(xyv);
-  for (int s = t.k0 - 2; s <= t.k1 + 3; ++s) {
-    const bool produce = s <= t.k1 + 2;
-    const bool own = s >= t.k0 && s <= t.k1;
-    const int k = s - 3;  // plane whose y^T, x^T run in this step
-    unsigned char *bxs = bxr[s & (MT_RING - 1)];
-    unsigned char *bys = byr[s & (MT_RING - 1)];
-    const unsigned char *bxk = bxr[k & (MT_RING - 1)];
-    const unsigned char *byk = byr[k & (MT_RING - 1)];
-    // A: this plane's coarse values and the fine adjoints of parent s - 1
-    // (used in C; the barriers wait for LDS traffic only)
-    double cl[MT_NCL];
-    if (produce)
+  // plane s's coarse values and the fine adjoints of parent s - 1, loaded
+  // one step ahead so their latency overlaps the previous step
+  double cl[MT_NCL], g0[MT_NPOS], g1[MT_NPOS];
+  auto load_step = [&](int s) {
+    if (s <= t.k1 + 2)
 #pragma unroll
       for (int n = 0; n < MT_NCL; ++n) {
         const int q = tid + MT_T * n;
@@ -330,17 +323,32 @@ k_mm_fused(MtArgs a) {
     const int pz = s - 1;
     const bool zt =
         pz >= t.k0 - 1 && pz <= t.k1 + 1 && pz >= plo2 && pz <= phi2;
-    double g0[MT_NPOS], g1[MT_NPOS];
-    {
-      const int fz0 = 2 * pz;
-      const bool in0 = zt && fz0 >= a.rlo[2], in1 = zt && fz0 + 1 <= a.rhi[2];
-      const double *pl0 = fp + (fz0 - a.f_lo[2]) * a.f_sz;
-      const double *pl1 = pl0 + a.f_sz;
+    const int fz0 = 2 * pz;
+    const bool in0 = zt && fz0 >= a.rlo[2], in1 = zt && fz0 + 1 <= a.rhi[2];
+    const double *pl0 = fp + (fz0 - a.f_lo[2]) * a.f_sz;
+    const double *pl1 = pl0 + a.f_sz;
 #pragma unroll
-      for (int m = 0; m < MT_NPOS; ++m) {
-        g0[m] = pos.in[m] && in0 ? pl0[pos.off[m]] : 0.0;
-        g1[m] = pos.in[m] && in1 ? pl1[pos.off[m]] : 0.0;
-      }
+    for (int m = 0; m < MT_NPOS; ++m) {
+      g0[m] = pos.in[m] && in0 ? pl0[pos.off[m]] : 0.0;
+      g1[m] = pos.in[m] && in1 ? pl1[pos.off[m]] : 0.0;
+    }
+  };
+  load_step(t.k0 - 2);
+  for (int s = t.k0 - 2; s <= t.k1 + 3; ++s) {
+    const bool produce = s <= t.k1 + 2;
+    const bool own = s >= t.k0 && s <= t.k1;
+    const int k = s - 3;  // plane whose y^T, x^T run in this step
+    unsigned char *bxs = bxr[s & (MT_RING - 1)];
+    unsigned char *bys = byr[s & (MT_RING - 1)];
+    const unsigned char *bxk = bxr[k & (MT_RING - 1)];
+    const unsigned char *byk = byr[k & (MT_RING - 1)];
+    // A: this plane's coarse values to LDS; the fine adjoints of parent
+    // s - 1 are used in C
+    double gc0[MT_NPOS], gc1[MT_NPOS];
+#pragma unroll
+    for (int m = 0; m < MT_NPOS; ++m) {
+      gc0[m] = g0[m];
+      gc1[m] = g1[m];
     }
     if (produce)
 #pragma unroll
@@ -349,6 +357,8 @@ k_mm_fused(MtArgs a) {
         if (q < MT_CW * MT_CW)
           cpl[q] = cl[n];
       }
+    if (s < t.k1 + 3)
+      load_step(s + 1);
     __syncthreads();
     if (produce)
       mt_x(tid, cpl_ld, xpl_st, own,
@@ -370,7 +380,7 @@ k_mm_fused(MtArgs a) {
     const bool store = s - 2 >= t.k0 && s - 2 <= t.k1;
 #pragma unroll
     for (int m = 0; m < MT_NPOS; ++m) {
-      const double done = mt_zt(win, m, bz[m], g0[m], g1[m]);
+      const double done = mt_zt(win, m, bz[m], gc0[m], gc1[m]);
       if (store)
         xy[tid + MT_T * m] = done;
     }
