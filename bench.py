@@ -729,6 +729,11 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (load_traffic("uneven_512") or {}).get("bytes_per_launch"),
+            # the same algorithmic bytes over the time of the list's store
+            # pattern alone (one store per fill point, no reads): the most
+            # this layout of partial-line writes allows
+            "frac_ceiling_store_floor": round(
+                st["algo_bytes"] / (res["store_floor_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
         "extra": {
             "wall_ms_per_step": round(res["wall_ms"], 4),

@@ -598,11 +598,13 @@ int adjoint_point_run(const cxp_op_s *op, const cxp_fab *crse_in,
                       const int rlo[3], const int rhi[3], int per_group,
                       unsigned char *codes, hipStream_t s) {
   if (eno_ccc_fits(op->d, per_group)) {
+    // one code per coarse parent: an eighth of the bytes of the per-point
+    // record, written and read contiguously
     if (const int e = eno_ccc_launch(crse_in, crse_comp, nullptr, 0, ncomp,
-                                     rlo, rhi, codes, true, s))
+                                     rlo, rhi, codes, true, s, true))
       return e;
     return eno_ccc_adj_launch(crse_bar, crse_comp, fine_bar, fine_comp, ncomp,
-                              rlo, rhi, codes, s);
+                              rlo, rhi, codes, s, true);
   }
   const AdjPointTask t =
       make_adj_point_task(op, crse_in, crse_bar, crse_comp, fine_bar,

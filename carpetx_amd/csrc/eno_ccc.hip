@@ -1,9 +1,12 @@
 // CCC ENO kernels (routines and their description in eno_ccc.hpp) and
 // their launches (eno_launch.hpp):
-//  * k_eno_ccc<CODES>  forward: values (and, given a code array, each fine
-//                      point's branch code beside them), or with CODES the
-//                      branch codes alone;
-//  * k_eno_ccc_adj_tiled  crse_bar += J^T fine_bar from the branch codes.
+//  * k_eno_ccc<CODES, PARENT>  forward: values (and, given a code array,
+//                      each fine point's branch code beside them), or with
+//                      CODES the branch codes alone, per fine point or (with
+//                      PARENT, for the adjoint that recomputes them) one per
+//                      coarse parent;
+//  * k_eno_ccc_adj_tiled<PARENT>  crse_bar += J^T fine_bar from the branch
+//                      codes of either layout.
 // One component per grid row y.
 #include <hip/hip_runtime.h>
 
@@ -26,9 +29,13 @@ struct EcArgs {
   int rlo[3], rhi[3], plo[3], phi[3], olo[3], ohi[3];
   int ntx, nty;
   long long npts;  // fine points of the region (codes per component)
+  long long npar;  // coarse parents of the region (compact codes per component)
 };
 
-template <bool CODES>
+// CODES: branch codes alone (no values); PARENT: one code per coarse
+// parent, parent-linear over [plo, phi] (else one per fine point of the
+// region).
+template <bool CODES, bool PARENT = false>
 __global__ void __launch_bounds__(EC_T)
     k_eno_ccc(EcArgs a, unsigned char *__restrict__ codes) {
   __shared__ double cw[EC_NC];
@@ -76,6 +83,12 @@ __global__ void __launch_bounds__(EC_T)
   if (p[0] > a.phi[0] || p[1] > a.phi[1] || p[2] > a.phi[2])
     return;
   const int code = ec_search(l, ml);
+  if constexpr (PARENT) {
+    const long long npx = a.phi[0] - a.plo[0] + 1, npy = a.phi[1] - a.plo[1] + 1;
+    codes[comp * a.npar + (p[0] - a.plo[0]) + (p[1] - a.plo[1]) * npx +
+          (p[2] - a.plo[2]) * npx * npy] = (unsigned char)code;
+    return;
+  }
   double v[2][2][2];
   if constexpr (!CODES)
     ec_children(l, code, cl, v);
@@ -104,6 +117,7 @@ __global__ void __launch_bounds__(EC_T)
 // range; zc output planes per workgroup.  Thread t loads parent slots t and
 // t + EA_NT (EA_NP <= 2 EA_NT); the next plane's loads are issued before
 // the gather of the current one, so their latency overlaps it.
+template <bool PARENT>
 __global__ void __launch_bounds__(EA_NT, 2)
     k_eno_ccc_adj_tiled(EcArgs a, const unsigned char *__restrict__ codes,
                         int ntx, int nty, int zc) {
@@ -121,13 +135,17 @@ __global__ void __launch_bounds__(EA_NT, 2)
     ws[tid] = ec_w(tid / 6 - 1, (tid / 3) % 2, tid % 3 - 1);
   if (tid < 2)
     hs[2 * EA_NP + tid] = 0.0;
-  const unsigned char *cd = codes + comp * a.npts;
+  const unsigned char *cd = codes + comp * (PARENT ? a.npar : a.npts);
   const double *fp = a.fine + comp * a.fcs;
   const long long rnx = a.rhi[0] - a.rlo[0] + 1;
   const long long rny = a.rhi[1] - a.rlo[1] + 1;
-  // a parent's code sits at each of its fine points in the region; read
-  // the first
+  const long long npx = a.phi[0] - a.plo[0] + 1, npy = a.phi[1] - a.plo[1] + 1;
+  // per fine point: a parent's code sits at each of its fine points in the
+  // region, read the first; per parent: parent-linear
   auto code = [&](int px, int py, int pz) -> int {
+    if constexpr (PARENT)
+      return cd[(px - a.plo[0]) + (py - a.plo[1]) * npx +
+                (pz - a.plo[2]) * npx * npy];
     const int fx = max(2 * px, a.rlo[0]), fy = max(2 * py, a.rlo[1]),
               fz = max(2 * pz, a.rlo[2]);
     return cd[(fx - a.rlo[0]) + (fy - a.rlo[1]) * rnx +
@@ -197,9 +215,10 @@ __global__ void __launch_bounds__(EA_NT, 2)
     }
     ea_gather(
         lx, ly, [&](int n) { return cs[n]; },
-        [&](int i, double &h0, double &h1) {
-          h0 = hs[2 * i];
-          h1 = hs[2 * i + 1];
+        [&](int b, double &h0, double &h1) {
+          const double *q = (const double *)((const char *)hs + b);
+          h0 = q[0];
+          h1 = q[1];
         },
         acc);
     const int qz = pz - 2;
@@ -237,6 +256,8 @@ EcArgs ec_args(const cxp_fab *crse, int crse_comp, const cxp_fab *fine,
     a.ohi[d] = a.phi[d] + 2;
     a.npts *= rhi[d] - rlo[d] + 1;
   }
+  a.npar = (long long)(a.phi[0] - a.plo[0] + 1) * (a.phi[1] - a.plo[1] + 1) *
+           (a.phi[2] - a.plo[2] + 1);
   a.ntx = (a.phi[0] - a.plo[0] + EC_PX) / EC_PX;
   a.nty = (a.phi[1] - a.plo[1] + EC_PY) / EC_PY;
   return a;
@@ -258,8 +279,8 @@ long long ea_slots() {
   long long n = 512;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                             dev) == hipSuccess &&
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eno_ccc_adj_tiled,
-                                                   EA_NT, 0) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, k_eno_ccc_adj_tiled<false>, EA_NT, 0) == hipSuccess &&
       cus > 0 && per_cu > 0)
     n = (long long)cus * per_cu;
   cache[dev] = n;
@@ -276,12 +297,14 @@ bool eno_ccc_fits(const OpDesc &op, int per_group) {
 int eno_ccc_launch(const cxp_fab *crse, int crse_comp, const cxp_fab *fine,
                    int fine_comp, int ncomp, const int rlo[3],
                    const int rhi[3], unsigned char *codes, bool codes_only,
-                   hipStream_t s) {
+                   hipStream_t s, bool parent_codes) {
   const EcArgs a = ec_args(crse, crse_comp, fine ? fine : crse,
                            fine ? fine_comp : 0, rlo, rhi);
   const int ntz = (a.phi[2] - a.plo[2] + EC_PZ) / EC_PZ;
   const dim3 grid((unsigned)(a.ntx * a.nty * ntz), ncomp);
-  if (codes_only)
+  if (codes_only && parent_codes)
+    hipLaunchKernelGGL((k_eno_ccc<true, true>), grid, dim3(EC_T), 0, s, a, codes);
+  else if (codes_only)
     hipLaunchKernelGGL(k_eno_ccc<true>, grid, dim3(EC_T), 0, s, a, codes);
   else
     hipLaunchKernelGGL(k_eno_ccc<false>, grid, dim3(EC_T), 0, s, a, codes);
@@ -291,7 +314,8 @@ int eno_ccc_launch(const cxp_fab *crse, int crse_comp, const cxp_fab *fine,
 int eno_ccc_adj_launch(const cxp_fab *crse_bar, int crse_comp,
                        const cxp_fab *fine_bar, int fine_comp, int ncomp,
                        const int rlo[3], const int rhi[3],
-                       const unsigned char *codes, hipStream_t s) {
+                       const unsigned char *codes, hipStream_t s,
+                       bool parent_codes) {
   const EcArgs a =
       ec_args(crse_bar, crse_comp, fine_bar, fine_comp, rlo, rhi);
   const int ntx = (a.ohi[0] - a.olo[0] + EA_TX) / EA_TX;
@@ -306,9 +330,13 @@ int eno_ccc_adj_launch(const cxp_fab *crse_bar, int crse_comp,
   long long zc = (nk + nkc - 1) / nkc;
   zc = zc < 4 ? (nk < 4 ? nk : 4) : zc;
   const long long ntz = (nk + zc - 1) / zc;
-  hipLaunchKernelGGL(k_eno_ccc_adj_tiled,
-                     dim3((unsigned)(ntx * nty * ntz), ncomp), dim3(EA_NT), 0,
-                     s, a, codes, ntx, nty, (int)zc);
+  const dim3 grid((unsigned)(ntx * nty * ntz), ncomp);
+  if (parent_codes)
+    hipLaunchKernelGGL(k_eno_ccc_adj_tiled<true>, grid, dim3(EA_NT), 0, s, a,
+                       codes, ntx, nty, (int)zc);
+  else
+    hipLaunchKernelGGL(k_eno_ccc_adj_tiled<false>, grid, dim3(EA_NT), 0, s, a,
+                       codes, ntx, nty, (int)zc);
   return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
 }
 

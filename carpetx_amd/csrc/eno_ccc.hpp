@@ -328,16 +328,19 @@ __host__ __device__ inline unsigned ea_sums(int c, const double (&f)[2][2][2],
 }
 
 // G: output column (lx, ly) of the tile.  cl(n) reads slot n's info word;
-// hp(i, h0, h1) reads H pair i (H[0][k], H[1][k] of slot n at i = k
-// (EA_NP + 1) + n; pair EA_NP holds zeros); acc[u] is output plane
-// pz - 2 + u.  A row of five slots at a time: their info words, then for
-// each slot and each z-shift class the pair it adds to that class's sums:
-// its own where the slot covers the column and has that class, else the
-// zero pair.  So every sum takes one read and one add per slot, with no
-// branches and no selects of values.
+// hp(b, h0, h1) reads the H pair at byte offset b = 16 i of the pair array
+// (pair i = k (EA_NP + 1) + n holds H[0][k], H[1][k] of slot n; pair EA_NP
+// holds zeros); acc[u] is output plane pz - 2 + u.  A row of five slots at
+// a time: their info words, then for each slot and each z-shift class the
+// pair it adds to that class's sums: its own where the slot covers the
+// column and has that class, else the zero pair.  So every sum takes one
+// read and one add per slot, with no branches and no selects of values;
+// offsets are formed in bytes, so masking them is the only index work.
 template <class CL, class HP>
 __host__ __device__ inline void ea_gather(int lx, int ly, const CL &cl,
                                           const HP &hp, double (&acc)[5]) {
+  constexpr int PB = 16 * (EA_NP + 1);  // bytes per H index k
+  constexpr int ZB = 16 * EA_NP;        // the zero pair
   // slot of offset (dx, dy): nb + (2 - dy) EA_PX + (2 - dx), offsets >= 0
   const int nb = ly * EA_PX + lx;
   double b[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};  // [sz + 1][oz]
@@ -349,7 +352,7 @@ __host__ __device__ inline void ea_gather(int lx, int ly, const CL &cl,
 #pragma unroll
     for (int dx = 2; dx >= -2; --dx)
       inf[2 - dx] = cl(nb + (2 - dy) * EA_PX + (2 - dx));
-    int idx[5][3];
+    int off[5][3];
 #pragma unroll
     for (int dx = 2; dx >= -2; --dx) {
       const int n = nb + (2 - dy) * EA_PX + (2 - dx);
@@ -357,26 +360,27 @@ __host__ __device__ inline void ea_gather(int lx, int ly, const CL &cl,
       // all ones when the slot covers the column, else 0
       const unsigned vm =
           0u - ((info >> ((dx + 2) + 5 * (dy + 2))) & 1u);
-      const int own = ((int)((info >> 25) & 15u) + 3 * dy + dx) * (EA_NP + 1) + n;
+      // own pair minus the zero pair, in bytes
+      const int own = (int)((info >> 25) & 15u) * PB +
+                      16 * (n + (3 * dy + dx) * (EA_NP + 1)) - ZB;
       // byte c of sel is 0xff for the slot's class c (none when it does
       // not cover the column)
       const unsigned sel = (0xffu << (8u * (info >> 29))) & vm;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const unsigned m = (unsigned)((int)(sel << (24 - 8 * c)) >> 24);
-        idx[2 - dx][c] = EA_NP + (int)((unsigned)(own - EA_NP) & m);
+        off[2 - dx][c] = ZB + (int)((unsigned)own & m);
       }
     }
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
+    for (int i = 0; i < 5; ++i)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         double h0, h1;
-        hp(idx[i][c], h0, h1);
+        hp(off[i][c], h0, h1);
         b[c][0] += h0;
         b[c][1] += h1;
       }
-    }
   }
 #pragma unroll
   for (int sz = -1; sz <= 1; ++sz)

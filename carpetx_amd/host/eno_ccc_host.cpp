@@ -279,8 +279,13 @@ long long cxp_host_eno_ccc_adjoint_tiled(const unsigned char *codes,
               }
               return cs[n];
             },
-            [&](int i, double &h0, double &h1) {
+            [&](int b, double &h0, double &h1) {
               h0 = h1 = 0.0;
+              if (b % 16 != 0) {
+                ck.report("H pair offset not a pair", b, pz);
+                return;
+              }
+              const int i = b / 16;
               if (i < 0 || i >= 9 * (EA_NP + 1)) {
                 ck.report("H pair read outside the array", i, pz);
                 return;
