@@ -965,14 +965,51 @@ void schedule_tiles(cxp_worklist_s *wl, int grid) {
       }
       gt[it->second].push_back(i);
     }
+    // A group heavier than a quarter of one XCD's share would leave XCDs
+    // idle on lists of few FABs: it is split into its patches, and a patch
+    // still that heavy into runs of its tiles.
+    long long total = 0;
+    for (size_t i = 0; i < nt; ++i)
+      total += cost[i];
+    const int nxcd = grid >= 8 ? 8 : 1;
+    const long long cap = std::max(1LL, total / (4LL * nxcd));
     member.reserve(nt);
-    for (const auto &v : gt) {
-      Group g{member.size(), v.size(), 0};
+    auto emit = [&](const std::vector<size_t> &v) {
+      Group g{member.size(), 0, 0};
       for (size_t i : v) {
+        if (g.count > 0 && g.cost + cost[i] > cap) {
+          groups.push_back(g);
+          g = Group{member.size(), 0, 0};
+        }
         member.push_back(i);
+        ++g.count;
         g.cost += cost[i];
       }
-      groups.push_back(g);
+      if (g.count > 0)
+        groups.push_back(g);
+    };
+    for (const auto &v : gt) {
+      long long c = 0;
+      for (size_t i : v)
+        c += cost[i];
+      if (c <= cap) {
+        Group g{member.size(), v.size(), c};
+        for (size_t i : v)
+          member.push_back(i);
+        groups.push_back(g);
+        continue;
+      }
+      // by patch (task), in order
+      std::map<int, std::vector<size_t>> by_task;
+      std::vector<int> task_order;
+      for (size_t i : v) {
+        const int tk = wl->tiles[i].task;
+        if (!by_task.count(tk))
+          task_order.push_back(tk);
+        by_task[tk].push_back(i);
+      }
+      for (int tk : task_order)
+        emit(by_task[tk]);
     }
   }
   const int nx = grid >= 8 ? 8 : 1;  // XCD classes of workgroups
