@@ -70,6 +70,10 @@ def test_fixtures_separate_the_rules():
         names["eno_overshoot_falls_back_per_var_only"]["per_group"]
     assert names["eno_s0_read_outside_the_shifted_stencil"]["per_var"] != \
         names["eno_s0_read_outside_the_shifted_stencil"]["per_group"]
+    # a nonzero shift whose value overshoots: a per-group fallback that took
+    # ENO directions would change it (to the unshifted row or to CONS 1)
+    c = names["eno_shifted_overshoot_falls_back_per_var_only"]
+    assert c["per_var"] != c["per_group"] and c["per_group"] != "-1/8"
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
