@@ -233,10 +233,14 @@ bool cfgs_checked() {
 const bool kCfgsOk = cfgs_checked();
 
 // Size class of a patch: thin direction (extent <= 4) and a coarse x-size
-// bucket.  The tuner keys on (sids, class).
+// bucket, 0 .. 15; a bulky patch (every extent above 64) is class 16 of its
+// own, so tuning the boxes of a level (38^3 in a regrid, say) does not
+// retune the launches of a 192^3 box.  The tuner keys on (sids, class).
 int size_class(const int rlo[3], const int rhi[3]) {
   const int n[3] = {rhi[0] - rlo[0] + 1, rhi[1] - rlo[1] + 1,
                     rhi[2] - rlo[2] + 1};
+  if (n[0] > 64 && n[1] > 64 && n[2] > 64)
+    return 16;
   int thin = 0;
   if (n[0] <= 4)
     thin = 1;
