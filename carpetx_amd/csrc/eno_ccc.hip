@@ -38,9 +38,14 @@ struct EcArgs {
 template <bool CODES, bool PARENT = false>
 __global__ void __launch_bounds__(EC_T)
     k_eno_ccc(EcArgs a, unsigned char *__restrict__ codes) {
-  __shared__ double cw[EC_NC];
-  __shared__ double d2[EC_ND];
-  __shared__ double mall[EC_NM];
+  // With PARENT the window is dead once the |D2| arrays are built (no
+  // children to compute), so Mall takes its place: less LDS, more
+  // workgroups per compute unit.
+  __shared__ double lds[EC_NC + EC_ND + (PARENT ? 0 : EC_NM)];
+  static_assert(EC_NM <= EC_NC, "Mall fits in the window's place");
+  double *const cw = lds;
+  double *const d2 = lds + EC_NC;
+  double *const mall = PARENT ? lds : lds + EC_NC + EC_ND;
   const int tid = threadIdx.x;
   const int comp = blockIdx.y;
   int p0[3];
