@@ -93,19 +93,61 @@ def region_view(t, box, region):
     return t[(slice(None),) + sl]
 
 
+class _FillGhosts(torch.autograd.Function):
+    """The six slabs of a ghost shell in one node.  The forward writes each
+    slab's prolongation into a copy of ``fine`` (recording the branches of
+    operators that have them); the backward adds every slab's adjoint into
+    one coarse gradient, slab after slab on the stream, and passes the fine
+    gradient through where the fill did not write."""
+
+    @staticmethod
+    def forward(ctx, crse, fine, P, crse_box, fine_box, interior):
+        from .workloads import shell_slabs
+        crse = crse.contiguous()
+        out = torch.empty_like(fine, memory_format=torch.contiguous_format)
+        out.copy_(fine)
+        ncomp = crse.shape[0]
+        slabs = shell_slabs(fine_box, interior)
+        cf, of = _fab(crse, crse_box), _fab(out, fine_box)
+        branches = []
+        for slab in slabs:
+            nb = P.branch_bytes(slab, ncomp)
+            br = None
+            if nb > 0:
+                br = torch.empty(nb, dtype=torch.uint8, device=crse.device)
+                P.interp_record(cf, 0, of, 0, ncomp, slab, br)
+            else:
+                P.interp(cf, 0, of, 0, ncomp, slab)
+            branches.append(br)
+        ctx.P, ctx.crse_box, ctx.fine_box = P, crse_box, fine_box
+        ctx.slabs, ctx.branches, ctx.cshape = slabs, branches, crse.shape
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        gout = gout.contiguous()
+        ncomp = ctx.cshape[0]
+        gcrse = torch.zeros(ctx.cshape, dtype=torch.float64, device=gout.device)
+        gc, gf = _fab(gcrse, ctx.crse_box), _fab(gout, ctx.fine_box)
+        for slab, br in zip(ctx.slabs, ctx.branches):
+            ctx.P.interp_adjoint_recorded(br, gc, 0, gf, 0, ncomp, slab)
+        gfine = gout.clone()
+        for slab in ctx.slabs:
+            gfine[(slice(None),) + _slices(slab, ctx.fine_box)] = 0.0
+        return gcrse, gfine, None, None, None, None
+
+
+def _slices(region, box):
+    return tuple(slice(region.lo[d] - box.lo[d], region.hi[d] - box.lo[d] + 1)
+                 for d in (2, 1, 0))
+
+
 def fill_ghosts(P, crse, crse_box, fine, fine_box, interior):
     """Differentiable ghost-shell fill: returns ``fine`` with every point of
     ``fine_box`` outside ``interior`` replaced by the prolongation of
     ``crse``.  The shell is split into the six slabs of
     cxp_worklist_append_shell."""
-    from .workloads import shell_slabs
-    out = fine.clone()
-    for slab in shell_slabs(fine_box, interior):
-        vals = prolongate(P, crse, crse_box, slab)
-        sl = tuple(slice(slab.lo[d] - fine_box.lo[d], slab.hi[d] - fine_box.lo[d] + 1)
-                   for d in (2, 1, 0))
-        out[(slice(None),) + sl] = vals
-    return out
+    return _FillGhosts.apply(crse, fine, P, crse_box, fine_box, interior)
 
 
 __all__ = ["prolongate", "restrict", "fill_ghosts", "region_view", "Box"]

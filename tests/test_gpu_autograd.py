@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import torch
 
-from carpetx_amd.autograd import prolongate, restrict
+from carpetx_amd.autograd import fill_ghosts, prolongate, restrict
 from carpetx_amd.interp import Box, get_interpolator
 
 pytestmark = pytest.mark.gpu
@@ -41,6 +41,47 @@ def test_prolongate_gradient_fd(fam, order, it):
         fd = float(_fd_directional(L, u, v))
         ad = float((x.grad * v).sum())
         assert abs(fd - ad) <= 1e-7 * max(1.0, abs(ad)), (fd, ad)
+
+
+@pytest.mark.parametrize("fam,order,it", [("natural", 5, (0, 0, 0)), ("eno", 3, (1, 1, 1))])
+def test_fill_ghosts_equals_its_slabs(fam, order, it):
+    """The ghost-shell fill as one autograd node: values bit for bit those
+    of the six slabs' prolongations copied in; the coarse gradient the sum
+    of the slabs' gradients (one buffer, slab after slab, so equal to
+    rounding); the fine gradient passes where the fill did not write and is
+    zero on the shell."""
+    from carpetx_amd.workloads import shell_slabs
+    torch.manual_seed(6)
+    P = get_interpolator(fam, order, it)
+    interior = Box((4, 2, 6), (27, 21, 19))
+    fbox = interior.grow(3)
+    cb = P.CoarseBox(fbox)
+    u = torch.rand((2,) + tuple(reversed(cb.size)), dtype=torch.float64, device="cuda") * 2 - 1
+    f = torch.rand((2,) + tuple(reversed(fbox.size)), dtype=torch.float64, device="cuda") * 2 - 1
+    g = torch.rand_like(f) * 2 - 1
+    x = u.clone().requires_grad_(True)
+    y = f.clone().requires_grad_(True)
+    out = fill_ghosts(P, x, cb, y, fbox, interior)
+    out.backward(g)
+
+    def sl(b):
+        return (slice(None),) + tuple(slice(b.lo[d] - fbox.lo[d], b.hi[d] - fbox.lo[d] + 1)
+                                      for d in (2, 1, 0))
+    x2 = u.clone().requires_grad_(True)
+    want = f.clone()
+    parts = []
+    for slab in shell_slabs(fbox, interior):
+        v = prolongate(P, x2, cb, slab)
+        want[sl(slab)] = v.detach()
+        parts.append((v, g[sl(slab)]))
+    assert torch.equal(out.detach(), want)
+    torch.autograd.backward([p for p, _ in parts], [q for _, q in parts])
+    err = float((x.grad - x2.grad).abs().max())
+    assert err <= 1e-14 * max(1.0, float(x2.grad.abs().max())), err
+    gf = g.clone()
+    for slab in shell_slabs(fbox, interior):
+        gf[sl(slab)] = 0.0
+    assert torch.equal(y.grad, gf)
 
 
 def test_restrict_gradient_fd():
