@@ -656,12 +656,16 @@ inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
     const dim3 grid(ntx * nty * nkc, ncomp);
     bool done = false;
     if (op.sid[0] == op.sid[1] && op.sid[1] == op.sid[2]) {
-      // one stencil in every direction: the z-first kernel, with z-chunks
-      // of at least 8 planes (a chunk re-reads span_z parent planes; at
-      // 96^3 -> 192^3 VVV order 5, chunks of 8 took 30 us, of 13 32 us, of
-      // 26 43 us: parallelism outweighs the re-reads)
+      // one stencil in every direction: the z-first kernel.  z-chunks of
+      // at least 8 planes when that still gives 512 workgroups (a chunk
+      // re-reads span_z parent planes; at 96^3 -> 192^3 VVV order 5, chunks
+      // of 8 took 30 us, of 13 32 us, of 26 43 us, of 4 37 us); thinner
+      // outputs (the slabs of a ghost shell, a few tiles across) take
+      // chunks down to 2 planes, which took the two-level wave's backward
+      // from 3.64 to 3.30 ms per coarse step
       int kz = (int)((nk * cols + 1023) / 1024);
-      kz = kz < 8 ? 8 : (kz > nk ? nk : kz);
+      const int kmin = cols * ((nk + 7) / 8) >= 512 ? 8 : 2;
+      kz = kz < kmin ? kmin : (kz > nk ? nk : kz);
       const dim3 gz(ntx * nty * ((nk + kz - 1) / kz), ncomp);
       done = true;
       switch (op.sid[0]) {

@@ -76,6 +76,8 @@ CASES = [("interpolate", 1, (0, 0, 0)), ("interpolate", 3, (0, 0, 0)),
 @pytest.mark.parametrize("typ,order,it", CASES)
 @pytest.mark.parametrize("rlo,rhi,kchunk", [((-3, 1, -2), (20, 17, 25), 8),
                                              ((5, -4, 3), (40, 9, 6), 3),
+                                             ((1, 2, -5), (30, 27, -3), 2),
+                                             ((-4, 0, 0), (-2, 21, 17), 2),
                                              ((0, 0, 0), (33, 35, 31), 100)])
 def test_host_zfirst_adjoint_equals_the_transpose(typ, order, it, rlo, rhi, kchunk):
     op = oracle.lookup(typ, order, it)
