@@ -217,9 +217,14 @@ class TwoLevelWave:
         inner = s[:, 2:-2, 2:-2, 2:-2]
         return torch.stack([inner[1], lap])
 
-    def fill(self, sf, crse_state):
-        sub = region_view(crse_state, Box((0, 0, 0), (self.n - 1,) * 3), self.crse_sub)
-        return fill_ghosts(self.P, sub, self.crse_sub, sf, self.fine_box, self.fine_int)
+    def fill(self, sf, crse_sub_state):
+        """Ghost fill of the fine state from the coarse state on crse_sub
+        (the coarse box the fill reads)."""
+        return fill_ghosts(self.P, crse_sub_state, self.crse_sub, sf, self.fine_box,
+                           self.fine_int)
+
+    def coarse_sub(self, crse_state):
+        return region_view(crse_state, Box((0, 0, 0), (self.n - 1,) * 3), self.crse_sub)
 
     # --- time stepping ------------------------------------------------------
     def rk4_coarse(self, s):
@@ -240,7 +245,8 @@ class TwoLevelWave:
     def rk4_fine(self, sf, c0, c1, theta0, dtf):
         """One fine RK4 step from time theta0 (in units of the coarse step);
         ghost values at each stage from the coarse state interpolated
-        linearly between c0 (t_n) and c1 (t_{n+1})."""
+        linearly between c0 (t_n) and c1 (t_{n+1}), both on crse_sub (the
+        interpolation touches only the box the fills read)."""
         def at(th):
             return (1.0 - th) * c0 + th * c1
 
@@ -271,8 +277,9 @@ class TwoLevelWave:
     def step(self, sc, sf):
         c1 = self.rk4_coarse(sc)
         dtf = 0.5 * self.dt
-        sf = self.rk4_fine(sf, sc, c1, 0.0, dtf)
-        sf = self.rk4_fine(sf, sc, c1, 0.5, dtf)
+        c0s, c1s = self.coarse_sub(sc), self.coarse_sub(c1)
+        sf = self.rk4_fine(sf, c0s, c1s, 0.0, dtf)
+        sf = self.rk4_fine(sf, c0s, c1s, 0.5, dtf)
         # restriction: coarse points under the fine level from fine injection
         inj = restrict(sf, self.fine_box, self.crse_under, (0, 0, 0))
         cu = self.crse_under
