@@ -20,6 +20,11 @@
 
 namespace {
 
+// Periodic index of i + o, |o| <= 2 < n (no integer division).
+__device__ __forceinline__ int wrap(int i, int n) {
+  return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
 __global__ void __launch_bounds__(256)
     k_wave_stage(int stage, const double *__restrict__ y, double *dst,
                  const double *base, double *acc, int nx, int ny, int nz,
@@ -34,9 +39,12 @@ __global__ void __launch_bounds__(256)
   auto at = [&](int dx, int dy, int dz) {
     int x = ix + dx, yy = iy + dy, z = iz + dz;
     if (periodic) {
-      x = (x + nx) % nx;
-      yy = (yy + ny) % ny;
-      z = (z + nz) % nz;
+      if (dx)
+        x = wrap(x, nx);
+      if (dy)
+        yy = wrap(yy, ny);
+      if (dz)
+        z = wrap(z, nz);
     }
     return y[x + yy * sy + z * sz];
   };
@@ -95,7 +103,9 @@ __global__ void __launch_bounds__(256)
     su += -90.0 * kbar(1, qx, qy, qz);
   const int off[4] = {-2, -1, 1, 2};
   const double wt[4] = {-1.0, 16.0, 16.0, -1.0};
+#pragma unroll
   for (int dir = 0; dir < 3; ++dir)
+#pragma unroll
     for (int m = 0; m < 4; ++m) {
       int x = qx, yy = qy, z = qz;
       if (dir == 0)
@@ -105,9 +115,12 @@ __global__ void __launch_bounds__(256)
       else
         z += off[m];
       if (periodic) {
-        x = (x + nx) % nx;
-        yy = (yy + ny) % ny;
-        z = (z + nz) % nz;
+        if (dir == 0)
+          x = wrap(x, nx);
+        else if (dir == 1)
+          yy = wrap(yy, ny);
+        else
+          z = wrap(z, nz);
       }
       if (inside(x, yy, z))
         su += wt[m] * kbar(1, x, yy, z);
