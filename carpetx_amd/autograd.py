@@ -109,16 +109,16 @@ class _FillGhosts(torch.autograd.Function):
         ncomp = crse.shape[0]
         slabs = shell_slabs(fine_box, interior)
         cf, of = _fab(crse, crse_box), _fab(out, fine_box)
-        branches = []
-        for slab in slabs:
+        branches = [None] * len(slabs)
+        for i, slab in enumerate(slabs):
             nb = P.branch_bytes(slab, ncomp)
-            br = None
             if nb > 0:
-                br = torch.empty(nb, dtype=torch.uint8, device=crse.device)
-                P.interp_record(cf, 0, of, 0, ncomp, slab, br)
-            else:
-                P.interp(cf, 0, of, 0, ncomp, slab)
-            branches.append(br)
+                branches[i] = torch.empty(nb, dtype=torch.uint8, device=crse.device)
+                P.interp_record(cf, 0, of, 0, ncomp, slab, branches[i])
+        # operators without branches: every slab in one call
+        linear = [s for s, br in zip(slabs, branches) if br is None]
+        if linear:
+            P.interp_regions(cf, 0, of, 0, ncomp, linear)
         ctx.P, ctx.crse_box, ctx.fine_box = P, crse_box, fine_box
         ctx.slabs, ctx.branches, ctx.cshape = slabs, branches, crse.shape
         return out

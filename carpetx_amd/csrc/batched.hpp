@@ -108,6 +108,46 @@ __global__ void __launch_bounds__(CXP_NT, 4)
   }
 }
 
+// Several regions of one (coarse, fine) FAB pair in one launch, without a
+// work list: up to CXP_MAX_REGIONS regions, region i owning blocks
+// [start[i], start[i + 1]) of grid.x (thin tiles, or separable tiles of
+// width 2^lx[i] and one row pair per thread with z-chunks of zchunk[i]
+// planes); grid.y = component.  The regions must not overlap: each tile
+// writes its own points.
+#define CXP_MAX_REGIONS 8
+struct RegionLaunch {
+  SepTask t[CXP_MAX_REGIONS];
+  int start[CXP_MAX_REGIONS + 1];
+  int ntx[CXP_MAX_REGIONS], nty[CXP_MAX_REGIONS];
+  int lx[CXP_MAX_REGIONS];  // -1: thin tiles (16 x 16, the whole z range)
+  int zchunk[CXP_MAX_REGIONS];
+  int n;
+};
+
+__global__ void __launch_bounds__(CXP_NT) k_sep_regions(RegionLaunch r) {
+  extern __shared__ double reg_smem[];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  int i = 0;
+  while (i + 1 < r.n && b >= r.start[i + 1])
+    ++i;
+  const SepTask &t = r.t[i];
+  const int local = b - r.start[i];
+  const int ntx = r.ntx[i];
+  if (r.lx[i] < 0) {
+    const int x0 = (t.rlo[0] & ~1) + 16 * (local % ntx);
+    const int y0 = (t.rlo[1] & ~1) + 16 * (local / ntx);
+    thin_tile(t, blockIdx.y, x0, y0, reg_smem);
+    return;
+  }
+  const Geom g = make_geom(r.lx[i], 1);
+  const int nty = r.nty[i], zc = r.zchunk[i];
+  const int x0 = (t.rlo[0] & ~1) + g.tx * (local % ntx);
+  const int y0 = (t.rlo[1] & ~1) + g.ty * ((local / ntx) % nty);
+  const int z0 = t.rlo[2] + zc * (local / (ntx * nty));
+  const int z1 = min(t.rhi[2], z0 + zc - 1);
+  sep_tile<-1, 1>(g, t, blockIdx.y, x0, y0, z0, z1, reg_smem);
+}
+
 // Calls f(kernel) with the k_fill_batched instance for uniform stencil su
 // (or the dispatching one for su < 0).
 template <class F> inline auto with_fill_kernel(int su, F &&f) {

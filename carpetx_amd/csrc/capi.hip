@@ -1588,6 +1588,97 @@ int cxp_prolongate(cxp_op op, const cxp_fab *crse, int crse_comp,
                          (hipStream_t)stream);
 }
 
+int cxp_prolongate_regions(cxp_op op, const cxp_fab *crse, int crse_comp,
+                           const cxp_fab *fine, int fine_comp, int ncomp,
+                           int nregions, const int *region_lo,
+                           const int *region_hi, int per_group, void *stream) {
+  if (!kCfgsOk)
+    return CXP_ERR_STATE;
+  if (nregions < 0 || (nregions > 0 && (!region_lo || !region_hi)))
+    return CXP_ERR_ARG;
+  for (int i = 0; i < nregions; ++i)
+    if (const int e = check_prolong(op, crse, crse_comp, fine, fine_comp, ncomp,
+                                    region_lo + 3 * i, region_hi + 3 * i,
+                                    per_group))
+      return e;
+  // no two regions may share a point (each tile writes its own)
+  for (int i = 0; i < nregions; ++i)
+    for (int j = i + 1; j < nregions; ++j) {
+      bool meet = true;
+      for (int d = 0; d < 3; ++d)
+        meet = meet && region_lo[3 * i + d] <= region_hi[3 * j + d] &&
+               region_lo[3 * j + d] <= region_hi[3 * i + d];
+      if (meet)
+        return CXP_ERR_REGION;
+    }
+  hipStream_t s = (hipStream_t)stream;
+  if (nregions == 0)
+    return CXP_OK;
+  // one launch for the separable operators; the others region by region
+  if (!op_is_separable(op->d, per_group != 0) || nregions > CXP_MAX_REGIONS) {
+    for (int i = 0; i < nregions; ++i)
+      if (const int e = prolongate_impl(op, crse, crse_comp, fine, fine_comp,
+                                        ncomp, region_lo + 3 * i,
+                                        region_hi + 3 * i, per_group, s))
+        return e;
+    return CXP_OK;
+  }
+  RegionLaunch r{};
+  r.n = nregions;
+  int nz[CXP_MAX_REGIONS];
+  size_t lds = 8 * (size_t)kLdsDoubles;
+  for (int i = 0; i < nregions; ++i) {
+    const SepTask t = make_sep_task(op->d, crse, crse_comp, fine, fine_comp,
+                                    region_lo + 3 * i, region_hi + 3 * i);
+    r.t[i] = t;
+    nz[i] = t.rhi[2] - t.rlo[2] + 1;
+    if (thin_eligible(t)) {
+      r.lx[i] = -1;
+      r.ntx[i] = (t.rhi[0] - (t.rlo[0] & ~1) + 16) / 16;
+      r.nty[i] = (t.rhi[1] - (t.rlo[1] & ~1) + 16) / 16;
+      r.zchunk[i] = nz[i];
+      lds = std::max(lds, 8 * (size_t)thin_lds_doubles(t.whi[1] - t.wlo[1],
+                                                       t.whi[2] - t.wlo[2]));
+      continue;
+    }
+    int cfg, zc;
+    choose_cfg(t, cfg, zc);
+    // one row pair per thread: the shape of the same width
+    if (kCfgs[cfg].rp != 1)
+      for (int c = 0; c < kNumTileConfigs; ++c)
+        if (kCfgs[c].rp == 1 && kCfgs[c].tx == kCfgs[cfg].tx)
+          cfg = c;
+    r.lx[i] = kCfgs[cfg].lx;
+    r.ntx[i] = (t.rhi[0] - (t.rlo[0] & ~1) + kCfgs[cfg].tx) / kCfgs[cfg].tx;
+    r.nty[i] = (t.rhi[1] - (t.rlo[1] & ~1) + kCfgs[cfg].ty) / kCfgs[cfg].ty;
+    r.zchunk[i] = zc > 0 ? std::min(nz[i], zc) : nz[i];
+  }
+  // as for one box: shorten the untuned z-chunks until the launch has
+  // enough workgroups
+  auto blocks = [&](int i) {
+    return r.ntx[i] * r.nty[i] * ((nz[i] + r.zchunk[i] - 1) / r.zchunk[i]);
+  };
+  for (;;) {
+    long long total = 0;
+    int widest = -1;
+    for (int i = 0; i < nregions; ++i) {
+      total += blocks(i);
+      if (r.lx[i] >= 0 && r.zchunk[i] > 8 &&
+          (widest < 0 || r.zchunk[i] > r.zchunk[widest]))
+        widest = i;
+    }
+    if (total * ncomp >= 2048 || widest < 0)
+      break;
+    r.zchunk[widest] = (r.zchunk[widest] + 1) / 2;
+  }
+  r.start[0] = 0;
+  for (int i = 0; i < nregions; ++i)
+    r.start[i + 1] = r.start[i] + blocks(i);
+  hipLaunchKernelGGL(k_sep_regions, dim3((unsigned)r.start[nregions], ncomp),
+                     dim3(CXP_NT), lds, s, r);
+  return hip_status(hipGetLastError());
+}
+
 int cxp_restrict(const int indextype[3], const cxp_fab *fine, int fine_comp,
                  const cxp_fab *crse, int crse_comp, int ncomp,
                  const int rlo[3], const int rhi[3], void *stream) {

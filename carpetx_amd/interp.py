@@ -142,6 +142,19 @@ class Prolongate3DRF2:
             ncomp, i3(fine_region.lo), i3(fine_region.hi), int(bool(per_group)),
             _stream(stream)))
 
+    def interp_regions(self, crse, crse_comp, fine, fine_comp, ncomp, regions,
+                       per_group=False, stream=None):
+        """interp over several regions of one FAB pair (the six slabs of a
+        ghost shell, say), in one launch for separable operators
+        (cxp_prolongate_regions; no reference counterpart)."""
+        c, f = crse.c(), fine.c()
+        n = len(regions)
+        lo = (ctypes.c_int * (3 * max(n, 1)))(*[v for r in regions for v in r.lo])
+        hi = (ctypes.c_int * (3 * max(n, 1)))(*[v for r in regions for v in r.hi])
+        check("cxp_prolongate_regions", _lib.lib().cxp_prolongate_regions(
+            self.handle, ctypes.byref(c), crse_comp, ctypes.byref(f), fine_comp,
+            ncomp, n, lo, hi, int(bool(per_group)), _stream(stream)))
+
     def interp_face(self, crse, crse_comp, fine, fine_comp, ncomp, fine_region,
                     ratio=(2, 2, 2), solve_mask=None, crse_geom=None,
                     fine_geom=None, bcr=None, bccomp=0, gpu_or_cpu=None,

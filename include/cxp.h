@@ -110,6 +110,19 @@ int cxp_prolongate(cxp_op op, const cxp_fab *crse, int crse_comp,
                    const int region_lo[3], const int region_hi[3],
                    int per_group, void *stream);
 
+/* cxp_prolongate over several regions of one (crse, fine) FAB pair, such as
+ * the six slabs of a box's ghost shell: region i is [region_lo[3 i ..
+ * 3 i + 2], region_hi[3 i .. 3 i + 2]].  No reference counterpart (there,
+ * interp is called once per region); the results are cxp_prolongate's,
+ * region by region, bit for bit.  For a separable operator and at most 8
+ * regions it is one launch.  Errors: those of cxp_prolongate for any
+ * region, CXP_ERR_ARG for a negative count or null arrays, CXP_ERR_REGION
+ * when two regions share a point. */
+int cxp_prolongate_regions(cxp_op op, const cxp_fab *crse, int crse_comp,
+                           const cxp_fab *fine, int fine_comp, int ncomp,
+                           int nregions, const int *region_lo,
+                           const int *region_hi, int per_group, void *stream);
+
 /* Restrict() -> AMReX average_down_nodal/_edges/_faces/average_down by the
  * number of cell-centred directions (CarpetX/src/schedule.cxx:2791-2816):
  * injection in VC directions, mean of the two children in CC directions,

@@ -92,3 +92,34 @@ def test_every_shape_in_a_list(typ, order, it):
             if mixed:
                 assert np.array_equal(f2.t.cpu().numpy(), expect2), cfg
             wl.close()
+
+
+@pytest.mark.parametrize("typ,order,it", OPS + [("eno", 3, (1, 1, 1))])
+def test_regions_in_one_call_equal_region_by_region(typ, order, it):
+    """cxp_prolongate_regions (a ghost shell's six slabs; then one whole box,
+    whose z-chunks the launch shortens) gives cxp_prolongate's bits region by
+    region, also with a two-row-pair shape in the tile table (the one-launch
+    kernel takes the one-pair shape of the same width) and for an operator
+    that is not separable (region by region inside)."""
+    from carpetx_amd.workloads import shell_slabs
+    L = _lib.lib()
+    P = get_interpolator(typ, order, it)
+    interior = Box((4, 2, 6), (43, 37, 29))
+    fbox = interior.grow(3)
+    cb = P.CoarseBox(fbox)
+    rng = np.random.default_rng(13)
+    crse = FArrayBox(cb, 2, tensor=torch.from_numpy(
+        rng.uniform(-1, 1, (2,) + tuple(reversed(cb.size)))).cuda())
+    for regions in (shell_slabs(fbox, interior), [interior]):
+        for tuned in (False, True):
+            L.cxp_tune_clear()
+            if tuned:
+                for r in regions:
+                    assert L.cxp_tune_set(P.handle, _i3(r.lo), _i3(r.hi), 5, 0) == 0
+            want = FArrayBox(fbox, 2)
+            for r in regions:
+                P.interp(crse, 0, want, 0, 2, r)
+            got = FArrayBox(fbox, 2)
+            P.interp_regions(crse, 0, got, 0, 2, regions)
+            torch.cuda.synchronize()
+            assert torch.equal(got.t, want.t), (typ, len(regions), tuned)

@@ -120,6 +120,41 @@ def test_validation_errors_without_device_work():
     assert L.cxp_status_string(-5) == b"coarse FAB does not cover CoarseBox(region)"
 
 
+def test_prolongate_regions_checks_before_device_work():
+    """cxp_prolongate_regions: every region meets cxp_prolongate's checks,
+    regions must not share a point, and a negative count or null arrays
+    are argument errors; all caught before any device call (zero regions do
+    nothing)."""
+    L = _lib.lib()
+    h = _lookup("natural", 3, (0, 0, 0))[1]
+    fake = ctypes.c_void_p(0x1000)
+
+    def fab(lo, hi):
+        f = _lib.CxpFab()
+        f.data = fake
+        f.lo, f.hi = i3(lo), i3(hi)
+        f.comp_stride = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+        f.ncomp = 1
+        return f
+    crse, fine = fab((-4, -4, -4), (20, 20, 20)), fab((0, 0, 0), (15, 15, 15))
+
+    def call(regions, n=None):
+        n = len(regions) if n is None else n
+        lo = (ctypes.c_int * (3 * max(1, len(regions))))(*[v for r in regions for v in r[0]])
+        hi = (ctypes.c_int * (3 * max(1, len(regions))))(*[v for r in regions for v in r[1]])
+        return L.cxp_prolongate_regions(h, ctypes.byref(crse), 0, ctypes.byref(fine), 0, 1,
+                                        n, lo, hi, 0, None)
+    a = ((0, 0, 0), (15, 15, 2))
+    b = ((0, 0, 13), (15, 15, 15))
+    assert call([]) == 0
+    assert call([a], n=-1) == -3
+    assert L.cxp_prolongate_regions(h, ctypes.byref(crse), 0, ctypes.byref(fine), 0, 1,
+                                    1, None, None, 0, None) == -3
+    assert call([a, ((0, 0, 0), (16, 15, 15))]) == -4     # second region outside fine
+    assert call([a, ((0, 0, 2), (3, 3, 5))]) == -4        # regions share a point
+    assert call([a, b, ((3, 0, 0), (2, 15, 15))]) == -3   # an empty region
+
+
 def test_adjoint_argument_checks_before_device_work():
     """ENO, fallback and MINMOD operators linearise at crse_in, so a call
     without it is rejected (CXP_ERR_ARG) before any device work."""
