@@ -77,6 +77,9 @@ _SIGS = {
     "cxp_prolongate_regions": (c_int, [c_void_p, ctypes.POINTER(CxpFab), c_int,
                                        ctypes.POINTER(CxpFab), c_int, c_int, c_int,
                                        P_INT, P_INT, c_int, c_void_p]),
+    "cxp_prolongate_adjoint_regions": (c_int, [c_void_p, ctypes.POINTER(CxpFab), c_int,
+                                               ctypes.POINTER(CxpFab), c_int, c_int,
+                                               c_int, P_INT, P_INT, c_int, c_void_p]),
     "cxp_restrict": (c_int, [P_INT, ctypes.POINTER(CxpFab), c_int,
                              ctypes.POINTER(CxpFab), c_int, c_int, P_INT, P_INT,
                              c_void_p]),

@@ -129,8 +129,12 @@ class _FillGhosts(torch.autograd.Function):
         ncomp = ctx.cshape[0]
         gcrse = torch.zeros(ctx.cshape, dtype=torch.float64, device=gout.device)
         gc, gf = _fab(gcrse, ctx.crse_box), _fab(gout, ctx.fine_box)
-        for slab, br in zip(ctx.slabs, ctx.branches):
-            ctx.P.interp_adjoint_recorded(br, gc, 0, gf, 0, ncomp, slab)
+        if all(br is None for br in ctx.branches):
+            # slabs whose coarse windows are apart share a launch
+            ctx.P.interp_adjoint_regions(gc, 0, gf, 0, ncomp, ctx.slabs)
+        else:
+            for slab, br in zip(ctx.slabs, ctx.branches):
+                ctx.P.interp_adjoint_recorded(br, gc, 0, gf, 0, ncomp, slab)
         gfine = gout.clone()
         for slab in ctx.slabs:
             gfine[(slice(None),) + _slices(slab, ctx.fine_box)] = 0.0

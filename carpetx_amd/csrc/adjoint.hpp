@@ -282,17 +282,17 @@ __global__ void __launch_bounds__(256)
 // parent plane ahead (points outside the fine region read as 0); the old
 // crse_bar value of a complete plane is loaded before that plane's LDS
 // passes.
+// Tile b of task t (ntx x nty column tiles, z-chunks of kchunk planes);
+// Zs, Ys: the workgroup's LDS buffers (AzGeom<S>::NZ and ::NY doubles).
 template <int S>
-__global__ void __launch_bounds__(AZ_NT)
-    k_adj_zf(AdjTask t, int ntx, int nty, int kchunk) {
+__device__ __forceinline__ void adj_zf_tile(const AdjTask &t, int b, int ntx,
+                                            int nty, int kchunk, double *Zs,
+                                            double *Ys) {
   using G = AzGeom<S>;
-  __shared__ double Zs[G::NZ];
-  __shared__ double Ys[G::NY];
   const int tid = threadIdx.x;
   const long long comp = blockIdx.y;
   int i0, j0, kca;
-  az_tile(t.olo, xcd_block(blockIdx.x, gridDim.x), ntx, nty, kchunk, i0, j0,
-          kca);
+  az_tile(t.olo, b, ntx, nty, kchunk, i0, j0, kca);
   const int kcb = min(t.ohi[2], kca + kchunk - 1);
   const __amdgpu_buffer_rsrc_t rf =
       make_rsrc(t.fbar.p + comp * t.fcs, t.fbytes);
@@ -377,6 +377,39 @@ __global__ void __launch_bounds__(AZ_NT)
       z[r][G::span] = 0.0;
     }
   }
+}
+
+template <int S>
+__global__ void __launch_bounds__(AZ_NT)
+    k_adj_zf(AdjTask t, int ntx, int nty, int kchunk) {
+  __shared__ double Zs[AzGeom<S>::NZ];
+  __shared__ double Ys[AzGeom<S>::NY];
+  adj_zf_tile<S>(t, xcd_block(blockIdx.x, gridDim.x), ntx, nty, kchunk, Zs,
+                 Ys);
+}
+
+// Several regions' adjoints in one launch, region i owning blocks
+// [start[i], start[i + 1]).  Their coarse output windows must be disjoint:
+// every output point then takes the additions of one region only.
+#define CXP_ADJ_MAX_REGIONS 8
+struct AdjRegions {
+  AdjTask t[CXP_ADJ_MAX_REGIONS];
+  int start[CXP_ADJ_MAX_REGIONS + 1];
+  int ntx[CXP_ADJ_MAX_REGIONS], nty[CXP_ADJ_MAX_REGIONS];
+  int kchunk[CXP_ADJ_MAX_REGIONS];
+  int n;
+};
+
+template <int S>
+__global__ void __launch_bounds__(AZ_NT) k_adj_zf_regions(AdjRegions r) {
+  __shared__ double Zs[AzGeom<S>::NZ];
+  __shared__ double Ys[AzGeom<S>::NY];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  int i = 0;
+  while (i + 1 < r.n && b >= r.start[i + 1])
+    ++i;
+  adj_zf_tile<S>(r.t[i], b - r.start[i], r.ntx[i], r.nty[i], r.kchunk[i], Zs,
+                 Ys);
 }
 
 // --------------------------------------------------------------------------
@@ -608,6 +641,50 @@ inline long long adj_component_bytes(const cxp_fab *f) {
          (f->hi[2] - f->lo[2] + 1);
 }
 
+// The linear adjoint's task for region [rlo, rhi]: fine_bar in, crse_bar
+// accumulated over the coarse output window.
+inline AdjTask adj_task(const OpDesc &op, const cxp_fab *crse_bar,
+                        int crse_comp, const cxp_fab *fine_bar, int fine_comp,
+                        const int rlo[3], const int rhi[3]) {
+  AdjTask t;
+  t.fbar = adj_dfab(fine_bar, fine_comp);
+  t.cbar = adj_dfab(crse_bar, crse_comp);
+  for (int d = 0; d < 3; ++d) {
+    t.rlo[d] = rlo[d];
+    t.rhi[d] = rhi[d];
+    t.sid[d] = op.sid[d];
+    t.wlo[d] = sid_wlo(op.sid[d]);
+    t.whi[d] = sid_whi(op.sid[d]);
+    t.olo[d] = (rlo[d] >> 1) + sid_wlo(op.sid[d]);
+    t.ohi[d] = (rhi[d] >> 1) + sid_whi(op.sid[d]);
+  }
+  t.fcs = fine_bar->comp_stride;
+  t.ccs = crse_bar->comp_stride;
+  // component volumes; the caller has checked both stay below 2 GiB
+  t.fbytes = (int)adj_component_bytes(fine_bar);
+  t.cbytes = (int)adj_component_bytes(crse_bar);
+  return t;
+}
+
+// The z-first kernel's grid for task t: column tiles and z-chunks.  Chunks
+// of at least 8 planes when that still gives 512 workgroups (a chunk
+// re-reads span_z parent planes; at 96^3 -> 192^3 VVV order 5, chunks of 8
+// took 30 us, of 13 32 us, of 26 43 us, of 4 37 us); thinner outputs (the
+// slabs of a ghost shell, a few tiles across) take chunks down to 2
+// planes, which took the two-level wave's backward from 3.64 to 3.3 ms per
+// coarse step.
+inline void adj_zf_grid(const AdjTask &t, int ncomp, int &ntx, int &nty,
+                        int &kz, int &nkc) {
+  ntx = (t.ohi[0] - t.olo[0] + AZ_T) / AZ_T;
+  nty = (t.ohi[1] - t.olo[1] + AZ_T) / AZ_T;
+  const int nk = t.ohi[2] - t.olo[2] + 1;
+  const long long cols = (long long)ntx * nty * ncomp;
+  kz = (int)((nk * cols + 1023) / 1024);
+  const int kmin = cols * ((nk + 7) / 8) >= 512 ? 8 : 2;
+  kz = kz < kmin ? kmin : (kz > nk ? nk : kz);
+  nkc = (nk + kz - 1) / kz;
+}
+
 inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
                           const cxp_fab *crse_bar, int crse_comp,
                           const cxp_fab *fine_bar, int fine_comp, int ncomp,
@@ -626,23 +703,8 @@ inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
     ohi[d] = (rhi[d] >> 1) + sid_whi(op.sid[d]);
   }
   if (!minmod) {
-    AdjTask t;
-    t.fbar = adj_dfab(fine_bar, fine_comp);
-    t.cbar = adj_dfab(crse_bar, crse_comp);
-    for (int d = 0; d < 3; ++d) {
-      t.rlo[d] = rlo[d];
-      t.rhi[d] = rhi[d];
-      t.sid[d] = op.sid[d];
-      t.wlo[d] = sid_wlo(op.sid[d]);
-      t.whi[d] = sid_whi(op.sid[d]);
-      t.olo[d] = olo[d];
-      t.ohi[d] = ohi[d];
-    }
-    t.fcs = fine_bar->comp_stride;
-    t.ccs = crse_bar->comp_stride;
-    // component volumes; the caller has checked both stay below 2 GiB
-    t.fbytes = (int)adj_component_bytes(fine_bar);
-    t.cbytes = (int)adj_component_bytes(crse_bar);
+    const AdjTask t =
+        adj_task(op, crse_bar, crse_comp, fine_bar, fine_comp, rlo, rhi);
     const int ntx = (ohi[0] - olo[0] + CXP_ADJ_T) / CXP_ADJ_T;
     const int nty = (ohi[1] - olo[1] + CXP_ADJ_T) / CXP_ADJ_T;
     const int nk = ohi[2] - olo[2] + 1;
@@ -656,22 +718,15 @@ inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
     const dim3 grid(ntx * nty * nkc, ncomp);
     bool done = false;
     if (op.sid[0] == op.sid[1] && op.sid[1] == op.sid[2]) {
-      // one stencil in every direction: the z-first kernel.  z-chunks of
-      // at least 8 planes when that still gives 512 workgroups (a chunk
-      // re-reads span_z parent planes; at 96^3 -> 192^3 VVV order 5, chunks
-      // of 8 took 30 us, of 13 32 us, of 26 43 us, of 4 37 us); thinner
-      // outputs (the slabs of a ghost shell, a few tiles across) take
-      // chunks down to 2 planes, which took the two-level wave's backward
-      // from 3.64 to 3.30 ms per coarse step
-      int kz = (int)((nk * cols + 1023) / 1024);
-      const int kmin = cols * ((nk + 7) / 8) >= 512 ? 8 : 2;
-      kz = kz < kmin ? kmin : (kz > nk ? nk : kz);
-      const dim3 gz(ntx * nty * ((nk + kz - 1) / kz), ncomp);
+      // one stencil in every direction: the z-first kernel
+      int ztx, zty, kz, zkc;
+      adj_zf_grid(t, ncomp, ztx, zty, kz, zkc);
+      const dim3 gz(ztx * zty * zkc, ncomp);
       done = true;
       switch (op.sid[0]) {
 #define CXP_AL(S)                                                              \
   case S:                                                                      \
-    hipLaunchKernelGGL(k_adj_zf<S>, gz, dim3(AZ_NT), 0, s, t, ntx, nty, kz);   \
+    hipLaunchKernelGGL(k_adj_zf<S>, gz, dim3(AZ_NT), 0, s, t, ztx, zty, kz);   \
     break;
         CXP_ADJ_LINEAR_SIDS(CXP_AL)
 #undef CXP_AL
@@ -688,6 +743,74 @@ inline int adjoint_launch(const OpDesc &op, const cxp_fab *crse_in,
   // needs the forward input; the C ABI routes them there.
   (void)crse_in;
   return CXP_ERR_UNSUPPORTED;
+}
+
+// Linear adjoints of several regions (lo[3 i ..], hi[3 i ..]) into one
+// crse_bar.  Regions run in order; a run of consecutive regions whose
+// coarse output windows are pairwise disjoint shares one launch when the
+// operator has one stencil in every direction (every output point then
+// takes the same additions in the same order as region by region).
+inline int adjoint_regions_launch(const OpDesc &op, const cxp_fab *crse_bar,
+                                  int crse_comp, const cxp_fab *fine_bar,
+                                  int fine_comp, int ncomp, int nregions,
+                                  const int *lo, const int *hi, hipStream_t s) {
+  bool uniform = op.sid[0] == op.sid[1] && op.sid[1] == op.sid[2];
+  switch (op.sid[0]) {
+#define CXP_AU(S)                                                              \
+  case S:                                                                      \
+    break;
+    CXP_ADJ_LINEAR_SIDS(CXP_AU)
+#undef CXP_AU
+  default:
+    uniform = false;
+  }
+  if (!uniform) {
+    for (int i = 0; i < nregions; ++i)
+      if (const int e = adjoint_launch(op, nullptr, crse_bar, crse_comp,
+                                       fine_bar, fine_comp, ncomp, lo + 3 * i,
+                                       hi + 3 * i, 0, s))
+        return e;
+    return CXP_OK;
+  }
+  auto meet = [&](const AdjTask &a, const AdjTask &b) {
+    for (int d = 0; d < 3; ++d)
+      if (a.ohi[d] < b.olo[d] || b.ohi[d] < a.olo[d])
+        return false;
+    return true;
+  };
+  int i = 0;
+  while (i < nregions) {
+    AdjRegions r{};
+    r.start[0] = 0;
+    r.n = 0;
+    while (i < nregions && r.n < CXP_ADJ_MAX_REGIONS) {
+      const AdjTask t = adj_task(op, crse_bar, crse_comp, fine_bar, fine_comp,
+                                 lo + 3 * i, hi + 3 * i);
+      bool apart = true;
+      for (int k = 0; k < r.n; ++k)
+        apart = apart && !meet(r.t[k], t);
+      if (!apart)
+        break;
+      int nkc;
+      r.t[r.n] = t;
+      adj_zf_grid(t, ncomp, r.ntx[r.n], r.nty[r.n], r.kchunk[r.n], nkc);
+      r.start[r.n + 1] = r.start[r.n] + r.ntx[r.n] * r.nty[r.n] * nkc;
+      ++r.n;
+      ++i;
+    }
+    const dim3 g((unsigned)r.start[r.n], ncomp);
+    switch (op.sid[0]) {
+#define CXP_AR(S)                                                              \
+  case S:                                                                      \
+    hipLaunchKernelGGL(k_adj_zf_regions<S>, g, dim3(AZ_NT), 0, s, r);          \
+    break;
+      CXP_ADJ_LINEAR_SIDS(CXP_AR)
+#undef CXP_AR
+    }
+    if (hipGetLastError() != hipSuccess)
+      return CXP_ERR_HIP;
+  }
+  return CXP_OK;
 }
 
 } // namespace cxp

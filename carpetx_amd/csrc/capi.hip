@@ -1883,6 +1883,28 @@ int cxp_prolongate_record(cxp_op op, const cxp_fab *crse, int crse_comp,
   return launch_point(t, ncomp, per_group, (hipStream_t)stream, branches);
 }
 
+int cxp_prolongate_adjoint_regions(cxp_op op, const cxp_fab *crse_bar,
+                                   int crse_comp, const cxp_fab *fine_bar,
+                                   int fine_comp, int ncomp, int nregions,
+                                   const int *region_lo, const int *region_hi,
+                                   int per_group, void *stream) {
+  if (nregions < 0 || (nregions > 0 && (!region_lo || !region_hi)))
+    return CXP_ERR_ARG;
+  for (int i = 0; i < nregions; ++i)
+    if (const int e = check_prolong(op, crse_bar, crse_comp, fine_bar,
+                                    fine_comp, ncomp, region_lo + 3 * i,
+                                    region_hi + 3 * i, per_group))
+      return e;
+  const OpDesc &od = op->d;
+  // linear operators only: the others are linearised at a forward input
+  if (op_has_minmod(od) || op_has_eno(od) || op_fallback_per_var(od) ||
+      op_fallback_per_group(od))
+    return CXP_ERR_UNSUPPORTED;
+  return adjoint_regions_launch(od, crse_bar, crse_comp, fine_bar, fine_comp,
+                                ncomp, nregions, region_lo, region_hi,
+                                (hipStream_t)stream);
+}
+
 int cxp_prolongate_adjoint_recorded(cxp_op op, const unsigned char *branches,
                                     const cxp_fab *crse_bar, int crse_comp,
                                     const cxp_fab *fine_bar, int fine_comp,

@@ -155,6 +155,19 @@ class Prolongate3DRF2:
             self.handle, ctypes.byref(c), crse_comp, ctypes.byref(f), fine_comp,
             ncomp, n, lo, hi, int(bool(per_group)), _stream(stream)))
 
+    def interp_adjoint_regions(self, crse_bar, crse_comp, fine_bar, fine_comp, ncomp,
+                               regions, per_group=False, stream=None):
+        """crse_bar += P^T fine_bar over several regions, for operators
+        without branches (cxp_prolongate_adjoint_regions; no reference
+        counterpart)."""
+        cb, fb = crse_bar.c(), fine_bar.c()
+        n = len(regions)
+        lo = (ctypes.c_int * (3 * max(n, 1)))(*[v for r in regions for v in r.lo])
+        hi = (ctypes.c_int * (3 * max(n, 1)))(*[v for r in regions for v in r.hi])
+        check("cxp_prolongate_adjoint_regions", _lib.lib().cxp_prolongate_adjoint_regions(
+            self.handle, ctypes.byref(cb), crse_comp, ctypes.byref(fb), fine_comp,
+            ncomp, n, lo, hi, int(bool(per_group)), _stream(stream)))
+
     def interp_face(self, crse, crse_comp, fine, fine_comp, ncomp, fine_region,
                     ratio=(2, 2, 2), solve_mask=None, crse_geom=None,
                     fine_geom=None, bcr=None, bccomp=0, gpu_or_cpu=None,

@@ -123,6 +123,22 @@ int cxp_prolongate_regions(cxp_op op, const cxp_fab *crse, int crse_comp,
                            int nregions, const int *region_lo,
                            const int *region_hi, int per_group, void *stream);
 
+/* cxp_prolongate_adjoint over several regions of one FAB pair (the
+ * transpose of cxp_prolongate_regions): crse_bar += P^T fine_bar for each
+ * region in turn, for operators without branches (POLY, HERMITE, CONS;
+ * CXP_ERR_UNSUPPORTED for MINMOD, ENO and linear-fallback operators, whose
+ * adjoints need the forward input or its branch record).  No reference
+ * counterpart.  Regions run in order; consecutive regions whose coarse
+ * output windows are apart share a launch when the operator has one
+ * stencil in every direction, so the sums are the region-by-region calls'
+ * bit for bit.  Errors: those of cxp_prolongate_adjoint for any region,
+ * CXP_ERR_ARG for a negative count or null arrays. */
+int cxp_prolongate_adjoint_regions(cxp_op op, const cxp_fab *crse_bar,
+                                   int crse_comp, const cxp_fab *fine_bar,
+                                   int fine_comp, int ncomp, int nregions,
+                                   const int *region_lo, const int *region_hi,
+                                   int per_group, void *stream);
+
 /* Restrict() -> AMReX average_down_nodal/_edges/_faces/average_down by the
  * number of cell-centred directions (CarpetX/src/schedule.cxx:2791-2816):
  * injection in VC directions, mean of the two children in CC directions,

@@ -123,3 +123,29 @@ def test_regions_in_one_call_equal_region_by_region(typ, order, it):
             P.interp_regions(crse, 0, got, 0, 2, regions)
             torch.cuda.synchronize()
             assert torch.equal(got.t, want.t), (typ, len(regions), tuned)
+
+
+@pytest.mark.parametrize("typ,order,it", [("interpolate", 5, (0, 0, 0)),
+                                          ("conservative", 1, (1, 1, 1)),
+                                          ("natural", 3, (1, 0, 1))])
+def test_adjoint_regions_equal_region_by_region(typ, order, it):
+    """cxp_prolongate_adjoint_regions over a ghost shell's six slabs (lo/hi
+    pairs apart: shared launches), over the shell of a box too thin for
+    that (windows that meet: one region after another) and over one whole
+    box gives the region-by-region adjoints' sums bit for bit."""
+    from carpetx_amd.workloads import shell_slabs
+    P = get_interpolator(typ, order, it)
+    rng = np.random.default_rng(17)
+    for interior in (Box((4, 2, 6), (43, 37, 29)), Box((2, 4, 0), (9, 11, 5))):
+        fbox = interior.grow(3)
+        cb = P.CoarseBox(fbox)
+        fbar = FArrayBox(fbox, 2, tensor=torch.from_numpy(
+            rng.uniform(-1, 1, (2,) + tuple(reversed(fbox.size)))).cuda())
+        for regions in (shell_slabs(fbox, interior), [interior]):
+            want = FArrayBox(cb, 2)
+            for r in regions:
+                P.interp_adjoint(want, 0, fbar, 0, 2, r)
+            got = FArrayBox(cb, 2)
+            P.interp_adjoint_regions(got, 0, fbar, 0, 2, regions)
+            torch.cuda.synchronize()
+            assert torch.equal(got.t, want.t), (typ, interior, len(regions))

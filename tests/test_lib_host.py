@@ -155,6 +155,37 @@ def test_prolongate_regions_checks_before_device_work():
     assert call([a, b, ((3, 0, 0), (2, 15, 15))]) == -3   # an empty region
 
 
+def test_adjoint_regions_checks_before_device_work():
+    """cxp_prolongate_adjoint_regions: a negative count or null arrays are
+    argument errors, every region meets cxp_prolongate_adjoint's checks,
+    and operators with branches are refused; no device call."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+
+    def fab(lo, hi):
+        f = _lib.CxpFab()
+        f.data = fake
+        f.lo, f.hi = i3(lo), i3(hi)
+        f.comp_stride = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+        f.ncomp = 1
+        return f
+    cbar, fbar = fab((-4, -4, -4), (20, 20, 20)), fab((0, 0, 0), (15, 15, 15))
+    lo = (ctypes.c_int * 3)(0, 0, 0)
+    hi = (ctypes.c_int * 3)(15, 15, 2)
+    bad = (ctypes.c_int * 3)(16, 15, 2)
+
+    def call(h, n=1, lo_=lo, hi_=hi):
+        return L.cxp_prolongate_adjoint_regions(h, ctypes.byref(cbar), 0, ctypes.byref(fbar),
+                                                0, 1, n, lo_, hi_, 0, None)
+    h = _lookup("natural", 3, (0, 0, 0))[1]
+    assert call(h, n=0) == 0
+    assert call(h, n=-1) == -3
+    assert call(h, lo_=None) == -3
+    assert call(h, hi_=bad) == -4
+    assert call(_lookup("minmod", 1, (1, 1, 1))[1]) == -11
+    assert call(_lookup("eno", 3, (1, 1, 1))[1]) == -11
+
+
 def test_adjoint_argument_checks_before_device_work():
     """ENO, fallback and MINMOD operators linearise at crse_in, so a call
     without it is rejected (CXP_ERR_ARG) before any device work."""
