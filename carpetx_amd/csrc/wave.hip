@@ -75,16 +75,24 @@ __global__ void __launch_bounds__(256)
 // distance 2 along an axis.  One thread per point q of y (ghosts
 // included; with periodic, the wrapped neighbours).  The pointwise parts
 // (base_bar, acc_in_bar) are done by the caller.
+//
+// A wave covers 64 consecutive x of one (y, z) row, from two points before
+// its 60 outputs to two after: each lane forms k_bar[rho] of its own point
+// once, and the x-direction taps take it from the lanes beside it; the y
+// and z taps read the gradients themselves.
+#define WA_OUT 60  // outputs per wave (lanes 2 .. 61)
+
 __global__ void __launch_bounds__(256)
     k_wave_stage_adjoint(int stage, const double *__restrict__ dst_bar,
                          const double *__restrict__ acc_bar,
                          double *__restrict__ y_bar, int nx, int ny, int nz,
                          int g, int periodic, double c, double a) {
-  const int qx = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int lane = threadIdx.x & 63;
+  const int qx = blockIdx.x * WA_OUT - 2 + lane;
   const int qy = blockIdx.y * 4 + (threadIdx.x >> 6);
   const int qz = blockIdx.z;
-  if (qx >= nx || qy >= ny || qz >= nz)
-    return;
+  if (qy >= ny)
+    return;  // whole waves: the shuffles below see only live lanes
   const long long sy = nx, sz = (long long)nx * ny, cs = sz * nz;
   const double w = stage == 1 ? 1.0 : 2.0;
   auto inside = [&](int x, int yy, int z) {
@@ -96,13 +104,27 @@ __global__ void __launch_bounds__(256)
     const long long p = m * cs + x + yy * sy + z * sz;
     return stage == 4 ? a * dst_bar[p] : a * dst_bar[p] + w * acc_bar[p];
   };
+  // this lane's point, wrapped (periodic) or as is; k_bar[rho] there, 0
+  // where it is not interior or lies beyond the reach of any output
+  int ox = qx;
+  bool reach = qx >= -2 && qx <= nx + 1;
+  if (periodic && reach)
+    ox = wrap(qx, nx);
+  const double kb_self =
+      (reach && inside(ox, qy, qz)) ? kbar(1, ox, qy, qz) : 0.0;
+  const int off[4] = {-2, -1, 1, 2};
+  const double wt[4] = {-1.0, 16.0, 16.0, -1.0};
+  double kx[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+    kx[m] = __shfl(kb_self, lane + off[m] < 0 ? 0 : (lane + off[m] > 63 ? 63 : lane + off[m]));
+  if (lane < 2 || lane > 61 || qx >= nx)
+    return;
   const long long q = qx + qy * sy + qz * sz;
   double su = 0.0;
   // the point itself
   if (inside(qx, qy, qz))
-    su += -90.0 * kbar(1, qx, qy, qz);
-  const int off[4] = {-2, -1, 1, 2};
-  const double wt[4] = {-1.0, 16.0, 16.0, -1.0};
+    su += -90.0 * kb_self;
 #pragma unroll
   for (int dir = 0; dir < 3; ++dir)
 #pragma unroll
@@ -123,7 +145,7 @@ __global__ void __launch_bounds__(256)
           z = wrap(z, nz);
       }
       if (inside(x, yy, z))
-        su += wt[m] * kbar(1, x, yy, z);
+        su += wt[m] * (dir == 0 ? kx[m] : kbar(1, x, yy, z));
     }
   y_bar[q] += c * su;
   if (inside(qx, qy, qz))
@@ -143,7 +165,7 @@ extern "C" int cxp_wave_rk4_stage_adjoint(int stage, const double *dst_bar,
       nx - 2 * ghosts < 1 || ny - 2 * ghosts < 1 || nz - 2 * ghosts < 1 ||
       (periodic && (nx < 3 || ny < 3 || nz < 3)))
     return CXP_ERR_ARG;
-  const dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
+  const dim3 grid((nx + WA_OUT - 1) / WA_OUT, (ny + 3) / 4, nz);
   hipLaunchKernelGGL(k_wave_stage_adjoint, grid, dim3(256), 0,
                      (hipStream_t)stream, stage, dst_bar, acc_bar, y_bar, nx,
                      ny, nz, ghosts, periodic, c, a);
