@@ -246,6 +246,8 @@ def bench_level_fill(args, stream, log):
             for _ in range(3):
                 g.launch(stream)
             mb = back_to_back(lambda: g.launch(stream), 50, stream)
+            # the chain launched directly (gather, fill: no graph)
+            md = back_to_back(lambda: wl.run(stream), 50, stream)
             wl.run_per_patch(stream)
             mp = back_to_back(lambda: wl.run_per_patch(stream), 10, stream)
             wl.poison(stream=stream)
@@ -254,8 +256,8 @@ def bench_level_fill(args, stream, log):
         out[name] = dict(
             patches=st["patches"], fine_pts=st["fine_points"],
             algo_MB=round(st["algo_bytes"] / 1e6, 2),
-            batched_ms=round(mb, 4), per_patch_ms=round(mp, 4),
-            store_floor_ms=round(mf, 4),
+            batched_ms=round(mb, 4), batched_direct_ms=round(md, 4),
+            per_patch_ms=round(mp, 4), store_floor_ms=round(mf, 4),
             batched_pts_per_s=st["fine_points"] / (mb * 1e-3),
             batched_hbm_frac=round(st["algo_bytes"] / (mb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
         if bc is None and not args.no_cpu:
@@ -297,6 +299,7 @@ def bench_regrid(args, stream, log):
                 for _ in range(3):
                     g.launch(stream)
                 mb = back_to_back(lambda: g.launch(stream), 30, stream)
+                md = back_to_back(lambda: wl.run(stream), 30, stream)
                 wl.run_per_patch(stream)
                 mp = back_to_back(lambda: wl.run_per_patch(stream), 5, stream)
             st = wl.stats()
@@ -304,7 +307,8 @@ def bench_regrid(args, stream, log):
             out[key] = dict(
                 patches=st["patches"], fine_pts=st["fine_points"],
                 algo_MB=round(st["algo_bytes"] / 1e6, 2),
-                batched_ms=round(mb, 4), per_patch_ms=round(mp, 4),
+                batched_ms=round(mb, 4), batched_direct_ms=round(md, 4),
+                per_patch_ms=round(mp, 4),
                 batched_pts_per_s=st["fine_points"] / (mb * 1e-3),
                 batched_hbm_frac=round(st["algo_bytes"] / (mb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
             if kind == "new_level" and not args.no_cpu:
