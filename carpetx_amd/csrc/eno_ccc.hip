@@ -36,20 +36,22 @@ struct EcArgs {
 // parent, parent-linear over [plo, phi] (else one per fine point of the
 // region).
 template <bool CODES, bool PARENT = false>
-__global__ void __launch_bounds__(EC_T)
+__global__ void __launch_bounds__(EcGeom<PARENT ? EC_PZ_CODES : EC_PZ>::T)
     k_eno_ccc(EcArgs a, unsigned char *__restrict__ codes) {
+  constexpr int PZ = PARENT ? EC_PZ_CODES : EC_PZ;
+  using Gm = EcGeom<PZ>;
   // With PARENT the window is dead once the |D2| arrays are built (no
   // children to compute), so Mall takes its place: less LDS, more
   // workgroups per compute unit.
-  __shared__ double lds[EC_NC + EC_ND + (PARENT ? 0 : EC_NM)];
-  static_assert(EC_NM <= EC_NC, "Mall fits in the window's place");
+  __shared__ double lds[Gm::NC + Gm::ND + (PARENT ? 0 : Gm::NM)];
+  static_assert(Gm::NM <= Gm::NC, "Mall fits in the window's place");
   double *const cw = lds;
-  double *const d2 = lds + EC_NC;
-  double *const mall = PARENT ? lds : lds + EC_NC + EC_ND;
+  double *const d2 = lds + Gm::NC;
+  double *const mall = PARENT ? lds : lds + Gm::NC + Gm::ND;
   const int tid = threadIdx.x;
   const int comp = blockIdx.y;
   int p0[3];
-  ec_tile(a.plo, xcd_block(blockIdx.x, gridDim.x), a.ntx, a.nty, p0);
+  ec_tile<PZ>(a.plo, xcd_block(blockIdx.x, gridDim.x), a.ntx, a.nty, p0);
   const double *cp = a.crse + comp * a.ccs;
   auto cv = [&](int x, int y, int z) {
     return cp[(x - a.c_lo[0]) + (y - a.c_lo[1]) * a.c_sy +
@@ -60,27 +62,27 @@ __global__ void __launch_bounds__(EC_T)
   auto ml = [&](int q) { return mall[q]; };
   // A
   {
-    double v[(EC_NC + EC_T - 1) / EC_T];
+    double v[(Gm::NC + Gm::T - 1) / Gm::T];
 #pragma unroll
-    for (int n = 0; n < (EC_NC + EC_T - 1) / EC_T; ++n) {
-      const int q = tid + EC_T * n;
-      v[n] = q < EC_NC ? ec_coarse(a.olo, a.ohi, p0, q, cv) : 0.0;
+    for (int n = 0; n < (Gm::NC + Gm::T - 1) / Gm::T; ++n) {
+      const int q = tid + Gm::T * n;
+      v[n] = q < Gm::NC ? ec_coarse(a.olo, a.ohi, p0, q, cv) : 0.0;
     }
 #pragma unroll
-    for (int n = 0; n < (EC_NC + EC_T - 1) / EC_T; ++n) {
-      const int q = tid + EC_T * n;
-      if (q < EC_NC)
+    for (int n = 0; n < (Gm::NC + Gm::T - 1) / Gm::T; ++n) {
+      const int q = tid + Gm::T * n;
+      if (q < Gm::NC)
         cw[q] = v[n];
     }
   }
   __syncthreads();
   // B
-  for (int e = tid; e < EC_ND; e += EC_T)
-    d2[e] = ec_d2(e, cl);
+  for (int e = tid; e < Gm::ND; e += Gm::T)
+    d2[e] = ec_d2<PZ>(e, cl);
   __syncthreads();
   // C
-  for (int q = tid; q < EC_NM; q += EC_T)
-    mall[q] = ec_mall(q, dl);
+  for (int q = tid; q < Gm::NM; q += Gm::T)
+    mall[q] = ec_mall<PZ>(q, dl);
   __syncthreads();
   // D, E: one parent per thread
   const int l[3] = {tid % EC_PX, (tid / EC_PX) % EC_PY, tid / (EC_PX * EC_PY)};
@@ -305,11 +307,16 @@ int eno_ccc_launch(const cxp_fab *crse, int crse_comp, const cxp_fab *fine,
                    hipStream_t s, bool parent_codes) {
   const EcArgs a = ec_args(crse, crse_comp, fine ? fine : crse,
                            fine ? fine_comp : 0, rlo, rhi);
+  if (codes_only && parent_codes) {
+    const int ntz = (a.phi[2] - a.plo[2] + EC_PZ_CODES) / EC_PZ_CODES;
+    const dim3 grid((unsigned)(a.ntx * a.nty * ntz), ncomp);
+    hipLaunchKernelGGL((k_eno_ccc<true, true>), grid,
+                       dim3(EcGeom<EC_PZ_CODES>::T), 0, s, a, codes);
+    return hipGetLastError() == hipSuccess ? CXP_OK : CXP_ERR_HIP;
+  }
   const int ntz = (a.phi[2] - a.plo[2] + EC_PZ) / EC_PZ;
   const dim3 grid((unsigned)(a.ntx * a.nty * ntz), ncomp);
-  if (codes_only && parent_codes)
-    hipLaunchKernelGGL((k_eno_ccc<true, true>), grid, dim3(EC_T), 0, s, a, codes);
-  else if (codes_only)
+  if (codes_only)
     hipLaunchKernelGGL(k_eno_ccc<true>, grid, dim3(EC_T), 0, s, a, codes);
   else
     hipLaunchKernelGGL(k_eno_ccc<false>, grid, dim3(EC_T), 0, s, a, codes);

@@ -9,11 +9,13 @@
 // (D2x the second undivided difference along x; My, Mz alike): with ENO in
 // every direction the stencil lines are those of the shifted centre for
 // either fine offset, so the score reads one coarse field, Mall, at p + s,
-// and all eight children of p take the same shift.  A workgroup of EC_T
-// threads owns 8 x 8 x 4 parents (one per thread):
-//   A  the coarse window, parents - 2 .. parents + 2 (12 x 12 x 8)   -> LDS
+// and all eight children of p take the same shift.  A workgroup owns
+// 8 x 8 x PZ parents, one per thread (PZ = 4 for the forward, 8 for the
+// codes-only kernel; EcGeom below):
+//   A  the coarse window, parents - 2 .. parents + 2 (12 x 12 x (PZ + 4))
+//                                                                    -> LDS
 //   B  |D2| along x, y, z where Mall needs them                      -> LDS
-//   C  Mall on parents - 1 .. parents + 1 (10 x 10 x 6)              -> LDS
+//   C  Mall on parents - 1 .. parents + 1 (10 x 10 x (PZ + 2))       -> LDS
 //   D  per parent: the scan over the 27 shifts in the reference's
 //      order (sz outermost, sx innermost; the first strictly smallest
 //      score wins, none finite leaves shift 0);
@@ -42,29 +44,42 @@
 
 namespace cxp {
 
-#define EC_T 256
 #define EC_PX 8
 #define EC_PY 8
-#define EC_PZ 4
 #define EC_CX (EC_PX + 4)  // coarse window
 #define EC_CY (EC_PY + 4)
-#define EC_CZ (EC_PZ + 4)
 #define EC_MX (EC_PX + 2)  // Mall window
 #define EC_MY (EC_PY + 2)
-#define EC_MZ (EC_PZ + 2)
-#define EC_NC (EC_CX * EC_CY * EC_CZ)              // 1152
-#define EC_NDX (EC_MX * EC_CY * EC_CZ)             // 960
-#define EC_NDY (EC_CX * EC_MY * EC_CZ)             // 960
-#define EC_NDZ (EC_CX * EC_CY * EC_MZ)             // 864
-#define EC_ND (EC_NDX + EC_NDY + EC_NDZ)
-#define EC_NM (EC_MX * EC_MY * EC_MZ)              // 600
+
+// A tile of 8 x 8 x PZ parents, one thread each: the forward takes PZ = 4
+// (its eight children per thread hold many registers), the codes-only
+// kernel PZ = 8 (less window per parent: 3.4 coarse values and 8.4 |D2|
+// entries instead of 4.5 and 10.9).
+template <int PZ> struct EcGeom {
+  static constexpr int T = EC_PX * EC_PY * PZ;     // threads
+  static constexpr int CZ = PZ + 4, MZ = PZ + 2;   // window, Mall planes
+  static constexpr int NC = EC_CX * EC_CY * CZ;    // window values
+  static constexpr int NDX = EC_MX * EC_CY * CZ;   // |D2| along x
+  static constexpr int NDY = EC_CX * EC_MY * CZ;   // along y
+  static constexpr int NDZ = EC_CX * EC_CY * MZ;   // along z
+  static constexpr int ND = NDX + NDY + NDZ;
+  static constexpr int NM = EC_MX * EC_MY * MZ;    // Mall values
+};
+#define EC_PZ 4        // the forward's tile depth
+#define EC_PZ_CODES 8  // the codes-only kernel's
+#define EC_T (EcGeom<EC_PZ>::T)
+#define EC_NC (EcGeom<EC_PZ>::NC)                  // 1152
+#define EC_NDX (EcGeom<EC_PZ>::NDX)                // 960
+#define EC_ND (EcGeom<EC_PZ>::ND)
+#define EC_NM (EcGeom<EC_PZ>::NM)                  // 600
 
 // Parent tile b of the parents [plo, phi]: origin p0.
+template <int PZ = EC_PZ>
 __host__ __device__ inline void ec_tile(const int plo[3], int b, int ntx,
                                         int nty, int p0[3]) {
   p0[0] = plo[0] + EC_PX * (b % ntx);
   p0[1] = plo[1] + EC_PY * ((b / ntx) % nty);
-  p0[2] = plo[2] + EC_PZ * (b / (ntx * nty));
+  p0[2] = plo[2] + PZ * (b / (ntx * nty));
 }
 
 __host__ __device__ inline int ec_cidx(int x, int y, int z) {
@@ -90,11 +105,12 @@ __host__ __device__ inline double ec_undiv2(double m, double c, double p) {
   return dd;
 }
 
-// B: |D2| entry e < EC_ND (x array, then y, then z), from the window.
-template <class CL>
+// B: |D2| entry e < ND (x array, then y, then z), from the window.
+template <int PZ = EC_PZ, class CL>
 __host__ __device__ inline double ec_d2(int e, const CL &cl) {
+  constexpr int NDX = EcGeom<PZ>::NDX, NDY = EcGeom<PZ>::NDY;
   int a, b, c, s;
-  if (e < EC_NDX) {  // (x', y', z') = window (x' + 1, y', z')
+  if (e < NDX) {  // (x', y', z') = window (x' + 1, y', z')
     a = e % EC_MX;
     b = (e / EC_MX) % EC_CY;
     c = e / (EC_MX * EC_CY);
@@ -102,14 +118,14 @@ __host__ __device__ inline double ec_d2(int e, const CL &cl) {
     const int q = ec_cidx(a, b, c);
     return fmax(0.0, fabs(ec_undiv2(cl(q), cl(q + s), cl(q + 2 * s))));
   }
-  if (e < EC_NDX + EC_NDY) {  // window (x', y' + 1, z')
-    e -= EC_NDX;
+  if (e < NDX + NDY) {  // window (x', y' + 1, z')
+    e -= NDX;
     a = e % EC_CX;
     b = (e / EC_CX) % EC_MY;
     c = e / (EC_CX * EC_MY);
     s = EC_CX;
   } else {  // window (x', y', z' + 1)
-    e -= EC_NDX + EC_NDY;
+    e -= NDX + NDY;
     a = e % EC_CX;
     b = (e / EC_CX) % EC_CY;
     c = e / (EC_CX * EC_CY);
@@ -121,8 +137,9 @@ __host__ __device__ inline double ec_d2(int e, const CL &cl) {
 
 // C: Mall at window point m = (mx, my, mz) of the Mall box (parents - 1
 // + m), from the |D2| arrays (dl(e) reads entry e).
-template <class DL>
+template <int PZ = EC_PZ, class DL>
 __host__ __device__ inline double ec_mall(int q, const DL &dl) {
+  constexpr int NDX = EcGeom<PZ>::NDX, NDY = EcGeom<PZ>::NDY;
   const int mx = q % EC_MX, my = (q / EC_MX) % EC_MY, mz = q / (EC_MX * EC_MY);
   double ddx = 0.0, ddy = 0.0, ddz = 0.0;
   for (int dk = -1; dk <= 1; ++dk)
@@ -130,11 +147,11 @@ __host__ __device__ inline double ec_mall(int q, const DL &dl) {
       ddx = fmax(ddx, dl(((mz + 1 + dk) * EC_CY + (my + 1 + dj)) * EC_MX + mx));
   for (int dk = -1; dk <= 1; ++dk)
     for (int di = -1; di <= 1; ++di)
-      ddy = fmax(ddy, dl(EC_NDX + ((mz + 1 + dk) * EC_MY + my) * EC_CX +
+      ddy = fmax(ddy, dl(NDX + ((mz + 1 + dk) * EC_MY + my) * EC_CX +
                          (mx + 1 + di)));
   for (int dj = -1; dj <= 1; ++dj)
     for (int di = -1; di <= 1; ++di)
-      ddz = fmax(ddz, dl(EC_NDX + EC_NDY + (mz * EC_CY + (my + 1 + dj)) * EC_CX +
+      ddz = fmax(ddz, dl(NDX + NDY + (mz * EC_CY + (my + 1 + dj)) * EC_CX +
                          (mx + 1 + di)));
   return fmax(fmax(ddx, ddy), ddz);
 }
