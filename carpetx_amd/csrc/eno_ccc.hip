@@ -13,6 +13,8 @@
 #include <map>
 #include <mutex>
 
+#include <type_traits>
+
 #include "eno_ccc.hpp"
 #include "eno_launch.hpp"
 #include "xcd.hpp"
@@ -40,14 +42,17 @@ __global__ void __launch_bounds__(EcGeom<PARENT ? EC_PZ_CODES : EC_PZ>::T)
     k_eno_ccc(EcArgs a, unsigned char *__restrict__ codes) {
   constexpr int PZ = PARENT ? EC_PZ_CODES : EC_PZ;
   using Gm = EcGeom<PZ>;
-  // With PARENT the window is dead once the |D2| arrays are built (no
-  // children to compute), so Mall takes its place: less LDS, more
-  // workgroups per compute unit.
-  __shared__ double lds[Gm::NC + Gm::ND + (PARENT ? 0 : Gm::NM)];
-  static_assert(Gm::NM <= Gm::NC, "Mall fits in the window's place");
+  // With PARENT (no children to compute) Mall is built one direction at a
+  // time through one |D2| array of the largest direction's size: 33 KB of
+  // LDS instead of 47.5 KB, four workgroups per compute unit instead of
+  // three.
+  constexpr int NDMAX = Gm::NDX > Gm::NDY
+                            ? (Gm::NDX > Gm::NDZ ? Gm::NDX : Gm::NDZ)
+                            : (Gm::NDY > Gm::NDZ ? Gm::NDY : Gm::NDZ);
+  __shared__ double lds[Gm::NC + (PARENT ? NDMAX : Gm::ND) + Gm::NM];
   double *const cw = lds;
   double *const d2 = lds + Gm::NC;
-  double *const mall = PARENT ? lds : lds + Gm::NC + Gm::ND;
+  double *const mall = lds + Gm::NC + (PARENT ? NDMAX : Gm::ND);
   const int tid = threadIdx.x;
   const int comp = blockIdx.y;
   int p0[3];
@@ -76,14 +81,33 @@ __global__ void __launch_bounds__(EcGeom<PARENT ? EC_PZ_CODES : EC_PZ>::T)
     }
   }
   __syncthreads();
-  // B
-  for (int e = tid; e < Gm::ND; e += Gm::T)
-    d2[e] = ec_d2<PZ>(e, cl);
-  __syncthreads();
-  // C
-  for (int q = tid; q < Gm::NM; q += Gm::T)
-    mall[q] = ec_mall<PZ>(q, dl);
-  __syncthreads();
+  if constexpr (PARENT) {
+    // B, C per direction: Mall = fmax(fmax(Mx, My), Mz), each thread
+    // updating its own Mall points (the chain ec_mall takes)
+    auto pass = [&](auto dirc, int off, int nd) {
+      constexpr int DIR = decltype(dirc)::value;
+      for (int e = tid; e < nd; e += Gm::T)
+        d2[e] = ec_d2<PZ>(off + e, cl);
+      __syncthreads();
+      for (int q = tid; q < Gm::NM; q += Gm::T) {
+        const double m = ec_mall_dir<DIR>(q, dl);
+        mall[q] = DIR == 0 ? m : fmax(mall[q], m);
+      }
+      __syncthreads();
+    };
+    pass(std::integral_constant<int, 0>{}, 0, Gm::NDX);
+    pass(std::integral_constant<int, 1>{}, Gm::NDX, Gm::NDY);
+    pass(std::integral_constant<int, 2>{}, Gm::NDX + Gm::NDY, Gm::NDZ);
+  } else {
+    // B
+    for (int e = tid; e < Gm::ND; e += Gm::T)
+      d2[e] = ec_d2<PZ>(e, cl);
+    __syncthreads();
+    // C
+    for (int q = tid; q < Gm::NM; q += Gm::T)
+      mall[q] = ec_mall<PZ>(q, dl);
+    __syncthreads();
+  }
   // D, E: one parent per thread
   const int l[3] = {tid % EC_PX, (tid / EC_PX) % EC_PY, tid / (EC_PX * EC_PY)};
   const int p[3] = {p0[0] + l[0], p0[1] + l[1], p0[2] + l[2]};
@@ -96,27 +120,31 @@ __global__ void __launch_bounds__(EcGeom<PARENT ? EC_PZ_CODES : EC_PZ>::T)
           (p[2] - a.plo[2]) * npx * npy] = (unsigned char)code;
     return;
   }
-  double v[2][2][2];
-  if constexpr (!CODES)
-    ec_children(l, code, cl, v);
   double *fp = a.fine + comp * a.fcs;
   unsigned char *cd = codes ? codes + comp * a.npts : nullptr;
   const long long rnx = a.rhi[0] - a.rlo[0] + 1;
   const long long rny = a.rhi[1] - a.rlo[1] + 1;
-  for (int oz = 0; oz < 2; ++oz)
-    for (int oy = 0; oy < 2; ++oy)
-      for (int ox = 0; ox < 2; ++ox) {
-        const int f[3] = {2 * p[0] + ox, 2 * p[1] + oy, 2 * p[2] + oz};
-        if (f[0] < a.rlo[0] || f[0] > a.rhi[0] || f[1] < a.rlo[1] ||
-            f[1] > a.rhi[1] || f[2] < a.rlo[2] || f[2] > a.rhi[2])
-          continue;
-        if constexpr (!CODES)
-          fp[(f[0] - a.f_lo[0]) + (f[1] - a.f_lo[1]) * a.f_sy +
-             (f[2] - a.f_lo[2]) * a.f_sz] = v[oz][oy][ox];
-        if (cd)
-          cd[(f[0] - a.rlo[0]) + (f[1] - a.rlo[1]) * rnx +
-             (f[2] - a.rlo[2]) * rnx * rny] = (unsigned char)code;
-      }
+  // child (ox, oy, oz): its value (unless CODES) and, given codes, its code
+  auto put = [&](int ox, int oy, int oz, double val) {
+    const int f[3] = {2 * p[0] + ox, 2 * p[1] + oy, 2 * p[2] + oz};
+    if (f[0] < a.rlo[0] || f[0] > a.rhi[0] || f[1] < a.rlo[1] ||
+        f[1] > a.rhi[1] || f[2] < a.rlo[2] || f[2] > a.rhi[2])
+      return;
+    if constexpr (!CODES)
+      fp[(f[0] - a.f_lo[0]) + (f[1] - a.f_lo[1]) * a.f_sy +
+         (f[2] - a.f_lo[2]) * a.f_sz] = val;
+    if (cd)
+      cd[(f[0] - a.rlo[0]) + (f[1] - a.rlo[1]) * rnx +
+         (f[2] - a.rlo[2]) * rnx * rny] = (unsigned char)code;
+  };
+  if constexpr (CODES) {
+    for (int oz = 0; oz < 2; ++oz)
+      for (int oy = 0; oy < 2; ++oy)
+        for (int ox = 0; ox < 2; ++ox)
+          put(ox, oy, oz, 0.0);
+  } else {
+    ec_children_each(l, code, cl, put);
+  }
 }
 
 // The tiled adjoint (eno_ccc.hpp, ea_*): crse_bar += J^T fine_bar over the

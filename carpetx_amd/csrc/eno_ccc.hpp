@@ -135,24 +135,35 @@ __host__ __device__ inline double ec_d2(int e, const CL &cl) {
   return fmax(0.0, fabs(ec_undiv2(cl(q), cl(q + s), cl(q + 2 * s))));
 }
 
-// C: Mall at window point m = (mx, my, mz) of the Mall box (parents - 1
-// + m), from the |D2| arrays (dl(e) reads entry e).
+// C: one direction's part of Mall at window point m = (mx, my, mz) of the
+// Mall box (parents - 1 + m): the fmax chain from 0 over the 3 x 3 |D2|
+// entries across that direction (dl(e) reads that direction's array, from
+// its own entry 0).
+template <int DIR, class DL>
+__host__ __device__ inline double ec_mall_dir(int q, const DL &dl) {
+  const int mx = q % EC_MX, my = (q / EC_MX) % EC_MY, mz = q / (EC_MX * EC_MY);
+  double dd = 0.0;
+  for (int a = -1; a <= 1; ++a)
+    for (int b = -1; b <= 1; ++b) {
+      if constexpr (DIR == 0)  // a = dk, b = dj
+        dd = fmax(dd, dl(((mz + 1 + a) * EC_CY + (my + 1 + b)) * EC_MX + mx));
+      else if constexpr (DIR == 1)  // a = dk, b = di
+        dd = fmax(dd, dl(((mz + 1 + a) * EC_MY + my) * EC_CX + (mx + 1 + b)));
+      else  // a = dj, b = di
+        dd = fmax(dd, dl((mz * EC_CY + (my + 1 + a)) * EC_CX + (mx + 1 + b)));
+    }
+  return dd;
+}
+
+// C: Mall at window point m from the three |D2| arrays (dl(e) reads entry
+// e of the x array, then y, then z): max(Mx, My, Mz).
 template <int PZ = EC_PZ, class DL>
 __host__ __device__ inline double ec_mall(int q, const DL &dl) {
   constexpr int NDX = EcGeom<PZ>::NDX, NDY = EcGeom<PZ>::NDY;
-  const int mx = q % EC_MX, my = (q / EC_MX) % EC_MY, mz = q / (EC_MX * EC_MY);
-  double ddx = 0.0, ddy = 0.0, ddz = 0.0;
-  for (int dk = -1; dk <= 1; ++dk)
-    for (int dj = -1; dj <= 1; ++dj)
-      ddx = fmax(ddx, dl(((mz + 1 + dk) * EC_CY + (my + 1 + dj)) * EC_MX + mx));
-  for (int dk = -1; dk <= 1; ++dk)
-    for (int di = -1; di <= 1; ++di)
-      ddy = fmax(ddy, dl(NDX + ((mz + 1 + dk) * EC_MY + my) * EC_CX +
-                         (mx + 1 + di)));
-  for (int dj = -1; dj <= 1; ++dj)
-    for (int di = -1; di <= 1; ++di)
-      ddz = fmax(ddz, dl(NDX + NDY + (mz * EC_CY + (my + 1 + dj)) * EC_CX +
-                         (mx + 1 + di)));
+  const double ddx = ec_mall_dir<0>(q, dl);
+  const double ddy = ec_mall_dir<1>(q, [&](int e) { return dl(NDX + e); });
+  const double ddz =
+      ec_mall_dir<2>(q, [&](int e) { return dl(NDX + NDY + e); });
   return fmax(fmax(ddx, ddy), ddz);
 }
 
@@ -178,31 +189,40 @@ __host__ __device__ inline int ec_search(const int l[3], const ML &ml) {
   return code;
 }
 
-// E: the eight children of local parent l with shift code `code`:
-// v[oz][oy][ox] (cl(q) reads the window).
-template <class CL>
-__host__ __device__ inline void ec_children(const int l[3], int code,
-                                            const CL &cl, double (&v)[2][2][2]) {
+// E: the eight children of local parent l with shift code `code`, handed
+// to out(ox, oy, oz, value) one x offset at a time (cl(q) reads the
+// window), so only one offset's x-pass results are live.
+template <class CL, class OUT>
+__host__ __device__ inline void ec_children_each(const int l[3], int code,
+                                                 const CL &cl, const OUT &out) {
   const int s[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
-  double X[2][3][3];  // [ox][dk - sz + 1][dj - sy + 1]
-  for (int ox = 0; ox < 2; ++ox)
+  for (int ox = 0; ox < 2; ++ox) {
+    double X[3][3];  // [dk - sz + 1][dj - sy + 1]
     for (int kk = 0; kk < 3; ++kk)
       for (int jj = 0; jj < 3; ++jj) {
         const int row = ec_cidx(l[0] + 2, l[1] + 2 + s[1] - 1 + jj,
                                 l[2] + 2 + s[2] - 1 + kk);
-        X[ox][kk][jj] =
-            eno2_apply([&](int o) { return cl(row + o); }, s[0], ox);
+        X[kk][jj] = eno2_apply([&](int o) { return cl(row + o); }, s[0], ox);
       }
-  for (int ox = 0; ox < 2; ++ox)
     for (int oy = 0; oy < 2; ++oy) {
       double Y[3];
       for (int kk = 0; kk < 3; ++kk)
-        Y[kk] = eno2_apply([&](int o) { return X[ox][kk][o - s[1] + 1]; }, s[1],
+        Y[kk] = eno2_apply([&](int o) { return X[kk][o - s[1] + 1]; }, s[1],
                            oy);
       for (int oz = 0; oz < 2; ++oz)
-        v[oz][oy][ox] =
-            eno2_apply([&](int o) { return Y[o - s[2] + 1]; }, s[2], oz);
+        out(ox, oy, oz,
+            eno2_apply([&](int o) { return Y[o - s[2] + 1]; }, s[2], oz));
     }
+  }
+}
+
+// E: the eight children as v[oz][oy][ox].
+template <class CL>
+__host__ __device__ inline void ec_children(const int l[3], int code,
+                                            const CL &cl, double (&v)[2][2][2]) {
+  ec_children_each(l, code, cl, [&](int ox, int oy, int oz, double val) {
+    v[oz][oy][ox] = val;
+  });
 }
 
 // Weight of coarse offset s + t (t in [-1, 1]) in eno2_apply(., s, o).
