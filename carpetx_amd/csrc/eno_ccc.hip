@@ -12,7 +12,6 @@
 
 #include <map>
 #include <mutex>
-
 #include <type_traits>
 
 #include "eno_ccc.hpp"
